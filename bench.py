@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Closed-loop benchmark of the sparse LIF network on one MI355X.
+
+Workload: 100,000 LIF voltage-jump neurons (80% excitatory), fixed in-degree 1,000
+(1e8 synapses, CSR, excitatory +0.1 mV / inhibitory -0.5 mV), integer delays
+U{1..20} in a delay ring, tau_m 20 ms, V_th -50, V_reset -65, refractory 2 ms,
+RK4 with dt 0.1 ms, Poisson background of 1,000 sources x 10 Hz per neuron (0.1 mV).
+A step is one simulated 0.1 ms tick of the whole network; inputs are resident in
+HBM before the timed region. Prints one JSON line.
+
+  value      neuron-steps/s (N x steps / time), with syn-events/s beside it
+  roofline   the dominant kernel (the persistent step kernel) timed with HIP events
+             on its own stream; algorithmic bytes = 24 B per neuron-step + 8 B per
+             synaptic event. A second entry covers replay delivery (injected spikes of
+             64 steps per launch, 8 B per event + 4 B per output arrival).
+  cpu_baseline  the CPU oracle (reference-faithful per-event float delivery), one core,
+             on the first --cpu-steps steps of the same network
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+STATE_BYTES = 24   # per neuron-step: read V, refractory counter, ring slot; write V, counter; clear slot
+EVENT_BYTES = 8    # per synaptic event: packed index/delay (4 B) + weight (4 B)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, steps: int) -> dict:
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import oracle as O
+    from harness import OracleRun
+    run = OracleRun(wl)
+    t0 = time.perf_counter()
+    run.run(steps, O.ACC_FLOAT_EVENTS, ring=False, raster=False)
+    dt = time.perf_counter() - t0
+    n = wl.spec.n
+    return {"value": n * steps / dt, "unit": "neuron-steps/s", "cores": 1, "kind": "port",
+            "syn_events_per_s": float(run.events[0]) / dt,
+            "sample": f"CPU oracle, reference-faithful per-event float delivery, steps 0..{steps - 1} of the "
+                      f"same {n}-neuron / {wl.syn.count}-synapse network, 1 thread, {dt:.1f} s"}
+
+
+def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd.hiptime import DeviceBuffer, Event
+    K, n = raster.shape[0], wl.spec.n
+    bits = np.unpackbits(raster.view(np.uint8), axis=1, bitorder="little")[:, :n]
+    spikes = [np.nonzero(b)[0].astype(np.int32) for b in bits]
+    tp = np.zeros(K + 1, np.int32)
+    tp[1:] = np.cumsum([s.size for s in spikes])
+    ids = np.concatenate(spikes).astype(np.int32) if tp[-1] else np.zeros(1, np.int32)
+    d_ids, d_tp, d_out = DeviceBuffer(ids.nbytes), DeviceBuffer(tp.nbytes), DeviceBuffer(K * n * 4)
+    d_ids.upload(ids)
+    d_tp.upload(tp)
+    deg = np.diff(wl.syn.row_ptr)
+    events = int(deg[ids[:tp[-1]]].sum())
+    stream = net.stream
+    L.call("nbx_replay_deliver_device", net.h, d_ids.ptr, d_tp.ptr, K, d_out.ptr)  # warm
+    net.sync()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        L.call("nbx_replay_deliver_device", net.h, d_ids.ptr, d_tp.ptr, K, d_out.ptr)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    alg = events * EVENT_BYTES + K * n * 4
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"kernel": "nbx_replay_blocked", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "steps_per_launch": K, "events_per_launch": events, "ms_per_launch": round(ms, 4),
+            "syn_events_per_s": events / (ms * 1e-3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "graph", "kernels"])
+    ap.add_argument("--integrator", default="rk4", choices=["rk4", "euler"])
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--replay-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-replay", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path runs on a single "
+              "MI355X; multi-GPU sharding is described in the design notes, not benchmarked)", file=sys.stderr)
+        return 2
+    import neuronbridge_amd as nb
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import Event
+
+    integ = nb.RK4 if args.integrator == "rk4" else nb.EULER
+    t0 = time.perf_counter()
+    wl = W.sparse_lif(integrator=integ, steps=args.warmup + args.steps)
+    log(f"built {wl.spec.n} neurons / {wl.syn.count} synapses in {time.perf_counter() - t0:.1f} s")
+    net = wl.make_net()
+    engines = {"auto": L.ENGINE_AUTO, "persistent": L.ENGINE_PERSISTENT, "graph": L.ENGINE_GRAPH,
+               "kernels": L.ENGINE_KERNELS}
+    engine = engines[args.engine]
+    if engine == L.ENGINE_AUTO:
+        engine = net.pick_engine()
+    ename = {v: k for k, v in engines.items()}[engine]
+    log(f"engine {ename}, fixed-point shift {net.fixed_shift}")
+    net.run(args.warmup, engine)
+    ev0 = net.events()
+    log(f"warmup {args.warmup} steps done")
+    stream = net.stream
+    e0, e1 = Event(), Event()
+    net.sync()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    net.run_async(args.steps, engine)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - w0
+    net.sync()
+    ms = e0.elapsed_ms(e1)
+    events = net.events() - ev0
+    n = wl.spec.n
+    secs = ms * 1e-3
+    nsteps_s = n * args.steps / secs
+    log(f"timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms), {events} events")
+    alg = STATE_BYTES * n * args.steps + EVENT_BYTES * events
+    ach = alg / secs / 1e9
+    kname = "nbx_persistent" if engine == L.ENGINE_PERSISTENT else "step graph"
+    out = {
+        "metric": "neuron-steps/s & syn-events/s, 1 MI355X; batched+adjoint vs plain at tol",
+        "value": nsteps_s, "unit": "neuron-steps/s",
+        "syn_events_per_s": events / secs,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded fixed in-degree topology, Philox Poisson background)",
+        "config": {"workload": f"sparse LIF 1e5 neurons, in-degree 1000, delays U{{1..20}}, "
+                               f"{args.integrator} dt 0.1 ms", "neurons": n, "synapses": int(wl.syn.count),
+                   "engine": ename, "fixed_shift": net.fixed_shift, "parallelism": "single GPU"},
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launch_ms": round(ms, 3), "algorithmic_bytes": alg},
+    }
+    if not args.no_replay:
+        K = 64
+        t_rec = net.time
+        net.record(t_rec, K, counts=True, raster=True)
+        net.run(K, engine)
+        out["replay_roofline"] = replay_roofline(net, wl, net.raster(), args.replay_reps)
+        log(f"replay: {out['replay_roofline']}")
+    if not args.no_cpu and args.cpu_steps > 0:
+        log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,220 @@
+/*
+ * nbx_core.h — C ABI of the MI355X (gfx950) simulation core for NeuronBridge's
+ * per-timestep hot path: neuron update (LIF voltage-jump, LIF exponential-decay,
+ * Izhikevich; forward Euler or RK4), spike delivery (CSR fan-out into an integer
+ * delay ring, pull-at-arrival delivery from a firing-history ring, dense W·s),
+ * and lazy pair STDP.
+ *
+ * Plain C types only: host pointers for uploads/downloads, opaque handles, int
+ * status codes, and a thread-local last-error string. Every call is synchronous
+ * on return unless its comment says otherwise, which is the contract the
+ * reference's callers rely on (every dense-runtime call ends with a device sync,
+ * src/native/gpu/src/gpu_runtime/GpuPropagationRuntime.cu:1687-1827).
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   G/ = src/native/gpu/include/, L/ = src/native/legacy/source_file_realtime_v1_async/
+ *   npgr::GpuPropagationRuntime           G/gpu_runtime/GpuPropagationRuntime.h:79-144
+ *   npgr::GpuPropagationLayout            G/gpu_runtime/GpuPropagationLayout.h:20-70
+ *   npgr::DenseUnifiedNeuronRuntime       G/dense_subnetwork/DenseUnifiedNeuronRuntime.h:15-41
+ *   npgr::DeviceCommonBuffersView         G/dense_subnetwork/DenseNeuronDeviceViews.h:6-13
+ *   legacy GPU neuron models (the *_GPU_Interface classes in L/NeuralModel/inc/TimeDrivenGPU/)
+ *                                         (UpdateState/ProcessSpike, NeuronModel.h:66-149)
+ */
+#ifndef NBX_CORE_H
+#define NBX_CORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define NBX_OK 0
+#define NBX_ERR_INVALID (-1)     /* bad argument or layout (reference: false + *reason) */
+#define NBX_ERR_HIP (-2)         /* HIP runtime error */
+#define NBX_ERR_UNSUPPORTED (-3) /* valid request this build does not implement */
+#define NBX_ERR_STATE (-4)       /* call out of order, or a device-side error flag */
+
+/* Message of the last failing call on this thread (reference: std::string* reason). */
+const char* nbx_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int32_t nbx_version(void);
+/* Number of visible HIP devices (0 when none). */
+int32_t nbx_device_count(void);
+
+/* ------------------------------------------------------------------ models */
+enum {
+    NBX_MODEL_LIF_VJ = 0,  /* TimeDrivenLIF_Voltage_jump (TimeDrivenLIF_Voltage_jump.cpp:49-115) */
+    NBX_MODEL_LIF_EXP = 1, /* TimeDrivenLIF_Exponential_Decay (.cpp:62-121) */
+    NBX_MODEL_IZH = 2,     /* TimeDrivenIzhikevic_Exponential_Decay, AMPA/GABA conductances (.cpp:59-131) */
+    NBX_MODEL_IZH_CUR = 3  /* Izhikevich with current-based synapses (signed weights, one-tick current) */
+};
+enum { NBX_EULER = 0, NBX_RK4 = 1 };
+
+/* Neuron population description (reference: DenseNeuronModelSpec / legacy model parameters). */
+typedef struct nbx_model_desc {
+    int32_t model;       /* NBX_MODEL_* */
+    int32_t integrator;  /* NBX_EULER (the reference's only integrator) or NBX_RK4 */
+    int32_t n;           /* neurons */
+    int32_t t_ref;       /* LIF refractory period in steps (reference t_ref, int) */
+    float dt;            /* step in ms */
+    float v_rest, tau, v_th, r, v_reset, g_tau, e_rev;             /* LIF, model-wide */
+    float izh_v_th, izh_r, e_ampa, e_gaba, ampa_tau, gaba_tau;     /* Izhikevich, model-wide */
+    const float *izh_a, *izh_b, *izh_c, *izh_d;                    /* Izhikevich, per neuron (host) */
+} nbx_model_desc;
+
+/* Synapses as CSR by presynaptic neuron, each row sorted by (delay, post) —
+ * the reference's SynapseArrays + PreDelaySlices (GpuPropagationLayout.h:20-49)
+ * folded into one ordering. Host pointers; copied at nbx_net_set_sparse. */
+typedef struct nbx_sparse_desc {
+    int64_t count;           /* synapses */
+    int32_t max_delay;       /* steps; delay ring has max_delay + 1 slots */
+    const int64_t* row_ptr;  /* n + 1 */
+    const int32_t* post;     /* count */
+    const uint8_t* delay;    /* count, 1 .. max_delay */
+    const uint8_t* type;     /* count, 0 excitatory, 1 inhibitory (reference Interconnections::type) */
+    const float* weight;     /* count; magnitudes (signed for NBX_MODEL_IZH_CUR) */
+    const uint8_t* plastic;  /* count or NULL; 1 = STDP synapse */
+} nbx_sparse_desc;
+
+/* Pair STDP parameters (reference STDP.h:14-20, STDP.cpp:19-61). */
+typedef struct nbx_stdp_desc {
+    float max_ltp, ltp_tau, max_ltd, ltd_tau, w_max;
+} nbx_stdp_desc;
+
+/* Engines for nbx_net_run. */
+enum {
+    NBX_ENGINE_KERNELS = 0,    /* one launch per phase per step */
+    NBX_ENGINE_GRAPH = 1,      /* the same launches captured in a hipGraph of many steps */
+    NBX_ENGINE_PERSISTENT = 2, /* one cooperative launch; post-blocked delivery into an LDS ring */
+    NBX_ENGINE_AUTO = 3        /* persistent when the network fits it, else graph */
+};
+
+typedef struct nbx_net nbx_net;
+
+/* Create a network on `device` (reference: DenseUnifiedNeuronRuntime::Initialize +
+ * AllocateDeviceBuffers, legacy InitStateVector). State starts like
+ * Neuron_State_Vector::InitNeuronState with sigma 0 (LastSpike = 10000). */
+int32_t nbx_net_create(const nbx_model_desc* model, int32_t device, nbx_net** out);
+int32_t nbx_net_destroy(nbx_net* net);
+
+/* Sparse synapses (reference: GpuPropagationRuntime::Initialize(layout)). */
+int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn);
+/* Dense all-to-all weights wt[pre * n + post] with one delay (FP32, host). */
+int32_t nbx_net_set_dense(nbx_net* net, const float* wt, int32_t delay);
+/* Poisson background: per (neuron, step) count ~ Poisson(lambda) of excitatory
+ * arrivals of weight w_bg, drawn from Philox4x32-10 keyed by the seed. */
+int32_t nbx_net_set_poisson(nbx_net* net, uint32_t seed_lo, uint32_t seed_hi, double lambda, float w_bg);
+/* Enable pair STDP on the synapses flagged plastic. */
+int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* stdp);
+/* External current per step: cur[(t - t_begin) * n + i] replaces I_ext at step t
+ * (reference: ProcessCurrent / InterfaceChannelValue with overwrite). Host array, copied. */
+int32_t nbx_net_set_current(nbx_net* net, const float* cur, int32_t t_begin, int32_t nsteps);
+/* Arrivals accumulate as int32 in units of 2^-shift; 0 picks the largest shift
+ * that cannot overflow for this network. Must precede nbx_net_finalize. */
+int32_t nbx_net_set_fixed_shift(nbx_net* net, int32_t shift);
+int32_t nbx_net_get_fixed_shift(const nbx_net* net, int32_t* shift);
+/* Upload and build device layouts; required before stepping. */
+int32_t nbx_net_finalize(nbx_net* net);
+
+/* State in the reference's AoS order x[i * nstate + j] (nstate 2 / 3 / 6 per model)
+ * plus LastSpike. */
+int32_t nbx_net_nstate(const nbx_net* net);
+int32_t nbx_net_set_state(nbx_net* net, const float* x, const int32_t* last);
+int32_t nbx_net_get_state(nbx_net* net, float* x, int32_t* last);
+
+/* Record per-step spike counts and/or a spike raster (bit i of word
+ * [(t - t_begin) * ceil(n/32) + i/32]) for steps [t_begin, t_begin + nsteps). */
+enum { NBX_RECORD_COUNTS = 1, NBX_RECORD_RASTER = 2 };
+int32_t nbx_net_record(nbx_net* net, int32_t t_begin, int32_t nsteps, int32_t flags);
+int32_t nbx_net_get_counts(nbx_net* net, int32_t* counts, int32_t nsteps);
+int32_t nbx_net_get_raster(nbx_net* net, uint32_t* raster, int64_t nwords);
+
+/* Advance the network by nsteps closed-loop steps (reference: Simulation::RunSimulationStep
+ * driving DenseSubnetworkRuntimeGpu::RunStep once per step). */
+int32_t nbx_net_run(nbx_net* net, int32_t nsteps, int32_t engine);
+/* Same, but returns once the work is queued on the network's stream (graph-capturable
+ * engines only); nbx_net_sync waits for it. */
+int32_t nbx_net_run_async(nbx_net* net, int32_t nsteps, int32_t engine);
+int32_t nbx_net_sync(nbx_net* net);
+/* Engine nbx_net_run would use for NBX_ENGINE_AUTO. */
+int32_t nbx_net_pick_engine(const nbx_net* net, int32_t* engine);
+/* Current step (the next step to be simulated). */
+int32_t nbx_net_time(const nbx_net* net, int32_t* t);
+/* Synaptic events delivered so far (one per synapse per presynaptic spike). */
+int32_t nbx_net_events(nbx_net* net, int64_t* events);
+
+/* ---------------- per-phase calls (the reference runtime's step sequence) ----------------
+ * GpuPropagationRuntime::BeginStep / RunPropagationOnly / CommitCurrentDeviceFiringsFromExternal /
+ * ApplyPostLearningFromDeviceFirings and DenseUnifiedNeuronRuntime::Step, in the order
+ * DenseSubnetworkRuntimeGpu::RunStep calls them (DenseSubnetworkRuntimeGpu.cpp:625-663). */
+int32_t nbx_net_begin_step(nbx_net* net, int32_t t);
+/* Deliver arrivals due at the current step (pull delivery and pre-side STDP). */
+int32_t nbx_net_run_propagation(nbx_net* net);
+/* Neuron update of the current step; optionally copies out the firing ids (ascending)
+ * and count (reference: DenseUnifiedNeuronRuntime::Step output_firing_ids). */
+int32_t nbx_net_neuron_step(nbx_net* net, int32_t* ids_out, int32_t ids_cap, int32_t* count_out);
+/* Post-side STDP for the neurons that fired this step. */
+int32_t nbx_net_apply_post_learning(nbx_net* net);
+/* Push this step's firings into the delay ring and close the step. */
+int32_t nbx_net_commit_firings(nbx_net* net);
+/* Inject spikes as if `ids` fired at the current step (reference: QueueInterfaceSpike /
+ * AddExternalSpikeActivity); call between nbx_net_neuron_step and nbx_net_commit_firings. */
+int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count);
+
+/* Delay ring contents, int32 fixed point ring[slot][channel][n] (slot = step % slots). */
+int32_t nbx_net_ring_shape(const nbx_net* net, int32_t* slots, int32_t* channels);
+int32_t nbx_net_get_ring(nbx_net* net, int32_t* ring, int64_t count);
+/* Spike list of step t if still held in the firing-history ring
+ * (reference: ExportFiringHistoryRing). Returns count in *count_out. */
+int32_t nbx_net_get_history(nbx_net* net, int32_t t, int32_t* ids, int32_t cap, int32_t* count_out);
+
+/* Pending-input view for a foreign neuron kernel (reference: GetDeviceCommonBuffersView).
+ * d_pending is int32 fixed point, channel-major d_pending[ch * stride + i] for the current step. */
+typedef struct nbx_pending_view {
+    int32_t* d_pending;
+    int32_t channel_count;
+    int32_t channel_stride;
+    int32_t neuron_count;
+    float inv_scale;
+} nbx_pending_view;
+int32_t nbx_net_pending_view(nbx_net* net, nbx_pending_view* view);
+
+/* Weights and STDP state (reference: Download/UploadSynapticWeights, Get/SetSynapseWeight). */
+int32_t nbx_net_get_weights(nbx_net* net, float* w, int64_t count);
+int32_t nbx_net_set_weights(nbx_net* net, const float* w, int64_t count);
+int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t* last_update, int64_t count);
+
+/* ---------------- replay delivery (multi-step, injected spikes) ----------------
+ * Delivers the spikes of steps [t0, t0 + nsteps) given as CSR lists
+ * (ids[tick_ptr[k] .. tick_ptr[k+1]) fired at step t0 + k) through the sparse
+ * synapses, and writes per-step arrivals out[k][channel][n] (int32 fixed point)
+ * for the steps [t0 + 1, t0 + nsteps] — what a closed-loop run would read from the
+ * ring, given only these spikes. Host arrays. */
+int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t nsteps,
+                           int32_t* out);
+/* Device-resident variant for timing: all pointers are device pointers and the
+ * call only enqueues the launch on the network's stream. */
+int32_t nbx_replay_deliver_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps,
+                                  int32_t* d_out);
+
+/* ---------------- host-side network builder ----------------
+ * Fixed in-degree random topology: each post neuron j draws k_in distinct presynaptic
+ * neurons (Floyd's sampling; Philox4x32-10 with counter (j, m, "TOPO", 0) and key
+ * (seed_lo, seed_hi)) and a delay in [d_min, d_max] per synapse. Output is the CSR by
+ * presynaptic neuron that nbx_net_set_sparse takes, rows sorted by (delay, post);
+ * type = 1 for presynaptic ids >= n_exc. Caller allocates row_ptr[n+1] and
+ * post/delay/type[n*k_in]. Host only (no device needed). Replaces the per-source,
+ * per-delay connection runs of Network::CaculateOutputConnection (Network.cu:483-560). */
+int32_t nbx_build_fixed_indegree(int32_t n, int32_t k_in, int32_t d_min, int32_t d_max, uint32_t seed_lo,
+                                 uint32_t seed_hi, int32_t n_exc, int32_t nthreads, int64_t* row_ptr, int32_t* post,
+                                 uint8_t* delay, uint8_t* type);
+
+/* HIP stream handle (hipStream_t) the network runs on, for event timing by callers. */
+void* nbx_net_stream(nbx_net* net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+"""ctypes binding of libnbx.so (the C ABI in include/nbx_core.h).
+
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libnbx.so")
+
+NBX_OK = 0
+NBX_ERR_INVALID = -1
+NBX_ERR_HIP = -2
+NBX_ERR_UNSUPPORTED = -3
+NBX_ERR_STATE = -4
+
+LIF_VJ, LIF_EXP, IZH, IZH_CUR = 0, 1, 2, 3
+EULER, RK4 = 0, 1
+ENGINE_KERNELS, ENGINE_GRAPH, ENGINE_PERSISTENT, ENGINE_AUTO = 0, 1, 2, 3
+RECORD_COUNTS, RECORD_RASTER = 1, 2
+
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+
+class NbxError(RuntimeError):
+    """A libnbx call returned a non-zero status (message from nbx_last_error)."""
+
+    def __init__(self, fn: str, status: int, message: str):
+        super().__init__(f"{fn} failed with status {status}: {message}")
+        self.status = status
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("integrator", C.c_int32), ("n", C.c_int32), ("t_ref", C.c_int32),
+        ("dt", C.c_float),
+        ("v_rest", C.c_float), ("tau", C.c_float), ("v_th", C.c_float), ("r", C.c_float),
+        ("v_reset", C.c_float), ("g_tau", C.c_float), ("e_rev", C.c_float),
+        ("izh_v_th", C.c_float), ("izh_r", C.c_float), ("e_ampa", C.c_float), ("e_gaba", C.c_float),
+        ("ampa_tau", C.c_float), ("gaba_tau", C.c_float),
+        ("izh_a", _f32p), ("izh_b", _f32p), ("izh_c", _f32p), ("izh_d", _f32p),
+    ]
+
+
+class SparseDesc(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64), ("max_delay", C.c_int32),
+        ("row_ptr", _i64p), ("post", _i32p), ("delay", _u8p), ("type", _u8p), ("weight", _f32p),
+        ("plastic", _u8p),
+    ]
+
+
+class StdpDesc(C.Structure):
+    _fields_ = [("max_ltp", C.c_float), ("ltp_tau", C.c_float), ("max_ltd", C.c_float),
+                ("ltd_tau", C.c_float), ("w_max", C.c_float)]
+
+
+class PendingView(C.Structure):
+    _fields_ = [("d_pending", _vp), ("channel_count", C.c_int32), ("channel_stride", C.c_int32),
+                ("neuron_count", C.c_int32), ("inv_scale", C.c_float)]
+
+
+_NET = _vp
+# name -> argtypes (all return int32 status unless listed in _RESTYPES)
+SIGNATURES = {
+    "nbx_last_error": [],
+    "nbx_version": [],
+    "nbx_device_count": [],
+    "nbx_net_create": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(_vp)],
+    "nbx_net_destroy": [_NET],
+    "nbx_net_set_sparse": [_NET, C.POINTER(SparseDesc)],
+    "nbx_net_set_dense": [_NET, _f32p, C.c_int32],
+    "nbx_net_set_poisson": [_NET, C.c_uint32, C.c_uint32, C.c_double, C.c_float],
+    "nbx_net_set_stdp": [_NET, C.POINTER(StdpDesc)],
+    "nbx_net_set_current": [_NET, _f32p, C.c_int32, C.c_int32],
+    "nbx_net_set_fixed_shift": [_NET, C.c_int32],
+    "nbx_net_get_fixed_shift": [_NET, _i32p],
+    "nbx_net_finalize": [_NET],
+    "nbx_net_nstate": [_NET],
+    "nbx_net_set_state": [_NET, _f32p, _i32p],
+    "nbx_net_get_state": [_NET, _f32p, _i32p],
+    "nbx_net_record": [_NET, C.c_int32, C.c_int32, C.c_int32],
+    "nbx_net_get_counts": [_NET, _i32p, C.c_int32],
+    "nbx_net_get_raster": [_NET, _u32p, C.c_int64],
+    "nbx_net_run": [_NET, C.c_int32, C.c_int32],
+    "nbx_net_run_async": [_NET, C.c_int32, C.c_int32],
+    "nbx_net_sync": [_NET],
+    "nbx_net_pick_engine": [_NET, _i32p],
+    "nbx_net_time": [_NET, _i32p],
+    "nbx_net_events": [_NET, _i64p],
+    "nbx_net_begin_step": [_NET, C.c_int32],
+    "nbx_net_run_propagation": [_NET],
+    "nbx_net_neuron_step": [_NET, _i32p, C.c_int32, _i32p],
+    "nbx_net_apply_post_learning": [_NET],
+    "nbx_net_commit_firings": [_NET],
+    "nbx_net_inject_spikes": [_NET, _i32p, C.c_int32],
+    "nbx_net_ring_shape": [_NET, _i32p, _i32p],
+    "nbx_net_get_ring": [_NET, _i32p, C.c_int64],
+    "nbx_net_get_history": [_NET, C.c_int32, _i32p, C.c_int32, _i32p],
+    "nbx_net_pending_view": [_NET, C.POINTER(PendingView)],
+    "nbx_net_get_weights": [_NET, _f32p, C.c_int64],
+    "nbx_net_set_weights": [_NET, _f32p, C.c_int64],
+    "nbx_net_get_stdp_state": [_NET, _f32p, _f32p, _i32p, C.c_int64],
+    "nbx_replay_deliver": [_NET, _i32p, _i32p, C.c_int32, C.c_int32, _i32p],
+    "nbx_replay_deliver_device": [_NET, _vp, _vp, C.c_int32, _vp],
+    "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
+                                 C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
+    "nbx_net_stream": [_NET],
+}
+_RESTYPES = {"nbx_last_error": C.c_char_p, "nbx_net_stream": _vp}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libnbx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, C.c_int32)
+        _lib = L
+    return _lib
+
+
+def call(name: str, *args) -> int:
+    """Call a status-returning entry point; raise NbxError on failure."""
+    L = lib()
+    rc = getattr(L, name)(*args)
+    if rc != NBX_OK:
+        msg = L.nbx_last_error()
+        raise NbxError(name, rc, msg.decode() if msg else "")
+    return rc

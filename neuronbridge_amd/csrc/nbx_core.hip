@@ -1,0 +1,891 @@
+// Host side of the C ABI declared in include/nbx_core.h: validation, device
+// allocation and upload, engine orchestration (per-phase launches, hipGraph
+// capture of many steps, the persistent engine) and downloads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "nbx_net.h"
+
+namespace nbx {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+}  // namespace nbx
+
+using namespace nbx;
+
+#define NBX_CHECK(cond, msg)                     \
+    do {                                         \
+        if (!(cond)) {                           \
+            ::nbx::set_error(msg);               \
+            return NBX_ERR_INVALID;              \
+        }                                        \
+    } while (0)
+
+namespace {
+
+// exp(-dt * (1/tau)) with float arithmetic and libm expf, the reference's lookup value
+// (TimeDrivenLIF_Exponential_Decay.cpp:37-39).
+float decay_factor(float dt, float tau) {
+    const float inv = (tau != 0.0f) ? (1.0f / tau) : 0.0f;
+    return std::exp(-dt * inv);
+}
+
+int32_t poisson_table(double lambda, std::vector<uint32_t>* tab) {
+    tab->clear();
+    double p = std::exp(-lambda), cdf = 0.0;
+    for (int k = 0; k < 64; ++k) {
+        cdf += p;
+        const double v = std::floor(cdf * 4294967296.0);
+        const uint32_t th = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+        tab->push_back(th);
+        if (th == 0xFFFFFFFFu) return k;
+        p = p * lambda / (double)(k + 1);
+    }
+    return 63;
+}
+
+int32_t device_scope(nbx_net* net) {
+    NBX_HIP_TRY(hipSetDevice(net->device));
+    return NBX_OK;
+}
+
+int32_t require_finalized(const nbx_net* net) {
+    if (!net) {
+        set_error("null network handle");
+        return NBX_ERR_INVALID;
+    }
+    if (!net->finalized) {
+        set_error("network is not finalized (call nbx_net_finalize)");
+        return NBX_ERR_STATE;
+    }
+    return NBX_OK;
+}
+
+int32_t check_device_error(nbx_net* net) {
+    int32_t e = 0;
+    NBX_HIP_TRY(hipMemcpy(&e, net->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e != 0) {
+        set_error("device-side error flag " + std::to_string(e) + " (grid barrier time-out)");
+        return NBX_ERR_STATE;
+    }
+    return NBX_OK;
+}
+
+// Launch the kernels of one step; tbase == nullptr means k is the absolute step.
+hipError_t enqueue_step(nbx_net* net, const StepArgs& A, const int32_t* tbase, int32_t k) {
+    hipError_t e = hipSuccess;
+    if (net->pull_mode && net->has_sparse) {
+        e = launch_pull(A, tbase, k, net->stream);
+        if (e != hipSuccess) return e;
+    }
+    e = launch_update(A, tbase, k, net->stream);
+    if (e != hipSuccess) return e;
+    if (net->stdp && net->d_in_ptr) {
+        e = launch_ltp(A, tbase, k, net->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (!net->pull_mode && net->has_sparse) {
+        e = launch_deliver_push(A, tbase, k, net->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (net->has_dense) e = launch_deliver_dense(A, tbase, k, net->stream);
+    return e;
+}
+
+constexpr int32_t kGraphChunk = 64;
+
+int32_t run_graph(nbx_net* net, int32_t nsteps) {
+    const StepArgs A = net->args();
+    int32_t done = 0;
+    if (nsteps >= kGraphChunk) {
+        if (!net->gexec) {
+            hipGraph_t g = nullptr;
+            NBX_HIP_TRY(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int32_t k = 0; k < kGraphChunk && e == hipSuccess; ++k) e = enqueue_step(net, A, net->d_tbase, k);
+            if (e == hipSuccess) e = launch_advance(net->d_tbase, kGraphChunk, net->stream);
+            hipError_t ec = hipStreamEndCapture(net->stream, &g);
+            if (e != hipSuccess || ec != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                set_error(std::string("graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+                return NBX_ERR_HIP;
+            }
+            hipError_t ei = hipGraphInstantiate(&net->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            NBX_HIP_TRY(ei);
+            net->gchunk = kGraphChunk;
+        }
+        NBX_HIP_TRY(hipMemcpyAsync(net->d_tbase, &net->t, sizeof(int32_t), hipMemcpyHostToDevice, net->stream));
+        // the host value must stay alive until the copy has run
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        while (nsteps - done >= net->gchunk) {
+            NBX_HIP_TRY(hipGraphLaunch(net->gexec, net->stream));
+            done += net->gchunk;
+        }
+    }
+    for (; done < nsteps; ++done) NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + done));
+    net->t += nsteps;
+    return NBX_OK;
+}
+
+int32_t run_kernels(nbx_net* net, int32_t nsteps) {
+    const StepArgs A = net->args();
+    for (int32_t k = 0; k < nsteps; ++k) NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + k));
+    net->t += nsteps;
+    return NBX_OK;
+}
+
+int32_t auto_shift(nbx_net* net) {
+    // largest shift such that the sum of |contributions| into one (step, channel, neuron)
+    // stays below 2^31 for any spike pattern
+    std::vector<double> bound((size_t)net->n, 0.0);
+    if (net->has_sparse) {
+        for (int64_t j = 0; j < net->S; ++j) {
+            double w = std::fabs((double)net->h_w[(size_t)j]);
+            if (net->stdp && !net->h_plastic.empty() && net->h_plastic[(size_t)j]) w = std::max(w, (double)net->sd.w_max);
+            bound[(size_t)syn_post(net->h_syn[(size_t)j])] += w;
+        }
+    }
+    if (net->has_dense) {
+        for (int32_t p = 0; p < net->n; ++p) {
+            const float* col = net->h_wt.data() + (size_t)p * net->n;
+            for (int32_t i = 0; i < net->n; ++i) bound[(size_t)i] += std::fabs((double)col[i]);
+        }
+    }
+    double B = 0.0;
+    for (double b : bound) B = std::max(B, b);
+    if (net->pois) B += (double)net->kmax * std::fabs((double)net->w_bg);
+    int32_t s = 24;
+    // ceil-safe: also leave room for rounding of each quantized term (0.5 ulp of 2^-s each)
+    while (s > 0 && B * std::ldexp(1.0, s) + 1e6 >= 2147483647.0) --s;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nbx_last_error(void) { return g_error.c_str(); }
+
+int32_t nbx_version(void) { return 10000; }
+
+int32_t nbx_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int32_t nbx_net_create(const nbx_model_desc* md, int32_t device, nbx_net** out) {
+    NBX_CHECK(md && out, "null argument");
+    *out = nullptr;
+    NBX_CHECK(md->model >= NBX_MODEL_LIF_VJ && md->model <= NBX_MODEL_IZH_CUR, "unknown model id");
+    NBX_CHECK(md->integrator == NBX_EULER || md->integrator == NBX_RK4, "unknown integrator");
+    NBX_CHECK(md->n > 0 && md->n < kMaxNeurons, "neuron count must be in [1, 2^24)");
+    NBX_CHECK(md->dt > 0.0f, "dt must be positive");
+    const bool izh = md->model == NBX_MODEL_IZH || md->model == NBX_MODEL_IZH_CUR;
+    NBX_CHECK(!izh || (md->izh_a && md->izh_b && md->izh_c && md->izh_d), "Izhikevich models need per-neuron a,b,c,d");
+    NBX_CHECK(izh || md->tau != 0.0f, "LIF tau must be non-zero");
+    int ndev = 0;
+    NBX_HIP_TRY(hipGetDeviceCount(&ndev));
+    NBX_CHECK(device >= 0 && device < ndev, "device index out of range");
+    std::unique_ptr<nbx_net> net(new nbx_net());
+    net->device = device;
+    net->md = *md;
+    net->md.izh_a = net->md.izh_b = net->md.izh_c = net->md.izh_d = nullptr;
+    net->n = md->n;
+    net->ns = md->model == NBX_MODEL_LIF_VJ ? 2 : (md->model == NBX_MODEL_LIF_EXP ? 3 : 6);
+    net->nch = md->model == NBX_MODEL_IZH ? 2 : 1;
+    if (izh) {
+        net->h_a.assign(md->izh_a, md->izh_a + md->n);
+        net->h_b.assign(md->izh_b, md->izh_b + md->n);
+        net->h_c.assign(md->izh_c, md->izh_c + md->n);
+        net->h_d.assign(md->izh_d, md->izh_d + md->n);
+    }
+    ModelParams& P = net->P;
+    P.model = md->model;
+    P.integrator = md->integrator;
+    P.n = md->n;
+    P.t_ref = md->t_ref;
+    P.dt = md->dt;
+    P.hdt = md->dt * 0.5f;
+    P.dt6 = md->dt / 6.0f;
+    P.v_rest = md->v_rest;
+    P.tau = md->tau;
+    P.v_th = md->v_th;
+    P.r = md->r;
+    P.v_reset = md->v_reset;
+    P.e_rev = md->e_rev;
+    P.izh_v_th = md->izh_v_th;
+    P.izh_r = md->izh_r;
+    P.e_ampa = md->e_ampa;
+    P.e_gaba = md->e_gaba;
+    P.g_decay = decay_factor(md->dt, md->g_tau);
+    P.ampa_decay = decay_factor(md->dt, md->ampa_tau);
+    P.gaba_decay = decay_factor(md->dt, md->gaba_tau);
+    NBX_HIP_TRY(hipSetDevice(device));
+    NBX_HIP_TRY(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+    *out = net.release();
+    return NBX_OK;
+}
+
+int32_t nbx_net_destroy(nbx_net* net) {
+    if (!net) return NBX_OK;
+    hipSetDevice(net->device);
+    if (net->stream) hipStreamSynchronize(net->stream);
+    net->release();
+    delete net;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn) {
+    NBX_CHECK(net && syn, "null argument");
+    NBX_CHECK(!net->finalized, "synapses must be set before nbx_net_finalize");
+    NBX_CHECK(syn->count >= 0, "negative synapse count");
+    NBX_CHECK(syn->max_delay >= 1 && syn->max_delay <= kMaxDelay, "max_delay must be in [1, 63]");
+    NBX_CHECK(syn->row_ptr && (syn->count == 0 || (syn->post && syn->delay && syn->type && syn->weight)),
+              "missing synapse arrays");
+    const int32_t n = net->n;
+    NBX_CHECK(syn->row_ptr[0] == 0 && syn->row_ptr[n] == syn->count, "row_ptr must start at 0 and end at count");
+    net->h_row_ptr.assign(syn->row_ptr, syn->row_ptr + n + 1);
+    net->h_syn.resize((size_t)syn->count);
+    net->h_w.assign(syn->weight, syn->weight + syn->count);
+    if (syn->plastic) net->h_plastic.assign(syn->plastic, syn->plastic + syn->count);
+    else net->h_plastic.clear();
+    for (int32_t p = 0; p < n; ++p) {
+        const int64_t b = syn->row_ptr[p], e = syn->row_ptr[p + 1];
+        NBX_CHECK(b <= e, "row_ptr must be non-decreasing");
+        int32_t prev_d = 0;
+        for (int64_t j = b; j < e; ++j) {
+            const int32_t d = syn->delay[j], post = syn->post[j], ty = syn->type[j];
+            NBX_CHECK(d >= 1 && d <= syn->max_delay, "synapse delay outside [1, max_delay]");
+            NBX_CHECK(post >= 0 && post < n, "synapse post index out of range");
+            NBX_CHECK(ty == 0 || ty == 1, "synapse type must be 0 (excitatory) or 1 (inhibitory)");
+            NBX_CHECK(d >= prev_d, "each row must be sorted by delay");
+            prev_d = d;
+            net->h_syn[(size_t)j] = pack_syn(post, d, ty);
+        }
+    }
+    net->S = syn->count;
+    net->max_delay = std::max(net->max_delay, syn->max_delay);
+    net->has_sparse = true;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_dense(nbx_net* net, const float* wt, int32_t delay) {
+    NBX_CHECK(net && wt, "null argument");
+    NBX_CHECK(!net->finalized, "dense weights must be set before nbx_net_finalize");
+    NBX_CHECK(delay >= 1 && delay <= kMaxDelay, "dense delay must be in [1, 63]");
+    net->h_wt.assign(wt, wt + (size_t)net->n * net->n);
+    net->dense_delay = delay;
+    net->max_delay = std::max(net->max_delay, delay);
+    net->has_dense = true;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_poisson(nbx_net* net, uint32_t seed_lo, uint32_t seed_hi, double lambda, float w_bg) {
+    NBX_CHECK(net, "null argument");
+    NBX_CHECK(!net->finalized, "Poisson input must be set before nbx_net_finalize");
+    NBX_CHECK(lambda >= 0.0 && lambda <= 20.0, "Poisson lambda must be in [0, 20]");
+    net->pois = lambda > 0.0;
+    net->seed_lo = seed_lo;
+    net->seed_hi = seed_hi;
+    net->lambda = lambda;
+    net->w_bg = w_bg;
+    net->kmax = poisson_table(lambda, &net->pois_tab);
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* sd) {
+    NBX_CHECK(net && sd, "null argument");
+    NBX_CHECK(!net->finalized, "STDP must be set before nbx_net_finalize");
+    NBX_CHECK(sd->ltp_tau > 0.0f && sd->ltd_tau > 0.0f, "STDP time constants must be positive");
+    net->stdp = true;
+    net->sd = *sd;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_current(nbx_net* net, const float* cur, int32_t t_begin, int32_t nsteps) {
+    NBX_CHECK(net && (cur || nsteps == 0), "null argument");
+    NBX_CHECK(nsteps >= 0, "negative step count");
+    net->h_cur.assign(cur, cur + (size_t)nsteps * net->n);
+    net->cur_t0 = t_begin;
+    net->cur_n = nsteps;
+    if (net->finalized) {
+        int32_t rc = device_scope(net);
+        if (rc) return rc;
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        if (net->d_cur) {
+            auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)net->d_cur);
+            if (it != net->allocs.end()) net->allocs.erase(it);
+            NBX_HIP_TRY(hipFree(net->d_cur));
+            net->d_cur = nullptr;
+        }
+        if (nsteps) {
+            rc = net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size());
+            if (rc) return rc;
+        }
+        if (net->gexec) {
+            hipGraphExecDestroy(net->gexec);
+            net->gexec = nullptr;
+        }
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_fixed_shift(nbx_net* net, int32_t shift) {
+    NBX_CHECK(net, "null argument");
+    NBX_CHECK(!net->finalized, "fixed shift must be set before nbx_net_finalize");
+    NBX_CHECK(shift >= 0 && shift <= 30, "fixed shift must be in [0, 30]");
+    net->shift_req = shift;
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_fixed_shift(const nbx_net* net, int32_t* shift) {
+    NBX_CHECK(net && shift, "null argument");
+    *shift = net->finalized ? net->shift : (net->shift_req ? net->shift_req : -1);
+    return NBX_OK;
+}
+
+int32_t nbx_net_nstate(const nbx_net* net) { return net ? net->ns : 0; }
+
+int32_t nbx_net_finalize(nbx_net* net) {
+    NBX_CHECK(net, "null argument");
+    NBX_CHECK(!net->finalized, "network already finalized");
+    int32_t rc = device_scope(net);
+    if (rc) return rc;
+    const int32_t n = net->n;
+    // max_delay + 1 slots (the reference's delay_slot_count, DenseSubnetworkBuildFinalizer.cpp:62),
+    // at least 3 so the slot being filled, the one being read and the one being freed differ
+    net->D = std::max(net->max_delay + 1, 3);
+    net->pull_mode = net->stdp;
+    NBX_CHECK(!(net->stdp && !net->has_sparse), "STDP needs sparse synapses");
+    net->shift = net->shift_req ? net->shift_req : auto_shift(net);
+    net->words = (n + 31) / 32;
+    // state (SoA) initialised like Neuron_State_Vector::InitNeuronState with sigma 0
+    std::vector<float> v((size_t)n), s1((size_t)n, 0.0f), zero((size_t)n, 0.0f);
+    std::vector<int32_t> last((size_t)n, 10000);
+    for (int32_t i = 0; i < n; ++i) {
+        if (net->md.model == NBX_MODEL_LIF_VJ) v[(size_t)i] = net->md.v_reset;
+        else v[(size_t)i] = net->md.v_rest;
+        if (net->md.model == NBX_MODEL_IZH || net->md.model == NBX_MODEL_IZH_CUR)
+            s1[(size_t)i] = net->h_b[(size_t)i] * net->md.v_rest;
+    }
+#define NBX_RC(x)            \
+    do {                     \
+        rc = (x);            \
+        if (rc) return rc;   \
+    } while (0)
+    NBX_RC(net->upload(&net->d_v, v.data(), (size_t)n));
+    NBX_RC(net->upload(&net->d_s1, s1.data(), (size_t)n));
+    NBX_RC(net->upload(&net->d_ga, zero.data(), (size_t)n));
+    NBX_RC(net->upload(&net->d_gg, zero.data(), (size_t)n));
+    NBX_RC(net->upload(&net->d_iext, zero.data(), (size_t)n));
+    NBX_RC(net->upload(&net->d_last, last.data(), (size_t)n));
+    if (!net->h_a.empty()) {
+        NBX_RC(net->upload(&net->d_a, net->h_a.data(), (size_t)n));
+        NBX_RC(net->upload(&net->d_b, net->h_b.data(), (size_t)n));
+        NBX_RC(net->upload(&net->d_c, net->h_c.data(), (size_t)n));
+        NBX_RC(net->upload(&net->d_d, net->h_d.data(), (size_t)n));
+    }
+    NBX_RC(net->alloc(&net->d_ring, (size_t)net->D * net->nch * n));
+    NBX_HIP_TRY(hipMemset(net->d_ring, 0, sizeof(int32_t) * (size_t)net->D * net->nch * n));
+    NBX_RC(net->alloc(&net->d_hist, (size_t)net->D * n));
+    NBX_RC(net->alloc(&net->d_hist_cnt, (size_t)net->D));
+    NBX_HIP_TRY(hipMemset(net->d_hist_cnt, 0, sizeof(int32_t) * net->D));
+    NBX_RC(net->alloc(&net->d_did_fire, (size_t)n));
+    NBX_HIP_TRY(hipMemset(net->d_did_fire, 0, (size_t)n));
+    NBX_RC(net->alloc(&net->d_events, 1));
+    NBX_HIP_TRY(hipMemset(net->d_events, 0, sizeof(unsigned long long)));
+    NBX_RC(net->alloc(&net->d_tbase, 1));
+    NBX_RC(net->alloc(&net->d_err, 1));
+    NBX_HIP_TRY(hipMemset(net->d_err, 0, sizeof(int32_t)));
+    if (net->has_sparse) {
+        NBX_RC(net->upload(&net->d_row_ptr, net->h_row_ptr.data(), net->h_row_ptr.size()));
+        NBX_RC(net->upload(&net->d_syn, net->h_syn.data(), net->h_syn.size()));
+        NBX_RC(net->upload(&net->d_w, net->h_w.data(), net->h_w.size()));
+        // start of each delay inside each row: dptr[p * (D + 1) + d]
+        const int32_t D = net->D;
+        std::vector<int64_t> dptr((size_t)n * (D + 1));
+        for (int32_t p = 0; p < n; ++p) {
+            int64_t j = net->h_row_ptr[(size_t)p];
+            const int64_t e = net->h_row_ptr[(size_t)p + 1];
+            for (int32_t d = 0; d <= D; ++d) {
+                while (j < e && syn_delay(net->h_syn[(size_t)j]) < d) ++j;
+                dptr[(size_t)p * (D + 1) + d] = j;
+            }
+            dptr[(size_t)p * (D + 1) + D] = e;
+        }
+        NBX_RC(net->upload(&net->d_dptr, dptr.data(), dptr.size()));
+        if (!net->h_plastic.empty()) NBX_RC(net->upload(&net->d_plastic, net->h_plastic.data(), net->h_plastic.size()));
+    }
+    if (net->has_dense) NBX_RC(net->upload(&net->d_wt, net->h_wt.data(), net->h_wt.size()));
+    if (net->pois) NBX_RC(net->upload(&net->d_pois_tab, net->pois_tab.data(), net->pois_tab.size()));
+    if (net->cur_n) NBX_RC(net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size()));
+    if (net->stdp) {
+        NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
+        const size_t S = (size_t)net->S;
+        std::vector<float> zf(S, 0.0f);
+        std::vector<int32_t> zi(S, 0);
+        NBX_RC(net->upload(&net->d_apre, zf.data(), S));
+        NBX_RC(net->upload(&net->d_apost, zf.data(), S));
+        NBX_RC(net->upload(&net->d_slast, zi.data(), S));
+        // decay tables by integer step delta: exp(-(delta * dt) * (1/tau)) in float, the
+        // arithmetic of STDP_State::SetUpdate (STDP_State.cpp:17-26); the last entry is 0
+        // or covers every delta a run can reach
+        const float inv_ltp = (float)(1.0 / (double)net->sd.ltp_tau);
+        const float inv_ltd = (float)(1.0 / (double)net->sd.ltd_tau);
+        std::vector<float> tp, td;
+        for (int32_t k = 0;; ++k) {
+            const float dT = (float)k * net->md.dt;
+            tp.push_back(std::exp(-dT * inv_ltp));
+            td.push_back(std::exp(-dT * inv_ltd));
+            if ((tp.back() == 0.0f && td.back() == 0.0f) || k >= (1 << 22)) break;
+        }
+        net->tab_len = (int32_t)tp.size();
+        NBX_RC(net->upload(&net->d_ltp_tab, tp.data(), tp.size()));
+        NBX_RC(net->upload(&net->d_ltd_tab, td.data(), td.size()));
+        // plastic inputs of each post neuron (post-side STDP index)
+        std::vector<int64_t> in_ptr((size_t)n + 1, 0);
+        for (size_t j = 0; j < S; ++j)
+            if (net->h_plastic[j]) in_ptr[(size_t)syn_post(net->h_syn[j]) + 1]++;
+        for (int32_t i = 0; i < n; ++i) in_ptr[(size_t)i + 1] += in_ptr[(size_t)i];
+        std::vector<int64_t> fill(in_ptr.begin(), in_ptr.end() - 1), in_syn((size_t)in_ptr[(size_t)n]);
+        for (size_t j = 0; j < S; ++j)
+            if (net->h_plastic[j]) in_syn[(size_t)fill[(size_t)syn_post(net->h_syn[j])]++] = (int64_t)j;
+        NBX_RC(net->upload(&net->d_in_ptr, in_ptr.data(), in_ptr.size()));
+        NBX_RC(net->upload(&net->d_in_syn, in_syn.data(), in_syn.size()));
+    }
+    net->finalized = true;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_state(nbx_net* net, const float* x, const int32_t* last) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(x && last, "null argument");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    const int32_t n = net->n, ns = net->ns;
+    std::vector<float> cols[6];
+    for (int j = 0; j < ns; ++j) {
+        cols[j].resize((size_t)n);
+        for (int32_t i = 0; i < n; ++i) cols[j][(size_t)i] = x[(size_t)i * ns + j];
+    }
+    auto put = [&](float* d, int j) -> hipError_t {
+        return hipMemcpy(d, cols[j].data(), sizeof(float) * n, hipMemcpyHostToDevice);
+    };
+    NBX_HIP_TRY(put(net->d_v, 0));
+    if (net->md.model == NBX_MODEL_LIF_VJ) {
+        NBX_HIP_TRY(put(net->d_iext, 1));
+    } else if (net->md.model == NBX_MODEL_LIF_EXP) {
+        NBX_HIP_TRY(put(net->d_s1, 1));
+        NBX_HIP_TRY(put(net->d_iext, 2));
+    } else {
+        NBX_HIP_TRY(put(net->d_s1, 1));
+        NBX_HIP_TRY(put(net->d_ga, 2));
+        NBX_HIP_TRY(put(net->d_gg, 3));
+        NBX_HIP_TRY(put(net->d_iext, 5));
+    }
+    NBX_HIP_TRY(hipMemcpy(net->d_last, last, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_state(nbx_net* net, float* x, int32_t* last) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(x && last, "null argument");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    const int32_t n = net->n, ns = net->ns;
+    std::vector<float> col((size_t)n);
+    auto get = [&](const float* d, int j) -> hipError_t {
+        hipError_t e = hipMemcpy(col.data(), d, sizeof(float) * n, hipMemcpyDeviceToHost);
+        for (int32_t i = 0; i < n; ++i) x[(size_t)i * ns + j] = col[(size_t)i];
+        return e;
+    };
+    for (size_t k = 0; k < (size_t)n * ns; ++k) x[k] = 0.0f;
+    NBX_HIP_TRY(get(net->d_v, 0));
+    if (net->md.model == NBX_MODEL_LIF_VJ) {
+        NBX_HIP_TRY(get(net->d_iext, 1));
+    } else if (net->md.model == NBX_MODEL_LIF_EXP) {
+        NBX_HIP_TRY(get(net->d_s1, 1));
+        NBX_HIP_TRY(get(net->d_iext, 2));
+    } else {
+        NBX_HIP_TRY(get(net->d_s1, 1));
+        NBX_HIP_TRY(get(net->d_ga, 2));
+        NBX_HIP_TRY(get(net->d_gg, 3));
+        NBX_HIP_TRY(get(net->d_iext, 5));
+    }
+    NBX_HIP_TRY(hipMemcpy(last, net->d_last, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_record(nbx_net* net, int32_t t_begin, int32_t nsteps, int32_t flags) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(nsteps >= 0 && t_begin >= 0, "negative record window");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    for (void** p : {(void**)&net->d_counts, (void**)&net->d_raster}) {
+        if (*p) {
+            auto it = std::find(net->allocs.begin(), net->allocs.end(), *p);
+            if (it != net->allocs.end()) net->allocs.erase(it);
+            NBX_HIP_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+    }
+    net->rec_t0 = t_begin;
+    net->rec_n = nsteps;
+    if (nsteps && (flags & NBX_RECORD_COUNTS)) {
+        NBX_RC(net->alloc(&net->d_counts, (size_t)nsteps));
+        NBX_HIP_TRY(hipMemset(net->d_counts, 0, sizeof(int32_t) * (size_t)nsteps));
+    }
+    if (nsteps && (flags & NBX_RECORD_RASTER)) {
+        const size_t nw = (size_t)nsteps * net->words;
+        NBX_RC(net->alloc(&net->d_raster, nw));
+        NBX_HIP_TRY(hipMemset(net->d_raster, 0, sizeof(uint32_t) * nw));
+    }
+    if (net->gexec) {  // captured kernels hold the old recording pointers
+        hipGraphExecDestroy(net->gexec);
+        net->gexec = nullptr;
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_counts(nbx_net* net, int32_t* counts, int32_t nsteps) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->d_counts, "spike counts are not being recorded");
+    NBX_CHECK(counts && nsteps >= 0 && nsteps <= net->rec_n, "bad counts buffer");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    NBX_HIP_TRY(hipMemcpy(counts, net->d_counts, sizeof(int32_t) * (size_t)nsteps, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_raster(nbx_net* net, uint32_t* raster, int64_t nwords) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->d_raster, "raster is not being recorded");
+    NBX_CHECK(raster && nwords >= 0 && nwords <= (int64_t)net->rec_n * net->words, "bad raster buffer");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    NBX_HIP_TRY(hipMemcpy(raster, net->d_raster, sizeof(uint32_t) * (size_t)nwords, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_pick_engine(const nbx_net* net, int32_t* engine) {
+    NBX_CHECK(net && engine, "null argument");
+    *engine = persistent_supported(net) ? NBX_ENGINE_PERSISTENT : NBX_ENGINE_GRAPH;
+    return NBX_OK;
+}
+
+int32_t nbx_net_run_async(nbx_net* net, int32_t nsteps, int32_t engine) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(nsteps >= 0, "negative step count");
+    NBX_CHECK(net->phase == 0, "a per-phase step is open (finish it with nbx_net_commit_firings)");
+    NBX_RC(device_scope(net));
+    if (engine == NBX_ENGINE_AUTO) NBX_RC(nbx_net_pick_engine(net, &engine));
+    switch (engine) {
+        case NBX_ENGINE_KERNELS: return run_kernels(net, nsteps);
+        case NBX_ENGINE_GRAPH: return run_graph(net, nsteps);
+        case NBX_ENGINE_PERSISTENT:
+            if (!persistent_supported(net)) {
+                set_error("persistent engine unsupported for this network: " + persistent_reason(net));
+                return NBX_ERR_UNSUPPORTED;
+            }
+            return run_persistent(net, nsteps);
+        default: set_error("unknown engine"); return NBX_ERR_INVALID;
+    }
+}
+
+int32_t nbx_net_sync(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    return check_device_error(net);
+}
+
+int32_t nbx_net_run(nbx_net* net, int32_t nsteps, int32_t engine) {
+    int32_t rc = nbx_net_run_async(net, nsteps, engine);
+    if (rc) return rc;
+    return nbx_net_sync(net);
+}
+
+int32_t nbx_net_time(const nbx_net* net, int32_t* t) {
+    NBX_CHECK(net && t, "null argument");
+    *t = net->t;
+    return NBX_OK;
+}
+
+int32_t nbx_net_events(nbx_net* net, int64_t* events) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(events, "null argument");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    unsigned long long e = 0;
+    NBX_HIP_TRY(hipMemcpy(&e, net->d_events, sizeof(e), hipMemcpyDeviceToHost));
+    *events = (int64_t)e;
+    return NBX_OK;
+}
+
+// ---------------------------------------------------------------- per-phase calls
+
+int32_t nbx_net_begin_step(nbx_net* net, int32_t t) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 0) {
+        set_error("previous step not committed");
+        return NBX_ERR_STATE;
+    }
+    if (t != net->t) {
+        set_error("BeginStep time " + std::to_string(t) + " != network time " + std::to_string(net->t));
+        return NBX_ERR_STATE;
+    }
+    net->phase = 1;
+    return NBX_OK;
+}
+
+int32_t nbx_net_run_propagation(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 1) {
+        set_error("RunPropagation must follow BeginStep");
+        return NBX_ERR_STATE;
+    }
+    NBX_RC(device_scope(net));
+    if (net->pull_mode && net->has_sparse) NBX_HIP_TRY(launch_pull(net->args(), nullptr, net->t, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    net->phase = 2;
+    return NBX_OK;
+}
+
+int32_t nbx_net_neuron_step(nbx_net* net, int32_t* ids_out, int32_t ids_cap, int32_t* count_out) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 2) {
+        set_error("neuron step must follow RunPropagation");
+        return NBX_ERR_STATE;
+    }
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(launch_update(net->args(), nullptr, net->t, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    net->phase = 3;
+    if (ids_out || count_out) {
+        int32_t cnt = 0;
+        NBX_RC(nbx_net_get_history(net, net->t, ids_out, ids_cap, &cnt));
+        if (count_out) *count_out = cnt;
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 3 && net->phase != 4) {
+        set_error("inject spikes between the neuron step and the commit");
+        return NBX_ERR_STATE;
+    }
+    NBX_CHECK(count >= 0 && (ids || count == 0), "bad spike list");
+    if (count == 0) return NBX_OK;
+    for (int32_t k = 0; k < count; ++k) NBX_CHECK(ids[k] >= 0 && ids[k] < net->n, "injected id out of range");
+    NBX_RC(device_scope(net));
+    int32_t cur = 0;
+    const int32_t slot = net->t % net->D;
+    NBX_HIP_TRY(hipMemcpy(&cur, net->d_hist_cnt + slot, sizeof(int32_t), hipMemcpyDeviceToHost));
+    NBX_CHECK(cur + count <= net->n, "too many spikes in one step");
+    int32_t* d_ids = nullptr;
+    NBX_HIP_TRY(hipMalloc(&d_ids, sizeof(int32_t) * count));
+    hipError_t e = hipMemcpy(d_ids, ids, sizeof(int32_t) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_inject(net->args(), d_ids, count, net->t, net->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
+    hipFree(d_ids);
+    NBX_HIP_TRY(e);
+    return NBX_OK;
+}
+
+int32_t nbx_net_apply_post_learning(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 3) {
+        set_error("post learning must follow the neuron step");
+        return NBX_ERR_STATE;
+    }
+    NBX_RC(device_scope(net));
+    if (net->stdp && net->d_in_ptr) NBX_HIP_TRY(launch_ltp(net->args(), nullptr, net->t, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    net->phase = 4;
+    return NBX_OK;
+}
+
+int32_t nbx_net_commit_firings(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 3 && net->phase != 4) {
+        set_error("commit must follow the neuron step");
+        return NBX_ERR_STATE;
+    }
+    NBX_RC(device_scope(net));
+    const StepArgs A = net->args();
+    if (net->phase == 3 && net->stdp && net->d_in_ptr) NBX_HIP_TRY(launch_ltp(A, nullptr, net->t, net->stream));
+    if (!net->pull_mode && net->has_sparse) NBX_HIP_TRY(launch_deliver_push(A, nullptr, net->t, net->stream));
+    if (net->has_dense) NBX_HIP_TRY(launch_deliver_dense(A, nullptr, net->t, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    net->t += 1;
+    net->phase = 0;
+    return NBX_OK;
+}
+
+int32_t nbx_net_ring_shape(const nbx_net* net, int32_t* slots, int32_t* channels) {
+    NBX_CHECK(net && slots && channels, "null argument");
+    if (!net->finalized) {
+        set_error("network is not finalized");
+        return NBX_ERR_STATE;
+    }
+    *slots = net->D;
+    *channels = net->nch;
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_ring(nbx_net* net, int32_t* ring, int64_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(ring && count == (int64_t)net->D * net->nch * net->n, "ring buffer size must be slots*channels*n");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    NBX_HIP_TRY(hipMemcpy(ring, net->d_ring, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_history(nbx_net* net, int32_t t, int32_t* ids, int32_t cap, int32_t* count_out) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(count_out, "null argument");
+    // the update of step u writes slot u % D and frees slot (u + 1) % D, so D - 1 steps are held
+    const int32_t newest = net->phase >= 3 ? net->t : net->t - 1;
+    if (t < 0 || t > newest || t < newest - net->D + 2) {
+        set_error("step " + std::to_string(t) + " is not held in the firing-history ring");
+        return NBX_ERR_INVALID;
+    }
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    const int32_t slot = t % net->D;
+    int32_t cnt = 0;
+    NBX_HIP_TRY(hipMemcpy(&cnt, net->d_hist_cnt + slot, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *count_out = cnt;
+    if (ids) {
+        NBX_CHECK(cap >= cnt, "id buffer too small");
+        NBX_HIP_TRY(hipMemcpy(ids, net->d_hist + (size_t)slot * net->n, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        std::sort(ids, ids + cnt);
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_pending_view(nbx_net* net, nbx_pending_view* view) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(view, "null argument");
+    view->d_pending = net->d_ring + (size_t)(net->t % net->D) * net->nch * net->n;
+    view->channel_count = net->nch;
+    view->channel_stride = net->n;
+    view->neuron_count = net->n;
+    view->inv_scale = 1.0f / std::ldexp(1.0f, net->shift);
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_weights(nbx_net* net, float* w, int64_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "network has no sparse synapses");
+    NBX_CHECK(w && count == net->S, "weight buffer size must equal the synapse count");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    NBX_HIP_TRY(hipMemcpy(w, net->d_w, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_weights(nbx_net* net, const float* w, int64_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "network has no sparse synapses");
+    NBX_CHECK(w && count == net->S, "weight buffer size must equal the synapse count");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    NBX_HIP_TRY(hipMemcpy(net->d_w, w, sizeof(float) * (size_t)count, hipMemcpyHostToDevice));
+    net->h_w.assign(w, w + count);
+    rc = refresh_persistent_weights(net);
+    return rc;
+}
+
+int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t* last_update, int64_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->stdp, "STDP is not enabled");
+    NBX_CHECK(count == net->S, "buffer size must equal the synapse count");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    if (apre) NBX_HIP_TRY(hipMemcpy(apre, net->d_apre, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
+    if (apost) NBX_HIP_TRY(hipMemcpy(apost, net->d_apost, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
+    if (last_update)
+        NBX_HIP_TRY(hipMemcpy(last_update, net->d_slast, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+void* nbx_net_stream(nbx_net* net) { return net ? (void*)net->stream : nullptr; }
+
+int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t nsteps,
+                           int32_t* out) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "replay delivery needs sparse synapses");
+    NBX_CHECK(tick_ptr && out && nsteps > 0 && t0 >= 0, "bad replay arguments");
+    NBX_CHECK(tick_ptr[0] == 0, "tick_ptr must start at 0");
+    const int32_t total = tick_ptr[nsteps];
+    for (int32_t k = 0; k < nsteps; ++k) NBX_CHECK(tick_ptr[k] <= tick_ptr[k + 1], "tick_ptr must be non-decreasing");
+    for (int32_t j = 0; j < total; ++j) NBX_CHECK(ids[j] >= 0 && ids[j] < net->n, "spike id out of range");
+    NBX_RC(device_scope(net));
+    int32_t *d_ids = nullptr, *d_tp = nullptr, *d_out = nullptr;
+    const size_t out_count = (size_t)nsteps * net->nch * net->n;
+    hipError_t e = hipMalloc(&d_ids, sizeof(int32_t) * std::max(total, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_tp, sizeof(int32_t) * (nsteps + 1));
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(int32_t) * out_count);
+    if (e == hipSuccess && total) e = hipMemcpy(d_ids, ids, sizeof(int32_t) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_tp, tick_ptr, sizeof(int32_t) * (nsteps + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = nbx_replay_deliver_device(net, d_ids, d_tp, nsteps, d_out);
+        if (rc == NBX_OK) e = hipStreamSynchronize(net->stream);
+    }
+    if (e == hipSuccess && rc == NBX_OK) e = hipMemcpy(out, d_out, sizeof(int32_t) * out_count, hipMemcpyDeviceToHost);
+    hipFree(d_ids);
+    hipFree(d_tp);
+    hipFree(d_out);
+    if (rc) return rc;
+    NBX_HIP_TRY(e);
+    return NBX_OK;
+}
+
+int32_t nbx_replay_deliver_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps,
+                                  int32_t* d_out) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "replay delivery needs sparse synapses");
+    NBX_CHECK(d_tick_ptr && d_out && nsteps > 0, "bad replay arguments");
+    NBX_RC(device_scope(net));
+    return launch_replay(net, d_ids, d_tick_ptr, nsteps, d_out);
+}
+
+}  // extern "C"

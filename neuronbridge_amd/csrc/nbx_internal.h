@@ -1,0 +1,268 @@
+// Internal declarations shared by the HIP translation units of libnbx.
+//
+// Numerics contract (what makes device results bit-identical to the CPU
+// restatement of the reference): every file is compiled with -ffp-contract=off
+// and correctly rounded fp32 division, no transcendental runs on the device
+// (decay factors and STDP exponentials are host-computed tables), and synaptic
+// arrivals accumulate as int32 fixed point so their sum is independent of the
+// order in which atomics land.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nbx_core.h"
+
+namespace nbx {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+#define NBX_HIP_TRY(expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            ::nbx::set_error(std::string(#expr " failed: ") + hipGetErrorString(e_));            \
+            return NBX_ERR_HIP;                                                                  \
+        }                                                                                        \
+    } while (0)
+
+// ----------------------------------------------------------------- packed synapse entry
+// bits 0..23 post neuron, 24..29 delay (1..63), 30..31 type (0 exc, 1 inh)
+constexpr uint32_t kPostMask = 0x00FFFFFFu;
+constexpr int kMaxNeurons = 1 << 24;
+constexpr int kMaxDelay = 63;
+__host__ __device__ inline uint32_t pack_syn(int32_t post, int32_t delay, int32_t type) {
+    return (uint32_t)post | ((uint32_t)delay << 24) | ((uint32_t)type << 30);
+}
+__host__ __device__ inline int32_t syn_post(uint32_t e) { return (int32_t)(e & kPostMask); }
+__host__ __device__ inline int32_t syn_delay(uint32_t e) { return (int32_t)((e >> 24) & 63u); }
+__host__ __device__ inline int32_t syn_type(uint32_t e) { return (int32_t)(e >> 30); }
+
+// ----------------------------------------------------------------- model parameters (by value)
+struct ModelParams {
+    int32_t model, integrator, n, t_ref;
+    float dt, hdt, dt6;
+    float v_rest, tau, v_th, r, v_reset, e_rev;
+    float izh_v_th, izh_r, e_ampa, e_gaba;
+    float g_decay, ampa_decay, gaba_decay;
+};
+
+// Device pointers and scalars the step kernels need; passed by value.
+struct StepArgs {
+    ModelParams P;
+    int32_t n, nch, D, words;
+    // state, SoA
+    float* v;
+    float* s1;    // LIF_EXP: g ; IZH: u
+    float* ga;    // IZH: g_ampa
+    float* gg;    // IZH: g_gaba
+    float* iext;  // I_ext
+    int32_t* last;
+    const float *a, *b, *c, *d;
+    // delivery
+    int32_t* ring;      // [D][nch][n]
+    int32_t* hist;      // [D][n]
+    int32_t* hist_cnt;  // [D]
+    uint8_t* did_fire;  // [n] or null
+    const int64_t* row_ptr;
+    const uint32_t* syn;
+    float* w;
+    const int64_t* dptr;  // [n][D+1] start of each delay inside a row
+    const uint8_t* plastic;
+    const float* wt;  // dense [n][n]
+    int32_t dense_delay;
+    float scale, inv_scale;
+    unsigned long long* events;
+    // Poisson background
+    int32_t pois;
+    uint32_t seed_lo, seed_hi;
+    int32_t kmax;
+    const uint32_t* pois_tab;
+    int32_t q_bg;
+    // external current
+    const float* cur;
+    int32_t cur_t0, cur_n;
+    // STDP
+    int32_t stdp;
+    float max_ltp, max_ltd, w_max;
+    float* apre;
+    float* apost;
+    int32_t* slast;
+    const float* ltp_tab;
+    const float* ltd_tab;
+    int32_t tab_len;
+    const int64_t* in_ptr;
+    const int64_t* in_syn;
+    // recording
+    int32_t* counts;
+    uint32_t* raster;
+    int32_t rec_t0, rec_n;
+    // error flag (device-side failures such as a barrier time-out)
+    int32_t* err;
+};
+
+// ----------------------------------------------------------------- device helpers
+#define NBX_POIS_STREAM 0x504F4953u
+
+__device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                     uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+__device__ inline int32_t poisson_count(uint32_t seed_lo, uint32_t seed_hi, int32_t i, int32_t t,
+                                        const uint32_t* __restrict__ tab, int32_t kmax) {
+    uint32_t o[4];
+    philox4x32_10((uint32_t)i, (uint32_t)t, NBX_POIS_STREAM, 0u, seed_lo, seed_hi, o);
+    int32_t k = 0;
+    while (k < kmax && o[0] >= tab[k]) ++k;
+    return k;
+}
+
+__device__ inline int32_t quantize(float w, float scale) { return __float2int_rn(w * scale); }
+
+// ---- LIF right-hand side: R*I/tau [+ g*(E-V)/tau] + (V_rest - V)/tau
+template <bool WITH_G>
+__device__ inline float lif_rhs(const ModelParams& P, float V, float I, float g) {
+    float cur = P.r * I / P.tau;
+    if (WITH_G) cur += g * (P.e_rev - V) / P.tau;
+    return cur + (P.v_rest - V) / P.tau;
+}
+
+template <bool COND>
+__device__ inline void izh_rhs(const ModelParams& P, float a, float b, float V, float U, float ga, float gg, float I,
+                               float& dv, float& du) {
+    float tot = 0.0f;
+    if (COND) {
+        tot += ga * (P.e_ampa - V);
+        tot += gg * (P.e_gaba - V);
+    }
+    tot += I;
+    dv = 0.04f * V * V + 5.0f * V + 140.0f - U + P.izh_r * tot;
+    du = a * (b * V - U);
+}
+
+// One LIF step after arrivals are applied; returns true on a spike.
+template <bool WITH_G>
+__device__ inline bool lif_step(const ModelParams& P, float& V, float& g, float I, int32_t& last) {
+    if (last > P.t_ref) {
+        if (P.integrator == NBX_EULER) {
+            V += lif_rhs<WITH_G>(P, V, I, g) * P.dt;
+        } else {
+            const float k1 = lif_rhs<WITH_G>(P, V, I, g);
+            const float k2 = lif_rhs<WITH_G>(P, V + k1 * P.hdt, I, g);
+            const float k3 = lif_rhs<WITH_G>(P, V + k2 * P.hdt, I, g);
+            const float k4 = lif_rhs<WITH_G>(P, V + k3 * P.dt, I, g);
+            float acc = k1 + 2.0f * k2;
+            acc = acc + 2.0f * k3;
+            acc = acc + k4;
+            V += acc * P.dt6;
+        }
+    }
+    if (WITH_G) g *= P.g_decay;
+    last += 1;
+    if (V > P.v_th) {
+        V = P.v_reset;
+        last = 0;
+        return true;
+    }
+    return false;
+}
+
+template <bool COND>
+__device__ inline bool izh_step(const ModelParams& P, float a, float b, float c, float d, float& V, float& U,
+                                float& ga, float& gg, float I, int32_t& last) {
+    if (last > 0) {
+        if (P.integrator == NBX_EULER) {
+            float dv, du;
+            izh_rhs<COND>(P, a, b, V, U, ga, gg, I, dv, du);
+            V += dv * P.dt;
+            U += du * P.dt;
+        } else {
+            float v1, u1, v2, u2, v3, u3, v4, u4;
+            izh_rhs<COND>(P, a, b, V, U, ga, gg, I, v1, u1);
+            izh_rhs<COND>(P, a, b, V + v1 * P.hdt, U + u1 * P.hdt, ga, gg, I, v2, u2);
+            izh_rhs<COND>(P, a, b, V + v2 * P.hdt, U + u2 * P.hdt, ga, gg, I, v3, u3);
+            izh_rhs<COND>(P, a, b, V + v3 * P.dt, U + u3 * P.dt, ga, gg, I, v4, u4);
+            float av = v1 + 2.0f * v2;
+            av = av + 2.0f * v3;
+            av = av + v4;
+            float au = u1 + 2.0f * u2;
+            au = au + 2.0f * u3;
+            au = au + u4;
+            V += av * P.dt6;
+            U += au * P.dt6;
+        }
+    }
+    if (COND) {
+        ga *= P.ampa_decay;
+        gg *= P.gaba_decay;
+    }
+    last += 1;
+    if (V >= P.izh_v_th) {
+        V = c;
+        U += d;
+        last = 0;
+        return true;
+    }
+    return false;
+}
+
+// Applies one neuron's arrivals (a0/a1 fixed point) and external inputs for step t,
+// then advances it one step. Shared by every engine so their arithmetic is identical.
+template <int MODEL>
+__device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, float& V,
+                                   float& S1, float& GA, float& GG, float& I, int32_t& last) {
+    if (A.cur && t >= A.cur_t0 && t < A.cur_t0 + A.cur_n) I = A.cur[(int64_t)(t - A.cur_t0) * A.n + i];
+    if (A.pois) a0 += poisson_count(A.seed_lo, A.seed_hi, i, t, A.pois_tab, A.kmax) * A.q_bg;
+    const float f0 = (float)a0 * A.inv_scale;
+    if (MODEL == NBX_MODEL_LIF_VJ) {
+        if (last > A.P.t_ref) V += f0;
+        float g = 0.0f;
+        return lif_step<false>(A.P, V, g, I, last);
+    } else if (MODEL == NBX_MODEL_LIF_EXP) {
+        S1 += f0;
+        return lif_step<true>(A.P, V, S1, I, last);
+    } else if (MODEL == NBX_MODEL_IZH) {
+        GA += f0;
+        GG += (float)a1 * A.inv_scale;
+        return izh_step<true>(A.P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, I, last);
+    } else {
+        const float Itot = I + f0;
+        return izh_step<false>(A.P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, Itot, last);
+    }
+}
+
+// ----------------------------------------------------------------- launchers (host)
+hipError_t launch_update(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_deliver_push(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_deliver_dense(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s);
+hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
+
+}  // namespace nbx

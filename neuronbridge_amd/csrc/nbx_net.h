@@ -1,0 +1,197 @@
+// The network object behind the opaque nbx_net handle of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "nbx_internal.h"
+
+using nbx::ModelParams;
+using nbx::StepArgs;
+
+// Post-blocked layout of the persistent engine: block b owns neurons
+// [b * P, min((b + 1) * P, n)); the synapses of presynaptic neuron i that target
+// block b are seg[off[i * (B + 1) + b] .. off[i * (B + 1) + b + 1]).
+struct PersistentLayout {
+    bool built = false;
+    bool fits = false;
+    std::string reason;
+    int32_t B = 0, P = 0, threads = 0, npt = 0, lds_bytes = 0;
+    int32_t* d_off = nullptr;   // [n][B + 1]
+    uint2* d_seg = nullptr;     // {post_local | delay << 16 | channel << 24, q}
+    int32_t* d_bar = nullptr;   // grid barrier counter
+};
+
+struct nbx_net {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    nbx_model_desc md{};
+    ModelParams P{};
+    int32_t n = 0, ns = 0, nch = 1;
+    std::vector<float> h_a, h_b, h_c, h_d;
+
+    // sparse synapses (host copies until finalize)
+    bool has_sparse = false;
+    int64_t S = 0;
+    int32_t max_delay = 1;
+    std::vector<int64_t> h_row_ptr;
+    std::vector<uint32_t> h_syn;
+    std::vector<float> h_w;
+    std::vector<uint8_t> h_plastic;
+    // dense
+    bool has_dense = false;
+    std::vector<float> h_wt;
+    int32_t dense_delay = 1;
+    // Poisson background
+    bool pois = false;
+    uint32_t seed_lo = 0, seed_hi = 0;
+    double lambda = 0.0;
+    float w_bg = 0.0f;
+    std::vector<uint32_t> pois_tab;
+    int32_t kmax = 0;
+    // STDP
+    bool stdp = false;
+    nbx_stdp_desc sd{};
+    // current
+    std::vector<float> h_cur;
+    int32_t cur_t0 = 0, cur_n = 0;
+
+    int32_t shift_req = 0, shift = 0;
+    bool finalized = false;
+    bool pull_mode = false;
+    int32_t D = 2;
+
+    // device buffers
+    std::vector<void*> allocs;
+    float *d_v = nullptr, *d_s1 = nullptr, *d_ga = nullptr, *d_gg = nullptr, *d_iext = nullptr;
+    int32_t* d_last = nullptr;
+    float *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_d = nullptr;
+    int32_t *d_ring = nullptr, *d_hist = nullptr, *d_hist_cnt = nullptr;
+    uint8_t* d_did_fire = nullptr;
+    int64_t* d_row_ptr = nullptr;
+    uint32_t* d_syn = nullptr;
+    float* d_w = nullptr;
+    uint8_t* d_plastic = nullptr;
+    int64_t* d_dptr = nullptr;
+    float* d_wt = nullptr;
+    uint32_t* d_pois_tab = nullptr;
+    float* d_cur = nullptr;
+    float *d_apre = nullptr, *d_apost = nullptr;
+    int32_t* d_slast = nullptr;
+    float *d_ltp_tab = nullptr, *d_ltd_tab = nullptr;
+    int32_t tab_len = 1;
+    int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
+    unsigned long long* d_events = nullptr;
+    int32_t* d_tbase = nullptr;
+    int32_t* d_err = nullptr;
+    int32_t* d_counts = nullptr;
+    uint32_t* d_raster = nullptr;
+    int32_t rec_t0 = 0, rec_n = 0, words = 0;
+
+    int32_t t = 0;
+    // graph engine
+    hipGraphExec_t gexec = nullptr;
+    int32_t gchunk = 0;
+    // per-phase sequence: 0 idle, 1 begun, 2 propagated, 3 updated, 4 learned
+    int32_t phase = 0;
+    // persistent engine
+    PersistentLayout pl;
+
+    StepArgs args() const {
+        StepArgs A{};
+        A.P = P;
+        A.n = n;
+        A.nch = nch;
+        A.D = D;
+        A.words = words;
+        A.v = d_v;
+        A.s1 = d_s1;
+        A.ga = d_ga;
+        A.gg = d_gg;
+        A.iext = d_iext;
+        A.last = d_last;
+        A.a = d_a;
+        A.b = d_b;
+        A.c = d_c;
+        A.d = d_d;
+        A.ring = d_ring;
+        A.hist = d_hist;
+        A.hist_cnt = d_hist_cnt;
+        A.did_fire = d_did_fire;
+        A.row_ptr = d_row_ptr;
+        A.syn = d_syn;
+        A.w = d_w;
+        A.dptr = d_dptr;
+        A.plastic = d_plastic;
+        A.wt = d_wt;
+        A.dense_delay = dense_delay;
+        A.scale = std::ldexp(1.0f, shift);
+        A.inv_scale = 1.0f / A.scale;
+        A.events = d_events;
+        A.pois = pois ? 1 : 0;
+        A.seed_lo = seed_lo;
+        A.seed_hi = seed_hi;
+        A.kmax = kmax;
+        A.pois_tab = d_pois_tab;
+        A.q_bg = pois ? (int32_t)std::lrint(w_bg * A.scale) : 0;
+        A.cur = d_cur;
+        A.cur_t0 = cur_t0;
+        A.cur_n = cur_n;
+        A.stdp = stdp ? 1 : 0;
+        A.max_ltp = sd.max_ltp;
+        A.max_ltd = sd.max_ltd;
+        A.w_max = sd.w_max;
+        A.apre = d_apre;
+        A.apost = d_apost;
+        A.slast = d_slast;
+        A.ltp_tab = d_ltp_tab;
+        A.ltd_tab = d_ltd_tab;
+        A.tab_len = tab_len;
+        A.in_ptr = d_in_ptr;
+        A.in_syn = d_in_syn;
+        A.counts = d_counts;
+        A.raster = d_raster;
+        A.rec_t0 = rec_t0;
+        A.rec_n = rec_n;
+        A.err = d_err;
+        return A;
+    }
+
+    template <typename T>
+    int32_t alloc(T** p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        void* q = nullptr;
+        NBX_HIP_TRY(hipMalloc(&q, count * sizeof(T)));
+        allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        return NBX_OK;
+    }
+    template <typename T>
+    int32_t upload(T** p, const T* src, size_t count) {
+        int32_t rc = alloc(p, count);
+        if (rc) return rc;
+        if (count) NBX_HIP_TRY(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+        return NBX_OK;
+    }
+    void release() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+
+namespace nbx {
+bool persistent_supported(const nbx_net* net);
+std::string persistent_reason(const nbx_net* net);
+int32_t build_persistent_layout(nbx_net* net);
+int32_t refresh_persistent_weights(nbx_net* net);
+int32_t run_persistent(nbx_net* net, int32_t nsteps);
+int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
+}  // namespace nbx

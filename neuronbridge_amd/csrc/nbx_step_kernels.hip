@@ -1,0 +1,323 @@
+// Per-step kernels of the launch-per-phase and graph engines.
+//
+//   nbx_update        neuron update of one step over SoA state; reads and clears the
+//                     step's delay-ring slot, adds Poisson background and external
+//                     current, integrates (Euler/RK4), and compacts spikes with a
+//                     wavefront ballot + LDS prefix into the firing-history ring
+//                     (reference: UpdateDenseUnifiedKernel, DenseUnifiedNeuronRuntime.cu:45-149,
+//                     and the legacy *_GPU_Interface_Update kernels)
+//   nbx_deliver_push  CSR fan-out of the step's spikes into ring[(t+d) % D]
+//                     (reference: TimeDrivenInternalSpike.cpp:37-69 + PropogatedSpikeGroup.cpp:33-71)
+//   nbx_deliver_dense dense W·s column gather into the ring (all-to-all networks)
+//   nbx_pull          arrival-time delivery from the firing-history ring with pre-side
+//                     STDP (reference: Mark/Drain + ApplyDenseLearningPreKernel,
+//                     GpuPropagationRuntime.cu:150-316)
+//   nbx_ltp           post-side STDP over each fired neuron's plastic inputs
+//                     (reference: STDP::ApplyPostSynaticSpike, STDP.cpp:37-61)
+#include "nbx_internal.h"
+
+namespace nbx {
+
+namespace {
+
+constexpr int kUpdateThreads = 256;
+constexpr int kWaves = kUpdateThreads / 64;
+
+__device__ inline int32_t step_of(const int32_t* tbase, int32_t k) { return tbase ? (*tbase + k) : k; }
+
+template <int MODEL>
+__global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    const int32_t i = blockIdx.x * kUpdateThreads + threadIdx.x;
+    const int32_t slot = t % A.D;
+    bool fired = false;
+    if (i < A.n) {
+        int32_t* r = A.ring + (int64_t)slot * A.nch * A.n;
+        const int32_t a0 = r[i];
+        r[i] = 0;
+        int32_t a1 = 0;
+        if (A.nch > 1) {
+            a1 = r[A.n + i];
+            r[A.n + i] = 0;
+        }
+        float V = A.v[i];
+        float S1 = (MODEL != NBX_MODEL_LIF_VJ) ? A.s1[i] : 0.0f;
+        float GA = (MODEL == NBX_MODEL_IZH) ? A.ga[i] : 0.0f;
+        float GG = (MODEL == NBX_MODEL_IZH) ? A.gg[i] : 0.0f;
+        float I = A.iext[i];
+        int32_t last = A.last[i];
+        fired = neuron_tick<MODEL>(A, i, t, a0, a1, V, S1, GA, GG, I, last);
+        A.v[i] = V;
+        if (MODEL != NBX_MODEL_LIF_VJ) A.s1[i] = S1;
+        if (MODEL == NBX_MODEL_IZH) {
+            A.ga[i] = GA;
+            A.gg[i] = GG;
+        }
+        A.iext[i] = I;
+        A.last[i] = last;
+        if (A.did_fire) A.did_fire[i] = fired ? 1 : 0;
+    }
+    // ---- spike compaction: ballot per wavefront, prefix over the block's wavefronts
+    const uint64_t mask = __ballot(fired);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    __shared__ int32_t s_wcnt[kWaves];
+    __shared__ int32_t s_base;
+    __shared__ unsigned long long s_ev;
+    if (threadIdx.x == 0) s_ev = 0;
+    if (lane == 0) s_wcnt[wave] = __popcll(mask);
+    if (A.raster && lane == 0 && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n) {
+        // 64 consecutive neurons per wavefront -> two aligned raster words
+        const int32_t i0 = blockIdx.x * kUpdateThreads + wave * 64;
+        uint32_t* row = A.raster + (int64_t)(t - A.rec_t0) * A.words;
+        if (i0 < A.n) row[i0 >> 5] = (uint32_t)mask;
+        if (i0 + 32 < A.n) row[(i0 >> 5) + 1] = (uint32_t)(mask >> 32);
+    }
+    __syncthreads();
+    // synaptic events are counted at the spike: out-degree (+ n for dense delivery)
+    if (fired) {
+        unsigned long long deg = A.row_ptr ? (unsigned long long)(A.row_ptr[i + 1] - A.row_ptr[i]) : 0ull;
+        if (A.wt) deg += (unsigned long long)A.n;
+        if (deg) atomicAdd(&s_ev, deg);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const int32_t c = s_wcnt[w];
+            s_wcnt[w] = tot;
+            tot += c;
+        }
+        if (s_ev) atomicAdd(A.events, s_ev);
+        s_base = tot ? atomicAdd(&A.hist_cnt[slot], tot) : 0;
+        if (tot && A.counts && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n) atomicAdd(&A.counts[t - A.rec_t0], tot);
+        // the slot of step t+1 held step t+1-D, which no later phase reads any more
+        if (blockIdx.x == 0) A.hist_cnt[(t + 1) % A.D] = 0;
+    }
+    __syncthreads();
+    if (fired) {
+        const int32_t pos = s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull));
+        A.hist[(int64_t)slot * A.n + pos] = i;
+    }
+}
+
+// Signed fixed-point contribution of one synapse and its ring channel.
+__device__ inline void syn_effect(const StepArgs& A, uint32_t e, float w, int32_t& ch, int32_t& q) {
+    q = quantize(w, A.scale);
+    ch = 0;
+    const int32_t type = syn_type(e);
+    if (A.P.model == NBX_MODEL_IZH) {
+        ch = (type == 1) ? 1 : 0;
+    } else if (A.P.model != NBX_MODEL_IZH_CUR) {
+        if (type == 1) q = -q;
+    }
+}
+
+constexpr int kDeliverThreads = 256;
+
+__global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_push(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    const int32_t slot = t % A.D;
+    const int32_t cnt = A.hist_cnt[slot];
+    const int32_t* ids = A.hist + (int64_t)slot * A.n;
+    for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const int32_t pre = ids[s];
+        const int64_t b = A.row_ptr[pre], e = A.row_ptr[pre + 1];
+        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+            const uint32_t en = A.syn[j];
+            int32_t ch, q;
+            syn_effect(A, en, A.w[j], ch, q);
+            const int32_t dslot = (t + syn_delay(en)) % A.D;
+            atomicAdd(&A.ring[((int64_t)dslot * A.nch + ch) * A.n + syn_post(en)], q);
+        }
+    }
+}
+
+// Dense delivery: thread per postsynaptic neuron, ascending-id loop over the
+// step's spikes reading column wt[pre][*] coalesced across the block.
+__global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_dense(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    const int32_t slot = t % A.D;
+    const int32_t cnt = A.hist_cnt[slot];
+    const int32_t* ids = A.hist + (int64_t)slot * A.n;
+    const int32_t i = blockIdx.x * kDeliverThreads + threadIdx.x;
+    if (i >= A.n || cnt == 0) return;
+    int32_t pos = 0, neg = 0;
+    for (int32_t s = 0; s < cnt; ++s) {
+        const float w = A.wt[(int64_t)ids[s] * A.n + i];
+        if (A.P.model == NBX_MODEL_IZH && w < 0.0f) neg += quantize(-w, A.scale);
+        else pos += quantize(w, A.scale);
+    }
+    const int32_t dslot = (t + A.dense_delay) % A.D;
+    int32_t* r = A.ring + (int64_t)dslot * A.nch * A.n;
+    atomicAdd(&r[i], pos);
+    if (A.nch > 1 && neg) atomicAdd(&r[A.n + i], neg);
+}
+
+// STDP_State::SetUpdate with the exponential looked up by the integer step delta.
+__device__ inline void stdp_decay(const StepArgs& A, int64_t s, int32_t t, float& apre, float& apost) {
+    int32_t dt = t - A.slast[s];
+    if (dt > A.tab_len - 1) dt = A.tab_len - 1;
+    apre = A.apre[s] * A.ltp_tab[dt];
+    apost = A.apost[s] * A.ltd_tab[dt];
+    A.slast[s] = t;
+}
+
+__device__ inline float stdp_clip(const StepArgs& A, float w) {
+    if (w < 0.0f) w = 0.0f;
+    else if (w > A.w_max) w = A.w_max;
+    return w;
+}
+
+// Arrival-time delivery: work items are (delay d, spike of step t-d) pairs; each
+// block walks the slice of synapses with that delay in the spiking neuron's row.
+__global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    __shared__ int32_t s_pref[kMaxDelay + 2];
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        s_pref[0] = 0;
+        for (int d = 1; d < A.D; ++d) {
+            s_pref[d] = tot;
+            if (t - d >= 0) tot += A.hist_cnt[(t - d) % A.D];
+        }
+        s_pref[A.D] = tot;
+    }
+    __syncthreads();
+    const int32_t total = s_pref[A.D];
+    int32_t* racc = A.ring + (int64_t)(t % A.D) * A.nch * A.n;
+    for (int32_t p = blockIdx.x; p < total; p += gridDim.x) {
+        int d = 1;
+        while (d + 1 < A.D && s_pref[d + 1] <= p) ++d;
+        const int32_t pre = A.hist[(int64_t)((t - d) % A.D) * A.n + (p - s_pref[d])];
+        const int64_t b = A.dptr[(int64_t)pre * (A.D + 1) + d];
+        const int64_t e = A.dptr[(int64_t)pre * (A.D + 1) + d + 1];
+        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+            const uint32_t en = A.syn[j];
+            const float w = A.w[j];
+            int32_t ch, q;
+            syn_effect(A, en, w, ch, q);
+            atomicAdd(&racc[(int64_t)ch * A.n + syn_post(en)], q);
+            if (A.stdp && A.plastic[j]) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
+                float apre, apost;
+                stdp_decay(A, j, t, apre, apost);
+                apre += 1.0f;
+                A.apre[j] = apre;
+                A.apost[j] = apost;
+                A.w[j] = stdp_clip(A, w + -A.max_ltd * apost);
+            }
+        }
+    }
+    // hist_cnt[t % D] belongs to step t-D, no longer read; the update of step t reuses it
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.hist_cnt[t % A.D] = 0;
+}
+
+__global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    const int32_t slot = t % A.D;
+    const int32_t cnt = A.hist_cnt[slot];
+    const int32_t* ids = A.hist + (int64_t)slot * A.n;
+    for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const int32_t p = ids[s];
+        const int64_t b = A.in_ptr[p], e = A.in_ptr[p + 1];
+        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+            const int64_t sid = A.in_syn[j];
+            float apre, apost;
+            stdp_decay(A, sid, t, apre, apost);
+            apost += 1.0f;
+            A.apre[sid] = apre;
+            A.apost[sid] = apost;
+            A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre);
+        }
+    }
+}
+
+__global__ void nbx_advance(int32_t* tbase, int32_t by) {
+    if (threadIdx.x == 0) *tbase += by;
+}
+
+// Injected spikes of step t: appended to the step's firing list (ids are distinct
+// from the neurons that fired on their own only if the caller makes them so).
+__global__ void nbx_inject(StepArgs A, const int32_t* ids, int32_t count, int32_t t) {
+    const int32_t slot = t % A.D;
+    __shared__ int32_t s_base;
+    if (threadIdx.x == 0) s_base = atomicAdd(&A.hist_cnt[slot], count);
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < count; j += blockDim.x) {
+        const int32_t id = ids[j];
+        A.hist[(int64_t)slot * A.n + s_base + j] = id;
+        if (A.did_fire) A.did_fire[id] = 1;
+        if (A.raster && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n)
+            atomicOr(&A.raster[(int64_t)(t - A.rec_t0) * A.words + (id >> 5)], 1u << (id & 31));
+    }
+    if (threadIdx.x == 0 && A.counts && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n)
+        atomicAdd(&A.counts[t - A.rec_t0], count);
+    // events of injected spikes, counted at the spike like those of the update kernel
+    unsigned long long ev = 0;
+    for (int32_t j = threadIdx.x; j < count; j += blockDim.x) {
+        const int32_t id = ids[j];
+        if (A.row_ptr) ev += (unsigned long long)(A.row_ptr[id + 1] - A.row_ptr[id]);
+        if (A.wt) ev += (unsigned long long)A.n;
+    }
+    if (ev) atomicAdd(A.events, ev);
+}
+
+__global__ void nbx_fill_i32(int32_t* p, int32_t value, int64_t count) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = value;
+}
+
+int deliver_grid() { return 1024; }
+
+}  // namespace
+
+hipError_t launch_update(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    const int grid = (A.n + kUpdateThreads - 1) / kUpdateThreads;
+    switch (A.P.model) {
+        case NBX_MODEL_LIF_VJ: nbx_update<NBX_MODEL_LIF_VJ><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+        case NBX_MODEL_LIF_EXP: nbx_update<NBX_MODEL_LIF_EXP><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+        case NBX_MODEL_IZH: nbx_update<NBX_MODEL_IZH><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+        default: nbx_update<NBX_MODEL_IZH_CUR><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_deliver_push(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_deliver_push<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_deliver_dense(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_deliver_dense<<<(A.n + kDeliverThreads - 1) / kDeliverThreads, kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_pull<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_ltp<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s) {
+    nbx_advance<<<1, 64, 0, s>>>(tbase, by);
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s) {
+    nbx_inject<<<1, 256, 0, s>>>(A, d_ids, count, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    nbx_fill_i32<<<(int)blocks, 256, 0, s>>>(p, value, count);
+    return hipGetLastError();
+}
+
+}  // namespace nbx

@@ -1,0 +1,292 @@
+"""Host-side wrapper of one simulated network on the GPU (numpy in, numpy out).
+
+Mirrors the reference runtime's per-step interface on this path:
+``GpuPropagationRuntime`` (BeginStep / RunPropagationOnly /
+CommitCurrentDeviceFiringsFromExternal / ApplyPostLearningFromDeviceFirings,
+src/native/gpu/include/gpu_runtime/GpuPropagationRuntime.h:85-144) and
+``DenseUnifiedNeuronRuntime::Step`` (DenseUnifiedNeuronRuntime.h:28-33), plus
+whole-run stepping (``Simulation.run``, python/src/neuronbridge/__init__.py:191-193).
+Every method goes through libnbx.so; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _p(a, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class ModelSpec:
+    """Neuron population parameters (reference model defaults where the model has them)."""
+    model: int
+    n: int
+    dt: float
+    integrator: int = L.EULER
+    t_ref: int = 0
+    # LIF (TimeDrivenLIF_Voltage_jump.h:22-32 / TimeDrivenLIF_Exponential_Decay.h defaults differ;
+    # callers state their own)
+    v_rest: float = -65.0
+    tau: float = 20.0
+    v_th: float = -50.0
+    r: float = 1.0
+    v_reset: float = -65.0
+    g_tau: float = 12.0
+    e_rev: float = 0.0
+    # Izhikevich (TimeDrivenIzhikevic_Exponential_Decay.h:28-52)
+    izh_v_th: float = 30.0
+    izh_r: float = 1.0
+    e_ampa: float = 0.0
+    e_gaba: float = -80.0
+    ampa_tau: float = 5.0
+    gaba_tau: float = 10.0
+    a: np.ndarray | None = None
+    b: np.ndarray | None = None
+    c: np.ndarray | None = None
+    d: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def desc(self) -> L.ModelDesc:
+        arrs = []
+        for v in (self.a, self.b, self.c, self.d):
+            arrs.append(None if v is None else np.ascontiguousarray(v, dtype=np.float32))
+        self._keep = arrs
+        return L.ModelDesc(
+            model=self.model, integrator=self.integrator, n=self.n, t_ref=self.t_ref, dt=self.dt,
+            v_rest=self.v_rest, tau=self.tau, v_th=self.v_th, r=self.r, v_reset=self.v_reset,
+            g_tau=self.g_tau, e_rev=self.e_rev, izh_v_th=self.izh_v_th, izh_r=self.izh_r,
+            e_ampa=self.e_ampa, e_gaba=self.e_gaba, ampa_tau=self.ampa_tau, gaba_tau=self.gaba_tau,
+            izh_a=_p(arrs[0], C.c_float), izh_b=_p(arrs[1], C.c_float),
+            izh_c=_p(arrs[2], C.c_float), izh_d=_p(arrs[3], C.c_float))
+
+    @property
+    def nstate(self) -> int:
+        return {L.LIF_VJ: 2, L.LIF_EXP: 3}.get(self.model, 6)
+
+
+@dataclass
+class Synapses:
+    """CSR by presynaptic neuron, rows sorted by (delay, post)."""
+    row_ptr: np.ndarray  # int64 [n+1]
+    post: np.ndarray     # int32 [S]
+    delay: np.ndarray    # uint8 [S]
+    type: np.ndarray     # uint8 [S]
+    weight: np.ndarray   # float32 [S]
+    max_delay: int
+    plastic: np.ndarray | None = None  # uint8 [S]
+
+    @property
+    def count(self) -> int:
+        return int(self.post.shape[0])
+
+
+def build_fixed_indegree(n: int, k_in: int, d_min: int, d_max: int, seed: int, n_exc: int,
+                         threads: int = 0) -> Synapses:
+    """Random fixed in-degree topology built by libnbx's host builder (weights left 0)."""
+    S = n * k_in
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    post = np.empty(S, dtype=np.int32)
+    delay = np.empty(S, dtype=np.uint8)
+    typ = np.empty(S, dtype=np.uint8)
+    L.call("nbx_build_fixed_indegree", n, k_in, d_min, d_max, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF,
+           n_exc, threads, _p(row_ptr, C.c_int64), _p(post, C.c_int32), _p(delay, C.c_uint8), _p(typ, C.c_uint8))
+    return Synapses(row_ptr, post, delay, typ, np.zeros(S, dtype=np.float32), d_max)
+
+
+class Net:
+    """One network resident on a GPU."""
+
+    def __init__(self, spec: ModelSpec, device: int = 0):
+        self.spec = spec
+        h = C.c_void_p()
+        desc = spec.desc()
+        L.call("nbx_net_create", C.byref(desc), device, C.byref(h))
+        self.h = h
+        self.n = spec.n
+        self.nstate = spec.nstate
+        self.syn: Synapses | None = None
+        self._rec = (0, 0, 0)
+
+    # ------------------------------------------------------------ configuration
+    def set_sparse(self, syn: Synapses) -> None:
+        self.syn = syn
+        d = L.SparseDesc(count=syn.count, max_delay=int(syn.max_delay),
+                         row_ptr=_p(syn.row_ptr, C.c_int64), post=_p(syn.post, C.c_int32),
+                         delay=_p(syn.delay, C.c_uint8), type=_p(syn.type, C.c_uint8),
+                         weight=_p(syn.weight, C.c_float),
+                         plastic=_p(syn.plastic, C.c_uint8) if syn.plastic is not None else None)
+        L.call("nbx_net_set_sparse", self.h, C.byref(d))
+
+    def set_dense(self, wt: np.ndarray, delay: int = 1) -> None:
+        wt = np.ascontiguousarray(wt, dtype=np.float32)
+        assert wt.shape == (self.n, self.n)
+        L.call("nbx_net_set_dense", self.h, _p(wt, C.c_float), int(delay))
+
+    def set_poisson(self, seed: int, lam: float, w_bg: float) -> None:
+        L.call("nbx_net_set_poisson", self.h, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF, float(lam), float(w_bg))
+
+    def set_stdp(self, max_ltp, ltp_tau, max_ltd, ltd_tau, w_max) -> None:
+        d = L.StdpDesc(max_ltp=max_ltp, ltp_tau=ltp_tau, max_ltd=max_ltd, ltd_tau=ltd_tau, w_max=w_max)
+        L.call("nbx_net_set_stdp", self.h, C.byref(d))
+
+    def set_current(self, cur: np.ndarray, t_begin: int = 0) -> None:
+        cur = np.ascontiguousarray(cur, dtype=np.float32)
+        assert cur.ndim == 2 and cur.shape[1] == self.n
+        L.call("nbx_net_set_current", self.h, _p(cur, C.c_float), int(t_begin), int(cur.shape[0]))
+
+    def set_fixed_shift(self, shift: int) -> None:
+        L.call("nbx_net_set_fixed_shift", self.h, int(shift))
+
+    def finalize(self) -> "Net":
+        L.call("nbx_net_finalize", self.h)
+        return self
+
+    @property
+    def fixed_shift(self) -> int:
+        s = C.c_int32()
+        L.call("nbx_net_get_fixed_shift", self.h, C.byref(s))
+        return s.value
+
+    # ------------------------------------------------------------ state
+    def set_state(self, x: np.ndarray, last: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.n, self.nstate)
+        last = np.ascontiguousarray(last, dtype=np.int32)
+        L.call("nbx_net_set_state", self.h, _p(x, C.c_float), _p(last, C.c_int32))
+
+    def get_state(self) -> tuple[np.ndarray, np.ndarray]:
+        x = np.zeros((self.n, self.nstate), dtype=np.float32)
+        last = np.zeros(self.n, dtype=np.int32)
+        L.call("nbx_net_get_state", self.h, _p(x, C.c_float), _p(last, C.c_int32))
+        return x, last
+
+    # ------------------------------------------------------------ running
+    def record(self, t_begin: int, nsteps: int, counts=True, raster=False) -> None:
+        flags = (L.RECORD_COUNTS if counts else 0) | (L.RECORD_RASTER if raster else 0)
+        L.call("nbx_net_record", self.h, int(t_begin), int(nsteps), flags)
+        self._rec = (t_begin, nsteps, flags)
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(self._rec[1], dtype=np.int32)
+        L.call("nbx_net_get_counts", self.h, _p(out, C.c_int32), out.shape[0])
+        return out
+
+    def raster(self) -> np.ndarray:
+        words = (self.n + 31) // 32
+        out = np.zeros((self._rec[1], words), dtype=np.uint32)
+        L.call("nbx_net_get_raster", self.h, _p(out, C.c_uint32), out.size)
+        return out
+
+    def run(self, nsteps: int, engine: int = L.ENGINE_AUTO) -> None:
+        L.call("nbx_net_run", self.h, int(nsteps), int(engine))
+
+    def run_async(self, nsteps: int, engine: int = L.ENGINE_AUTO) -> None:
+        L.call("nbx_net_run_async", self.h, int(nsteps), int(engine))
+
+    def sync(self) -> None:
+        L.call("nbx_net_sync", self.h)
+
+    def pick_engine(self) -> int:
+        e = C.c_int32()
+        L.call("nbx_net_pick_engine", self.h, C.byref(e))
+        return e.value
+
+    @property
+    def time(self) -> int:
+        t = C.c_int32()
+        L.call("nbx_net_time", self.h, C.byref(t))
+        return t.value
+
+    def events(self) -> int:
+        e = C.c_int64()
+        L.call("nbx_net_events", self.h, C.byref(e))
+        return e.value
+
+    @property
+    def stream(self) -> int:
+        return int(L.lib().nbx_net_stream(self.h) or 0)
+
+    # ------------------------------------------------------------ per-phase calls
+    def begin_step(self, t: int) -> None:
+        L.call("nbx_net_begin_step", self.h, int(t))
+
+    def run_propagation(self) -> None:
+        L.call("nbx_net_run_propagation", self.h)
+
+    def neuron_step(self) -> np.ndarray:
+        ids = np.zeros(self.n, dtype=np.int32)
+        cnt = C.c_int32()
+        L.call("nbx_net_neuron_step", self.h, _p(ids, C.c_int32), self.n, C.byref(cnt))
+        return ids[:cnt.value].copy()
+
+    def apply_post_learning(self) -> None:
+        L.call("nbx_net_apply_post_learning", self.h)
+
+    def inject_spikes(self, ids) -> None:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        L.call("nbx_net_inject_spikes", self.h, _p(ids, C.c_int32), int(ids.shape[0]))
+
+    def commit_firings(self) -> None:
+        L.call("nbx_net_commit_firings", self.h)
+
+    def history(self, t: int) -> np.ndarray:
+        ids = np.zeros(self.n, dtype=np.int32)
+        cnt = C.c_int32()
+        L.call("nbx_net_get_history", self.h, int(t), _p(ids, C.c_int32), self.n, C.byref(cnt))
+        return ids[:cnt.value].copy()
+
+    # ------------------------------------------------------------ buffers
+    def ring(self) -> np.ndarray:
+        slots, ch = C.c_int32(), C.c_int32()
+        L.call("nbx_net_ring_shape", self.h, C.byref(slots), C.byref(ch))
+        out = np.zeros((slots.value, ch.value, self.n), dtype=np.int32)
+        L.call("nbx_net_get_ring", self.h, _p(out, C.c_int32), out.size)
+        return out
+
+    def weights(self) -> np.ndarray:
+        out = np.zeros(self.syn.count, dtype=np.float32)
+        L.call("nbx_net_get_weights", self.h, _p(out, C.c_float), out.size)
+        return out
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        L.call("nbx_net_set_weights", self.h, _p(w, C.c_float), w.size)
+
+    def stdp_state(self):
+        S = self.syn.count
+        apre = np.zeros(S, np.float32)
+        apost = np.zeros(S, np.float32)
+        last = np.zeros(S, np.int32)
+        L.call("nbx_net_get_stdp_state", self.h, _p(apre, C.c_float), _p(apost, C.c_float), _p(last, C.c_int32), S)
+        return apre, apost, last
+
+    def replay(self, spikes_per_step: list, t0: int = 0) -> np.ndarray:
+        """Deliver injected spikes of steps t0..t0+K-1; returns arrivals [K][ch][n] of steps t0+1..t0+K."""
+        K = len(spikes_per_step)
+        tp = np.zeros(K + 1, dtype=np.int32)
+        tp[1:] = np.cumsum([len(s) for s in spikes_per_step])
+        ids = np.concatenate([np.asarray(s, dtype=np.int32) for s in spikes_per_step]) if tp[-1] else np.zeros(1, np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        ch = 2 if self.spec.model == L.IZH else 1
+        out = np.zeros((K, ch, self.n), dtype=np.int32)
+        L.call("nbx_replay_deliver", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), K, _p(out, C.c_int32))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.lib().nbx_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

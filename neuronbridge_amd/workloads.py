@@ -1,0 +1,142 @@
+"""Synthetic, seeded workloads of the benchmark configurations (inputs only).
+
+Each function returns a ``Workload``: model parameters, synapses, input settings
+and the initial state, at the configuration's full size by default and at any
+smaller size for tests. Seeds are ``0x5EED0000 + config number``.
+
+  sparse_lif  (config 2) LIF voltage-jump, fixed in-degree CSR, integer delays
+              U{1..20}, w_E = +0.1 mV, w_I = -0.5 mV (stored as type 1, +0.5),
+              tau_m 20 ms, V_th -50, V_reset = V_rest -65, t_ref 2 ms, dt 0.1 ms,
+              Poisson background 1000 sources x 10 Hz -> lambda = 1.0 per step, w 0.1 mV
+  dense_izh   (config 1) Izhikevich 800 RS / 200 FS, dense all-to-all signed
+              weights (exc U[0,0.5], inh U[-1,0]), current input 5N(0,1) / 2N(0,1)
+              per step, dt 0.1 ms
+  stdp_izh    (config 3) current-based Izhikevich RS/FS 80/20 (the polychronization
+              network form: each arrival adds its signed weight to one step's current),
+              in-degree 500, E->E plastic pair STDP (A+ 0.01, A- 0.012, tau 20 ms,
+              w in [0,1], initial U[0,0.5]), inhibitory U[-1,0], delays U{1..10},
+              Poisson input at 5 Hz from 1000 sources per neuron (w 1.3), dt 0.5 ms
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .network import ModelSpec, Net, Synapses, build_fixed_indegree
+
+SEED_BASE = 0x5EED0000
+
+
+@dataclass
+class Workload:
+    name: str
+    spec: ModelSpec
+    syn: Synapses | None = None
+    dense_wt: np.ndarray | None = None
+    dense_delay: int = 1
+    pois_seed: int = 0
+    pois_lambda: float = 0.0
+    w_bg: float = 0.0
+    cur: np.ndarray | None = None
+    stdp: dict | None = None
+    x0: np.ndarray | None = None
+    last0: np.ndarray | None = None
+    steps: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def d_slots(self) -> int:
+        md = 1
+        if self.syn is not None:
+            md = max(md, int(self.syn.max_delay))
+        if self.dense_wt is not None:
+            md = max(md, self.dense_delay)
+        return max(md + 1, 3)
+
+    def make_net(self, device: int = 0, fixed_shift: int = 0) -> Net:
+        net = Net(self.spec, device)
+        if self.syn is not None:
+            net.set_sparse(self.syn)
+        if self.dense_wt is not None:
+            net.set_dense(self.dense_wt, self.dense_delay)
+        if self.pois_lambda > 0:
+            net.set_poisson(self.pois_seed, self.pois_lambda, self.w_bg)
+        if self.stdp is not None:
+            net.set_stdp(**self.stdp)
+        if self.cur is not None:
+            net.set_current(self.cur, 0)
+        if fixed_shift:
+            net.set_fixed_shift(fixed_shift)
+        net.finalize()
+        if self.x0 is not None:
+            net.set_state(self.x0, self.last0)
+        return net
+
+
+def sparse_lif(n: int = 100_000, k_in: int = 1000, integrator: int = L.RK4, steps: int = 10_000,
+               seed: int = SEED_BASE + 2, threads: int = 0) -> Workload:
+    n_exc = int(round(0.8 * n))
+    spec = ModelSpec(model=L.LIF_VJ, n=n, dt=0.1, integrator=integrator, t_ref=20,
+                     v_rest=-65.0, tau=20.0, v_th=-50.0, r=1.0, v_reset=-65.0)
+    syn = build_fixed_indegree(n, k_in, 1, 20, seed, n_exc, threads)
+    syn.weight = np.where(syn.type == 0, np.float32(0.1), np.float32(0.5)).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    x0 = np.zeros((n, 2), dtype=np.float32)
+    x0[:, 0] = rng.uniform(-65.0, -50.0, n).astype(np.float32)
+    last0 = np.full(n, 10000, dtype=np.int32)
+    return Workload("sparse_lif", spec, syn=syn, pois_seed=seed, pois_lambda=1000 * 10.0 * 0.1e-3, w_bg=0.1,
+                    x0=x0, last0=last0, steps=steps, extra={"n_exc": n_exc, "k_in": k_in})
+
+
+def izh_params(n: int, n_exc: int, rng) -> tuple:
+    """Izhikevich (2003) RS/FS parameter draws: RS a=.02 b=.2 c=-65+15r^2 d=8-6r^2; FS a=.02+.08r b=.25-.05r c=-65 d=2."""
+    re = rng.random(n_exc)
+    ri = rng.random(n - n_exc)
+    a = np.concatenate([np.full(n_exc, 0.02), 0.02 + 0.08 * ri]).astype(np.float32)
+    b = np.concatenate([np.full(n_exc, 0.2), 0.25 - 0.05 * ri]).astype(np.float32)
+    c = np.concatenate([-65.0 + 15.0 * re ** 2, np.full(n - n_exc, -65.0)]).astype(np.float32)
+    d = np.concatenate([8.0 - 6.0 * re ** 2, np.full(n - n_exc, 2.0)]).astype(np.float32)
+    return a, b, c, d
+
+
+def dense_izh(n: int = 1000, steps: int = 10_000, integrator: int = L.RK4, seed: int = SEED_BASE + 1,
+              noise_scale: float = 1.0) -> Workload:
+    """noise_scale multiplies the per-step current draws (1.0 = the configuration as stated; at dt 0.1 ms
+    sqrt(10) gives the per-millisecond noise variance of the original 1 ms Izhikevich network)."""
+    n_exc = int(round(0.8 * n))
+    rng = np.random.default_rng(seed)
+    a, b, c, d = izh_params(n, n_exc, rng)
+    spec = ModelSpec(model=L.IZH_CUR, n=n, dt=0.1, integrator=integrator, v_rest=-65.0, izh_v_th=30.0, izh_r=1.0,
+                     a=a, b=b, c=c, d=d)
+    wt = np.empty((n, n), dtype=np.float32)  # wt[pre][post]
+    wt[:n_exc] = rng.uniform(0.0, 0.5, (n_exc, n)).astype(np.float32)
+    wt[n_exc:] = rng.uniform(-1.0, 0.0, (n - n_exc, n)).astype(np.float32)
+    cur = np.empty((steps, n), dtype=np.float32)
+    cur[:, :n_exc] = (noise_scale * 5.0 * rng.standard_normal((steps, n_exc))).astype(np.float32)
+    cur[:, n_exc:] = (noise_scale * 2.0 * rng.standard_normal((steps, n - n_exc))).astype(np.float32)
+    return Workload("dense_izh", spec, dense_wt=wt, dense_delay=1, cur=cur, steps=steps, extra={"n_exc": n_exc})
+
+
+def stdp_izh(n: int = 50_000, k_in: int = 500, steps: int = 20_000, integrator: int = L.EULER,
+             seed: int = SEED_BASE + 3, pois_rate_hz: float = 5.0, pois_sources: int = 1000, w_bg: float = 1.3,
+             w_inh: float = 1.0, threads: int = 0) -> Workload:
+    n_exc = int(round(0.8 * n))
+    rng = np.random.default_rng(seed)
+    a, b, c, d = izh_params(n, n_exc, rng)
+    dt = 0.5
+    spec = ModelSpec(model=L.IZH_CUR, n=n, dt=dt, integrator=integrator, v_rest=-65.0, izh_v_th=30.0, izh_r=1.0,
+                     a=a, b=b, c=c, d=d)
+    syn = build_fixed_indegree(n, k_in, 1, 10, seed, n_exc, threads)
+    S = syn.count
+    pre = np.repeat(np.arange(n, dtype=np.int32), np.diff(syn.row_ptr))
+    w = rng.uniform(0.0, 0.5, S).astype(np.float32)
+    inh = syn.type == 1
+    w[inh] = rng.uniform(-w_inh, 0.0, int(inh.sum())).astype(np.float32)  # signed current weights
+    syn.weight = w
+    syn.plastic = ((pre < n_exc) & (syn.post < n_exc)).astype(np.uint8)
+    stdp = dict(max_ltp=0.01, ltp_tau=20.0, max_ltd=0.012, ltd_tau=20.0, w_max=1.0)
+    lam = pois_sources * pois_rate_hz * dt * 1e-3
+    return Workload("stdp_izh", spec, syn=syn, pois_seed=seed, pois_lambda=lam, w_bg=w_bg, stdp=stdp,
+                    steps=steps, extra={"n_exc": n_exc, "k_in": k_in})

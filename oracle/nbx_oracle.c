@@ -1,0 +1,433 @@
+/*
+ * CPU oracle for the NeuronBridge per-timestep hot path (see nbx_oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY — never linked into or called by the product library.
+ * Built with -ffp-contract=off so every float operation rounds exactly as written,
+ * matching the reference's x86-64 build (no FMA contraction, SSE float arithmetic).
+ *
+ * Parity pins, per function (what checks it, so a dropped term or wrong index fails):
+ *   orc_philox4x32_10      Random123 known-answer vectors (tests/test_oracle.py)
+ *   orc_hash_uniform01     reference poisson_hash restatement in numerical_accuracy_tests.py:118-128
+ *                          and its expected sequence rule (:139); brute force vs Python
+ *   orc_poisson_*          chi-square against the Poisson pmf; thresholds vs closed form
+ *   orc_update_f32 (Euler) reference KATs: numerical_accuracy_report.md (Izhikevich first run
+ *                          V=-65.03999328613281, U=-13.01200008392334; voltage jump -5 -> -4.5 -> -3.55)
+ *                          and the LIF closed form of numerical_accuracy_tests.py:54-64
+ *   orc_update_f32 (RK4)   closed form for the linear LIF ODE (growth factor 1-h+h^2/2-h^3/6+h^4/24),
+ *                          4th-order convergence, FP64 instantiation
+ *   orc_run delivery       brute-force push-ring image checked against the pull delivery every tick
+ *                          (returns -2 on mismatch); delay KAT of tests/native/dense_cuda_smoke.cpp:109-118
+ *   STDP                   closed-form pair updates; replay scenario of
+ *                          tests/model_codegen/pair_stdp_gpu_trajectory.cpp:52-66
+ */
+#include "nbx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int orc_nstate(int model) {
+    switch (model) {
+        case ORC_LIF_VJ: return 2;  /* V, I_ext            (TimeDrivenLIF_Voltage_jump.h:34) */
+        case ORC_LIF_EXP: return 3; /* V, g, I_ext         (TimeDrivenLIF_Exponential_Decay.h) */
+        default: return 6;          /* v, u, g_ampa, g_gaba, g_nmda, I_ext (Izhikevic .h) */
+    }
+}
+
+/* exp(-dt * (1/tau)) the way ResetConductanceDecayLookup computes it
+ * (TimeDrivenLIF_Exponential_Decay.cpp:37-39, TimeDrivenIzhikevic_Exponential_Decay.cpp:32-39). */
+static float ref_decay(float dt, float tau) {
+    const float inv = (tau != 0.0f) ? (1.0f / tau) : 0.0f;
+    return expf(-dt * inv);
+}
+
+void orc_model_prepare(orc_model* m) {
+    m->g_decay = ref_decay(m->dt, m->g_tau);
+    m->ampa_decay = ref_decay(m->dt, m->ampa_tau);
+    m->gaba_decay = ref_decay(m->dt, m->gaba_tau);
+}
+
+#define REAL float
+#define FN(name) name##_f32
+#include "orc_model.inc"
+#undef REAL
+#undef FN
+#define REAL double
+#define FN(name) name##_f64
+#include "orc_model.inc"
+#undef REAL
+#undef FN
+
+int32_t orc_update_f32(const orc_model* m, float* x, int32_t* last, const float* isyn, int32_t* spikes) {
+    return update_f32(m, x, last, isyn, spikes);
+}
+int32_t orc_update_f64(const orc_model* m, double* x, int32_t* last, const double* isyn, int32_t* spikes) {
+    return update_f64(m, x, last, isyn, spikes);
+}
+
+/* ---------------------------------------------------------------- random numbers */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* PoissonRate.cpp:7-21 */
+float orc_hash_uniform01(int32_t neuron, int32_t t) {
+    uint32_t x = (uint32_t)neuron * 747796405u + (uint32_t)(t + 1) * 2891336453u + 0x9e3779b9u;
+    x ^= x >> 16;
+    x *= 2246822519u;
+    x ^= x >> 13;
+    x *= 3266489917u;
+    x ^= x >> 16;
+    return (float)(x & 0x00FFFFFFu) / (float)0x01000000u;
+}
+
+int32_t orc_poisson_thresholds(double lambda, uint32_t* tab, int32_t cap) {
+    double p = exp(-lambda), cdf = 0.0;
+    int32_t k = 0;
+    for (; k < cap; ++k) {
+        cdf += p;
+        double v = floor(cdf * 4294967296.0);
+        uint32_t t = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+        tab[k] = t;
+        if (t == 0xFFFFFFFFu) return k;
+        p = p * lambda / (double)(k + 1);
+    }
+    return cap - 1;
+}
+
+#define ORC_POIS_STREAM 0x504F4953u /* "POIS" */
+
+int32_t orc_poisson_count(uint32_t seed_lo, uint32_t seed_hi, int32_t neuron, int32_t t, const uint32_t* tab,
+                          int32_t kmax) {
+    const uint32_t ctr[4] = {(uint32_t)neuron, (uint32_t)t, ORC_POIS_STREAM, 0u};
+    const uint32_t key[2] = {seed_lo, seed_hi};
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    int32_t k = 0;
+    while (k < kmax && out[0] >= tab[k]) ++k;
+    return k;
+}
+
+int32_t orc_quantize(float w, int32_t shift) { return (int32_t)lrintf(w * ldexpf(1.0f, shift)); }
+
+void orc_init_state(const orc_model* m, float* x, int32_t* last) {
+    const int ns = orc_nstate(m->model);
+    for (int i = 0; i < m->n; ++i) {
+        float* s = x + (size_t)i * ns;
+        memset(s, 0, sizeof(float) * (size_t)ns);
+        if (m->model == ORC_LIF_VJ) {
+            s[0] = m->v_reset; /* TimeDrivenLIF_Voltage_jump.cpp:37 */
+        } else if (m->model == ORC_LIF_EXP) {
+            s[0] = m->v_rest; /* TimeDrivenLIF_Exponential_Decay.cpp:51 */
+        } else {
+            s[0] = m->v_rest;            /* V_rest + init[0], Izhikevic .cpp:49 */
+            s[1] = m->b[i] * m->v_rest;  /* b * V_rest + init[1] */
+        }
+        last[i] = 10000; /* Neuron_State_Vector.cpp:43-67 */
+    }
+}
+
+/* ---------------------------------------------------------------- network run */
+
+typedef struct {
+    orc_net* net;
+    int32_t acc_mode;
+    float* x;
+    int32_t* last;
+    int32_t* acc0;
+    int32_t* acc1;
+    float* isyn;
+    float scale;
+    int ns;
+} run_ctx;
+
+/* One synaptic arrival. ProcessSpike semantics:
+ *   voltage jump: V += w (exc) / V -= w (inh) only outside refractory (Voltage_jump.cpp:100-113)
+ *   exp decay:    g += w / g -= w, ungated (Exponential_Decay.cpp:108-119)
+ *   Izhikevich:   g_ampa += w (exc), g_gaba += w (inh) (Izhikevic .cpp:119-131)
+ *   current-based Izhikevich: signed w into this tick's synaptic current. */
+static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
+    const orc_model* m = &c->net->m;
+    float* s = c->x + (size_t)post * c->ns;
+    if (c->acc_mode == ORC_ACC_FIXED) {
+        const int32_t q = (int32_t)lrintf(w * c->scale);
+        switch (m->model) {
+            case ORC_LIF_VJ:
+            case ORC_LIF_EXP: c->acc0[post] += (type == 1) ? -q : q; break;
+            case ORC_IZH:
+                if (type == 1) c->acc1[post] += q;
+                else c->acc0[post] += q;
+                break;
+            default: c->acc0[post] += q; break;
+        }
+        return;
+    }
+    switch (m->model) {
+        case ORC_LIF_VJ:
+            if (c->last[post] > m->t_ref) s[0] += (type == 1) ? -w : w;
+            break;
+        case ORC_LIF_EXP: s[1] += (type == 1) ? -w : w; break;
+        case ORC_IZH:
+            if (type == 1) s[3] += w;
+            else s[2] += w;
+            break;
+        default: c->isyn[post] += w; break;
+    }
+}
+
+/* Fixed-point arrivals are applied once per tick, before the update. */
+static void apply_fixed(run_ctx* c) {
+    const orc_model* m = &c->net->m;
+    const float inv = 1.0f / c->scale;
+    for (int i = 0; i < m->n; ++i) {
+        float* s = c->x + (size_t)i * c->ns;
+        const float a0 = (float)c->acc0[i] * inv;
+        switch (m->model) {
+            case ORC_LIF_VJ:
+                if (c->last[i] > m->t_ref) s[0] += a0;
+                break;
+            case ORC_LIF_EXP: s[1] += a0; break;
+            case ORC_IZH:
+                s[2] += a0;
+                s[3] += (float)c->acc1[i] * inv;
+                break;
+            default: c->isyn[i] = a0; break;
+        }
+    }
+}
+
+/* STDP_State::SetUpdate (STDP_State.cpp:17-26) */
+static void stdp_set_update(orc_net* net, int64_t s, int32_t t, float inv_ltp, float inv_ltd) {
+    const float dT = (float)(t - net->slast[s]) * net->m.dt;
+    net->apre[s] *= expf(-dT * inv_ltp);
+    net->apost[s] *= expf(-dT * inv_ltd);
+    net->slast[s] = t;
+}
+
+static void stdp_clip(orc_net* net, int64_t s) {
+    if (net->w[s] < 0.0f) net->w[s] = 0.0f;
+    else if (net->w[s] > net->w_max) net->w[s] = net->w_max;
+}
+
+int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t* hist, int32_t* hist_cnt,
+                int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
+                int64_t* events) {
+    orc_model* m = &net->m;
+    orc_model_prepare(m);
+    const int n = m->n;
+    const int D = net->d_slots;
+    const int words = (n + 31) / 32;
+    run_ctx c;
+    c.net = net;
+    c.acc_mode = acc_mode;
+    c.x = x;
+    c.last = last;
+    c.ns = orc_nstate(m->model);
+    c.scale = ldexpf(1.0f, net->fixed_shift);
+    c.acc0 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    c.acc1 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    c.isyn = (float*)calloc((size_t)n, sizeof(float));
+    int32_t* spikes = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    uint32_t tab[64];
+    int32_t kmax = 0;
+    if (net->pois_on) kmax = orc_poisson_thresholds(net->pois_lambda, tab, 64);
+    const float inv_ltp = (float)(1.0 / (double)net->ltp_tau); /* STDP_State.cpp:6-7 */
+    const float inv_ltd = (float)(1.0 / (double)net->ltd_tau);
+    const int iidx = (m->model == ORC_LIF_VJ) ? 1 : (m->model == ORC_LIF_EXP ? 2 : 5);
+    int64_t ev = 0;
+    int32_t status = 0;
+
+    for (int32_t t = t0; t < t0 + nticks; ++t) {
+        memset(c.acc0, 0, sizeof(int32_t) * (size_t)n);
+        memset(c.acc1, 0, sizeof(int32_t) * (size_t)n);
+        memset(c.isyn, 0, sizeof(float) * (size_t)n);
+        /* external current replaces I_ext for the tick (stateful current input) */
+        if (net->cur) {
+            const float* cr = net->cur + (size_t)(t - net->cur_t_begin) * n;
+            for (int i = 0; i < n; ++i) x[(size_t)i * c.ns + iidx] = cr[i];
+        }
+        /* sparse arrivals: spikes fired at t-d through synapses of delay d
+         * (group time = t_spike + delay, TimeDrivenInternalSpike.cpp:17; delivery
+         * then pre-learning per synapse, PropogatedSpikeGroup.cpp:42-58) */
+        if (net->row_ptr) {
+            for (int d = 1; d < D; ++d) {
+                const int32_t ts = t - d;
+                if (ts < 0) continue;
+                const int slot = ts % D;
+                const int32_t* hl = hist + (size_t)slot * n;
+                for (int k = 0; k < hist_cnt[slot]; ++k) {
+                    const int32_t pre = hl[k];
+                    /* rows are sorted by delay: binary search the start of the delay-d run */
+                    int64_t lo = net->row_ptr[pre], hi = net->row_ptr[pre + 1];
+                    while (lo < hi) {
+                        const int64_t mid = lo + (hi - lo) / 2;
+                        if (net->delay[mid] < d) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    for (int64_t s = lo; s < net->row_ptr[pre + 1] && net->delay[s] == d; ++s) {
+                        arrive(&c, net->post[s], net->type[s], net->w[s]);
+                        if (net->stdp_on && net->plastic[s]) { /* STDP.cpp:19-35 */
+                            stdp_set_update(net, s, t, inv_ltp, inv_ltd);
+                            net->apre[s] += 1.0f;
+                            net->w[s] += -net->max_ltd * net->apost[s];
+                            stdp_clip(net, s);
+                        }
+                    }
+                }
+            }
+        }
+        /* dense all-to-all arrivals (column of the fired presynaptic neuron) */
+        if (net->dense_wt && t - net->dense_delay >= 0) {
+            const int slot = (t - net->dense_delay) % D;
+            const int32_t* hl = hist + (size_t)slot * n;
+            for (int k = 0; k < hist_cnt[slot]; ++k) {
+                const float* col = net->dense_wt + (size_t)hl[k] * n;
+                for (int i = 0; i < n; ++i) {
+                    float w = col[i];
+                    int32_t type = 0;
+                    if (m->model != ORC_IZH_CUR && w < 0.0f) {
+                        type = 1;
+                        w = -w;
+                    }
+                    arrive(&c, i, type, w);
+                }
+            }
+        }
+        /* push-ring image: must hold exactly the pull result for this tick */
+        if (ring && acc_mode == ORC_ACC_FIXED) {
+            int32_t* r = ring + (size_t)(t % D) * 2 * n;
+            for (int i = 0; i < n; ++i) {
+                if (r[i] != c.acc0[i] || r[n + i] != c.acc1[i]) status = -2;
+                r[i] = 0;
+                r[n + i] = 0;
+            }
+        }
+        /* Poisson background: count excitatory arrivals of weight w_bg */
+        if (net->pois_on) {
+            for (int i = 0; i < n; ++i) {
+                const int32_t cnt = orc_poisson_count(net->seed_lo, net->seed_hi, i, t, tab, kmax);
+                for (int k = 0; k < cnt; ++k) arrive(&c, i, 0, net->w_bg);
+            }
+        }
+        if (acc_mode == ORC_ACC_FIXED) apply_fixed(&c);
+
+        const int32_t nspk = update_f32(m, x, last, m->model == ORC_IZH_CUR ? c.isyn : NULL, spikes);
+
+        /* post-synaptic learning at the spike tick (TimeDrivenInternalSpike.cpp:70-82, STDP.cpp:37-61) */
+        if (net->stdp_on && net->in_ptr) {
+            for (int k = 0; k < nspk; ++k) {
+                const int32_t p = spikes[k];
+                for (int64_t j = net->in_ptr[p]; j < net->in_ptr[p + 1]; ++j) {
+                    const int64_t s = net->in_syn[j];
+                    stdp_set_update(net, s, t, inv_ltp, inv_ltd);
+                    net->apost[s] += 1.0f;
+                    net->w[s] += net->max_ltp * net->apre[s];
+                    stdp_clip(net, s);
+                }
+            }
+        }
+        /* push-ring image of this tick's spikes */
+        if (ring && acc_mode == ORC_ACC_FIXED) {
+            for (int k = 0; k < nspk; ++k) {
+                const int32_t pre = spikes[k];
+                if (net->row_ptr) {
+                    for (int64_t s = net->row_ptr[pre]; s < net->row_ptr[pre + 1]; ++s) {
+                        int32_t* r = ring + (size_t)((t + net->delay[s]) % D) * 2 * n;
+                        const int32_t q = (int32_t)lrintf(net->w[s] * c.scale);
+                        const int32_t p = net->post[s];
+                        if (m->model == ORC_IZH) {
+                            if (net->type[s] == 1) r[n + p] += q;
+                            else r[p] += q;
+                        } else if (m->model == ORC_IZH_CUR) {
+                            r[p] += q;
+                        } else {
+                            r[p] += (net->type[s] == 1) ? -q : q;
+                        }
+                    }
+                }
+                if (net->dense_wt) {
+                    int32_t* r = ring + (size_t)((t + net->dense_delay) % D) * 2 * n;
+                    const float* col = net->dense_wt + (size_t)pre * n;
+                    for (int i = 0; i < n; ++i) {
+                        float w = col[i];
+                        if (m->model == ORC_IZH && w < 0.0f) r[n + i] += (int32_t)lrintf(-w * c.scale);
+                        else if (m->model == ORC_IZH_CUR || m->model == ORC_IZH) r[i] += (int32_t)lrintf(w * c.scale);
+                        else r[i] += (w < 0.0f) ? -(int32_t)lrintf(-w * c.scale) : (int32_t)lrintf(w * c.scale);
+                    }
+                }
+            }
+        }
+        /* synaptic events: one per outgoing synapse of every spike, counted at the spike */
+        for (int k = 0; k < nspk; ++k) {
+            if (net->row_ptr) ev += net->row_ptr[spikes[k] + 1] - net->row_ptr[spikes[k]];
+            if (net->dense_wt) ev += n;
+        }
+        /* record history, counts and raster */
+        const int slot = t % D;
+        memcpy(hist + (size_t)slot * n, spikes, sizeof(int32_t) * (size_t)nspk);
+        hist_cnt[slot] = nspk;
+        if (spike_counts) spike_counts[t - t0] = nspk;
+        if (raster) {
+            uint32_t* row = raster + (size_t)(t - t0) * words;
+            for (int k = 0; k < nspk; ++k) row[spikes[k] >> 5] |= 1u << (spikes[k] & 31);
+        }
+    }
+    if (events) *events += ev;
+    free(c.acc0);
+    free(c.acc1);
+    free(c.isyn);
+    free(spikes);
+    return status;
+}
+
+void orc_stdp_replay(float max_ltp, float ltp_tau, float max_ltd, float ltd_tau, float dt, float w0, float w_max,
+                     const uint8_t* pre, const uint8_t* post, int32_t nticks, float* w_out) {
+    float w = w0, apre = 0.0f, apost = 0.0f;
+    int32_t lastu = 0;
+    const float inv_ltp = (float)(1.0 / (double)ltp_tau);
+    const float inv_ltd = (float)(1.0 / (double)ltd_tau);
+    w_out[0] = w;
+    for (int32_t t = 0; t < nticks; ++t) {
+        if (pre[t] || post[t]) {
+            for (int side = 0; side < 2; ++side) {
+                if (!(side == 0 ? pre[t] : post[t])) continue;
+                const float dT = (float)(t - lastu) * dt;
+                apre *= expf(-dT * inv_ltp);
+                apost *= expf(-dT * inv_ltd);
+                lastu = t;
+                if (side == 0) {
+                    apre += 1.0f;
+                    w += -max_ltd * apost;
+                } else {
+                    apost += 1.0f;
+                    w += max_ltp * apre;
+                }
+                if (w < 0.0f) w = 0.0f;
+                else if (w > w_max) w = w_max;
+            }
+        }
+        w_out[t + 1] = w;
+    }
+}

@@ -1,0 +1,128 @@
+/*
+ * CPU oracle for the NeuronBridge per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
+ * legacy CPU/OpenMP semantics (neuron update, spike delivery through integer
+ * delays, lazy pair STDP) used to check the HIP library. Only tests, the smoke
+ * check and the CPU-baseline leg of the benchmark load it; the product library
+ * never links it and never falls back to it.
+ *
+ * Reference files restated (paths relative to the reference root;
+ * L/ = src/native/legacy/source_file_realtime_v1_async/):
+ *   L/Intergration/inc/FixStep/ForwardEulerMethod.h:29-55
+ *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Voltage_jump.cpp:49-115
+ *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Exponential_Decay.cpp:37-121
+ *   L/NeuralModel/src/TimeDriven/TimeDrivenIzhikevic_Exponential_Decay.cpp:32-131
+ *   L/NeuralModel/src/TimeDriven/PoissonRate.cpp:7-21
+ *   L/NeuralModel/src/NeuralStateVector/Neuron_State_Vector.cpp:43-67
+ *   L/Event/inc/Event.h:10 (deliveries before updates within a tick)
+ *   L/Event/src/Spike/PropogatedSpikeGroup.cpp:33-71 (ProcessSpike, then pre-learning)
+ *   L/Event/src/Spike/TimeDrivenInternalSpike.cpp:11-144 (group time = t + delay; post-learning)
+ *   L/LearningRule/src/Rule/STDP.cpp:19-61, L/LearningRule/src/State/STDP_State.cpp:17-34
+ *
+ * Parity status of each function is listed in the header comment of nbx_oracle.c.
+ */
+#ifndef NBX_ORACLE_H
+#define NBX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_LIF_VJ = 0, ORC_LIF_EXP = 1, ORC_IZH = 2, ORC_IZH_CUR = 3 };
+enum { ORC_EULER = 0, ORC_RK4 = 1 };
+enum { ORC_ACC_FIXED = 0, ORC_ACC_FLOAT_EVENTS = 1 };
+
+typedef struct orc_model {
+    int32_t model;       /* ORC_* model id */
+    int32_t integrator;  /* ORC_EULER | ORC_RK4 */
+    int32_t n;           /* neurons */
+    int32_t t_ref;       /* LIF refractory period in steps */
+    float dt;            /* ms */
+    /* LIF (model-wide) */
+    float v_rest, tau, v_th, r, v_reset, g_tau, e_rev;
+    /* Izhikevich (model-wide) */
+    float izh_v_th, izh_r, e_ampa, e_gaba, ampa_tau, gaba_tau;
+    /* Izhikevich per-neuron a, b, c, d */
+    const float *a, *b, *c, *d;
+    /* filled by orc_model_prepare: exp(-dt/tau) factors computed the reference way */
+    float g_decay, ampa_decay, gaba_decay;
+} orc_model;
+
+typedef struct orc_net {
+    orc_model m;
+    int32_t d_slots;         /* delay ring slots = max_delay + 1 */
+    int32_t fixed_shift;     /* arrivals accumulate as int32 in units of 2^-fixed_shift */
+    /* sparse synapses, CSR by presynaptic neuron, rows sorted by (delay, post) */
+    const int64_t* row_ptr;  /* n + 1 (NULL: no sparse synapses) */
+    const int32_t* post;
+    const uint8_t* delay;    /* 1 .. d_slots-1 */
+    const uint8_t* type;     /* 0 excitatory, 1 inhibitory */
+    float* w;                /* magnitudes (signed for ORC_IZH_CUR); STDP updates in place */
+    /* dense all-to-all delivery: wt[pre*n + post] (NULL: none), fixed delay */
+    const float* dense_wt;
+    int32_t dense_delay;
+    /* Poisson background: count ~ Poisson(lambda) per (neuron, tick), w_bg excitatory */
+    int32_t pois_on;
+    uint32_t seed_lo, seed_hi;
+    double pois_lambda;
+    float w_bg;
+    /* external current per tick: cur[(t - t_begin) * n + i] replaces I_ext (NULL: none) */
+    const float* cur;
+    int32_t cur_t_begin;
+    /* pair STDP (reference STDP::Apply*SynaticSpike) */
+    int32_t stdp_on;
+    const uint8_t* plastic;  /* per synapse */
+    float max_ltp, ltp_tau, max_ltd, ltd_tau, w_max;
+    float* apre;             /* per synapse */
+    float* apost;
+    int32_t* slast;
+    const int64_t* in_ptr;   /* n + 1: plastic incoming synapses of each post */
+    const int64_t* in_syn;   /* synapse ids (CSR positions) */
+} orc_net;
+
+int orc_nstate(int model);
+void orc_model_prepare(orc_model* m);
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* PoissonRate hash (PoissonRate.cpp:7-21): 24-bit uniform in [0,1). */
+float orc_hash_uniform01(int32_t neuron, int32_t t);
+/* Inverse-CDF thresholds of Poisson(lambda) in units of 2^-32; returns kmax. */
+int32_t orc_poisson_thresholds(double lambda, uint32_t* tab, int32_t cap);
+int32_t orc_poisson_count(uint32_t seed_lo, uint32_t seed_hi, int32_t neuron, int32_t t,
+                          const uint32_t* tab, int32_t kmax);
+/* fixed-point image of a weight: rint(w * 2^shift) */
+int32_t orc_quantize(float w, int32_t shift);
+
+/* initial state like Neuron_State_Vector::InitNeuronState with sigma = 0 */
+void orc_init_state(const orc_model* m, float* x, int32_t* last);
+
+/* one neuron-update pass (FP32 reference order / FP64); returns spike count */
+int32_t orc_update_f32(const orc_model* m, float* x, int32_t* last, const float* isyn, int32_t* spikes);
+int32_t orc_update_f64(const orc_model* m, double* x, int32_t* last, const double* isyn, int32_t* spikes);
+
+/*
+ * Closed-loop network run for ticks [t0, t0 + nticks).
+ *   acc_mode ORC_ACC_FIXED:        arrivals accumulate in int32 fixed point, applied once per tick
+ *   acc_mode ORC_ACC_FLOAT_EVENTS: every arrival is applied as its own float add (reference order)
+ * hist/hist_cnt carry the last d_slots ticks of spike lists between calls:
+ *   hist[(t % d_slots) * n + k], hist_cnt[t % d_slots].
+ * ring (optional, fixed mode, non-plastic): push-delivery image ring[slot * 2n + ch * n + i]
+ *   maintained alongside the pull delivery so tests can compare device ring contents.
+ * Outputs: spike_counts[nticks], raster (optional) bits [nticks][ceil(n/32)], events delivered.
+ */
+int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t* hist, int32_t* hist_cnt,
+                int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
+                int64_t* events);
+
+/* STDP replay on one synapse for a given event schedule (pair_stdp_gpu_trajectory.cpp:52-66). */
+void orc_stdp_replay(float max_ltp, float ltp_tau, float max_ltd, float ltd_tau, float dt, float w0, float w_max,
+                     const uint8_t* pre, const uint8_t* post, int32_t nticks, float* w_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+"""Test helpers: run a workload through the CPU oracle (the checker)."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+import oracle as O
+from neuronbridge_amd import _lib as L
+
+MODEL_MAP = {L.LIF_VJ: O.LIF_VJ, L.LIF_EXP: O.LIF_EXP, L.IZH: O.IZH, L.IZH_CUR: O.IZH_CUR}
+
+
+def auto_shift(wl) -> int:
+    """Largest fixed-point shift with no possible int32 overflow (same rule as libnbx)."""
+    n = wl.spec.n
+    bound = np.zeros(n, dtype=np.float64)
+    if wl.syn is not None:
+        w = np.abs(wl.syn.weight.astype(np.float64))
+        if wl.stdp is not None and wl.syn.plastic is not None:
+            w = np.where(wl.syn.plastic != 0, np.maximum(w, wl.stdp["w_max"]), w)
+        bound += np.bincount(wl.syn.post, weights=w, minlength=n)
+    if wl.dense_wt is not None:
+        bound += np.abs(wl.dense_wt.astype(np.float64)).sum(axis=0)
+    B = float(bound.max()) if n else 0.0
+    if wl.pois_lambda > 0:
+        tab = np.zeros(64, dtype=np.uint32)
+        kmax = O.lib().orc_poisson_thresholds(wl.pois_lambda, O.ptr(tab, C.c_uint32), 64)
+        B += kmax * abs(wl.w_bg)
+    s = 24
+    while s > 0 and B * math.ldexp(1.0, s) + 1e6 >= 2147483647.0:
+        s -= 1
+    return s
+
+
+def initial_state(wl):
+    """The state a freshly finalized network holds, then the workload's x0 if any."""
+    n, ns = wl.spec.n, wl.spec.nstate
+    if wl.x0 is not None:
+        return np.ascontiguousarray(wl.x0, dtype=np.float32).reshape(n, ns).copy(), wl.last0.astype(np.int32).copy()
+    x = np.zeros((n, ns), dtype=np.float32)
+    last = np.full(n, 10000, dtype=np.int32)
+    sp = wl.spec
+    if sp.model == L.LIF_VJ:
+        x[:, 0] = sp.v_reset
+    elif sp.model == L.LIF_EXP:
+        x[:, 0] = sp.v_rest
+    else:
+        x[:, 0] = sp.v_rest
+        x[:, 1] = (np.asarray(sp.b, np.float32) * np.float32(sp.v_rest)).astype(np.float32)
+    return x, last
+
+
+class OracleRun:
+    """Holds an OrcNet built from a workload plus the oracle-side mutable arrays."""
+
+    def __init__(self, wl, fixed_shift: int | None = None):
+        self.wl = wl
+        sp = wl.spec
+        self.keep = []
+        k = self.keep.append
+        m = O.OrcModel(model=MODEL_MAP[sp.model], integrator=sp.integrator, n=sp.n, t_ref=sp.t_ref, dt=sp.dt,
+                       v_rest=sp.v_rest, tau=sp.tau, v_th=sp.v_th, r=sp.r, v_reset=sp.v_reset, g_tau=sp.g_tau,
+                       e_rev=sp.e_rev, izh_v_th=sp.izh_v_th, izh_r=sp.izh_r, e_ampa=sp.e_ampa, e_gaba=sp.e_gaba,
+                       ampa_tau=sp.ampa_tau, gaba_tau=sp.gaba_tau)
+        if sp.a is not None:
+            for name in "abcd":
+                arr = np.ascontiguousarray(getattr(sp, name), dtype=np.float32)
+                k(arr)
+                setattr(m, name, O.ptr(arr, C.c_float))
+        net = O.OrcNet(m=m, d_slots=wl.d_slots, fixed_shift=auto_shift(wl) if fixed_shift is None else fixed_shift)
+        self.shift = net.fixed_shift
+        self.w = None
+        if wl.syn is not None:
+            s = wl.syn
+            self.w = s.weight.astype(np.float32).copy()
+            for arr in (s.row_ptr, s.post, s.delay, s.type, self.w):
+                k(arr)
+            net.row_ptr = O.ptr(s.row_ptr, C.c_int64)
+            net.post = O.ptr(s.post, C.c_int32)
+            net.delay = O.ptr(s.delay, C.c_uint8)
+            net.type = O.ptr(s.type, C.c_uint8)
+            net.w = O.ptr(self.w, C.c_float)
+        if wl.dense_wt is not None:
+            wt = np.ascontiguousarray(wl.dense_wt, dtype=np.float32)
+            k(wt)
+            net.dense_wt = O.ptr(wt, C.c_float)
+            net.dense_delay = wl.dense_delay
+        if wl.pois_lambda > 0:
+            net.pois_on = 1
+            net.seed_lo = wl.pois_seed & 0xFFFFFFFF
+            net.seed_hi = (wl.pois_seed >> 32) & 0xFFFFFFFF
+            net.pois_lambda = wl.pois_lambda
+            net.w_bg = wl.w_bg
+        if wl.cur is not None:
+            cur = np.ascontiguousarray(wl.cur, dtype=np.float32)
+            k(cur)
+            net.cur = O.ptr(cur, C.c_float)
+            net.cur_t_begin = 0
+        if wl.stdp is not None:
+            s = wl.syn
+            S = s.count
+            net.stdp_on = 1
+            plastic = np.ascontiguousarray(s.plastic, dtype=np.uint8)
+            self.apre = np.zeros(S, np.float32)
+            self.apost = np.zeros(S, np.float32)
+            self.slast = np.zeros(S, np.int32)
+            sel = np.nonzero(plastic)[0].astype(np.int64)
+            order = np.argsort(s.post[sel], kind="stable")
+            in_syn = np.ascontiguousarray(sel[order])
+            in_ptr = np.zeros(sp.n + 1, dtype=np.int64)
+            in_ptr[1:] = np.cumsum(np.bincount(s.post[sel], minlength=sp.n))
+            for arr in (plastic, self.apre, self.apost, self.slast, in_syn, in_ptr):
+                k(arr)
+            net.plastic = O.ptr(plastic, C.c_uint8)
+            for f in ("max_ltp", "ltp_tau", "max_ltd", "ltd_tau", "w_max"):
+                setattr(net, f, wl.stdp[f])
+            net.apre = O.ptr(self.apre, C.c_float)
+            net.apost = O.ptr(self.apost, C.c_float)
+            net.slast = O.ptr(self.slast, C.c_int32)
+            net.in_ptr = O.ptr(in_ptr, C.c_int64)
+            net.in_syn = O.ptr(in_syn, C.c_int64)
+        self.net = net
+        self.x, self.last = initial_state(wl)
+        D, n = wl.d_slots, sp.n
+        self.hist = np.zeros((D, n), dtype=np.int32)
+        self.hist_cnt = np.zeros(D, dtype=np.int32)
+        self.ring = np.zeros((D, 2, n), dtype=np.int32)
+        self.t = 0
+        self.events = np.zeros(1, dtype=np.int64)
+
+    def run(self, nsteps: int, acc_mode: int = O.ACC_FIXED, ring: bool = True, raster: bool = True):
+        n = self.wl.spec.n
+        counts = np.zeros(nsteps, dtype=np.int32)
+        words = (n + 31) // 32
+        ras = np.zeros((nsteps, words), dtype=np.uint32) if raster else None
+        use_ring = ring and acc_mode == O.ACC_FIXED and self.wl.stdp is None
+        st = O.lib().orc_run(C.byref(self.net), acc_mode, O.ptr(self.x, C.c_float), O.ptr(self.last, C.c_int32),
+                             O.ptr(self.hist, C.c_int32), O.ptr(self.hist_cnt, C.c_int32),
+                             O.ptr(self.ring, C.c_int32) if use_ring else None, self.t, nsteps,
+                             O.ptr(counts, C.c_int32), O.ptr(ras, C.c_uint32) if raster else None,
+                             O.ptr(self.events, C.c_int64))
+        assert st == 0, f"oracle self-check failed (status {st}): push-ring image != pull delivery"
+        self.t += nsteps
+        return counts, ras
+
+    def ring_as_device(self, nch: int) -> np.ndarray:
+        """Oracle ring image in the device layout [slot][channel][n]."""
+        return self.ring[:, :nch, :].copy()

@@ -1,0 +1,218 @@
+"""GPU parity: the HIP library against the CPU oracle on identical seeded inputs.
+
+Bar: bit-exact state (float32 bits), LastSpike, spike counts, rasters, delay-ring
+contents, synaptic event counts and STDP weights/traces.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from harness import OracleRun
+import neuronbridge_amd as nb
+from neuronbridge_amd import _lib as L
+from neuronbridge_amd import workloads as W
+from neuronbridge_amd.network import ModelSpec, Synapses
+from neuronbridge_amd.workloads import Workload
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = {"kernels": L.ENGINE_KERNELS, "graph": L.ENGINE_GRAPH, "persistent": L.ENGINE_PERSISTENT}
+
+
+def assert_bits_equal(a, b, what):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape, what
+    if a.dtype == np.float32:
+        bad = np.nonzero(a.view(np.uint32) != b.view(np.uint32))
+    else:
+        bad = np.nonzero(a != b)
+    assert bad[0].size == 0, f"{what}: {bad[0].size} mismatches, first at {[x[:3] for x in bad]}: " \
+                             f"{a[tuple(x[:1] for x in bad)]} vs {b[tuple(x[:1] for x in bad)]}"
+
+
+def compare(net, run, wl, counts_o, raster_o, ring=True):
+    x, last = net.get_state()
+    assert_bits_equal(x, run.x, "state")
+    assert_bits_equal(last, run.last, "LastSpike")
+    assert_bits_equal(net.counts(), counts_o, "spike counts")
+    assert_bits_equal(net.raster(), raster_o, "raster")
+    assert net.events() == int(run.events[0]), "synaptic events"
+    if ring:
+        nch = 2 if wl.spec.model == L.IZH else 1
+        assert_bits_equal(net.ring(), run.ring_as_device(nch), "delay ring")
+
+
+def small_sparse(model, integrator, n=2000, k_in=100, seed=21):
+    wl = W.sparse_lif(n=n, k_in=k_in, integrator=integrator, steps=400, seed=seed)
+    if model == L.LIF_VJ:
+        return wl
+    rng = np.random.default_rng(seed)
+    n_exc = wl.extra["n_exc"]
+    if model == L.LIF_EXP:
+        wl.spec = ModelSpec(model=L.LIF_EXP, n=n, dt=0.1, integrator=integrator, t_ref=20, v_rest=-65.0, tau=20.0,
+                            v_th=-50.0, r=1.0, v_reset=-65.0, g_tau=5.0, e_rev=0.0)
+        wl.syn.weight = np.where(wl.syn.type == 0, 0.02, 0.05).astype(np.float32)
+        wl.w_bg = 0.03
+        x0 = np.zeros((n, 3), np.float32)
+        x0[:, 0] = wl.x0[:, 0]
+        wl.x0 = x0
+    else:  # conductance Izhikevich
+        a, b, c, d = W.izh_params(n, n_exc, rng)
+        wl.spec = ModelSpec(model=L.IZH, n=n, dt=0.1, integrator=integrator, v_rest=-65.0, a=a, b=b, c=c, d=d)
+        wl.syn.weight = np.where(wl.syn.type == 0, 0.05, 0.1).astype(np.float32)
+        wl.w_bg = 0.2
+        x0 = np.zeros((n, 6), np.float32)
+        x0[:, 0] = rng.uniform(-70, -55, n)
+        x0[:, 1] = b * x0[:, 0]
+        wl.x0 = x0
+    return wl
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+@pytest.mark.parametrize("integrator", [L.EULER, L.RK4], ids=["euler", "rk4"])
+def test_sparse_lif_closed_loop_bitexact(gpu, engine, integrator):
+    wl = small_sparse(L.LIF_VJ, integrator)
+    T = 400
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, ENGINES[engine])
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 100, "workload too quiet to test delivery"
+    compare(net, run, wl, counts_o, raster_o)
+
+
+@pytest.mark.parametrize("model", [L.LIF_EXP, L.IZH], ids=["lif_exp", "izh_cond"])
+@pytest.mark.parametrize("engine", ["graph", "persistent"])
+def test_other_models_closed_loop_bitexact(gpu, model, engine):
+    wl = small_sparse(model, L.RK4 if model == L.IZH else L.EULER)
+    T = 300
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, ENGINES[engine])
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 20
+    compare(net, run, wl, counts_o, raster_o)
+
+
+def test_engines_interoperate(gpu):
+    # state, ring and firing history carry over between engines mid-run
+    wl = small_sparse(L.LIF_VJ, L.RK4, n=3000, k_in=120, seed=4)
+    net = wl.make_net()
+    net.record(0, 330, counts=True, raster=True)
+    net.run(100, L.ENGINE_GRAPH)
+    net.run(130, L.ENGINE_PERSISTENT)
+    net.run(37, L.ENGINE_KERNELS)
+    net.run(63, L.ENGINE_PERSISTENT)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(330)
+    compare(net, run, wl, counts_o, raster_o)
+
+
+def test_dense_izhikevich_current_config(gpu):
+    # dense all-to-all current-based Izhikevich at the configuration's full size (N=1000)
+    T = 1500
+    wl = W.dense_izh(n=1000, steps=T, noise_scale=10 ** 0.5)
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 100
+    compare(net, run, wl, counts_o, raster_o)
+
+
+@pytest.mark.parametrize("engine", ["kernels", "graph"])
+def test_stdp_weights_bitexact(gpu, engine):
+    wl = W.stdp_izh(n=1500, k_in=60, steps=1200, w_bg=2.0)
+    T = 1200
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, ENGINES[engine])
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 200, "too few spikes for plasticity"
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    w = net.weights()
+    assert_bits_equal(w, run.w, "STDP weights")
+    apre, apost, last = net.stdp_state()
+    assert_bits_equal(apre, run.apre, "Apre")
+    assert_bits_equal(apost, run.apost, "Apost")
+    assert_bits_equal(last, run.slast, "LastUpdate")
+    assert not np.array_equal(w, wl.syn.weight), "weights never changed"
+    pw = w[wl.syn.plastic == 1]
+    assert pw.min() >= 0.0 and pw.max() <= 1.0  # plastic weights stay clipped to [0, w_max]
+    assert np.array_equal(w[wl.syn.plastic == 0], wl.syn.weight[wl.syn.plastic == 0])
+
+
+def test_per_phase_delay_known_answer(gpu):
+    # dense_cuda_smoke.cpp:109-118 through the per-phase calls: an input spike injected at
+    # t=0 through a delay-1 synapse makes the target fire at t=1, not t=0
+    spec = ModelSpec(model=L.LIF_VJ, n=3, dt=0.1, t_ref=0, v_rest=-65.0, tau=20.0, v_th=-64.8, v_reset=-65.0)
+    syn = Synapses(row_ptr=np.array([0, 1, 2, 2], np.int64), post=np.array([1, 2], np.int32),
+                   delay=np.array([1, 1], np.uint8), type=np.zeros(2, np.uint8),
+                   weight=np.array([12.0, 8.0], np.float32), max_delay=1)
+    net = Workload("kat", spec, syn=syn).make_net()
+    fired = []
+    for t in range(3):
+        net.begin_step(t)
+        net.run_propagation()
+        ids = net.neuron_step()
+        if t == 0:
+            net.inject_spikes([0])
+        net.apply_post_learning()
+        net.commit_firings()
+        fired.append(sorted(net.history(t).tolist()))
+    assert fired == [[0], [1], [2]]
+    with pytest.raises(L.NbxError):
+        net.begin_step(7)  # out of order: the reference's BeginStep contract
+
+
+def test_replay_delivery_brute_force(gpu):
+    wl = small_sparse(L.LIF_VJ, L.EULER, n=2500, k_in=80, seed=9)
+    net = wl.make_net()
+    rng = np.random.default_rng(1)
+    K = 48
+    spikes = [np.sort(rng.choice(wl.spec.n, size=int(rng.integers(0, 60)), replace=False)).astype(np.int32)
+              for _ in range(K)]
+    out = net.replay(spikes)
+    s = wl.syn
+    q = np.rint(s.weight * np.float32(2.0 ** net.fixed_shift)).astype(np.int64)
+    q = np.where(s.type == 1, -q, q)
+    expect = np.zeros((K, 1, wl.spec.n), np.int64)
+    for k, ids in enumerate(spikes):
+        for pre in ids:
+            sl = slice(s.row_ptr[pre], s.row_ptr[pre + 1])
+            ko = k + s.delay[sl].astype(np.int64) - 1
+            ok = ko < K
+            np.add.at(expect[:, 0, :], (ko[ok], s.post[sl][ok]), q[sl][ok])
+    assert_bits_equal(out.astype(np.int64), expect, "replay arrivals")
+
+
+def test_full_size_sparse_properties(gpu):
+    # configuration-size network (1e5 neurons, 1e8 synapses): engines agree bit for bit,
+    # event count = sum over spikes of out-degree, raster popcount = spike count
+    wl = W.sparse_lif()
+    T = 150
+    a = wl.make_net()
+    a.record(0, T, counts=True, raster=True)
+    a.run(T, L.ENGINE_PERSISTENT)
+    b = wl.make_net()
+    b.record(0, T, counts=True, raster=True)
+    b.run(T, L.ENGINE_GRAPH)
+    xa, la = a.get_state()
+    xb, lb = b.get_state()
+    assert_bits_equal(xa, xb, "state persistent vs graph")
+    assert_bits_equal(la, lb, "LastSpike persistent vs graph")
+    ra, rb = a.raster(), b.raster()
+    assert_bits_equal(ra, rb, "raster persistent vs graph")
+    assert_bits_equal(a.ring(), b.ring(), "ring persistent vs graph")
+    counts = a.counts()
+    assert np.array_equal(counts, [int(np.unpackbits(r.view(np.uint8)).sum()) for r in ra])
+    deg = np.diff(wl.syn.row_ptr)
+    bits = np.unpackbits(ra.view(np.uint8), axis=1, bitorder="little")[:, :wl.spec.n]
+    assert a.events() == int((bits.astype(np.int64) * deg[None, :]).sum())
+    assert a.events() == b.events()
+    assert counts.sum() > 0
