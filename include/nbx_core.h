@@ -211,6 +211,52 @@ int32_t nbx_build_fixed_indegree(int32_t n, int32_t k_in, int32_t d_min, int32_t
                                  uint32_t seed_hi, int32_t n_exc, int32_t nthreads, int64_t* row_ptr, int32_t* post,
                                  uint8_t* delay, uint8_t* type);
 
+/* Dense N(0, sigma^2) weights rounded to BF16 (round to nearest even), w[pre * n + post]:
+ * Philox4x32-10 counter (pre, post, "WGHT", 0) -> Box-Muller in double on the host.
+ * Host only. Used for the batched multi-instance workload. */
+int32_t nbx_build_dense_normal_bf16(int32_t n, float sigma, uint32_t seed_lo, uint32_t seed_hi, int32_t nthreads,
+                                    uint16_t* out);
+
+/* ---------------- batched multi-instance engine (no reference counterpart) ----------------
+ * Hundreds of independent LIF voltage-jump networks of uneven size and parameters, each with
+ * dense all-to-all BF16 weights (delay 1 step), its own tau, Poisson background and
+ * fixed-point shift, advanced together: one workgroup per instance, state in registers, the
+ * fired rows of W gathered per step, chunks of steps replayed from a captured hipGraph.
+ * Arrivals are the same int32 fixed-point sums the single-network engines compute from the
+ * same (BF16-valued) weights, so each instance's raster equals the per-instance FP32 run. */
+typedef struct nbx_batch nbx_batch;
+typedef struct nbx_batch_instance {
+    int32_t n;               /* neurons, 1 .. 16384 */
+    float tau;               /* membrane time constant, ms */
+    double pois_lambda;      /* Poisson background count per step */
+    uint32_t seed_lo, seed_hi;
+    int32_t fixed_shift;     /* arrivals in units of 2^-shift */
+    const uint16_t* w_bf16;  /* host [n][n] pre-major */
+    const float* v0;         /* host initial V or NULL (V_reset) */
+} nbx_batch_instance;
+typedef struct nbx_batch_desc {
+    int32_t count;
+    int32_t integrator;      /* NBX_EULER | NBX_RK4 */
+    float dt;
+    int32_t t_ref;
+    float v_rest, v_th, r, v_reset, w_bg;
+    const nbx_batch_instance* inst;
+} nbx_batch_desc;
+int32_t nbx_batch_create(const nbx_batch_desc* desc, int32_t device, nbx_batch** out);
+int32_t nbx_batch_destroy(nbx_batch* b);
+/* Autotuner: times `steps` steps per candidate workgroup size (state restored), keeps the fastest. */
+int32_t nbx_batch_tune(nbx_batch* b, int32_t steps, int32_t* threads_out);
+int32_t nbx_batch_set_schedule(nbx_batch* b, int32_t threads, int32_t chunk);
+int32_t nbx_batch_get_schedule(const nbx_batch* b, int32_t* threads, int32_t* chunk);
+int32_t nbx_batch_record(nbx_batch* b, int32_t t_begin, int32_t nsteps);
+int32_t nbx_batch_run(nbx_batch* b, int32_t nsteps);
+int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps);
+int32_t nbx_batch_sync(nbx_batch* b);
+int32_t nbx_batch_get_raster(nbx_batch* b, int32_t instance, uint32_t* raster, int64_t nwords);
+int32_t nbx_batch_get_state(nbx_batch* b, int32_t instance, float* v, int32_t* last);
+int32_t nbx_batch_events(nbx_batch* b, int64_t* events);
+void* nbx_batch_stream(nbx_batch* b);
+
 /* HIP stream handle (hipStream_t) the network runs on, for event timing by callers. */
 void* nbx_net_stream(nbx_net* net);
 

@@ -63,6 +63,18 @@ class StdpDesc(C.Structure):
                 ("ltd_tau", C.c_float), ("w_max", C.c_float)]
 
 
+class BatchInstance(C.Structure):
+    _fields_ = [("n", C.c_int32), ("tau", C.c_float), ("pois_lambda", C.c_double), ("seed_lo", C.c_uint32),
+                ("seed_hi", C.c_uint32), ("fixed_shift", C.c_int32), ("w_bf16", C.POINTER(C.c_uint16)),
+                ("v0", C.POINTER(C.c_float))]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("count", C.c_int32), ("integrator", C.c_int32), ("dt", C.c_float), ("t_ref", C.c_int32),
+                ("v_rest", C.c_float), ("v_th", C.c_float), ("r", C.c_float), ("v_reset", C.c_float),
+                ("w_bg", C.c_float), ("inst", C.POINTER(BatchInstance))]
+
+
 class PendingView(C.Structure):
     _fields_ = [("d_pending", _vp), ("channel_count", C.c_int32), ("channel_stride", C.c_int32),
                 ("neuron_count", C.c_int32), ("inv_scale", C.c_float)]
@@ -114,8 +126,22 @@ SIGNATURES = {
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],
+    "nbx_build_dense_normal_bf16": [C.c_int32, C.c_float, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint16)],
+    "nbx_batch_create": [C.POINTER(BatchDesc), C.c_int32, C.POINTER(_vp)],
+    "nbx_batch_destroy": [_vp],
+    "nbx_batch_tune": [_vp, C.c_int32, _i32p],
+    "nbx_batch_set_schedule": [_vp, C.c_int32, C.c_int32],
+    "nbx_batch_get_schedule": [_vp, _i32p, _i32p],
+    "nbx_batch_record": [_vp, C.c_int32, C.c_int32],
+    "nbx_batch_run": [_vp, C.c_int32],
+    "nbx_batch_run_async": [_vp, C.c_int32],
+    "nbx_batch_sync": [_vp],
+    "nbx_batch_get_raster": [_vp, C.c_int32, _u32p, C.c_int64],
+    "nbx_batch_get_state": [_vp, C.c_int32, _f32p, _i32p],
+    "nbx_batch_events": [_vp, _i64p],
+    "nbx_batch_stream": [_vp],
 }
-_RESTYPES = {"nbx_last_error": C.c_char_p, "nbx_net_stream": _vp}
+_RESTYPES = {"nbx_last_error": C.c_char_p, "nbx_net_stream": _vp, "nbx_batch_stream": _vp}
 
 _lib = None
 

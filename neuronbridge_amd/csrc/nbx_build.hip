@@ -6,6 +6,7 @@
 // object per synapse: two passes over counter-based random draws and one counting
 // sort over (pre, delay) buckets, split over host threads.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -120,7 +121,6 @@ extern "C" int32_t nbx_build_fixed_indegree(int32_t n, int32_t k_in, int32_t d_m
                 Drawer dr(n, k_in, d_min, d_max, seed_lo, seed_hi);
                 std::vector<int32_t> pre((size_t)k_in);
                 std::vector<uint8_t> del((size_t)k_in);
-                std::vector<int32_t> ord((size_t)k_in);
                 for (int32_t j = lo; j < hi; ++j) {
                     dr.draw(j, pre.data(), del.data());
                     for (int32_t m = 0; m < k_in; ++m) {
@@ -134,5 +134,35 @@ extern "C" int32_t nbx_build_fixed_indegree(int32_t n, int32_t k_in, int32_t d_m
             });
         for (auto& x : th) x.join();
     }
+    return NBX_OK;
+}
+
+extern "C" int32_t nbx_build_dense_normal_bf16(int32_t n, float sigma, uint32_t seed_lo, uint32_t seed_hi,
+                                               int32_t nthreads, uint16_t* out) {
+    if (n <= 0 || !out) {
+        nbx::set_error("nbx_build_dense_normal_bf16: bad arguments");
+        return NBX_ERR_INVALID;
+    }
+    int T = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    T = std::max(1, std::min(T, std::min(64, n)));
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([=] {
+            const int32_t lo = (int32_t)((int64_t)n * t / T), hi = (int32_t)((int64_t)n * (t + 1) / T);
+            for (int32_t p = lo; p < hi; ++p)
+                for (int32_t q = 0; q < n; ++q) {
+                    uint32_t o[4];
+                    philox((uint32_t)p, (uint32_t)q, 0x57474854u /* "WGHT" */, 0u, seed_lo, seed_hi, o);
+                    const double u1 = ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+                    const double u2 = ((double)o[1] + 0.5) * (1.0 / 4294967296.0);
+                    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+                    const float f = (float)(z * (double)sigma);
+                    uint32_t bits;
+                    std::memcpy(&bits, &f, 4);
+                    bits += 0x7FFFu + ((bits >> 16) & 1u);  // round to nearest even
+                    out[(size_t)p * n + q] = (uint16_t)(bits >> 16);
+                }
+        });
+    for (auto& x : th) x.join();
     return NBX_OK;
 }

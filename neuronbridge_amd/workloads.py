@@ -140,3 +140,45 @@ def stdp_izh(n: int = 50_000, k_in: int = 500, steps: int = 20_000, integrator: 
     lam = pois_sources * pois_rate_hz * dt * 1e-3
     return Workload("stdp_izh", spec, syn=syn, pois_seed=seed, pois_lambda=lam, w_bg=w_bg, stdp=stdp,
                     steps=steps, extra={"n_exc": n_exc, "k_in": k_in})
+
+
+# ---------------------------------------------------------------------------------------------
+# config 4: batched independent LIF instances with dense BF16 recurrent weights
+
+BATCH_LIF = dict(integrator=L.RK4, dt=0.1, t_ref=20, v_rest=-65.0, v_th=-50.0, r=1.0, v_reset=-65.0, w_bg=0.1)
+MAX_ABS_NORMAL = 6.8  # |z| bound of the Box-Muller draws (u1 >= 2^-33)
+
+
+def batched_lif(count: int = 256, n_lo: int = 512, n_hi: int = 8192, steps: int = 5000,
+                seed: int = SEED_BASE + 4, threads: int = 0):
+    """256 instances, N ~ U{512..8192}, tau_m ~ U[10,30] ms, input rate ~ U[5,20] Hz (1000 sources, 0.1 mV),
+    W ~ N(0, 1/N) rounded to BF16, RK4 dt 0.1 ms. Returns (instances, shared parameters)."""
+    from .batch import Instance, dense_normal_bf16, poisson_kmax, shift_for_bound
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.integers(n_lo, n_hi + 1))
+        tau = float(rng.uniform(10.0, 30.0))
+        rate = float(rng.uniform(5.0, 20.0))
+        lam = 1000 * rate * BATCH_LIF["dt"] * 1e-3
+        sigma = 1.0 / np.sqrt(n)
+        iseed = (seed << 16) + k
+        w = dense_normal_bf16(n, sigma, iseed, threads)
+        bound = n * MAX_ABS_NORMAL * sigma * 1.01 + poisson_kmax(lam) * BATCH_LIF["w_bg"]
+        v0 = rng.uniform(-65.0, -50.0, n).astype(np.float32)
+        out.append(Instance(n=n, tau=tau, pois_lambda=lam, seed=iseed, fixed_shift=shift_for_bound(bound),
+                            w_bf16=w, v0=v0))
+    return out, dict(BATCH_LIF)
+
+
+def instance_workload(ins, shared) -> Workload:
+    """One batched instance as a single-network workload (the plain FP32 per-instance path)."""
+    from .batch import bf16_to_f32
+    spec = ModelSpec(model=L.LIF_VJ, n=ins.n, dt=shared["dt"], integrator=shared["integrator"], t_ref=shared["t_ref"],
+                     v_rest=shared["v_rest"], tau=ins.tau, v_th=shared["v_th"], r=shared["r"],
+                     v_reset=shared["v_reset"])
+    x0 = np.zeros((ins.n, 2), np.float32)
+    x0[:, 0] = ins.v0 if ins.v0 is not None else shared["v_reset"]
+    return Workload("batched_instance", spec, dense_wt=bf16_to_f32(ins.w_bf16), dense_delay=1,
+                    pois_seed=ins.seed, pois_lambda=ins.pois_lambda, w_bg=shared["w_bg"], x0=x0,
+                    last0=np.full(ins.n, 10000, np.int32), extra={"fixed_shift": ins.fixed_shift})
