@@ -32,31 +32,62 @@ constexpr int kMaxThreads = 1024;
 constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
 
-struct PersistArgs {
-    StepArgs A;
-    const int32_t* off;
-    const uint2* seg;
-    int32_t B, P;
-    int32_t t0, nsteps;
-    int32_t* bar;
-};
 
 __device__ inline int32_t block_of(int32_t w, int32_t B) {
     const int32_t per = B / kXcds;  // B is a multiple of kXcds
     return (w % kXcds) * per + (w / kXcds);
 }
 
-// Grid barrier: arrival counter that only grows; all workgroups wait until it reaches
-// `target`. Bounded spin: on time-out the error flag is raised and every waiter leaves.
-__device__ inline bool grid_barrier(int32_t* bar, int32_t target, int32_t* err, int32_t* s_abort) {
+// Grid barrier, two-level: workgroups of one XCD group (blockIdx % 8) arrive on their
+// group counter; the last arrival of a group arrives on the top counter; the last top
+// arrival raises every group's generation flag. Counters only grow (targets scale with
+// the step), each sits on its own 128-byte line, and the spin is bounded: on time-out
+// the error flag is raised and every waiter leaves.
+//
+// Memory ordering: the only data other workgroups read within a launch are the firing
+// ids and counts of the previous step. They are written with device-scope atomic
+// stores/adds (coherence-point writes, `sc1`) and read with device-scope atomic loads,
+// and each arrival is preceded by s_waitcnt 0, so every store of a workgroup has been
+// performed before its arrival is counted. No L2 write-back or invalidate is needed.
+constexpr int kLine = 32;  // int32 per 128-byte line
+struct Barrier {
+    int32_t group[kXcds * kLine];
+    int32_t top[kLine];
+    int32_t gen[kXcds * kLine];
+};
+
+struct PersistArgs {
+    StepArgs A;
+    const int32_t* off;
+    const uint2* seg;
+    int32_t B, P;
+    int32_t t0, nsteps;
+    Barrier* bar;
+};
+
+__device__ inline int32_t ld_dev(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(int32_t* p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool grid_barrier(Barrier* bar, int32_t k, int32_t nblocks, int32_t* err, int32_t* s_abort) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(bar, 1);
+        __builtin_amdgcn_s_waitcnt(0);  // this workgroup's published stores are performed
+        const int32_t g = blockIdx.x % kXcds;
+        const int32_t per = nblocks / kXcds;
+        const int32_t old = __hip_atomic_fetch_add(&bar->group[g * kLine], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == per * (k + 1) - 1) {
+            const int32_t o2 = __hip_atomic_fetch_add(&bar->top[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o2 == kXcds * (k + 1) - 1)
+                for (int q = 0; q < kXcds; ++q) st_dev(&bar->gen[q * kLine], k + 1);
+        }
         uint32_t spins = 0;
         bool abort = false;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        while (ld_dev(&bar->gen[g * kLine]) < k + 1) {
+            if (ld_dev(err) != 0) {
                 abort = true;
                 break;
             }
@@ -67,7 +98,6 @@ __device__ inline bool grid_barrier(int32_t* bar, int32_t target, int32_t* err, 
                 break;
             }
         }
-        __threadfence();
         *s_abort = abort ? 1 : 0;
     }
     __syncthreads();
@@ -123,20 +153,28 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         if (k > 0) {
             // spikes of step t-1 were complete when the previous barrier opened
             const int32_t ps = (t - 1) % D;
-            const int32_t cnt = __hip_atomic_load(&A.hist_cnt[ps], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t cnt = ld_dev(&A.hist_cnt[ps]);
             const int32_t* ids = A.hist + (int64_t)ps * n;
             if (b == 0 && tid == 0 && A.counts && t - 1 >= A.rec_t0 && t - 1 < A.rec_t0 + A.rec_n)
                 A.counts[t - 1 - A.rec_t0] = cnt;
             for (int32_t s = tid; s < cnt; s += T) {
-                const int32_t pre = ids[s];
+                const int32_t pre = ld_dev(&ids[s]);
                 const int32_t* op = X.off + (int64_t)pre * (X.B + 1) + b;
                 const int32_t o0 = op[0], o1 = op[1];
-                for (int32_t j = o0; j < o1; ++j) {
-                    const uint2 e = X.seg[j];
-                    const int32_t pl = (int32_t)(e.x & 0xFFFFu);
-                    const int32_t d = (int32_t)((e.x >> 16) & 0xFFu);
-                    const int32_t ch = (int32_t)(e.x >> 24);
-                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + ch) * P + pl], (int32_t)e.y);
+                constexpr int U = 8;  // segment entries loaded together
+                for (int32_t j0 = o0; j0 < o1; j0 += U) {
+                    uint2 e[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) e[u] = (j0 + u < o1) ? X.seg[j0 + u] : make_uint2(0u, 0u);
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (j0 + u < o1) {
+                            const int32_t pl = (int32_t)(e[u].x & 0xFFFFu);
+                            const int32_t d = (int32_t)((e[u].x >> 16) & 0xFFu);
+                            const int32_t ch = (int32_t)(e[u].x >> 24);
+                            atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + ch) * P + pl], (int32_t)e[u].y);
+                        }
+                    }
                 }
                 ev += (unsigned long long)(o1 - o0);
             }
@@ -179,12 +217,12 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 s_base = tot ? atomicAdd(&A.hist_cnt[slot], tot) : 0;
             }
             __syncthreads();
-            if (fired) A.hist[(int64_t)slot * n + s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+            if (fired) st_dev(&A.hist[(int64_t)slot * n + s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull))], i);
             __syncthreads();
         }
         // slot of step t+1 held step t+1-D: nobody reads it in this phase (D >= 3)
-        if (b == 0 && tid == 0) A.hist_cnt[(t + 1) % D] = 0;
-        if (!grid_barrier(X.bar, X.B * (k + 1), A.err, &s_abort)) break;
+        if (b == 0 && tid == 0) st_dev(&A.hist_cnt[(t + 1) % D], 0);
+        if (!grid_barrier(X.bar, k, X.B, A.err, &s_abort)) break;
     }
     // write back state and ring; the ring then holds every delivery up to the last step
 #pragma unroll
@@ -391,7 +429,7 @@ int32_t build_persistent_layout(nbx_net* net) {
     if (rc) return rc;
     rc = net->upload(&L.d_seg, seg.data(), seg.size());
     if (rc) return rc;
-    rc = net->alloc(&L.d_bar, 1);
+    rc = net->alloc(&L.d_bar, sizeof(Barrier) / sizeof(int32_t));
     if (rc) return rc;
     L.B = B;
     L.P = P;
@@ -429,7 +467,7 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     }
     if (L.lds_bytes > 64 * 1024)
         NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.lds_bytes));
-    nbx_zero_i32<<<1, 64, 0, net->stream>>>(L.d_bar, 1);
+    nbx_zero_i32<<<1, 256, 0, net->stream>>>(L.d_bar, (int64_t)(sizeof(Barrier) / sizeof(int32_t)));
     NBX_HIP_TRY(hipGetLastError());
     PersistArgs X;
     X.A = net->args();
@@ -439,7 +477,7 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.P = L.P;
     X.t0 = net->t;
     X.nsteps = nsteps;
-    X.bar = L.d_bar;
+    X.bar = reinterpret_cast<Barrier*>(L.d_bar);
     void* kargs[] = {&X};
     NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes, net->stream));
     net->t += nsteps;

@@ -25,6 +25,44 @@ constexpr int kWaves = kUpdateThreads / 64;
 
 __device__ inline int32_t step_of(const int32_t* tbase, int32_t k) { return tbase ? (*tbase + k) : k; }
 
+// Signed fixed-point contribution of one synapse and its ring channel.
+__device__ inline void syn_effect(const StepArgs& A, uint32_t e, float w, int32_t& ch, int32_t& q) {
+    q = quantize(w, A.scale);
+    ch = 0;
+    const int32_t type = syn_type(e);
+    if (A.P.model == NBX_MODEL_IZH) {
+        ch = (type == 1) ? 1 : 0;
+    } else if (A.P.model != NBX_MODEL_IZH_CUR) {
+        if (type == 1) q = -q;
+    }
+}
+
+// Fan out one presynaptic row [b, e) into the delay ring of step t. Loads are issued
+// UNROLL at a time ahead of the atomics so several row segments are in flight per thread.
+template <int THREADS>
+__device__ inline void push_row(const StepArgs& A, int32_t t, int64_t b, int64_t e) {
+    constexpr int UNROLL = 4;
+    for (int64_t j0 = b + threadIdx.x; j0 < e; j0 += (int64_t)THREADS * UNROLL) {
+        uint32_t en[UNROLL];
+        float w[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int64_t j = j0 + (int64_t)u * THREADS;
+            en[u] = j < e ? A.syn[j] : 0u;
+            w[u] = j < e ? A.w[j] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            if (j0 + (int64_t)u * THREADS < e) {
+                int32_t ch, q;
+                syn_effect(A, en[u], w[u], ch, q);
+                const int32_t dslot = (t + syn_delay(en[u])) % A.D;
+                atomicAdd(&A.ring[((int64_t)dslot * A.nch + ch) * A.n + syn_post(en[u])], q);
+            }
+        }
+    }
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const int32_t* tbase, int32_t k) {
     const int32_t t = step_of(tbase, k);
@@ -95,22 +133,7 @@ __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const i
         if (blockIdx.x == 0) A.hist_cnt[(t + 1) % A.D] = 0;
     }
     __syncthreads();
-    if (fired) {
-        const int32_t pos = s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull));
-        A.hist[(int64_t)slot * A.n + pos] = i;
-    }
-}
-
-// Signed fixed-point contribution of one synapse and its ring channel.
-__device__ inline void syn_effect(const StepArgs& A, uint32_t e, float w, int32_t& ch, int32_t& q) {
-    q = quantize(w, A.scale);
-    ch = 0;
-    const int32_t type = syn_type(e);
-    if (A.P.model == NBX_MODEL_IZH) {
-        ch = (type == 1) ? 1 : 0;
-    } else if (A.P.model != NBX_MODEL_IZH_CUR) {
-        if (type == 1) q = -q;
-    }
+    if (fired) A.hist[(int64_t)slot * A.n + s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = i;
 }
 
 constexpr int kDeliverThreads = 256;
@@ -122,14 +145,7 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_push(StepArgs A, 
     const int32_t* ids = A.hist + (int64_t)slot * A.n;
     for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
         const int32_t pre = ids[s];
-        const int64_t b = A.row_ptr[pre], e = A.row_ptr[pre + 1];
-        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
-            const uint32_t en = A.syn[j];
-            int32_t ch, q;
-            syn_effect(A, en, A.w[j], ch, q);
-            const int32_t dslot = (t + syn_delay(en)) % A.D;
-            atomicAdd(&A.ring[((int64_t)dslot * A.nch + ch) * A.n + syn_post(en)], q);
-        }
+        push_row<kDeliverThreads>(A, t, A.row_ptr[pre], A.row_ptr[pre + 1]);
     }
 }
 
