@@ -11,8 +11,8 @@ HBM before the timed region. Prints one JSON line.
   value      neuron-steps/s (N x steps / time), with syn-events/s beside it
   roofline   the dominant kernel (the persistent step kernel) timed with HIP events
              on its own stream; algorithmic bytes = 24 B per neuron-step + 8 B per
-             synaptic event. A second entry covers replay delivery (injected spikes of
-             64 steps per launch, 8 B per event + 4 B per output arrival).
+             synaptic event. A second entry covers replay delivery (the recorded spikes of
+             --replay-steps steps injected in one launch; 8 B per event + 4 B per output arrival).
   cpu_baseline  the CPU oracle (reference-faithful per-event float delivery), one core,
              on the first --cpu-steps steps of the same network
 """
@@ -91,9 +91,12 @@ def main() -> int:
     ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "graph", "kernels"])
     ap.add_argument("--integrator", default="rk4", choices=["rk4", "euler"])
     ap.add_argument("--cpu-steps", type=int, default=300)
-    ap.add_argument("--replay-reps", type=int, default=20)
+    ap.add_argument("--replay-reps", type=int, default=5)
+    ap.add_argument("--replay-steps", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path runs on a single "
@@ -153,12 +156,19 @@ def main() -> int:
                      "launch_ms": round(ms, 3), "algorithmic_bytes": alg},
     }
     if not args.no_replay:
-        K = 64
+        K = args.replay_steps
         t_rec = net.time
         net.record(t_rec, K, counts=True, raster=True)
         net.run(K, engine)
         out["replay_roofline"] = replay_roofline(net, wl, net.raster(), args.replay_reps)
         log(f"replay: {out['replay_roofline']}")
+    if args.calibrate_copy_mib > 0:
+        from neuronbridge_amd.hiptime import DeviceBuffer, hip
+        nbytes = args.calibrate_copy_mib << 20
+        src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+        rc = hip().hipMemcpy(dst.ptr, src.ptr, nbytes, 3)  # device to device
+        hip().hipDeviceSynchronize()
+        log(f"calibration copy of {args.calibrate_copy_mib} MiB (rc {rc})")
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,6 +32,7 @@ constexpr int kXcds = 8;
 constexpr int kMaxThreads = 1024;
 constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
+constexpr int kLanes = 4;                  // lanes sharing one spike's synapse segment
 
 
 __device__ inline int32_t block_of(int32_t w, int32_t B) {
@@ -119,6 +121,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const int32_t T = blockDim.x;
     const int32_t tid = threadIdx.x;
     const int32_t lane = tid & 63, wave = tid >> 6, nwaves = T >> 6;
+    const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
 
     // load this block's slice of the delay ring and its neurons' state
     for (int32_t j = tid; j < D * nch * P; j += T) {
@@ -157,26 +160,41 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             const int32_t* ids = A.hist + (int64_t)ps * n;
             if (b == 0 && tid == 0 && A.counts && t - 1 >= A.rec_t0 && t - 1 < A.rec_t0 + A.rec_n)
                 A.counts[t - 1 - A.rec_t0] = cnt;
-            for (int32_t s = tid; s < cnt; s += T) {
-                const int32_t pre = ld_dev(&ids[s]);
-                const int32_t* op = X.off + (int64_t)pre * (X.B + 1) + b;
-                const int32_t o0 = op[0], o1 = op[1];
-                constexpr int U = 8;  // segment entries loaded together
-                for (int32_t j0 = o0; j0 < o1; j0 += U) {
-                    uint2 e[U];
+            // lanes in groups of kLanes share one spike so its segment entries coalesce
+            for (int32_t base = grp; base < cnt; base += ngrp * 2) {
+                int32_t pre[2], o0[2], o1[2];
 #pragma unroll
-                    for (int u = 0; u < U; ++u) e[u] = (j0 + u < o1) ? X.seg[j0 + u] : make_uint2(0u, 0u);
+                for (int r = 0; r < 2; ++r) pre[r] = (base + r * ngrp < cnt) ? ld_dev(&ids[base + r * ngrp]) : -1;
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        if (j0 + u < o1) {
-                            const int32_t pl = (int32_t)(e[u].x & 0xFFFFu);
-                            const int32_t d = (int32_t)((e[u].x >> 16) & 0xFFu);
-                            const int32_t ch = (int32_t)(e[u].x >> 24);
-                            atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + ch) * P + pl], (int32_t)e[u].y);
-                        }
+                for (int r = 0; r < 2; ++r) {
+                    o0[r] = o1[r] = 0;
+                    if (pre[r] >= 0) {
+                        const int32_t* op = X.off + (int64_t)pre[r] * (X.B + 1) + b;
+                        o0[r] = op[0];
+                        o1[r] = op[1];
                     }
                 }
-                ev += (unsigned long long)(o1 - o0);
+                uint2 e[2][2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int32_t j = o0[r] + gl + u * kLanes;
+                        e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
+                    }
+                auto deposit = [&](const uint2 x) {
+                    const int32_t d = (int32_t)((x.x >> 16) & 0xFFu);
+                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)(x.x >> 24)) * P + (int32_t)(x.x & 0xFFFFu)],
+                              (int32_t)x.y);
+                };
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+                        if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u]);
+                    for (int32_t j = o0[r] + gl + 2 * kLanes; j < o1[r]; j += kLanes) deposit(X.seg[j]);
+                    if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
+                }
             }
         }
         if (k == X.nsteps) break;
@@ -250,48 +268,99 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     if (tid == 0 && s_ev) atomicAdd(A.events, s_ev);
 }
 
-// Replay delivery on the post-blocked layout: block b accumulates, in LDS, the
-// arrivals of its neurons for output steps [0, K) from the injected spikes of the
-// K input steps, then streams them out. ids/tick_ptr as in nbx_replay_deliver.
-__global__ __launch_bounds__(kMaxThreads) void nbx_replay_blocked(StepArgs A, const int32_t* __restrict__ off,
-                                                                   const uint2* __restrict__ seg, int32_t B, int32_t P,
-                                                                   const int32_t* __restrict__ ids,
-                                                                   const int32_t* __restrict__ tick_ptr, int32_t K,
-                                                                   int32_t* __restrict__ out) {
-    extern __shared__ int32_t s_acc[];  // [K][nch][P] followed by tick_ptr[K + 1]
-    const int32_t nch = A.nch, n = A.n;
+// Replay delivery on the post-blocked layout. Block b owns neurons [b*P, b*P + P) and
+// streams the injected spikes of all K input steps through an LDS window of
+// kReplayWin output steps plus a (D-1)-step carry: contributions that land beyond the
+// window stay in LDS and move to the front when the window is flushed, so every
+// synapse segment is read exactly once however many steps one launch covers.
+// Lanes work in groups of kLanes on one spike, so the consecutive entries of its
+// segment coalesce into one memory request; each group keeps kR spikes in flight
+// (ids, then segment offsets, then entries) before its LDS atomics.
+constexpr int kReplayWin = 64;
+constexpr int kR = 4;
+constexpr int kU = 2;  // entries per lane loaded ahead (kLanes * kU per segment)
+
+__global__ __launch_bounds__(1024) void nbx_replay_blocked(StepArgs A, const int32_t* __restrict__ off,
+                                                           const uint2* __restrict__ seg, int32_t B, int32_t P,
+                                                           const int32_t* __restrict__ ids,
+                                                           const int32_t* __restrict__ tick_ptr, int32_t K,
+                                                           int32_t* __restrict__ out) {
+    extern __shared__ int32_t s_acc[];  // [(kReplayWin + D - 1) * nch * P]
+    __shared__ int32_t s_tp[kReplayWin + 1];
+    const int32_t nch = A.nch, n = A.n, D = A.D;
     const int32_t b = block_of(blockIdx.x, B);
     const int32_t p0 = b * P;
     const int32_t np = max(0, min(P, n - p0));
-    int32_t* s_tp = s_acc + K * nch * P;
-    for (int32_t j = threadIdx.x; j < K * nch * P; j += blockDim.x) s_acc[j] = 0;
-    for (int32_t j = threadIdx.x; j <= K; j += blockDim.x) s_tp[j] = tick_ptr[j];
-    __syncthreads();
-    const int32_t total = s_tp[K];
+    const int32_t T = blockDim.x, tid = threadIdx.x;
+    const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
+    const int32_t span = (kReplayWin + D - 1) * nch * P;
+    const int32_t tick_words = nch * P;
+    for (int32_t j = tid; j < span; j += T) s_acc[j] = 0;
     unsigned long long ev = 0;
-    for (int32_t s = threadIdx.x; s < total; s += blockDim.x) {
-        // step of flat spike s: binary search in tick_ptr
-        int32_t lo = 0, hi = K;
-        while (hi - lo > 1) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (s_tp[mid] <= s) lo = mid;
-            else hi = mid;
+    for (int32_t w0 = 0; w0 < K; w0 += kReplayWin) {
+        const int32_t wl = min(kReplayWin, K - w0);
+        for (int32_t j = tid; j <= wl; j += T) s_tp[j] = tick_ptr[w0 + j];
+        __syncthreads();
+        const int32_t sb = s_tp[0], cnt = s_tp[wl] - sb;
+        auto deposit = [&](const uint2 x, int32_t tkr) {
+            const int32_t ko = tkr + (int32_t)((x.x >> 16) & 0xFFu) - 1;  // window-relative output step
+            if (w0 + ko < K)
+                atomicAdd(&s_acc[ko * tick_words + (int32_t)(x.x >> 24) * P + (int32_t)(x.x & 0xFFFFu)], (int32_t)x.y);
+        };
+        for (int32_t base = grp; base < cnt; base += ngrp * kR) {
+            int32_t pre[kR], o0[kR], o1[kR], tk[kR];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) pre[r] = (base + r * ngrp < cnt) ? ids[sb + base + r * ngrp] : -1;
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                o0[r] = o1[r] = 0;
+                if (pre[r] >= 0) {
+                    const int32_t* op = off + (int64_t)pre[r] * (B + 1) + b;
+                    o0[r] = op[0];
+                    o1[r] = op[1];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {  // window step of each spike
+                const int32_t f = sb + base + r * ngrp;
+                int32_t lo = 0, hi = wl;
+                while (hi - lo > 1) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (s_tp[mid] <= f) lo = mid;
+                    else hi = mid;
+                }
+                tk[r] = lo;
+            }
+            uint2 e[kR][kU];
+#pragma unroll
+            for (int r = 0; r < kR; ++r)
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int32_t j = o0[r] + gl + u * kLanes;
+                    e[r][u] = (j < o1[r]) ? seg[j] : make_uint2(0u, 0u);
+                }
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u], tk[r]);
+                for (int32_t j = o0[r] + gl + kU * kLanes; j < o1[r]; j += kLanes) deposit(seg[j], tk[r]);
+                if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
+            }
         }
-        const int32_t pre = ids[s];
-        const int32_t* op = off + (int64_t)pre * (B + 1) + b;
-        const int32_t o0 = op[0], o1 = op[1];
-        for (int32_t j = o0; j < o1; ++j) {
-            const uint2 e = seg[j];
-            const int32_t ko = lo + (int32_t)((e.x >> 16) & 0xFFu) - 1;
-            if (ko < K)
-                atomicAdd(&s_acc[(ko * nch + (int32_t)(e.x >> 24)) * P + (int32_t)(e.x & 0xFFFFu)], (int32_t)e.y);
+        __syncthreads();
+        // flush the window's steps, move the carry to the front, clear the rest
+        for (int32_t j = tid; j < wl * tick_words; j += T) {
+            const int32_t pl = j % P, kc = j / P;
+            if (pl < np) out[(int64_t)(w0 * nch + kc) * n + p0 + pl] = s_acc[j];
         }
-        ev += (unsigned long long)(o1 - o0);
-    }
-    __syncthreads();
-    for (int32_t j = threadIdx.x; j < K * nch * P; j += blockDim.x) {
-        const int32_t pl = j % P, kc = j / P;
-        if (pl < np) out[(int64_t)kc * n + p0 + pl] = s_acc[j];
+        if (w0 + kReplayWin >= K) break;  // last window: no carry needed
+        __syncthreads();
+        const int32_t carry = (D - 1) * tick_words;  // kReplayWin >= D - 1, so source and target do not overlap
+        for (int32_t j = tid; j < carry; j += T) s_acc[j] = s_acc[wl * tick_words + j];
+        __syncthreads();
+        for (int32_t j = carry + tid; j < span; j += T) s_acc[j] = 0;
+        __syncthreads();
     }
     if (ev) atomicAdd(A.events, ev);
 }
@@ -479,7 +548,15 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.nsteps = nsteps;
     X.bar = reinterpret_cast<Barrier*>(L.d_bar);
     void* kargs[] = {&X};
-    NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes, net->stream));
+    // Co-residency of all L.B workgroups is checked above (occupancy x CUs). The cooperative
+    // launch additionally has the runtime verify it; NBX_COOPERATIVE=0 selects a regular launch
+    // for tools that cannot trace cooperative dispatches (the barrier spin is bounded either way).
+    const char* coop = std::getenv("NBX_COOPERATIVE");
+    if (coop && coop[0] == '0')
+        NBX_HIP_TRY(hipLaunchKernel(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
+    else
+        NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes,
+                                               net->stream));
     net->t += nsteps;
     return NBX_OK;
 }
@@ -489,15 +566,15 @@ int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_
     // the blocked layout carries pre-quantized weights, so it is not used while STDP changes them
     if (!net->stdp && build_persistent_layout(net) == NBX_OK) {
         PersistentLayout& L = net->pl;
-        const size_t lds = ((size_t)nsteps * net->nch * L.P + nsteps + 1) * sizeof(int32_t);
+        const size_t lds = (size_t)(kReplayWin + net->D - 1) * net->nch * L.P * sizeof(int32_t);
         int max_lds = 0;
         NBX_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
-        if (lds <= (size_t)max_lds) {
+        if (lds + 1024 <= (size_t)max_lds) {
             if (lds > 64 * 1024)
                 NBX_HIP_TRY(hipFuncSetAttribute((const void*)nbx_replay_blocked,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            nbx_replay_blocked<<<L.B, L.threads, lds, net->stream>>>(A, L.d_off, L.d_seg, L.B, L.P, d_ids, d_tick_ptr,
-                                                                      nsteps, d_out);
+            nbx_replay_blocked<<<L.B, 1024, lds, net->stream>>>(A, L.d_off, L.d_seg, L.B, L.P, d_ids, d_tick_ptr,
+                                                                 nsteps, d_out);
             NBX_HIP_TRY(hipGetLastError());
             return NBX_OK;
         }

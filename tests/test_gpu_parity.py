@@ -174,7 +174,7 @@ def test_replay_delivery_brute_force(gpu):
     wl = small_sparse(L.LIF_VJ, L.EULER, n=2500, k_in=80, seed=9)
     net = wl.make_net()
     rng = np.random.default_rng(1)
-    K = 48
+    K = 150  # spans several internal replay windows (carry across windows)
     spikes = [np.sort(rng.choice(wl.spec.n, size=int(rng.integers(0, 60)), replace=False)).astype(np.int32)
               for _ in range(K)]
     out = net.replay(spikes)
