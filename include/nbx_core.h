@@ -257,6 +257,32 @@ int32_t nbx_batch_get_state(nbx_batch* b, int32_t instance, float* v, int32_t* l
 int32_t nbx_batch_events(nbx_batch* b, int64_t* events);
 void* nbx_batch_stream(nbx_batch* b);
 
+/* ---------------- differentiable recurrent LIF (no reference counterpart) ----------------
+ * Elementwise step kernels of a surrogate-gradient adjoint; the caller runs the products with
+ * the weight matrices as GEMMs between them. Device pointers, [n][batch] float32 row-major,
+ * enqueued on `stream` (a hipStream_t, NULL = default stream).
+ *   forward:  Vm = g*V + (1-g)*(V_rest + R*I);  s = Vm > V_th;  V = Vm - delta_reset*s
+ *   backward: ds = R_t + G_t - delta_reset*dV;  dVm = dV + sigma'(Vm - V_th)*ds;
+ *             dI = (1-g)*R*dVm;  dV <- g*dVm
+ * mode 1: Heaviside spike, surrogate sigma'(x) = 1/(1 + slope*|x|)^2 (fast sigmoid)
+ * mode 0: Heaviside spike, sigma' = 0 (the exact gradient away from threshold crossings)
+ * mode 2: smooth spike s = 1/2 + u/(2(1 + |u|)), u = slope*x, with its exact derivative. */
+typedef struct nbx_lif_adj_params {
+    int32_t n, batch;
+    float g, one_minus_g;  /* RK4 growth factor of the linear LIF ODE over one step, and 1 - g */
+    float r, v_rest, v_th, delta_reset, slope;
+    int32_t mode;
+} nbx_lif_adj_params;
+int32_t nbx_lif_fwd_step(const nbx_lif_adj_params* p, const float* d_I, float* d_V, float* d_Vm, float* d_s,
+                         void* stream);
+int32_t nbx_lif_bwd_step(const nbx_lif_adj_params* p, const float* d_G, const float* d_R, const float* d_Vm,
+                         float* d_dV, float* d_dI, void* stream);
+/* Leaky readout y_t = beta*y_{t-1} + (1-beta)*Z_t over [T][m] (state carried in d_state if given),
+ * and its adjoint dZ_t = (1-beta)*(gy_t + beta*(...)) as a reverse scan. */
+int32_t nbx_readout_fwd(int32_t T, int64_t m, float beta, const float* d_Z, float* d_y, float* d_state, void* stream);
+int32_t nbx_readout_bwd(int32_t T, int64_t m, float beta, const float* d_gy, float* d_dZ, float* d_carry,
+                        void* stream);
+
 /* HIP stream handle (hipStream_t) the network runs on, for event timing by callers. */
 void* nbx_net_stream(nbx_net* net);
 

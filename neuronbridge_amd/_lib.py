@@ -75,6 +75,12 @@ class BatchDesc(C.Structure):
                 ("w_bg", C.c_float), ("inst", C.POINTER(BatchInstance))]
 
 
+class LifAdjParams(C.Structure):
+    _fields_ = [("n", C.c_int32), ("batch", C.c_int32), ("g", C.c_float), ("one_minus_g", C.c_float),
+                ("r", C.c_float), ("v_rest", C.c_float), ("v_th", C.c_float), ("delta_reset", C.c_float),
+                ("slope", C.c_float), ("mode", C.c_int32)]
+
+
 class PendingView(C.Structure):
     _fields_ = [("d_pending", _vp), ("channel_count", C.c_int32), ("channel_stride", C.c_int32),
                 ("neuron_count", C.c_int32), ("inv_scale", C.c_float)]
@@ -140,6 +146,10 @@ SIGNATURES = {
     "nbx_batch_get_state": [_vp, C.c_int32, _f32p, _i32p],
     "nbx_batch_events": [_vp, _i64p],
     "nbx_batch_stream": [_vp],
+    "nbx_lif_fwd_step": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp],
+    "nbx_lif_bwd_step": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp, _vp],
+    "nbx_readout_fwd": [C.c_int32, C.c_int64, C.c_float, _vp, _vp, _vp, _vp],
+    "nbx_readout_bwd": [C.c_int32, C.c_int64, C.c_float, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"nbx_last_error": C.c_char_p, "nbx_net_stream": _vp, "nbx_batch_stream": _vp}
 

@@ -182,3 +182,24 @@ def instance_workload(ins, shared) -> Workload:
     return Workload("batched_instance", spec, dense_wt=bf16_to_f32(ins.w_bf16), dense_delay=1,
                     pois_seed=ins.seed, pois_lambda=ins.pois_lambda, w_bg=shared["w_bg"], x0=x0,
                     last0=np.full(ins.n, 10000, np.int32), extra={"fixed_shift": ins.fixed_shift})
+
+
+# ---------------------------------------------------------------------------------------------
+# config 5: differentiable recurrent LIF trained on sinusoid readout targets
+
+def diff_lif(n: int = 2048, batch: int = 64, steps: int = 500, n_in: int = 200, rate_hz: float = 20.0,
+             n_out: int = 8, dt: float = 1.0, seed: int = SEED_BASE + 5):
+    """W_rec ~ N(0, 1/N), W_in ~ N(0, 1/n_in), W_out ~ N(0, 1/N); Poisson inputs (rate_hz) as 0/1 per
+    (step, channel, trial); targets sin(2 pi f t + phi), f ~ U[0.5, 4] Hz, phi ~ U[0, 2 pi), shared by
+    all trials. Returns float32 numpy arrays (W_in, W_rec, W_out, x [T, n_in, B], target [T, n_out, B])."""
+    rng = np.random.default_rng(seed)
+    W_rec = (rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32)
+    W_in = (rng.standard_normal((n, n_in)) / np.sqrt(n_in)).astype(np.float32)
+    W_out = (rng.standard_normal((n_out, n)) / np.sqrt(n)).astype(np.float32)
+    x = (rng.random((steps, n_in, batch)) < rate_hz * dt * 1e-3).astype(np.float32)
+    f = rng.uniform(0.5, 4.0, n_out)
+    phi = rng.uniform(0.0, 2 * np.pi, n_out)
+    t = np.arange(steps) * dt * 1e-3
+    tgt = np.sin(2 * np.pi * f[None, :] * t[:, None] + phi[None, :]).astype(np.float32)
+    target = np.repeat(tgt[:, :, None], batch, axis=2)
+    return W_in, W_rec, W_out, x, target

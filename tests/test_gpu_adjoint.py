@@ -1,0 +1,83 @@
+"""Surrogate-gradient adjoint on the GPU against the FP64 checker (autograd unroll with the
+GPU's spike raster replayed, and finite differences in the smooth mode). Bar: gradients
+within 1e-4 relative of FP64."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_ref as R
+from neuronbridge_amd import workloads as W
+from neuronbridge_amd.adjoint import LifParams, loss_and_grads
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def to_dev(*arrs):
+    return [torch.tensor(a, device=DEV) for a in arrs]
+
+
+def rel_err(a: torch.Tensor, b: torch.Tensor) -> float:
+    a, b = a.double().cpu(), b.double().cpu()
+    return float((a - b).norm() / b.norm())
+
+
+def check_against_fp64(n, b, T, checkpoint, mode=1, seed=None, tol=1e-4):
+    kw = dict(n=n, batch=b, steps=T)
+    if seed is not None:
+        kw["seed"] = seed
+    Wi, Wr, Wo, x, tg = W.diff_lif(**kw)
+    p = LifParams(mode=mode)
+    loss, grads, S = loss_and_grads(*to_dev(Wi, Wr, Wo, x, tg), p, checkpoint)
+    torch.cuda.synchronize()
+    raster = S.cpu().double()
+    # FP64 forward without replay: how many threshold decisions differ from the FP32 GPU run
+    _, r64, _ = R.run(*[torch.tensor(a, dtype=torch.float64) for a in (Wi, Wr, Wo, x)], p)
+    flips = int((r64 != raster).sum()) if mode != 2 else 0
+    g64, loss64, _ = R.gradients(torch.tensor(Wi), torch.tensor(Wr), torch.tensor(Wo), torch.tensor(x), p,
+                                 torch.tensor(tg), spikes=raster if mode != 2 else None)
+    errs = [rel_err(g, r) for g, r in zip(grads, g64)]
+    assert abs(float(loss) - loss64) <= 1e-5 * abs(loss64), (float(loss), loss64)
+    assert max(errs) <= tol, errs
+    return errs, flips, float(raster.mean())
+
+
+@pytest.mark.parametrize("checkpoint", [0, 50])
+def test_surrogate_gradients_match_fp64(gpu, checkpoint):
+    errs, flips, rate = check_against_fp64(512, 16, 150, checkpoint)
+    assert rate > 1e-3
+    print(f"relative gradient errors {errs}, FP32/FP64 spike decisions differing {flips}")
+
+
+def test_smooth_mode_matches_fp64_and_finite_differences(gpu):
+    n, b, T = 96, 4, 60
+    Wi, Wr, Wo, x, tg = W.diff_lif(n=n, batch=b, steps=T, seed=123)
+    p = LifParams(mode=2)
+    _, grads, _ = loss_and_grads(*to_dev(Wi, Wr, Wo, x, tg), p, 16)
+    g64, _, _ = R.gradients(torch.tensor(Wi), torch.tensor(Wr), torch.tensor(Wo), torch.tensor(x), p, torch.tensor(tg))
+    for which in range(3):
+        assert rel_err(grads[which], g64[which]) <= 1e-4
+        flat = grads[which].reshape(-1).double().cpu()
+        idx = np.random.default_rng(which).choice(flat.numel(), size=6, replace=False)
+        fd, _ = R.finite_difference(torch.tensor(Wi), torch.tensor(Wr), torch.tensor(Wo), torch.tensor(x), p,
+                                    torch.tensor(tg), which, idx, eps=1e-6)
+        assert torch.max(torch.abs(flat[idx] - fd)) <= 1e-4 * torch.max(torch.abs(fd)) + 1e-8
+
+
+def test_checkpointing_does_not_change_gradients(gpu):
+    Wi, Wr, Wo, x, tg = W.diff_lif(n=256, batch=8, steps=120, seed=9)
+    p = LifParams()
+    ref = None
+    for k in (0, 50, 7):
+        loss, grads, _ = loss_and_grads(*to_dev(Wi, Wr, Wo, x, tg), p, k)
+        if ref is None:
+            ref = grads
+        else:
+            for g, r in zip(grads, ref):
+                assert torch.allclose(g, r, rtol=1e-6, atol=1e-9)
+
+
+def test_full_size_configuration(gpu):
+    # N=2048, 64 trials, T=500, checkpoint every 50 steps (the configuration's sizes)
+    errs, flips, rate = check_against_fp64(2048, 64, 500, 50)
+    print(f"full size: relative gradient errors {errs}, spike decisions differing {flips}, rate {rate * 1000:.1f} Hz")
