@@ -83,6 +83,87 @@ def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
             "syn_events_per_s": events / (ms * 1e-3)}
 
 
+def batched_bench(steps: int, sample: int) -> dict:
+    """Config 4: 256 independent LIF instances (N ~ U{512..8192}, dense BF16 weights) in one
+    launch schedule vs the plain per-instance FP32 path on a sample of instances (graph engine,
+    same BF16-valued weights); rasters of the sampled instances must be identical."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.batch import Batch
+    from neuronbridge_amd.hiptime import Event
+    t0 = time.perf_counter()
+    inst, shared = W.batched_lif(steps=steps)
+    log(f"batched: built {len(inst)} instances ({sum(i.n for i in inst)} neurons, "
+        f"{sum(i.n * i.n for i in inst) / 1e9:.2f} G BF16 weights) in {time.perf_counter() - t0:.1f} s")
+    b = Batch(inst, **shared)
+    thr = b.tune(100)
+    b.record(0, steps)
+    e0, e1 = Event(), Event()
+    e0.record(b.stream)
+    b.run_async(steps)
+    e1.record(b.stream)
+    e1.synchronize()
+    ms_b = e0.elapsed_ms(e1)
+    ev_b = b.events()
+    total_n = sum(i.n for i in inst)
+    order = np.argsort([i.n for i in inst])
+    pick = order[np.linspace(0, len(inst) - 1, sample).astype(int)]
+    plain_ms, plain_n, identical = 0.0, 0, True
+    warm = 64  # the first 64 steps capture and instantiate the step graph; they are not timed
+    for k in pick:
+        ins = inst[int(k)]
+        net = W.instance_workload(ins, shared).make_net(fixed_shift=ins.fixed_shift)
+        net.record(0, steps, counts=False, raster=True)
+        net.run(warm, L.ENGINE_GRAPH)
+        f0, f1 = Event(), Event()
+        f0.record(net.stream)
+        net.run_async(steps - warm, L.ENGINE_GRAPH)
+        f1.record(net.stream)
+        f1.synchronize()
+        net.sync()
+        plain_ms += f0.elapsed_ms(f1)
+        plain_n += ins.n
+        identical &= bool(np.array_equal(net.raster(), b.raster(int(k))))
+        net.close()
+    batched_rate = total_n * steps / (ms_b * 1e-3)
+    plain_rate = plain_n * (steps - warm) / (plain_ms * 1e-3)
+    return {"workload": f"{len(inst)} LIF instances, N~U{{512..8192}}, dense BF16 W, RK4 dt 0.1 ms, {steps} steps",
+            "value": batched_rate, "unit": "instance-neuron-steps/s", "ms": round(ms_b, 2),
+            "syn_events_per_s": ev_b / (ms_b * 1e-3), "workgroup_threads": thr,
+            "plain_per_instance": {"value": plain_rate, "unit": "instance-neuron-steps/s",
+                                   "sample_instances": [int(inst[int(k)].n) for k in pick]},
+            "speedup_vs_plain": batched_rate / plain_rate, "rasters_identical_on_sample": identical}
+
+
+def adjoint_bench(reps: int) -> dict:
+    """Config 5: forward + surrogate-gradient backward (checkpoint 50) of the 2048-neuron recurrent
+    LIF network, 64 trials x 500 steps; GEMM flop count over the timed training step."""
+    import torch
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.adjoint import LifParams, loss_and_grads
+    Wi, Wr, Wo, x, tg = W.diff_lif()
+    dev = "cuda:0"
+    args = [torch.tensor(a, device=dev) for a in (Wi, Wr, Wo, x, tg)]
+    p = LifParams()
+    loss_and_grads(*args, p, 50)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        loss, grads, S = loss_and_grads(*args, p, 50)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    T, cin, B = x.shape
+    N, M = Wr.shape[0], Wo.shape[0]
+    rec = 2.0 * N * N * B * T
+    flop = 4 * rec + 2 * 2.0 * N * cin * B * T + 3 * 2.0 * M * N * B * T
+    return {"workload": f"recurrent LIF N={N}, {B} trials, T={T}, dt 1 ms, 200 Poisson inputs, 8 readouts, "
+                        f"checkpoint 50", "ms_per_training_step": round(ms, 3),
+            "gemm_tflops": flop / (ms * 1e-3) / 1e12, "gemm_peak_tflops_fp32": 157.3,
+            "loss": float(loss), "spike_rate_hz": float(S.float().mean()) * 1000.0}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +176,10 @@ def main() -> int:
     ap.add_argument("--replay-steps", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=5000)
+    ap.add_argument("--batched-sample", type=int, default=8)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -169,6 +254,13 @@ def main() -> int:
         rc = hip().hipMemcpy(dst.ptr, src.ptr, nbytes, 3)  # device to device
         hip().hipDeviceSynchronize()
         log(f"calibration copy of {args.calibrate_copy_mib} MiB (rc {rc})")
+    if not args.no_batched:
+        net.close()
+        out["batched"] = batched_bench(args.batched_steps, args.batched_sample)
+        log(f"batched: {out['batched']}")
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(3)
+        log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)

@@ -107,13 +107,24 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
         int32_t acc[NPT];
 #pragma unroll
         for (int u = 0; u < NPT; ++u) acc[u] = 0;
-        for (int32_t s = 0; s < cnt; ++s) {
-            const uint16_t* row = B.w + I.w_off + (int64_t)ids[s] * n;
+        // kRows fired rows in flight per pass, so their loads overlap instead of one row per trip
+        constexpr int kRows = 4;
+        for (int32_t s0 = 0; s0 < cnt; s0 += kRows) {
+            uint16_t wv[kRows][NPT];
 #pragma unroll
-            for (int u = 0; u < NPT; ++u) {
-                const int32_t i = tid + u * T;
-                if (i < n) acc[u] += quantize(bf16_to_f32(row[i]), I.scale);
+            for (int r = 0; r < kRows; ++r) {
+                const bool ok = s0 + r < cnt;
+                const uint16_t* row = B.w + I.w_off + (int64_t)(ok ? ids[s0 + r] : 0) * n;
+#pragma unroll
+                for (int u = 0; u < NPT; ++u) {
+                    const int32_t i = tid + u * T;
+                    wv[r][u] = (ok && i < n) ? row[i] : (uint16_t)0;
+                }
             }
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+#pragma unroll
+                for (int u = 0; u < NPT; ++u) acc[u] += quantize(bf16_to_f32(wv[r][u]), I.scale);
         }
         if (tid == 0) s_cnt[cur ^ 1] = 0;
         __syncthreads();
