@@ -32,6 +32,22 @@ STATE_BYTES = 24   # per neuron-step: read V, refractory counter, ring slot; wri
 EVENT_BYTES = 8    # per synaptic event: packed index/delay (4 B) + weight (4 B)
 
 
+def measured_traffic(kernel: str, steps: int | None = None):
+    """HBM bytes per launch of `kernel` from the committed PMC profile (profiles/traffic.json),
+    scaled to `steps` steps when the launch length differs; None when not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    b = (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"]) * 1024.0
+    if steps is not None:
+        b *= steps / k["steps_per_launch"]
+    return int(b)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -78,7 +94,8 @@ def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
     alg = events * EVENT_BYTES + K * n * 4
     ach = alg / (ms * 1e-3) / 1e9
     return {"kernel": "nbx_replay_blocked", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("nbx_replay_blocked", K),
+            "algorithmic_bytes": alg,
             "steps_per_launch": K, "events_per_launch": events, "ms_per_launch": round(ms, 4),
             "syn_events_per_s": events / (ms * 1e-3)}
 
@@ -237,7 +254,8 @@ def main() -> int:
                                f"{args.integrator} dt 0.1 ms", "neurons": n, "synapses": int(wl.syn.count),
                    "engine": ename, "fixed_shift": net.fixed_shift, "parallelism": "single GPU"},
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("nbx_persistent", args.steps) if kname == "nbx_persistent" else None,
                      "launch_ms": round(ms, 3), "algorithmic_bytes": alg},
     }
     if not args.no_replay:

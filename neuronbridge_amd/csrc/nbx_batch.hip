@@ -18,7 +18,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <memory>
 #include <numeric>
@@ -70,7 +69,6 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
     extern __shared__ uint16_t s_ids[];  // two buffers of n ids
     __shared__ int32_t s_cnt[2];
     __shared__ int32_t s_wcnt[16];
-    __shared__ uint32_t s_ev_lo;
     const InstDev& I = B.inst[B.order[blockIdx.x]];
     const int32_t n = I.n;
     const int32_t T = blockDim.x, tid = threadIdx.x;
@@ -92,10 +90,7 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
     const int32_t inst_id = B.order[blockIdx.x];
     const int32_t c0 = B.spk_cnt[inst_id];
     for (int32_t j = tid; j < c0; j += T) s_ids[j] = B.spk[I.s_off + j];
-    if (tid == 0) {
-        s_cnt[0] = c0;
-        s_ev_lo = 0;
-    }
+    if (tid == 0) s_cnt[0] = c0;
     __syncthreads();
     unsigned long long ev = 0;
     int cur = 0;
@@ -191,8 +186,6 @@ void* batch_kernel(int npt) {
         default: return (void*)nbx_batch_kernel<16>;
     }
 }
-
-float decay_none(float) { return 1.0f; }
 
 }  // namespace
 
