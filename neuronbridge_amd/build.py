@@ -67,5 +67,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+NATIVE_TESTS = ["dense_runtime_smoke"]
+
+
+def build_native_tests() -> list:
+    """g++ builds of the C++ checks under tests/native against libnbx.so (host code only)."""
+    out = []
+    src_dir = os.path.join(ROOT, "tests", "native")
+    for name in NATIVE_TESTS:
+        exe = os.path.join(src_dir, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
+               os.path.join(PKG, "shim"), os.path.join(src_dir, name + ".cpp"), "-o", exe, "-L", PKG, "-lnbx",
+               "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../neuronbridge_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {name}:\n{r.stdout}{r.stderr}")
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
