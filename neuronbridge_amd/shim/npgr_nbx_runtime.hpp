@@ -27,7 +27,7 @@
 namespace npgr_nbx {
 
 struct RuntimeConfig {
-    int steps_to_keep = 0;  // firing-history depth (nbx keeps max_delay + 1 steps)
+    int steps_to_keep = 0;  // firing-history depth (nbx keeps max(max_delay, 2) committed steps)
     bool enable_learning_buffers = false;
     float dt_ms = 1.0f;
 };
@@ -147,7 +147,8 @@ public:
         if (nbx_net_ring_shape(net_, &slots, &ch) != NBX_OK) return Fail(reason);
         history->clear();
         std::vector<int32_t> ids((size_t)Neurons());
-        for (int s = t - 1; s >= 0 && s > t - slots + 1; --s) {
+        // after a commit the ring holds steps t-1 down to t-1-(slots-2): slots - 1 steps
+        for (int s = t - 1; s >= 0 && s >= t - slots + 1; --s) {
             int32_t cnt = 0;
             if (nbx_net_get_history(net_, s, ids.data(), (int32_t)ids.size(), &cnt) != NBX_OK) return Fail(reason);
             history->push_back({s, std::vector<int>(ids.begin(), ids.begin() + cnt)});

@@ -73,16 +73,17 @@ int main() {
         npgr_nbx::RuntimeConfig cfg;
         cfg.steps_to_keep = 4;
         if (!rt.Initialize(lif(3, -59.8f), chain(3, {12.0f, 8.0f}, {}), cfg, nullptr, &reason)) return fail(reason);
-        std::vector<std::vector<int>> history;
+        // the history ring holds the last max(max_delay, 2) committed steps: read each step's
+        // firings right after its commit
+        std::vector<int> at[3];
         for (int t = 0; t < 3; ++t) {
             std::vector<int> fired;
             if (!step(rt, t, t == 0 ? std::vector<int>{0} : std::vector<int>{}, &fired, &reason)) return fail(reason);
+            std::vector<npgr_nbx::FiringStepView> ring;
+            if (!rt.ExportFiringHistoryRing(&ring, &reason)) return fail(reason);
+            if (ring.empty() || ring.front().time_step != t) return fail("history ring must start at the last commit");
+            at[t] = ring.front().firing_ids;
         }
-        std::vector<npgr_nbx::FiringStepView> ring;
-        if (!rt.ExportFiringHistoryRing(&ring, &reason)) return fail(reason);
-        std::vector<int> at[3];
-        for (const auto& v : ring)
-            if (v.time_step >= 0 && v.time_step < 3) at[v.time_step] = v.firing_ids;
         if (at[0] != std::vector<int>{0}) return fail("t0 must hold only the injected spike");
         if (at[1] != std::vector<int>{1}) return fail("delay-1 target must fire at t1");
         if (at[2] != std::vector<int>{2}) return fail("second hop must fire at t2");
