@@ -29,6 +29,13 @@ void set_error(const std::string& msg);
         }                                                                                        \
     } while (0)
 
+// return a non-zero nbx status to the caller
+#define NBX_PASS(expr)                  \
+    do {                                \
+        const int32_t rc_ = (expr);     \
+        if (rc_ != 0) return rc_;       \
+    } while (0)
+
 // ----------------------------------------------------------------- packed synapse entry
 // bits 0..23 post neuron, 24..29 delay (1..63), 30..31 type (0 exc, 1 inh)
 constexpr uint32_t kPostMask = 0x00FFFFFFu;

@@ -11,17 +11,39 @@
 using nbx::ModelParams;
 using nbx::StepArgs;
 
-// Post-blocked layout of the persistent engine: block b owns neurons
-// [b * P, min((b + 1) * P, n)); the synapses of presynaptic neuron i that target
-// block b are seg[off[i * (B + 1) + b] .. off[i * (B + 1) + b + 1]).
+// Post-blocked layout of the persistent engine and of replay delivery: block b owns
+// neurons [b * P, min((b + 1) * P, n)). Blocks are grouped into regions of rb blocks
+// (rb = B / 8 rounded up to an even count, so pairs of blocks never straddle a region).
+// The synapses of presynaptic neuron i that target block b are seg[o0 .. o1) with
+//   o0 = off[i * nr * gs + (b / rb) * gs + b % rb],   o1 = the next entry of that region,
+// i.e. each region keeps rb + 1 running offsets in its own 128-byte aligned group of gs
+// ints, and each region's synapses start on a 128-byte boundary in seg (padding entries
+// between regions are never inside any [o0, o1)).
 struct PersistentLayout {
     bool built = false;
     bool fits = false;
     std::string reason;
     int32_t B = 0, P = 0, threads = 0, npt = 0, lds_bytes = 0;
-    int32_t* d_off = nullptr;   // [n][B + 1]
-    uint2* d_seg = nullptr;     // {post_local | delay << 16 | channel << 24, q}
-    int32_t* d_bar = nullptr;   // grid barrier counter
+    int32_t rb = 0, nr = 0, gs = 0;  // blocks per region, regions, ints per region in an offset row
+    int64_t seg_count = 0;           // entries in seg, region padding included
+    int64_t max_row = 0;             // longest presynaptic row
+    int32_t* d_off = nullptr;        // [n][nr * gs]
+    uint2* d_seg = nullptr;          // {post_local | delay << 16 | channel << 24 | (block & 127) << 25, q}
+    int32_t* d_bar = nullptr;        // grid barrier counter
+};
+
+// Fixed-slot layout of replay delivery: the synapses of presynaptic neuron i onto the m*P
+// neurons of post block q sit in one 128-byte slot of 16 entries, slots[(i * Bq + q) * 16 ..],
+// addressable without an offset lookup. Unused entries hold kSlotEmpty; a segment longer than
+// 16 keeps 15 entries in the slot and a link in the last one ({kSlotLink | count, start}) to the
+// rest, stored contiguously in ovf. Entry: {channel*m*P + post - q*m*P | (delay - 1) << 16, q}:
+// the entry's LDS word within one ring step, and its output step offset.
+struct ReplaySlots {
+    bool built = false;
+    int32_t m = 0, Bq = 0;
+    uint2* d_slots = nullptr;  // [n][Bq][16]
+    uint2* d_ovf = nullptr;
+    int64_t ovf_count = 0;
 };
 
 struct nbx_net {
@@ -98,6 +120,8 @@ struct nbx_net {
     int32_t phase = 0;
     // persistent engine
     PersistentLayout pl;
+    // replay delivery
+    ReplaySlots rs;
 
     StepArgs args() const {
         StepArgs A{};

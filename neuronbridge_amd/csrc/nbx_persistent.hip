@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nbx_net.h"
@@ -33,6 +34,9 @@ constexpr int kMaxThreads = 1024;
 constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
 constexpr int kLanes = 4;                  // lanes sharing one spike's synapse segment
+constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;  // replay slot entry: unused
+constexpr uint32_t kSlotLink = 0x80000000u;   // replay slot entry: link to the overflow list
+constexpr int kSlotCap = 16;                  // entries per replay slot (128 bytes)
 
 
 __device__ inline int32_t block_of(int32_t w, int32_t B) {
@@ -63,6 +67,7 @@ struct PersistArgs {
     const int32_t* off;
     const uint2* seg;
     int32_t B, P;
+    int32_t rb, gs, os;  // blocks per region, ints per region group, ints per offset row
     int32_t t0, nsteps;
     Barrier* bar;
 };
@@ -122,6 +127,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const int32_t tid = threadIdx.x;
     const int32_t lane = tid & 63, wave = tid >> 6, nwaves = T >> 6;
     const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
+    const int32_t i0 = (b / X.rb) * X.gs + b % X.rb;  // this block's entry in an offset row; i0 + 1 ends it
 
     // load this block's slice of the delay ring and its neurons' state
     for (int32_t j = tid; j < D * nch * P; j += T) {
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 for (int r = 0; r < 2; ++r) {
                     o0[r] = o1[r] = 0;
                     if (pre[r] >= 0) {
-                        const int32_t* op = X.off + (int64_t)pre[r] * (X.B + 1) + b;
+                        const int32_t* op = X.off + (int64_t)pre[r] * X.os + i0;
                         o0[r] = op[0];
                         o1[r] = op[1];
                     }
@@ -184,7 +190,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                     }
                 auto deposit = [&](const uint2 x) {
                     const int32_t d = (int32_t)((x.x >> 16) & 0xFFu);
-                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)(x.x >> 24)) * P + (int32_t)(x.x & 0xFFFFu)],
+                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)((x.x >> 24) & 1u)) * P +
+                                      (int32_t)(x.x & 0xFFFFu)],
                               (int32_t)x.y);
                 };
 #pragma unroll
@@ -268,101 +275,428 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     if (tid == 0 && s_ev) atomicAdd(A.events, s_ev);
 }
 
-// Replay delivery on the post-blocked layout. Block b owns neurons [b*P, b*P + P) and
-// streams the injected spikes of all K input steps through an LDS window of
-// kReplayWin output steps plus a (D-1)-step carry: contributions that land beyond the
-// window stay in LDS and move to the front when the window is flushed, so every
-// synapse segment is read exactly once however many steps one launch covers.
-// Lanes work in groups of kLanes on one spike, so the consecutive entries of its
-// segment coalesce into one memory request; each group keeps kR spikes in flight
-// (ids, then segment offsets, then entries) before its LDS atomics.
-constexpr int kReplayWin = 64;
-constexpr int kR = 4;
-constexpr int kU = 2;  // entries per lane loaded ahead (kLanes * kU per segment)
+// Replay delivery on the post-blocked layout.
+//
+// Workgroup (g, q) owns the neurons of m consecutive layout blocks [q*m, q*m + m) and the
+// output steps [g*Kg, g*Kg + Kg) of a K-step launch. Its LDS holds a circular ring of
+// R = W + D - 1 output steps for its neurons. It walks the input steps from D - 2 steps
+// before its first output (contributions landing earlier belong to the previous time
+// group) in windows of W steps; after the input steps [k0, k0 + W) the outputs
+// [k0, k0 + W) are complete (delays are >= 1), so they are written to HBM and their slots
+// cleared. Every synapse segment is read once per launch (plus a D - 2 step halo).
+//
+// The kernel is bound by memory requests in flight, so the per-window work is a three-
+// stage software pipeline: while the segments of window w are delivered, the segment
+// bounds (o0, o1) of window w+1 and the spike ids of window w+2 are already in flight.
+// Delivery itself keeps two batches of kRR spikes per lane group in flight; lanes work in
+// groups of LANES on one spike so the consecutive entries of its segment coalesce, and
+// arrivals are int32 fixed point added with LDS atomics, so the sum is order independent.
+constexpr int kMaxWin = 128;
+constexpr int kTpCap = 1024;  // input steps whose tick pointers one pass keeps in LDS
+constexpr int kRR = 4;        // spikes per lane group per batch
+constexpr int kRU = 2;        // entries per lane loaded ahead (LANES * kRU per segment)
+constexpr int kStageU = 4;    // spikes staged per thread per window (C <= kStageU * 1024)
 
-__global__ __launch_bounds__(1024) void nbx_replay_blocked(StepArgs A, const int32_t* __restrict__ off,
-                                                           const uint2* __restrict__ seg, int32_t B, int32_t P,
-                                                           const int32_t* __restrict__ ids,
-                                                           const int32_t* __restrict__ tick_ptr, int32_t K,
-                                                           int32_t* __restrict__ out) {
-    extern __shared__ int32_t s_acc[];  // [(kReplayWin + D - 1) * nch * P]
-    __shared__ int32_t s_tp[kReplayWin + 1];
-    const int32_t nch = A.nch, n = A.n, D = A.D;
-    const int32_t b = block_of(blockIdx.x, B);
-    const int32_t p0 = b * P;
-    const int32_t np = max(0, min(P, n - p0));
+struct ReplayArgs {
+    const int32_t* off;
+    const uint2* seg;
+    int32_t B, P, rb, gs, os;   // layout
+    int32_t m, Bq, G, W, R, C;  // merge factor, post blocks, time groups, window, ring slots, staged spikes
+    int32_t n, nch, D, K;
+    const int32_t* ids;
+    const int32_t* tick_ptr;
+    int32_t* out;
+    unsigned long long* events;
+    const uint4* slots;  // fixed-slot layout: 8 uint4 (16 entries) per (pre, post block)
+    const uint2* ovf;
+    const int64_t* row_ptr;  // CSR row starts (row lengths count the synaptic events)
+};
+
+template <int LANES>
+__global__ __launch_bounds__(1024) void nbx_replay_ring(ReplayArgs X) {
+    extern __shared__ int32_t s_mem[];
+    __shared__ int32_t s_tp[kTpCap + 1];
     const int32_t T = blockDim.x, tid = threadIdx.x;
-    const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
-    const int32_t span = (kReplayWin + D - 1) * nch * P;
-    const int32_t tick_words = nch * P;
-    for (int32_t j = tid; j < span; j += T) s_acc[j] = 0;
+    const int32_t gl = tid % LANES, grp = tid / LANES, ngrp = T / LANES;
+    const int32_t Pr = X.m * X.P, words = X.nch * Pr, R = X.R, C = X.C, W = X.W;
+    int32_t* s_ring = s_mem;                                           // [R][nch][Pr]
+    int2* s_span = reinterpret_cast<int2*>(s_mem + (int64_t)R * words);  // [2][C]: {o0, len | step << 24}
+
+    // consecutive post blocks of one time group share an XCD (and its L2)
+    const int32_t nw = gridDim.x;
+    const int32_t lin = (nw % kXcds == 0) ? (int32_t)(blockIdx.x % kXcds) * (nw / kXcds) + (int32_t)blockIdx.x / kXcds
+                                          : (int32_t)blockIdx.x;
+    const int32_t g = lin / X.Bq, q = lin % X.Bq;
+    const int32_t b0 = q * X.m, b1 = min(b0 + X.m, X.B);
+    const int32_t p0 = b0 * X.P;
+    const int32_t np = max(0, min(Pr, X.n - p0));
+    const int32_t Kg = (X.K + X.G - 1) / X.G;
+    const int32_t o_begin = g * Kg, o_end = min(X.K, o_begin + Kg);
+    if (np == 0 || o_begin >= o_end || b0 >= X.B) return;  // uniform over the workgroup
+    const int32_t k_begin = max(0, o_begin - (X.D - 2));
+    // offsets of [b0, b1) within a row: the pair never straddles a region (rb is even)
+    const int32_t i0 = (b0 / X.rb) * X.gs + b0 % X.rb;
+    const int32_t i1 = i0 + (b1 - b0);
+    const uint32_t b0lo = (uint32_t)b0 & 0x7Fu;
+
+    for (int32_t j = tid; j < R * words; j += T) s_ring[j] = 0;
     unsigned long long ev = 0;
-    for (int32_t w0 = 0; w0 < K; w0 += kReplayWin) {
-        const int32_t wl = min(kReplayWin, K - w0);
-        for (int32_t j = tid; j <= wl; j += T) s_tp[j] = tick_ptr[w0 + j];
+    int32_t phys = 0;  // ring slot of the first output step of the current window
+
+    for (int32_t s0 = k_begin; s0 < o_end; s0 += kTpCap) {
+        const int32_t sl = min(kTpCap, o_end - s0);
+        const int32_t nwin = (sl + W - 1) / W;
         __syncthreads();
-        const int32_t sb = s_tp[0], cnt = s_tp[wl] - sb;
-        auto deposit = [&](const uint2 x, int32_t tkr) {
-            const int32_t ko = tkr + (int32_t)((x.x >> 16) & 0xFFu) - 1;  // window-relative output step
-            if (w0 + ko < K)
-                atomicAdd(&s_acc[ko * tick_words + (int32_t)(x.x >> 24) * P + (int32_t)(x.x & 0xFFFFu)], (int32_t)x.y);
+        for (int32_t j = tid; j <= sl; j += T) s_tp[j] = X.tick_ptr[s0 + j];
+        __syncthreads();
+        auto win_lo = [&](int32_t w) { return s_tp[w * W]; };
+        auto win_cnt = [&](int32_t w) { return s_tp[min(w * W + W, sl)] - s_tp[w * W]; };
+        auto load_ids = [&](int32_t w, int32_t (&dst)[kStageU]) {
+            const int32_t lo = (w < nwin) ? win_lo(w) : 0;
+            const int32_t cnt = (w < nwin) ? min(win_cnt(w), C) : 0;
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) {
+                const int32_t f = tid + u * T;
+                dst[u] = (f < cnt) ? X.ids[lo + f] : -1;
+            }
         };
-        for (int32_t base = grp; base < cnt; base += ngrp * kR) {
-            int32_t pre[kR], o0[kR], o1[kR], tk[kR];
+        auto load_bounds = [&](const int32_t (&id)[kStageU], int32_t (&o0)[kStageU], int32_t (&o1)[kStageU]) {
 #pragma unroll
-            for (int r = 0; r < kR; ++r) pre[r] = (base + r * ngrp < cnt) ? ids[sb + base + r * ngrp] : -1;
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-                o0[r] = o1[r] = 0;
-                if (pre[r] >= 0) {
-                    const int32_t* op = off + (int64_t)pre[r] * (B + 1) + b;
-                    o0[r] = op[0];
-                    o1[r] = op[1];
+            for (int u = 0; u < kStageU; ++u) {
+                o0[u] = o1[u] = 0;
+                if (id[u] >= 0) {
+                    const int32_t* op = X.off + (int64_t)id[u] * X.os;
+                    o0[u] = op[i0];
+                    o1[u] = op[i1];
                 }
             }
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {  // window step of each spike
-                const int32_t f = sb + base + r * ngrp;
-                int32_t lo = 0, hi = wl;
-                while (hi - lo > 1) {
-                    const int32_t mid = (lo + hi) >> 1;
-                    if (s_tp[mid] <= f) lo = mid;
-                    else hi = mid;
-                }
-                tk[r] = lo;
+        };
+        // window step of spike fa (absolute index) within window w, by binary search
+        auto step_of = [&](int32_t w, int32_t fa) {
+            const int32_t kw0 = w * W, wl = min(W, sl - kw0);
+            int32_t lo = 0, hi = wl;
+            while (hi - lo > 1) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (s_tp[kw0 + mid] <= fa) lo = mid;
+                else hi = mid;
             }
-            uint2 e[kR][kU];
+            return lo;
+        };
+        auto commit = [&](int32_t w, const int32_t (&id)[kStageU], const int32_t (&o0)[kStageU],
+                          const int32_t (&o1)[kStageU], int32_t buf) {
+            if (w >= nwin) return;
+            const int32_t lo = win_lo(w);
 #pragma unroll
-            for (int r = 0; r < kR; ++r)
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    const int32_t j = o0[r] + gl + u * kLanes;
-                    e[r][u] = (j < o1[r]) ? seg[j] : make_uint2(0u, 0u);
-                }
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-#pragma unroll
-                for (int u = 0; u < kU; ++u)
-                    if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u], tk[r]);
-                for (int32_t j = o0[r] + gl + kU * kLanes; j < o1[r]; j += kLanes) deposit(seg[j], tk[r]);
-                if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
+            for (int u = 0; u < kStageU; ++u) {
+                const int32_t f = tid + u * T;
+                if (id[u] >= 0) s_span[buf * C + f] = make_int2(o0[u], (o1[u] - o0[u]) | (step_of(w, lo + f) << 24));
             }
+        };
+
+        // prologue: window 0 staged, ids of window 1 in flight
+        int32_t idn[kStageU], o0v[kStageU], o1v[kStageU];
+        {
+            int32_t id0[kStageU];
+            load_ids(0, id0);
+            load_bounds(id0, o0v, o1v);
+            load_ids(1, idn);
+            commit(0, id0, o0v, o1v, 0);
         }
         __syncthreads();
-        // flush the window's steps, move the carry to the front, clear the rest
-        for (int32_t j = tid; j < wl * tick_words; j += T) {
-            const int32_t pl = j % P, kc = j / P;
-            if (pl < np) out[(int64_t)(w0 * nch + kc) * n + p0 + pl] = s_acc[j];
+
+        for (int32_t w = 0; w < nwin; ++w) {
+            const int32_t buf = w & 1;
+            const int32_t kw0 = w * W, wl = min(W, sl - kw0), k0 = s0 + kw0;
+            // prefetch: bounds of window w+1, ids of window w+2
+            load_bounds(idn, o0v, o1v);
+            int32_t idn2[kStageU];
+            load_ids(w + 2, idn2);
+
+            auto deposit = [&](const uint2 x, int32_t tkr) {
+                const int32_t rel = tkr + (int32_t)((x.x >> 16) & 0xFFu) - 1;  // output step - k0
+                const int32_t ko = k0 + rel;
+                if (ko >= o_begin && ko < o_end) {
+                    int32_t slot = phys + rel;
+                    if (slot >= R) slot -= R;
+                    const int32_t sub = (int32_t)(((x.x >> 25) - b0lo) & 0x7Fu);
+                    atomicAdd(&s_ring[slot * words + (int32_t)((x.x >> 24) & 1u) * Pr + sub * X.P +
+                                      (int32_t)(x.x & 0xFFFFu)],
+                              (int32_t)x.y);
+                }
+            };
+            const int32_t lo = win_lo(w), cntw = win_cnt(w);
+            for (int32_t c0 = 0; c0 < cntw; c0 += C) {
+                const int32_t cc = min(C, cntw - c0);
+                if (c0 > 0) {  // more spikes than one staging buffer holds: stage this chunk now
+                    __syncthreads();
+                    for (int32_t f = tid; f < cc; f += T) {
+                        const int32_t fa = lo + c0 + f;
+                        const int32_t* op = X.off + (int64_t)X.ids[fa] * X.os;
+                        const int32_t a0 = op[i0], a1 = op[i1];
+                        s_span[buf * C + f] = make_int2(a0, (a1 - a0) | (step_of(w, fa) << 24));
+                    }
+                    __syncthreads();
+                }
+                const int32_t step = ngrp * kRR;
+                auto fetch = [&](int32_t base, int2 (&sp)[kRR], uint2 (&e)[kRR][kRU]) {
+#pragma unroll
+                    for (int r = 0; r < kRR; ++r) {
+                        const int32_t f = base + r * ngrp;
+                        sp[r] = (f < cc) ? s_span[buf * C + f] : make_int2(0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < kRR; ++r)
+#pragma unroll
+                        for (int u = 0; u < kRU; ++u) {
+                            const int32_t j = gl + u * LANES;
+                            e[r][u] = (j < (sp[r].y & 0xFFFFFF)) ? X.seg[sp[r].x + j] : make_uint2(0u, 0u);
+                        }
+                };
+                auto consume = [&](const int2 (&sp)[kRR], const uint2 (&e)[kRR][kRU]) {
+#pragma unroll
+                    for (int r = 0; r < kRR; ++r) {
+                        const int32_t len = sp[r].y & 0xFFFFFF, tk = (int32_t)((uint32_t)sp[r].y >> 24);
+#pragma unroll
+                        for (int u = 0; u < kRU; ++u)
+                            if (gl + u * LANES < len) deposit(e[r][u], tk);
+                        for (int32_t j = gl + kRU * LANES; j < len; j += LANES) deposit(X.seg[sp[r].x + j], tk);
+                        // each spike is counted by the time group that owns its input step
+                        if (gl == 0 && k0 + tk >= o_begin) ev += (unsigned long long)len;
+                    }
+                };
+                int2 spA[kRR], spB[kRR];
+                uint2 eA[kRR][kRU], eB[kRR][kRU];
+                int32_t base = grp;
+                if (base < cc) fetch(base, spA, eA);
+                while (base < cc) {
+                    const int32_t b2 = base + step;
+                    if (b2 < cc) fetch(b2, spB, eB);
+                    consume(spA, eA);
+                    if (b2 >= cc) break;
+                    const int32_t b3 = b2 + step;
+                    if (b3 < cc) fetch(b3, spA, eA);
+                    consume(spB, eB);
+                    base = b3;
+                }
+            }
+            // stage window w+1 into the other buffer (its bounds were loaded during delivery)
+            commit(w + 1, idn, o0v, o1v, buf ^ 1);
+            __syncthreads();
+            // flush the window's complete output steps and clear their slots
+            for (int32_t kk = 0; kk < wl; ++kk) {
+                int32_t slot = phys + kk;
+                if (slot >= R) slot -= R;
+                int32_t* row = s_ring + slot * words;
+                const bool own = k0 + kk >= o_begin;
+                for (int32_t j = tid; j < words; j += T) {
+                    const int32_t ch = (j >= Pr) ? 1 : 0, pl = j - ch * Pr;
+                    if (own && pl < np) X.out[((int64_t)(k0 + kk) * X.nch + ch) * X.n + p0 + pl] = row[j];
+                    row[j] = 0;
+                }
+            }
+            phys += wl;
+            if (phys >= R) phys -= R;
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) idn[u] = idn2[u];
+            __syncthreads();
         }
-        if (w0 + kReplayWin >= K) break;  // last window: no carry needed
-        __syncthreads();
-        const int32_t carry = (D - 1) * tick_words;  // kReplayWin >= D - 1, so source and target do not overlap
-        for (int32_t j = tid; j < carry; j += T) s_acc[j] = s_acc[wl * tick_words + j];
-        __syncthreads();
-        for (int32_t j = carry + tid; j < span; j += T) s_acc[j] = 0;
-        __syncthreads();
     }
-    if (ev) atomicAdd(A.events, ev);
+    if (ev) atomicAdd(X.events, ev);
+}
+
+// Replay delivery on the fixed-slot layout: the same workgroup/time-group/window structure as
+// nbx_replay_ring, but a spike's synapses onto this workgroup's neurons are one aligned
+// 128-byte slot at a computed address, so each (spike, workgroup) pair costs one memory
+// request and no offset lookup. Eight lanes share a slot (16 bytes, two entries each). A slot
+// entry carries its LDS word within a ring step (channel * m*P + neuron) and delay - 1, and
+// the ring has a power-of-two step count, so a deposit is one add, mask, multiply-add and LDS
+// atomic; range checks run only in windows near the ends of the time group. The window's
+// spike ids (with their window step) are staged in LDS, those of the next window are in
+// flight while the current one is delivered; synaptic events are counted once per spike from
+// its row length by the workgroup of post block 0.
+constexpr int kSR = 6;  // spikes per lane group per batch
+
+__global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
+    constexpr int LANES = 8;
+    extern __shared__ int32_t s_mem[];
+    __shared__ int32_t s_tp[kTpCap + 1];
+    const int32_t T = blockDim.x, tid = threadIdx.x;
+    const int32_t gl = tid % LANES, grp = tid / LANES, ngrp = T / LANES;
+    const int32_t Pr = X.m * X.P, words = X.nch * Pr, R = X.R, C = X.C, W = X.W;
+    const uint32_t rmask = (uint32_t)R - 1u;                 // R is a power of two
+    int32_t* s_ring = s_mem;                                 // [R][nch][Pr]
+    int32_t* s_sid = s_mem + (int64_t)R * words;             // [2][C]: pre | window step << 24
+
+    const int32_t nw = gridDim.x;
+    const int32_t lin = (nw % kXcds == 0) ? (int32_t)(blockIdx.x % kXcds) * (nw / kXcds) + (int32_t)blockIdx.x / kXcds
+                                          : (int32_t)blockIdx.x;
+    const int32_t g = lin / X.Bq, q = lin % X.Bq;
+    const int32_t p0 = q * Pr;
+    const int32_t np = max(0, min(Pr, X.n - p0));
+    const int32_t Kg = (X.K + X.G - 1) / X.G;
+    const int32_t o_begin = g * Kg, o_end = min(X.K, o_begin + Kg);
+    if (np == 0 || o_begin >= o_end) return;  // uniform over the workgroup
+    const int32_t k_begin = max(0, o_begin - (X.D - 2));
+    const bool counter = (q == 0);  // this workgroup counts the time group's synaptic events
+
+    for (int32_t j = tid; j < R * words; j += T) s_ring[j] = 0;
+    unsigned long long ev = 0;
+    int32_t phys = 0;
+
+    for (int32_t s0 = k_begin; s0 < o_end; s0 += kTpCap) {
+        const int32_t sl = min(kTpCap, o_end - s0);
+        const int32_t nwin = (sl + W - 1) / W;
+        __syncthreads();
+        for (int32_t j = tid; j <= sl; j += T) s_tp[j] = X.tick_ptr[s0 + j];
+        __syncthreads();
+        auto win_lo = [&](int32_t w) { return s_tp[w * W]; };
+        auto win_cnt = [&](int32_t w) { return s_tp[min(w * W + W, sl)] - s_tp[w * W]; };
+        auto load_ids = [&](int32_t w, int32_t (&dst)[kStageU]) {
+            const int32_t lo = (w < nwin) ? win_lo(w) : 0;
+            const int32_t cnt = (w < nwin) ? min(win_cnt(w), C) : 0;
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) {
+                const int32_t f = tid + u * T;
+                dst[u] = (f < cnt) ? X.ids[lo + f] : -1;
+            }
+        };
+        auto load_deg = [&](const int32_t (&id)[kStageU], int32_t (&deg)[kStageU]) {
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u)
+                deg[u] = (counter && id[u] >= 0) ? (int32_t)(X.row_ptr[id[u] + 1] - X.row_ptr[id[u]]) : 0;
+        };
+        auto step_of = [&](int32_t w, int32_t fa) {
+            const int32_t kw0 = w * W, wl = min(W, sl - kw0);
+            int32_t lo = 0, hi = wl;
+            while (hi - lo > 1) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (s_tp[kw0 + mid] <= fa) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        };
+        auto commit = [&](int32_t w, const int32_t (&id)[kStageU], const int32_t (&deg)[kStageU], int32_t buf) {
+            if (w >= nwin) return;
+            const int32_t lo = win_lo(w), kw = s0 + w * W;
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) {
+                const int32_t f = tid + u * T;
+                if (id[u] >= 0) {
+                    const int32_t tk = step_of(w, lo + f);
+                    s_sid[buf * C + f] = id[u] | (tk << 24);
+                    if (kw + tk >= o_begin) ev += (unsigned long long)deg[u];  // deg is 0 unless counter
+                }
+            }
+        };
+
+        int32_t idn[kStageU], degn[kStageU];
+        {
+            int32_t id0[kStageU], deg0[kStageU];
+            load_ids(0, id0);
+            load_ids(1, idn);
+            load_deg(id0, deg0);
+            commit(0, id0, deg0, 0);
+        }
+        __syncthreads();
+
+        for (int32_t w = 0; w < nwin; ++w) {
+            const int32_t buf = w & 1;
+            const int32_t kw0 = w * W, wl = min(W, sl - kw0), k0 = s0 + kw0;
+            int32_t idn2[kStageU];
+            load_deg(idn, degn);
+            load_ids(w + 2, idn2);
+            // every arrival of this window lands in [k0, k0 + wl + D - 3]: check only near the ends
+            const bool checked = (k0 < o_begin) || (k0 + wl + X.D - 3 >= o_end);
+
+            const int32_t lo = win_lo(w), cntw = win_cnt(w);
+            for (int32_t c0 = 0; c0 < cntw; c0 += C) {
+                const int32_t cc = min(C, cntw - c0);
+                if (c0 > 0) {  // more spikes than one staging buffer holds: stage this chunk now
+                    __syncthreads();
+                    for (int32_t f = tid; f < cc; f += T) {
+                        const int32_t fa = lo + c0 + f;
+                        const int32_t pre = X.ids[fa], tk = step_of(w, fa);
+                        s_sid[buf * C + f] = pre | (tk << 24);
+                        if (counter && k0 + tk >= o_begin)
+                            ev += (unsigned long long)(X.row_ptr[pre + 1] - X.row_ptr[pre]);
+                    }
+                    __syncthreads();
+                }
+                const int32_t step = ngrp * kSR;
+                auto fetch = [&](int32_t base, int32_t (&sid)[kSR], uint4 (&v)[kSR]) {
+#pragma unroll
+                    for (int r = 0; r < kSR; ++r) {
+                        const int32_t f = base + r * ngrp;
+                        sid[r] = (f < cc) ? s_sid[buf * C + f] : -1;
+                    }
+#pragma unroll
+                    for (int r = 0; r < kSR; ++r)
+                        v[r] = (sid[r] >= 0) ? X.slots[((int64_t)(sid[r] & 0xFFFFFF) * X.Bq + q) * (kSlotCap / 2) + gl]
+                                             : make_uint4(kSlotEmpty, 0u, kSlotEmpty, 0u);
+                };
+                auto consume = [&](const int32_t (&sid)[kSR], const uint4 (&v)[kSR], auto chk) {
+                    constexpr bool CHK = decltype(chk)::value;
+#pragma unroll
+                    for (int r = 0; r < kSR; ++r) {
+                        const uint32_t tk = (uint32_t)sid[r] >> 24;
+                        const uint32_t tkp = (uint32_t)phys + tk;
+                        auto deposit = [&](uint32_t meta, uint32_t qv) {
+                            if (CHK) {
+                                const int32_t ko = k0 + (int32_t)(tk + (meta >> 16));
+                                if (ko < o_begin || ko >= o_end) return;
+                            }
+                            atomicAdd(&s_ring[((tkp + (meta >> 16)) & rmask) * (uint32_t)words + (meta & 0xFFFFu)],
+                                      (int32_t)qv);
+                        };
+                        if (v[r].x != kSlotEmpty) deposit(v[r].x, v[r].y);
+                        if ((v[r].z & 0xFF000000u) == kSlotLink) {  // the rest of a long segment
+                            const uint32_t cnt = v[r].z & 0xFFFFFFu;
+                            for (uint32_t k = 0; k < cnt; ++k) {
+                                const uint2 e = X.ovf[(int64_t)v[r].w + k];
+                                deposit(e.x, e.y);
+                            }
+                        } else if (v[r].z != kSlotEmpty) {
+                            deposit(v[r].z, v[r].w);
+                        }
+                    }
+                };
+                auto run = [&](auto chk) {
+                    int32_t sidA[kSR], sidB[kSR];
+                    uint4 vA[kSR], vB[kSR];
+                    int32_t base = grp;
+                    if (base < cc) fetch(base, sidA, vA);
+                    while (base < cc) {
+                        const int32_t b2 = base + step;
+                        if (b2 < cc) fetch(b2, sidB, vB);
+                        consume(sidA, vA, chk);
+                        if (b2 >= cc) break;
+                        const int32_t b3 = b2 + step;
+                        if (b3 < cc) fetch(b3, sidA, vA);
+                        consume(sidB, vB, chk);
+                        base = b3;
+                    }
+                };
+                if (checked) run(std::true_type{});
+                else run(std::false_type{});
+            }
+            commit(w + 1, idn, degn, buf ^ 1);
+            __syncthreads();
+            for (int32_t kk = 0; kk < wl; ++kk) {
+                int32_t* row = s_ring + (((uint32_t)phys + (uint32_t)kk) & rmask) * (uint32_t)words;
+                const bool own = k0 + kk >= o_begin;
+                for (int32_t j = tid; j < words; j += T) {
+                    const int32_t ch = (j >= Pr) ? 1 : 0, pl = j - ch * Pr;
+                    if (own && pl < np) X.out[((int64_t)(k0 + kk) * X.nch + ch) * X.n + p0 + pl] = row[j];
+                    row[j] = 0;
+                }
+            }
+            phys = (int32_t)(((uint32_t)phys + (uint32_t)wl) & rmask);
+#pragma unroll
+            for (int u = 0; u < kStageU; ++u) idn[u] = idn2[u];
+            __syncthreads();
+        }
+    }
+    if (ev) atomicAdd(X.events, ev);
 }
 
 // Replay with global atomics (networks the post-blocked layout does not fit).
@@ -420,6 +754,13 @@ struct Geometry {
     std::string why;
 };
 
+struct ReplayGeometry {
+    bool ok = false;
+    bool slots = false;
+    int32_t m = 1, Bq = 0, G = 1, W = 0, R = 0, C = 0, lanes = 4;
+    int64_t lds = 0;
+};
+
 Geometry geometry(const nbx_net* net) {
     Geometry g;
     int cus = 0;
@@ -470,30 +811,59 @@ int32_t build_persistent_layout(nbx_net* net) {
     }
     const int32_t n = net->n, B = g.B, P = g.P;
     const float scale = std::ldexp(1.0f, net->shift);
-    std::vector<int32_t> off((size_t)n * (B + 1));
-    std::vector<uint2> seg((size_t)net->S);
+    // regions of rb blocks (an even count), each with its own 128-byte aligned group of
+    // rb + 1 offsets per row and a 128-byte aligned start in seg
+    const int32_t per = B / kXcds;
+    const int32_t rb = (per % 2 == 0) ? per : 2 * per;
+    const int32_t nr = B / rb;
+    const int32_t gs = (rb + 1 + 31) / 32 * 32;
+    const int32_t os = nr * gs;
+    constexpr int64_t kAlign = 16;  // uint2 entries per 128 bytes
+    std::vector<int32_t> off((size_t)n * os, 0);
     std::vector<int32_t> cnt((size_t)B);
     int64_t pos = 0;
-    for (int32_t p = 0; p < n; ++p) {
-        const int64_t rb = net->h_row_ptr[(size_t)p], re = net->h_row_ptr[(size_t)p + 1];
+    for (int32_t p = 0; p < n; ++p) {  // pass 1: offsets
+        const int64_t rbeg = net->h_row_ptr[(size_t)p], rend = net->h_row_ptr[(size_t)p + 1];
         std::fill(cnt.begin(), cnt.end(), 0);
-        for (int64_t j = rb; j < re; ++j) cnt[(size_t)(syn_post(net->h_syn[(size_t)j]) / P)]++;
-        int32_t* o = off.data() + (size_t)p * (B + 1);
-        o[0] = (int32_t)pos;
-        for (int32_t b = 0; b < B; ++b) o[b + 1] = o[b] + cnt[(size_t)b];
-        for (int32_t b = 0; b < B; ++b) cnt[(size_t)b] = o[b];
-        for (int64_t j = rb; j < re; ++j) {
+        for (int64_t j = rbeg; j < rend; ++j) cnt[(size_t)(syn_post(net->h_syn[(size_t)j]) / P)]++;
+        int32_t* o = off.data() + (size_t)p * os;
+        for (int32_t r = 0; r < nr; ++r) {
+            pos = (pos + kAlign - 1) / kAlign * kAlign;
+            if (pos >= ((int64_t)1 << 31) - 1) {
+                set_error("persistent layout: more than 2^31 entries");
+                return NBX_ERR_UNSUPPORTED;
+            }
+            o[r * gs] = (int32_t)pos;
+            for (int32_t j = 0; j < rb; ++j) {
+                pos += cnt[(size_t)(r * rb + j)];
+                o[r * gs + j + 1] = (int32_t)pos;
+            }
+        }
+    }
+    std::vector<uint2> seg((size_t)pos, make_uint2(0u, 0u));
+    for (int32_t p = 0; p < n; ++p) {  // pass 2: entries, in row order within each block
+        const int64_t rbeg = net->h_row_ptr[(size_t)p], rend = net->h_row_ptr[(size_t)p + 1];
+        const int32_t* o = off.data() + (size_t)p * os;
+        for (int32_t bb = 0; bb < B; ++bb) cnt[(size_t)bb] = o[(bb / rb) * gs + bb % rb];
+        for (int64_t j = rbeg; j < rend; ++j) {
             const uint32_t e = net->h_syn[(size_t)j];
-            const int32_t post = syn_post(e), ty = syn_type(e);
-            int32_t q = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
+            const int32_t post = syn_post(e), ty = syn_type(e), blk = post / P;
+            int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
             int32_t ch = 0;
             if (net->md.model == NBX_MODEL_IZH) ch = (ty == 1) ? 1 : 0;
-            else if (net->md.model != NBX_MODEL_IZH_CUR && ty == 1) q = -q;
-            const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 24);
-            seg[(size_t)cnt[(size_t)(post / P)]++] = make_uint2(meta, (uint32_t)q);
+            else if (net->md.model != NBX_MODEL_IZH_CUR && ty == 1) qv = -qv;
+            const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 24) |
+                                  (((uint32_t)blk & 0x7Fu) << 25);
+            seg[(size_t)cnt[(size_t)blk]++] = make_uint2(meta, (uint32_t)qv);
         }
-        pos = o[B];
     }
+    L.rb = rb;
+    L.nr = nr;
+    L.gs = gs;
+    L.seg_count = pos;
+    L.max_row = 0;
+    for (int32_t p = 0; p < n; ++p)
+        L.max_row = std::max<int64_t>(L.max_row, net->h_row_ptr[(size_t)p + 1] - net->h_row_ptr[(size_t)p]);
     int32_t rc = net->upload(&L.d_off, off.data(), off.size());
     if (rc) return rc;
     rc = net->upload(&L.d_seg, seg.data(), seg.size());
@@ -509,15 +879,79 @@ int32_t build_persistent_layout(nbx_net* net) {
     return NBX_OK;
 }
 
+static int32_t free_owned(nbx_net* net, void* p) {
+    if (!p) return NBX_OK;
+    auto it = std::find(net->allocs.begin(), net->allocs.end(), p);
+    if (it != net->allocs.end()) net->allocs.erase(it);
+    NBX_HIP_TRY(hipFree(p));
+    return NBX_OK;
+}
+
+// Both blocked layouts carry pre-quantized weights: drop them when the weights change.
 int32_t refresh_persistent_weights(nbx_net* net) {
     PersistentLayout& L = net->pl;
-    if (!L.built) return NBX_OK;
-    for (void* p : {(void*)L.d_off, (void*)L.d_seg, (void*)L.d_bar}) {
-        auto it = std::find(net->allocs.begin(), net->allocs.end(), p);
-        if (it != net->allocs.end()) net->allocs.erase(it);
-        NBX_HIP_TRY(hipFree(p));
+    if (L.built) {
+        for (void* p : {(void*)L.d_off, (void*)L.d_seg, (void*)L.d_bar}) NBX_PASS(free_owned(net, p));
+        L = PersistentLayout();
     }
-    L = PersistentLayout();
+    ReplaySlots& RS = net->rs;
+    if (RS.built) {
+        for (void* p : {(void*)RS.d_slots, (void*)RS.d_ovf}) NBX_PASS(free_owned(net, p));
+        RS = ReplaySlots();
+    }
+    return NBX_OK;
+}
+
+static int32_t build_replay_slots(nbx_net* net, int32_t m) {
+    ReplaySlots& RS = net->rs;
+    if (RS.built && RS.m == m) return NBX_OK;
+    if (RS.built) {
+        for (void* p : {(void*)RS.d_slots, (void*)RS.d_ovf}) NBX_PASS(free_owned(net, p));
+        RS = ReplaySlots();
+    }
+    const PersistentLayout& L = net->pl;
+    const int32_t n = net->n, Pr = m * L.P, Bq = (L.B + m - 1) / m;
+    const float scale = std::ldexp(1.0f, net->shift);
+    std::vector<uint2> slots((size_t)n * Bq * kSlotCap, make_uint2(kSlotEmpty, 0u));
+    std::vector<uint2> ovf;
+    std::vector<int32_t> cnt((size_t)Bq), fill((size_t)Bq);
+    for (int32_t p = 0; p < n; ++p) {
+        const int64_t rbeg = net->h_row_ptr[(size_t)p], rend = net->h_row_ptr[(size_t)p + 1];
+        std::fill(cnt.begin(), cnt.end(), 0);
+        std::fill(fill.begin(), fill.end(), 0);
+        for (int64_t j = rbeg; j < rend; ++j) cnt[(size_t)(syn_post(net->h_syn[(size_t)j]) / Pr)]++;
+        uint2* row = slots.data() + (size_t)p * Bq * kSlotCap;
+        for (int32_t qb = 0; qb < Bq; ++qb)
+            if (cnt[(size_t)qb] > kSlotCap) {  // 15 entries in the slot, the rest behind a link
+                row[(size_t)qb * kSlotCap + kSlotCap - 1] =
+                    make_uint2(kSlotLink | (uint32_t)(cnt[(size_t)qb] - (kSlotCap - 1)), (uint32_t)ovf.size());
+                ovf.resize(ovf.size() + (size_t)(cnt[(size_t)qb] - (kSlotCap - 1)));
+            }
+        for (int64_t j = rbeg; j < rend; ++j) {
+            const uint32_t e = net->h_syn[(size_t)j];
+            const int32_t post = syn_post(e), ty = syn_type(e), qb = post / Pr;
+            int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
+            int32_t ch = 0;
+            if (net->md.model == NBX_MODEL_IZH) ch = (ty == 1) ? 1 : 0;
+            else if (net->md.model != NBX_MODEL_IZH_CUR && ty == 1) qv = -qv;
+            const uint2 en = make_uint2((uint32_t)(ch * Pr + post - qb * Pr) | ((uint32_t)(syn_delay(e) - 1) << 16),
+                                        (uint32_t)qv);
+            const int32_t k = fill[(size_t)qb]++;
+            const bool linked = cnt[(size_t)qb] > kSlotCap;
+            if (!linked || k < kSlotCap - 1) row[(size_t)qb * kSlotCap + k] = en;
+            else ovf[(size_t)row[(size_t)qb * kSlotCap + kSlotCap - 1].y + (size_t)(k - (kSlotCap - 1))] = en;
+        }
+    }
+    if ((int64_t)ovf.size() >= ((int64_t)1 << 31)) {
+        set_error("replay slots: overflow list too long");
+        return NBX_ERR_UNSUPPORTED;
+    }
+    NBX_PASS(net->upload(&RS.d_slots, slots.data(), slots.size()));
+    NBX_PASS(net->upload(&RS.d_ovf, ovf.data(), ovf.size()));
+    RS.ovf_count = (int64_t)ovf.size();
+    RS.m = m;
+    RS.Bq = Bq;
+    RS.built = true;
     return NBX_OK;
 }
 
@@ -544,6 +978,9 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.seg = L.d_seg;
     X.B = L.B;
     X.P = L.P;
+    X.rb = L.rb;
+    X.gs = L.gs;
+    X.os = L.nr * L.gs;
     X.t0 = net->t;
     X.nsteps = nsteps;
     X.bar = reinterpret_cast<Barrier*>(L.d_bar);
@@ -561,21 +998,114 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     return NBX_OK;
 }
 
+static ReplayGeometry replay_geometry(const nbx_net* net) {
+    const PersistentLayout& L = net->pl;
+    ReplayGeometry r;
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device) != hipSuccess ||
+        max_lds <= 0)
+        max_lds = 64 * 1024;
+    const int64_t budget = (int64_t)max_lds - (int64_t)(kTpCap + 1) * 4 - 256;  // minus the static LDS
+    auto env = [](const char* name, int32_t dflt) {
+        const char* v = std::getenv(name);
+        return (v && *v) ? (int32_t)std::atoi(v) : dflt;
+    };
+    // launch-shape overrides (merge factor, window, time groups, staged spikes, slot path) for
+    // tuning and tests
+    const int32_t m_req = env("NBX_REPLAY_M", 0), w_req = env("NBX_REPLAY_W", 0), g_req = env("NBX_REPLAY_G", 0);
+    const int32_t c_req = env("NBX_REPLAY_C", 0), slots_req = env("NBX_REPLAY_SLOTS", -1);
+    const int32_t c_min = 2048, c_max = kStageU * 1024;
+    if (L.max_row >= (1 << 24)) return r;  // segment lengths share a word with the window step
+    if (net->n > (1 << 24)) return r;        // staged ids share a word with the window step
+    const double outdeg = (double)net->S / std::max<int32_t>(1, net->n);
+    for (int32_t m : {2, 1}) {
+        if (m_req && m != m_req) continue;
+        if (m > L.B) continue;
+        const double mean_seg = outdeg * m * L.P / std::max<int32_t>(1, net->n);
+        // a 16-entry slot suits mean segments of ~3..10 entries (longer ones overflow often,
+        // shorter ones waste most of the 128 bytes); otherwise the offset layout is used
+        const bool slots = slots_req >= 0 ? slots_req != 0 : (mean_seg >= 3.0 && mean_seg <= 10.5);
+        if (!slots && L.rb % m != 0) continue;
+        const int64_t words = (int64_t)net->nch * m * L.P;
+        if (slots && words > 65536) continue;  // a slot entry holds its LDS word in 16 bits
+        const int64_t stage_bytes = slots ? 8 : 16;  // per staged spike, two buffers
+        const int32_t steps_fit = (int32_t)((budget - c_min * stage_bytes) / (words * 4));
+        int32_t R = 0, W = 0;
+        if (slots) {  // power-of-two ring
+            R = 1;
+            while (2 * R <= steps_fit) R *= 2;
+            W = std::min(R - (net->D - 1), kMaxWin) / 4 * 4;
+        } else {
+            W = std::min(steps_fit - (net->D - 1), kMaxWin) / 4 * 4;
+        }
+        if (w_req) W = std::min(W, w_req);
+        if (W < 4) continue;
+        if (m == 2 && !m_req && W < 8) continue;  // a short window costs more than the larger segments gain
+        if (!slots) R = W + net->D - 1;
+        r.slots = slots;
+        r.m = m;
+        r.W = W;
+        r.R = R;
+        const int64_t ring = (int64_t)r.R * words * 4;
+        r.C = (int32_t)std::min<int64_t>(c_max, (budget - ring) / stage_bytes / 64 * 64);
+        if (c_req >= 64) r.C = std::min(r.C, c_req / 64 * 64);
+        r.lds = ring + (int64_t)r.C * stage_bytes;
+        r.Bq = (L.B + m - 1) / m;
+        r.G = g_req > 0 ? g_req : m;
+        r.lanes = mean_seg >= 12 ? 16 : (mean_seg >= 6 ? 8 : 4);
+        r.ok = true;
+        return r;
+    }
+    return r;
+}
+
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out) {
     const StepArgs A = net->args();
     // the blocked layout carries pre-quantized weights, so it is not used while STDP changes them
     if (!net->stdp && build_persistent_layout(net) == NBX_OK) {
-        PersistentLayout& L = net->pl;
-        const size_t lds = (size_t)(kReplayWin + net->D - 1) * net->nch * L.P * sizeof(int32_t);
-        int max_lds = 0;
-        NBX_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
-        if (lds + 1024 <= (size_t)max_lds) {
-            if (lds > 64 * 1024)
-                NBX_HIP_TRY(hipFuncSetAttribute((const void*)nbx_replay_blocked,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            nbx_replay_blocked<<<L.B, 1024, lds, net->stream>>>(A, L.d_off, L.d_seg, L.B, L.P, d_ids, d_tick_ptr,
-                                                                 nsteps, d_out);
-            NBX_HIP_TRY(hipGetLastError());
+        const PersistentLayout& L = net->pl;
+        const ReplayGeometry r = replay_geometry(net);
+        if (r.ok) {
+            ReplayArgs X;
+            X.off = L.d_off;
+            X.seg = L.d_seg;
+            X.B = L.B;
+            X.P = L.P;
+            X.rb = L.rb;
+            X.gs = L.gs;
+            X.os = L.nr * L.gs;
+            X.m = r.m;
+            X.Bq = r.Bq;
+            X.G = r.G;
+            X.W = r.W;
+            X.R = r.R;
+            X.C = r.C;
+            X.n = net->n;
+            X.nch = net->nch;
+            X.D = net->D;
+            X.K = nsteps;
+            X.ids = d_ids;
+            X.tick_ptr = d_tick_ptr;
+            X.out = d_out;
+            X.events = A.events;
+            X.slots = nullptr;
+            X.ovf = nullptr;
+            X.row_ptr = net->d_row_ptr;
+            const void* fn = r.lanes == 16 ? (const void*)nbx_replay_ring<16>
+                             : r.lanes == 8 ? (const void*)nbx_replay_ring<8>
+                                            : (const void*)nbx_replay_ring<4>;
+            if (r.slots) {
+                NBX_PASS(build_replay_slots(net, r.m));
+                X.slots = reinterpret_cast<const uint4*>(net->rs.d_slots);
+                X.ovf = net->rs.d_ovf;
+                X.Bq = net->rs.Bq;
+                fn = (const void*)nbx_replay_slots;
+            }
+            if (r.lds > 64 * 1024)
+                NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)r.lds));
+            void* kargs[] = {&X};
+            NBX_HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)(r.Bq * r.G)), dim3(1024), kargs, (size_t)r.lds,
+                                        net->stream));
             return NBX_OK;
         }
     }

@@ -170,25 +170,118 @@ def test_per_phase_delay_known_answer(gpu):
         net.begin_step(7)  # out of order: the reference's BeginStep contract
 
 
-def test_replay_delivery_brute_force(gpu):
+def replay_expect(wl, fixed_shift, spikes):
+    """Brute force: every synapse of every injected spike adds its fixed-point weight at step
+    k + delay - 1, on channel 1 for inhibitory synapses of conductance Izhikevich, negated for
+    inhibitory synapses of the LIF models."""
+    s = wl.syn
+    K = len(spikes)
+    model = wl.spec.model
+    q = np.rint(s.weight * np.float32(2.0 ** fixed_shift)).astype(np.int64)
+    if model in (L.LIF_VJ, L.LIF_EXP):
+        q = np.where(s.type == 1, -q, q)
+    ch = s.type.astype(np.int64) if model == L.IZH else np.zeros(s.type.shape, np.int64)
+    expect = np.zeros((K, 2 if model == L.IZH else 1, wl.spec.n), np.int64)
+    events = 0
+    for k, ids in enumerate(spikes):
+        if len(ids) == 0:
+            continue
+        sl = np.concatenate([np.arange(s.row_ptr[p], s.row_ptr[p + 1]) for p in ids])
+        events += sl.size
+        ko = k + s.delay[sl].astype(np.int64) - 1
+        ok = ko < K
+        np.add.at(expect, (ko[ok], ch[sl][ok], s.post[sl][ok]), q[sl][ok])
+    return expect, events
+
+
+# Replay launch shapes forced through the library's launch-time knobs: (merge factor, window,
+# time groups, staged spikes, slot layout); None = the geometry the library picks. "c64" makes
+# most windows hold more spikes than one staging buffer (the chunked path).
+REPLAY_KNOBS = ("NBX_REPLAY_M", "NBX_REPLAY_W", "NBX_REPLAY_G", "NBX_REPLAY_C", "NBX_REPLAY_SLOTS")
+REPLAY_SHAPES = {
+    "auto": None,
+    "ring-m2w8g2": ("2", "8", "2", "", "0"),
+    "ring-m1w8g3": ("1", "8", "3", "", "0"),
+    "ring-m2w16g5": ("2", "16", "5", "", "0"),
+    "ring-m1w12g2c64": ("1", "12", "2", "64", "0"),
+    "slots-m2w8g2": ("2", "8", "2", "", "1"),
+    "slots-m1w12g3c64": ("1", "12", "3", "64", "1"),
+}
+
+
+def set_replay_shape(monkeypatch, shape):
+    if shape:
+        for k, v in zip(REPLAY_KNOBS, shape):
+            monkeypatch.setenv(k, v)
+
+
+def random_spikes(rng, n, K, lo, hi):
+    return [np.sort(rng.choice(n, size=int(rng.integers(lo, hi)), replace=False)).astype(np.int32) for _ in range(K)]
+
+
+@pytest.mark.parametrize("shape", list(REPLAY_SHAPES.values()), ids=list(REPLAY_SHAPES))
+def test_replay_delivery_brute_force(gpu, shape, monkeypatch):
+    set_replay_shape(monkeypatch, shape)
     wl = small_sparse(L.LIF_VJ, L.EULER, n=2500, k_in=80, seed=9)
     net = wl.make_net()
     rng = np.random.default_rng(1)
-    K = 150  # spans several internal replay windows (carry across windows)
-    spikes = [np.sort(rng.choice(wl.spec.n, size=int(rng.integers(0, 60)), replace=False)).astype(np.int32)
-              for _ in range(K)]
+    K = 150  # spans several replay windows and, with time groups, their halos
+    spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
+    spikes[7] = spikes[7][:0]  # an empty step
+    e0 = net.events()
     out = net.replay(spikes)
-    s = wl.syn
-    q = np.rint(s.weight * np.float32(2.0 ** net.fixed_shift)).astype(np.int64)
-    q = np.where(s.type == 1, -q, q)
-    expect = np.zeros((K, 1, wl.spec.n), np.int64)
-    for k, ids in enumerate(spikes):
-        for pre in ids:
-            sl = slice(s.row_ptr[pre], s.row_ptr[pre + 1])
-            ko = k + s.delay[sl].astype(np.int64) - 1
-            ok = ko < K
-            np.add.at(expect[:, 0, :], (ko[ok], s.post[sl][ok]), q[sl][ok])
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
     assert_bits_equal(out.astype(np.int64), expect, "replay arrivals")
+    assert net.events() - e0 == events
+
+
+def test_replay_slot_overflow(gpu, monkeypatch):
+    # 640 neurons with in-degree 400: a pair of 32-neuron blocks receives ~40 synapses per
+    # presynaptic neuron, so most slots continue in the overflow list
+    set_replay_shape(monkeypatch, ("2", "8", "2", "", "1"))
+    wl = small_sparse(L.LIF_VJ, L.EULER, n=640, k_in=400, seed=4)
+    net = wl.make_net()
+    rng = np.random.default_rng(2)
+    K = 60
+    spikes = random_spikes(rng, wl.spec.n, K, 0, 40)
+    e0 = net.events()
+    out = net.replay(spikes)
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
+    assert_bits_equal(out.astype(np.int64), expect, "replay arrivals with slot overflow")
+    assert net.events() - e0 == events
+
+
+@pytest.mark.parametrize("slots", ["0", "1"], ids=["ring", "slots"])
+def test_replay_two_channels(gpu, slots, monkeypatch):
+    # conductance Izhikevich: inhibitory arrivals go to their own channel
+    set_replay_shape(monkeypatch, ("2", "8", "2", "", slots))
+    wl = small_sparse(L.IZH, L.EULER, n=2500, k_in=80, seed=6)
+    net = wl.make_net()
+    rng = np.random.default_rng(3)
+    K = 70
+    spikes = random_spikes(rng, wl.spec.n, K, 0, 50)
+    e0 = net.events()
+    out = net.replay(spikes)
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
+    assert_bits_equal(out.astype(np.int64), expect, "two-channel replay arrivals")
+    assert net.events() - e0 == events
+
+
+@pytest.mark.parametrize("slots", ["auto", "0"], ids=["auto", "ring"])
+def test_replay_full_size(gpu, slots, monkeypatch):
+    # configuration-size network (1e5 neurons, 1e8 synapses), 96 steps of 100-220 spikes
+    if slots != "auto":
+        monkeypatch.setenv("NBX_REPLAY_SLOTS", slots)
+    wl = W.sparse_lif()
+    net = wl.make_net()
+    rng = np.random.default_rng(5)
+    K = 96
+    spikes = random_spikes(rng, wl.spec.n, K, 100, 220)
+    e0 = net.events()
+    out = net.replay(spikes)
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
+    assert_bits_equal(out.astype(np.int64), expect, "full-size replay arrivals")
+    assert net.events() - e0 == events
 
 
 def test_full_size_sparse_properties(gpu):
