@@ -53,8 +53,9 @@ __device__ inline int32_t block_of(int32_t w, int32_t B) {
 // Memory ordering: the only data other workgroups read within a launch are the firing
 // ids and counts of the previous step. They are written with device-scope atomic
 // stores/adds (coherence-point writes, `sc1`) and read with device-scope atomic loads,
-// and each arrival is preceded by s_waitcnt 0, so every store of a workgroup has been
-// performed before its arrival is counted. No L2 write-back or invalidate is needed.
+// and every wave executes s_waitcnt 0 before the workgroup barrier that precedes the
+// arrival, so every store of the workgroup has been performed before its arrival is
+// counted. No L2 write-back or invalidate is needed.
 constexpr int kLine = 32;  // int32 per 128-byte line
 struct Barrier {
     int32_t group[kXcds * kLine];
@@ -80,15 +81,18 @@ __device__ inline void st_dev(int32_t* p, int32_t v) {
 }
 
 __device__ inline bool grid_barrier(Barrier* bar, int32_t k, int32_t nblocks, int32_t* err, int32_t* s_abort) {
+    // every wave waits for its own published stores (__syncthreads waits for LDS only), so all
+    // of the workgroup's stores are performed before its arrival is counted
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_s_waitcnt(0);  // this workgroup's published stores are performed
         const int32_t g = blockIdx.x % kXcds;
-        const int32_t per = nblocks / kXcds;
+        const int32_t per = nblocks / kXcds + (g < nblocks % kXcds ? 1 : 0);  // workgroups in group g
+        const int32_t groups = min(nblocks, kXcds);                            // non-empty groups
         const int32_t old = __hip_atomic_fetch_add(&bar->group[g * kLine], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == per * (k + 1) - 1) {
             const int32_t o2 = __hip_atomic_fetch_add(&bar->top[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (o2 == kXcds * (k + 1) - 1)
+            if (o2 == groups * (k + 1) - 1)
                 for (int q = 0; q < kXcds; ++q) st_dev(&bar->gen[q * kLine], k + 1);
         }
         uint32_t spins = 0;
@@ -757,6 +761,7 @@ struct Geometry {
 struct ReplayGeometry {
     bool ok = false;
     bool slots = false;
+    int32_t P = 0, B = 0;  // layout block size and count
     int32_t m = 1, Bq = 0, G = 1, W = 0, R = 0, C = 0, lanes = 4;
     int64_t lds = 0;
 };
@@ -909,7 +914,7 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
         for (void* p : {(void*)RS.d_slots, (void*)RS.d_ovf}) NBX_PASS(free_owned(net, p));
         RS = ReplaySlots();
     }
-    const PersistentLayout& L = net->pl;
+    const Geometry L = geometry(net);
     const int32_t n = net->n, Pr = m * L.P, Bq = (L.B + m - 1) / m;
     const float scale = std::ldexp(1.0f, net->shift);
     std::vector<uint2> slots((size_t)n * Bq * kSlotCap, make_uint2(kSlotEmpty, 0u));
@@ -953,6 +958,13 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
     RS.Bq = Bq;
     RS.built = true;
     return NBX_OK;
+}
+
+static int64_t longest_row(const nbx_net* net) {
+    int64_t r = 0;
+    for (int32_t p = 0; p < net->n; ++p)
+        r = std::max<int64_t>(r, net->h_row_ptr[(size_t)p + 1] - net->h_row_ptr[(size_t)p]);
+    return r;
 }
 
 int32_t run_persistent(nbx_net* net, int32_t nsteps) {
@@ -999,7 +1011,18 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
 }
 
 static ReplayGeometry replay_geometry(const nbx_net* net) {
-    const PersistentLayout& L = net->pl;
+    struct {
+        int32_t P, B, rb;
+        int64_t max_row;
+    } L;
+    {
+        const Geometry g = geometry(net);
+        const int32_t per = g.B / kXcds;
+        L.P = g.P;
+        L.B = g.B;
+        L.rb = (per % 2 == 0) ? per : 2 * per;  // the offset layout's region size
+        L.max_row = longest_row(net);
+    }
     ReplayGeometry r;
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device) != hipSuccess ||
@@ -1043,6 +1066,8 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
         if (m == 2 && !m_req && W < 8) continue;  // a short window costs more than the larger segments gain
         if (!slots) R = W + net->D - 1;
         r.slots = slots;
+        r.P = L.P;
+        r.B = L.B;
         r.m = m;
         r.W = W;
         r.R = R;
@@ -1061,16 +1086,16 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
 
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out) {
     const StepArgs A = net->args();
-    // the blocked layout carries pre-quantized weights, so it is not used while STDP changes them
-    if (!net->stdp && build_persistent_layout(net) == NBX_OK) {
+    // the blocked layouts carry pre-quantized weights, so they are not used while STDP changes them
+    const ReplayGeometry r = net->stdp ? ReplayGeometry() : replay_geometry(net);
+    if (r.ok && (r.slots || build_persistent_layout(net) == NBX_OK)) {
         const PersistentLayout& L = net->pl;
-        const ReplayGeometry r = replay_geometry(net);
-        if (r.ok) {
+        {
             ReplayArgs X;
-            X.off = L.d_off;
+            X.off = L.d_off;  // null on the slot path (the offset layout is not built for it)
             X.seg = L.d_seg;
-            X.B = L.B;
-            X.P = L.P;
+            X.B = r.B;
+            X.P = r.P;
             X.rb = L.rb;
             X.gs = L.gs;
             X.os = L.nr * L.gs;

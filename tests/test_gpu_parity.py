@@ -109,6 +109,16 @@ def test_engines_interoperate(gpu):
     run = OracleRun(wl, fixed_shift=net.fixed_shift)
     counts_o, raster_o = run.run(330)
     compare(net, run, wl, counts_o, raster_o)
+    assert_history(net, raster_o, wl.spec.n)
+
+
+def assert_history(net, raster_o, n):
+    """Every step the firing-history ring still holds lists exactly the raster's spikes."""
+    slots = net.ring().shape[0]
+    T = raster_o.shape[0]
+    bits = np.unpackbits(raster_o.view(np.uint8), axis=1, bitorder="little")[:, :n]
+    for t in range(T - slots + 2, T):
+        assert_bits_equal(np.sort(net.history(t)), np.nonzero(bits[t])[0].astype(np.int32), f"history of step {t}")
 
 
 def test_dense_izhikevich_current_config(gpu):
