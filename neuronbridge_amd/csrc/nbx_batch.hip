@@ -1,15 +1,19 @@
 // Batched multi-instance engine: hundreds of independent LIF networks of uneven
 // size and parameters in one launch.
 //
-// Workgroup w owns instance order[w] for the whole launch. Its neurons' state
-// lives in VGPRs (NPT neurons per thread), its spike list in LDS. Per step the
-// workgroup gathers the BF16 weight rows of the neurons that fired in the previous
-// step (dense all-to-all, delay 1), accumulates them as int32 fixed point (the same
-// integers the FP32 per-instance path produces from the same BF16-rounded weights,
-// so rasters are identical), integrates, and compacts its new spikes with a
-// wavefront ballot. Instances never communicate, so a step needs only workgroup
-// barriers, never a grid barrier. Chunks of steps are launched from a captured
-// hipGraph; the workgroup size is picked by a timing autotuner.
+// Workgroup w owns instance order[w] for the whole launch. Each thread owns NPT (8 or 16)
+// consecutive neurons, whose state lives in VGPRs; the spike list lives in LDS. Per step
+// the workgroup gathers the BF16 weight rows of the neurons that fired in the previous
+// step (dense all-to-all, delay 1): each thread reads the 16-byte pieces of a fired row
+// that cover its neurons, with kRows rows in flight, and accumulates them as int32 fixed
+// point (the same integers the FP32 per-instance path produces from the same BF16-rounded
+// weights, so rasters are identical). It then integrates and appends its new spikes to the
+// next list (one LDS atomic per thread that fired). Instances never communicate, so a step
+// needs only workgroup barriers, never a grid barrier. Chunks of steps are launched from a
+// captured hipGraph; the workgroup size is picked by a timing autotuner.
+//
+// Weight rows are padded to a multiple of 16 elements (32 bytes), so every thread's
+// 16-byte loads are aligned and stay inside its row.
 //
 // Delivery is a column gather, not a GEMM: each instance has its own W and a
 // binary spike vector with a handful of ones per step, so reading only the
@@ -33,10 +37,13 @@ namespace {
 constexpr int kMaxPoisTab = 32;
 constexpr int kMaxBatchN = 16384;
 
+constexpr int kRowPad = 16;  // weight row stride: a multiple of 16 elements
+
 struct InstDev {
     int32_t n;
+    int32_t ld;          // weight row stride (elements), n rounded up to kRowPad
     int32_t words;       // raster words per step
-    int64_t w_off;       // element offset of W[pre][post] (bf16)
+    int64_t w_off;       // element offset of W[pre][post] (bf16), row stride ld
     int64_t s_off;       // offset into the state arrays
     int64_t r_off;       // offset of this instance's raster rows (words)
     float tau;
@@ -60,132 +67,130 @@ struct BatchArgs {
     int32_t rec_t0, rec_n;
     unsigned long long* events;
     int32_t t0, nsteps;
+    int32_t max_n;        // spike-list stride in LDS
 };
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 template <int NPT>
 __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
-    extern __shared__ uint16_t s_ids[];  // two buffers of n ids
+    static_assert(NPT % 8 == 0, "a thread owns whole 16-byte pieces of a row");
+    extern __shared__ uint16_t s_ids[];  // [2][max_n] spike lists, then [blockDim] raster bits
     __shared__ int32_t s_cnt[2];
-    __shared__ int32_t s_wcnt[16];
-    const InstDev& I = B.inst[B.order[blockIdx.x]];
+    const int32_t inst_id = B.order[blockIdx.x];
+    const InstDev& I = B.inst[inst_id];
     const int32_t n = I.n;
     const int32_t T = blockDim.x, tid = threadIdx.x;
-    const int32_t lane = tid & 63, wave = tid >> 6, nwaves = T >> 6;
+    const int32_t c0 = tid * NPT;  // first neuron of this thread
+    const bool own = c0 < n;
+    uint16_t* s_rb = s_ids + 2 * B.max_n;
     ModelParams P = B.P;
     P.tau = I.tau;
     float V[NPT];
     int32_t last[NPT];
 #pragma unroll
-    for (int u = 0; u < NPT; ++u) {
-        const int32_t i = tid + u * T;
-        V[u] = 0.0f;
-        last[u] = 0;
+    for (int j = 0; j < NPT; ++j) {
+        const int32_t i = c0 + j;
+        V[j] = 0.0f;
+        last[j] = 0;
         if (i < n) {
-            V[u] = B.v[I.s_off + i];
-            last[u] = B.last[I.s_off + i];
+            V[j] = B.v[I.s_off + i];
+            last[j] = B.last[I.s_off + i];
         }
     }
-    const int32_t inst_id = B.order[blockIdx.x];
-    const int32_t c0 = B.spk_cnt[inst_id];
-    for (int32_t j = tid; j < c0; j += T) s_ids[j] = B.spk[I.s_off + j];
-    if (tid == 0) s_cnt[0] = c0;
+    const int32_t cnt0 = B.spk_cnt[inst_id];
+    for (int32_t j = tid; j < cnt0; j += T) s_ids[j] = B.spk[I.s_off + j];
+    if (tid == 0) s_cnt[0] = cnt0;
     __syncthreads();
     unsigned long long ev = 0;
     int cur = 0;
+    const uint16_t* wbase = B.w + I.w_off + c0;
     for (int32_t k = 0; k < B.nsteps; ++k) {
         const int32_t t = B.t0 + k;
         const int32_t cnt = s_cnt[cur];
-        const uint16_t* ids = s_ids + cur * n;
-        // gather the fired rows: thread owns columns tid + u*T
+        const uint16_t* ids = s_ids + cur * B.max_n;
+        // gather the fired rows over this thread's neurons, kRows rows in flight
         int32_t acc[NPT];
 #pragma unroll
-        for (int u = 0; u < NPT; ++u) acc[u] = 0;
-        // kRows fired rows in flight per pass, so their loads overlap instead of one row per trip
-        constexpr int kRows = 4;
+        for (int j = 0; j < NPT; ++j) acc[j] = 0;
+        constexpr int kRows = (NPT == 8) ? 16 : 4;
         for (int32_t s0 = 0; s0 < cnt; s0 += kRows) {
-            uint16_t wv[kRows][NPT];
+            uint4 wv[kRows][NPT / 8];
 #pragma unroll
             for (int r = 0; r < kRows; ++r) {
-                const bool ok = s0 + r < cnt;
-                const uint16_t* row = B.w + I.w_off + (int64_t)(ok ? ids[s0 + r] : 0) * n;
+                const bool ok = own && s0 + r < cnt;
+                const uint4* row = reinterpret_cast<const uint4*>(wbase + (int64_t)(ok ? ids[s0 + r] : 0) * I.ld);
 #pragma unroll
-                for (int u = 0; u < NPT; ++u) {
-                    const int32_t i = tid + u * T;
-                    wv[r][u] = (ok && i < n) ? row[i] : (uint16_t)0;
-                }
+                for (int h = 0; h < NPT / 8; ++h) wv[r][h] = ok ? row[h] : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
             for (int r = 0; r < kRows; ++r)
 #pragma unroll
-                for (int u = 0; u < NPT; ++u) acc[u] += quantize(bf16_to_f32(wv[r][u]), I.scale);
+                for (int h = 0; h < NPT / 8; ++h) {
+                    const uint32_t q4[4] = {wv[r][h].x, wv[r][h].y, wv[r][h].z, wv[r][h].w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        acc[h * 8 + 2 * e] += quantize(bf16_to_f32((uint16_t)(q4[e] & 0xFFFFu)), I.scale);
+                        acc[h * 8 + 2 * e + 1] += quantize(bf16_to_f32((uint16_t)(q4[e] >> 16)), I.scale);
+                    }
+                }
         }
         if (tid == 0) s_cnt[cur ^ 1] = 0;
         __syncthreads();
-        const bool rec = B.raster && t >= B.rec_t0 && t < B.rec_t0 + B.rec_n;
-        uint16_t* nids = s_ids + (cur ^ 1) * n;
+        uint32_t fired_bits = 0;
 #pragma unroll
-        for (int u = 0; u < NPT; ++u) {
-            const int32_t i = tid + u * T;
-            bool fired = false;
+        for (int j = 0; j < NPT; ++j) {
+            const int32_t i = c0 + j;
             if (i < n) {
-                int32_t a0 = acc[u] + poisson_count(I.seed_lo, I.seed_hi, i, t, I.tab, I.kmax) * I.q_bg;
-                if (last[u] > P.t_ref) V[u] += (float)a0 * I.inv_scale;
+                const int32_t a0 = acc[j] + poisson_count(I.seed_lo, I.seed_hi, i, t, I.tab, I.kmax) * I.q_bg;
+                if (last[j] > P.t_ref) V[j] += (float)a0 * I.inv_scale;
                 float g = 0.0f;
-                fired = lif_step<false>(P, V[u], g, 0.0f, last[u]);
+                if (lif_step<false>(P, V[j], g, 0.0f, last[j])) fired_bits |= 1u << j;
             }
-            const uint64_t mask = __ballot(fired);
-            if (lane == 0) s_wcnt[wave] = __popcll(mask);
-            if (rec && lane == 0) {
-                const int32_t i0 = u * T + wave * 64;
-                uint32_t* row = B.raster + I.r_off + (int64_t)(t - B.rec_t0) * I.words;
-                if (i0 < n) row[i0 >> 5] = (uint32_t)mask;
-                if (i0 + 32 < n) row[(i0 >> 5) + 1] = (uint32_t)(mask >> 32);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int32_t tot = s_cnt[cur ^ 1];
-                for (int w = 0; w < nwaves; ++w) {
-                    const int32_t c = s_wcnt[w];
-                    s_wcnt[w] = tot;
-                    tot += c;
-                }
-                s_cnt[cur ^ 1] = tot;
-            }
-            __syncthreads();
-            if (fired) nids[s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)i;
         }
+        uint16_t* nids = s_ids + (cur ^ 1) * B.max_n;
+        if (fired_bits) {
+            int32_t pos = atomicAdd(&s_cnt[cur ^ 1], __popc(fired_bits));
+            for (uint32_t m = fired_bits; m; m &= m - 1u) nids[pos++] = (uint16_t)(c0 + __builtin_ctz(m));
+        }
+        const bool rec = B.raster && t >= B.rec_t0 && t < B.rec_t0 + B.rec_n;
+        if (rec) s_rb[tid] = (uint16_t)fired_bits;
         __syncthreads();
+        if (rec) {
+            // raster word w holds the bits of threads [w * 32/NPT, (w + 1) * 32/NPT)
+            uint32_t* row = B.raster + I.r_off + (int64_t)(t - B.rec_t0) * I.words;
+            constexpr int kPer = 32 / NPT;
+            for (int32_t w = tid; w < I.words; w += T) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int q = 0; q < kPer; ++q) {
+                    const int32_t th = w * kPer + q;
+                    if (th * NPT < n) word |= (uint32_t)s_rb[th] << (q * NPT);
+                }
+                row[w] = word;
+            }
+        }
         ev += (unsigned long long)s_cnt[cur ^ 1] * (unsigned long long)n;
-        cur ^= 1;
+        cur ^= 1;  // s_rb and the old list are rewritten only after the next step's first barrier
     }
     // persist state and the last step's spikes for the next launch
 #pragma unroll
-    for (int u = 0; u < NPT; ++u) {
-        const int32_t i = tid + u * T;
+    for (int j = 0; j < NPT; ++j) {
+        const int32_t i = c0 + j;
         if (i < n) {
-            B.v[I.s_off + i] = V[u];
-            B.last[I.s_off + i] = last[u];
+            B.v[I.s_off + i] = V[j];
+            B.last[I.s_off + i] = last[j];
         }
     }
     const int32_t cl = s_cnt[cur];
-    for (int32_t j = tid; j < cl; j += T) B.spk[I.s_off + j] = s_ids[cur * n + j];
+    for (int32_t j = tid; j < cl; j += T) B.spk[I.s_off + j] = s_ids[cur * B.max_n + j];
     if (tid == 0) {
         B.spk_cnt[inst_id] = cl;
         if (ev) atomicAdd(B.events, ev);
     }
 }
 
-void* batch_kernel(int npt) {
-    switch (npt) {
-        case 1: return (void*)nbx_batch_kernel<1>;
-        case 2: return (void*)nbx_batch_kernel<2>;
-        case 4: return (void*)nbx_batch_kernel<4>;
-        case 8: return (void*)nbx_batch_kernel<8>;
-        default: return (void*)nbx_batch_kernel<16>;
-    }
-}
+void* batch_kernel(int npt) { return npt == 8 ? (void*)nbx_batch_kernel<8> : (void*)nbx_batch_kernel<16>; }
 
 }  // namespace
 
@@ -230,11 +235,10 @@ struct nbx_batch {
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
+    // neurons per thread: 8 or 16 (whole 16-byte row pieces); 32 means the size does not fit
     int npt_for(int32_t thr) const {
-        int npt = (max_n + thr - 1) / thr;
-        int p = 1;
-        while (p < npt) p <<= 1;
-        return p;
+        const int need = (max_n + thr - 1) / thr;
+        return need <= 8 ? 8 : (need <= 16 ? 16 : 32);
     }
     hipError_t launch(int32_t t0, int32_t nsteps, int32_t thr) {
         BatchArgs A{};
@@ -252,8 +256,9 @@ struct nbx_batch {
         A.events = d_events;
         A.t0 = t0;
         A.nsteps = nsteps;
+        A.max_n = max_n;
         void* fn = batch_kernel(npt_for(thr));
-        const size_t lds = (size_t)2 * max_n * sizeof(uint16_t);
+        const size_t lds = ((size_t)2 * max_n + (size_t)thr) * sizeof(uint16_t);
         void* args[] = {&A};
         return hipLaunchKernel(fn, dim3(count), dim3(thr), args, lds, stream);
     }
@@ -313,6 +318,7 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
         }
         InstDev I{};
         I.n = s.n;
+        I.ld = (s.n + kRowPad - 1) / kRowPad * kRowPad;
         I.words = (s.n + 31) / 32;
         I.w_off = b->total_w;
         I.s_off = b->total_n;
@@ -325,7 +331,7 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
         I.seed_hi = s.seed_hi;
         I.kmax = s.pois_lambda > 0.0 ? poisson_tab(s.pois_lambda, I.tab, kMaxPoisTab) : 0;
         b->h_inst.push_back(I);
-        b->total_w += (int64_t)s.n * s.n;
+        b->total_w += (int64_t)s.n * I.ld;
         b->total_n += s.n;
         b->total_words += I.words;
         b->max_n = std::max(b->max_n, s.n);
@@ -334,12 +340,14 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
     NBX_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     int32_t rc = b->alloc(&b->d_w, (size_t)b->total_w);
     if (rc) return rc;
+    NBX_HIP_TRY(hipMemset(b->d_w, 0, sizeof(uint16_t) * (size_t)b->total_w));  // row padding reads as 0
     std::vector<float> v((size_t)b->total_n);
     for (int32_t k = 0; k < d->count; ++k) {
         const nbx_batch_instance& s = d->inst[k];
         const InstDev& I = b->h_inst[(size_t)k];
-        NBX_HIP_TRY(hipMemcpy(b->d_w + I.w_off, s.w_bf16, sizeof(uint16_t) * (size_t)s.n * s.n,
-                              hipMemcpyHostToDevice));
+        NBX_HIP_TRY(hipMemcpy2D(b->d_w + I.w_off, sizeof(uint16_t) * (size_t)I.ld, s.w_bf16,
+                                sizeof(uint16_t) * (size_t)s.n, sizeof(uint16_t) * (size_t)s.n, (size_t)s.n,
+                                hipMemcpyHostToDevice));
         for (int32_t i = 0; i < s.n; ++i) v[(size_t)(I.s_off + i)] = s.v0 ? s.v0[i] : d->v_reset;
     }
     std::vector<int32_t> last((size_t)b->total_n, 10000);
@@ -507,9 +515,9 @@ int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps) {
     }
     if (nsteps == 0) return NBX_OK;
     NBX_HIP_TRY(hipSetDevice(b->device));
-    if (b->threads == 0) {
+    if (b->threads == 0) {  // untuned: the smallest workgroup that gives each thread at most 8 neurons
         int32_t thr = 1024;
-        while (thr > 256 && b->npt_for(thr / 2) <= 16 && b->max_n <= thr / 2 * 4) thr /= 2;
+        while (thr > 256 && b->max_n <= thr / 2 * 8) thr /= 2;
         b->threads = thr;
     }
     // step numbers are baked into the launches, so the graph is specific to (t, nsteps)
