@@ -93,8 +93,10 @@ def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
     ms = e0.elapsed_ms(e1) / reps
     alg = events * EVENT_BYTES + K * n * 4
     ach = alg / (ms * 1e-3) / 1e9
-    return {"kernel": "nbx_replay_ring", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("nbx_replay_ring", K),
+    kernel = {L.REPLAY_SLOTS: "nbx_replay_slots", L.REPLAY_OFFSETS: "nbx_replay_ring"}.get(net.replay_path(),
+                                                                                           "nbx_replay_global")
+    return {"kernel": kernel, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, K),
             "algorithmic_bytes": alg,
             "steps_per_launch": K, "events_per_launch": events, "ms_per_launch": round(ms, 4),
             "syn_events_per_s": events / (ms * 1e-3)}
@@ -124,7 +126,7 @@ def batched_bench(steps: int, sample: int) -> dict:
     ev_b = b.events()
     total_n = sum(i.n for i in inst)
     order = np.argsort([i.n for i in inst])
-    pick = order[np.linspace(0, len(inst) - 1, sample).astype(int)]
+    pick = order[np.linspace(0, len(inst) - 1, sample).astype(int)] if sample > 0 else []
     plain_ms, plain_n, identical = 0.0, 0, True
     warm = 64  # the first 64 steps capture and instantiate the step graph; they are not timed
     for k in pick:
@@ -143,13 +145,17 @@ def batched_bench(steps: int, sample: int) -> dict:
         identical &= bool(np.array_equal(net.raster(), b.raster(int(k))))
         net.close()
     batched_rate = total_n * steps / (ms_b * 1e-3)
-    plain_rate = plain_n * (steps - warm) / (plain_ms * 1e-3)
-    return {"workload": f"{len(inst)} LIF instances, N~U{{512..8192}}, dense BF16 W, RK4 dt 0.1 ms, {steps} steps",
-            "value": batched_rate, "unit": "instance-neuron-steps/s", "ms": round(ms_b, 2),
-            "syn_events_per_s": ev_b / (ms_b * 1e-3), "workgroup_threads": thr,
-            "plain_per_instance": {"value": plain_rate, "unit": "instance-neuron-steps/s",
-                                   "sample_instances": [int(inst[int(k)].n) for k in pick]},
-            "speedup_vs_plain": batched_rate / plain_rate, "rasters_identical_on_sample": identical}
+    out = {"workload": f"{len(inst)} LIF instances, N~U{{512..8192}}, dense BF16 W, RK4 dt 0.1 ms, {steps} steps",
+           "value": batched_rate, "unit": "instance-neuron-steps/s", "ms": round(ms_b, 2),
+           "syn_events_per_s": ev_b / (ms_b * 1e-3), "workgroup_threads": thr,
+           # BF16 weight-row bytes the gather reads (2 B per synaptic event) per second
+           "weight_row_gbs": round(ev_b * 2 / (ms_b * 1e-3) / 1e9, 1)}
+    if plain_n:
+        plain_rate = plain_n * (steps - warm) / (plain_ms * 1e-3)
+        out.update({"plain_per_instance": {"value": plain_rate, "unit": "instance-neuron-steps/s",
+                                           "sample_instances": [int(inst[int(k)].n) for k in pick]},
+                    "speedup_vs_plain": batched_rate / plain_rate, "rasters_identical_on_sample": identical})
+    return out
 
 
 def adjoint_bench(reps: int) -> dict:

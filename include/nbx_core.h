@@ -194,6 +194,12 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
  * ring, given only these spikes. Host arrays. */
 int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t nsteps,
                            int32_t* out);
+/* Which replay kernel nbx_replay_deliver uses for this network (launch-shape overrides
+ * in the environment included): NBX_REPLAY_GLOBAL (global atomics; STDP networks and
+ * layouts that do not fit), NBX_REPLAY_OFFSETS (post-blocked offset layout, LDS ring) or
+ * NBX_REPLAY_SLOTS (fixed 128-byte slot per presynaptic neuron and post block). */
+enum { NBX_REPLAY_GLOBAL = 0, NBX_REPLAY_OFFSETS = 1, NBX_REPLAY_SLOTS = 2 };
+int32_t nbx_replay_path(nbx_net* net, int32_t* path);
 /* Device-resident variant for timing: all pointers are device pointers and the
  * call only enqueues the launch on the network's stream. */
 int32_t nbx_replay_deliver_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps,

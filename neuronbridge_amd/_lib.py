@@ -20,6 +20,7 @@ NBX_ERR_STATE = -4
 LIF_VJ, LIF_EXP, IZH, IZH_CUR = 0, 1, 2, 3
 EULER, RK4 = 0, 1
 ENGINE_KERNELS, ENGINE_GRAPH, ENGINE_PERSISTENT, ENGINE_AUTO = 0, 1, 2, 3
+REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS = 0, 1, 2
 RECORD_COUNTS, RECORD_RASTER = 1, 2
 
 _f32p = C.POINTER(C.c_float)
@@ -129,6 +130,7 @@ SIGNATURES = {
     "nbx_net_get_stdp_state": [_NET, _f32p, _f32p, _i32p, C.c_int64],
     "nbx_replay_deliver": [_NET, _i32p, _i32p, C.c_int32, C.c_int32, _i32p],
     "nbx_replay_deliver_device": [_NET, _vp, _vp, C.c_int32, _vp],
+    "nbx_replay_path": [_NET, _i32p],
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],

@@ -878,6 +878,16 @@ int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick
     return NBX_OK;
 }
 
+int32_t nbx_replay_path(nbx_net* net, int32_t* path) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(path, "null argument");
+    NBX_CHECK(net->has_sparse, "replay delivery needs sparse synapses");
+    NBX_RC(device_scope(net));
+    *path = replay_path(net);
+    return NBX_OK;
+}
+
 int32_t nbx_replay_deliver_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps,
                                   int32_t* d_out) {
     int32_t rc = require_finalized(net);

@@ -218,4 +218,5 @@ int32_t build_persistent_layout(nbx_net* net);
 int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
+int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLOTS
 }  // namespace nbx

@@ -1084,6 +1084,14 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
     return r;
 }
 
+int32_t replay_path(const nbx_net* net) {
+    if (net->stdp) return NBX_REPLAY_GLOBAL;
+    const ReplayGeometry r = replay_geometry(net);
+    if (!r.ok) return NBX_REPLAY_GLOBAL;
+    if (r.slots) return NBX_REPLAY_SLOTS;
+    return persistent_reason(net).empty() || net->pl.built ? NBX_REPLAY_OFFSETS : NBX_REPLAY_GLOBAL;
+}
+
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out) {
     const StepArgs A = net->args();
     // the blocked layouts carry pre-quantized weights, so they are not used while STDP changes them

@@ -280,6 +280,12 @@ class Net:
         L.call("nbx_replay_deliver", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), K, _p(out, C.c_int32))
         return out
 
+    def replay_path(self) -> int:
+        """Replay kernel this network uses: REPLAY_GLOBAL, REPLAY_OFFSETS or REPLAY_SLOTS."""
+        p = C.c_int32()
+        L.call("nbx_replay_path", self.h, C.byref(p))
+        return p.value
+
     def close(self) -> None:
         if getattr(self, "h", None):
             L.lib().nbx_net_destroy(self.h)

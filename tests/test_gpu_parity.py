@@ -238,6 +238,8 @@ def test_replay_delivery_brute_force(gpu, shape, monkeypatch):
     K = 150  # spans several replay windows and, with time groups, their halos
     spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
     spikes[7] = spikes[7][:0]  # an empty step
+    if shape:
+        assert net.replay_path() == (L.REPLAY_SLOTS if shape[4] == "1" else L.REPLAY_OFFSETS)
     e0 = net.events()
     out = net.replay(spikes)
     expect, events = replay_expect(wl, net.fixed_shift, spikes)
@@ -284,6 +286,7 @@ def test_replay_full_size(gpu, slots, monkeypatch):
         monkeypatch.setenv("NBX_REPLAY_SLOTS", slots)
     wl = W.sparse_lif()
     net = wl.make_net()
+    assert net.replay_path() == (L.REPLAY_SLOTS if slots == "auto" else L.REPLAY_OFFSETS)
     rng = np.random.default_rng(5)
     K = 96
     spikes = random_spikes(rng, wl.spec.n, K, 100, 220)
