@@ -194,6 +194,28 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
  * ring, given only these spikes. Host arrays. */
 int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t nsteps,
                            int32_t* out);
+/* ---------------- .nbnet connection tables ----------------
+ * The reference's binary network description (python/src/neuronbridge/nbnet.py:34-38,
+ * read by src/native/shared/src/streaming_build/NbnetReader.cpp): header, metadata JSON,
+ * 32-byte connection records. Validation and error messages follow NbnetReader
+ * (magic, major version 1, record size 32, metadata <= 64 MiB, exact file size, optional
+ * SHA-256 over metadata + records, neuron ids in range, finite weights). */
+typedef struct {
+    int64_t file_bytes, metadata_bytes, connection_count;
+    int32_t version_major, version_minor, flags;
+    int32_t checksum_verified;
+} nbx_nbnet_header;
+/* Validates a file (and its checksum when verify_checksum != 0) and reports its header. */
+int32_t nbx_nbnet_info(const char* path, int32_t verify_checksum, nbx_nbnet_header* info);
+/* Copies the metadata JSON into buf (cap bytes); buf = NULL only reports its length. */
+int32_t nbx_nbnet_metadata(const char* path, char* buf, int64_t cap, int64_t* len);
+/* Replaces NbnetReader::ForEachConnectionBatch feeding the network builder: loads the
+ * connection table as the network's sparse synapses (before nbx_net_finalize). Records are
+ * streamed to the device and sorted there by (presynaptic, delay, postsynaptic), equal keys in
+ * file order. Supported subset: delays 1..63 steps, synapse type 0/1, no trigger rules;
+ * synapse_rule >= 0 marks a synapse plastic (STDP via nbx_net_set_stdp, whose w_max applies). */
+int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checksum);
+
 /* Which replay kernel nbx_replay_deliver uses for this network (launch-shape overrides
  * in the environment included): NBX_REPLAY_GLOBAL (global atomics; STDP networks and
  * layouts that do not fit), NBX_REPLAY_OFFSETS (post-blocked offset layout, LDS ring) or

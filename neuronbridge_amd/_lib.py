@@ -87,6 +87,12 @@ class PendingView(C.Structure):
                 ("neuron_count", C.c_int32), ("inv_scale", C.c_float)]
 
 
+class NbnetHeader(C.Structure):
+    _fields_ = [("file_bytes", C.c_int64), ("metadata_bytes", C.c_int64), ("connection_count", C.c_int64),
+                ("version_major", C.c_int32), ("version_minor", C.c_int32), ("flags", C.c_int32),
+                ("checksum_verified", C.c_int32)]
+
+
 _NET = _vp
 # name -> argtypes (all return int32 status unless listed in _RESTYPES)
 SIGNATURES = {
@@ -131,6 +137,9 @@ SIGNATURES = {
     "nbx_replay_deliver": [_NET, _i32p, _i32p, C.c_int32, C.c_int32, _i32p],
     "nbx_replay_deliver_device": [_NET, _vp, _vp, C.c_int32, _vp],
     "nbx_replay_path": [_NET, _i32p],
+    "nbx_nbnet_info": [C.c_char_p, C.c_int32, C.POINTER(NbnetHeader)],
+    "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
+    "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],

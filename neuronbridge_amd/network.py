@@ -11,6 +11,7 @@ Every method goes through libnbx.so; nothing here computes on the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -114,17 +115,25 @@ class Net:
         self.n = spec.n
         self.nstate = spec.nstate
         self.syn: Synapses | None = None
+        self.syn_count = 0
         self._rec = (0, 0, 0)
 
     # ------------------------------------------------------------ configuration
     def set_sparse(self, syn: Synapses) -> None:
         self.syn = syn
+        self.syn_count = syn.count
         d = L.SparseDesc(count=syn.count, max_delay=int(syn.max_delay),
                          row_ptr=_p(syn.row_ptr, C.c_int64), post=_p(syn.post, C.c_int32),
                          delay=_p(syn.delay, C.c_uint8), type=_p(syn.type, C.c_uint8),
                          weight=_p(syn.weight, C.c_float),
                          plastic=_p(syn.plastic, C.c_uint8) if syn.plastic is not None else None)
         L.call("nbx_net_set_sparse", self.h, C.byref(d))
+
+    def load_nbnet(self, path: str, verify_checksum: bool = True) -> int:
+        """Sparse synapses from a .nbnet connection table (sorted on the device); returns the count."""
+        L.call("nbx_net_load_nbnet", self.h, os.fsencode(path), int(bool(verify_checksum)))
+        self.syn_count = nbnet_info(path, False).connection_count
+        return self.syn_count
 
     def set_dense(self, wt: np.ndarray, delay: int = 1) -> None:
         wt = np.ascontiguousarray(wt, dtype=np.float32)
@@ -252,7 +261,7 @@ class Net:
         return out
 
     def weights(self) -> np.ndarray:
-        out = np.zeros(self.syn.count, dtype=np.float32)
+        out = np.zeros(self.syn_count, dtype=np.float32)
         L.call("nbx_net_get_weights", self.h, _p(out, C.c_float), out.size)
         return out
 
@@ -261,7 +270,7 @@ class Net:
         L.call("nbx_net_set_weights", self.h, _p(w, C.c_float), w.size)
 
     def stdp_state(self):
-        S = self.syn.count
+        S = self.syn_count
         apre = np.zeros(S, np.float32)
         apost = np.zeros(S, np.float32)
         last = np.zeros(S, np.int32)
@@ -296,3 +305,19 @@ class Net:
             self.close()
         except Exception:
             pass
+
+
+def nbnet_info(path: str, verify_checksum: bool = True) -> "L.NbnetHeader":
+    """Validated header of a .nbnet file (and its SHA-256 when verify_checksum)."""
+    h = L.NbnetHeader()
+    L.call("nbx_nbnet_info", os.fsencode(path), int(bool(verify_checksum)), C.byref(h))
+    return h
+
+
+def nbnet_metadata(path: str) -> bytes:
+    """The metadata JSON of a .nbnet file."""
+    n = C.c_int64()
+    L.call("nbx_nbnet_metadata", os.fsencode(path), None, 0, C.byref(n))
+    buf = C.create_string_buffer(max(1, n.value))
+    L.call("nbx_nbnet_metadata", os.fsencode(path), buf, n.value, C.byref(n))
+    return buf.raw[:n.value]

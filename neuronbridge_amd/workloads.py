@@ -55,9 +55,13 @@ class Workload:
             md = max(md, self.dense_delay)
         return max(md + 1, 3)
 
-    def make_net(self, device: int = 0, fixed_shift: int = 0) -> Net:
+    def make_net(self, device: int = 0, fixed_shift: int = 0, nbnet: str | None = None) -> Net:
+        """The workload as a finalized network; `nbnet` loads the synapses from that .nbnet file
+        (the connection table of this workload) instead of the in-memory CSR."""
         net = Net(self.spec, device)
-        if self.syn is not None:
+        if nbnet is not None:
+            net.load_nbnet(nbnet)
+        elif self.syn is not None:
             net.set_sparse(self.syn)
         if self.dense_wt is not None:
             net.set_dense(self.dense_wt, self.dense_delay)
