@@ -158,6 +158,84 @@ def batched_bench(steps: int, sample: int) -> dict:
     return out
 
 
+def stdp_bench(steps: int) -> dict:
+    """Config 3: STDP on current-based Izhikevich, 5e4 neurons, in-degree 500 (2.5e7 synapses, E->E plastic),
+    delays 1..10, dt 0.5 ms. Plastic updates = pre-side LTD at each arrival on a plastic synapse + post-side LTP on
+    each plastic input of a firing neuron, counted from the recorded raster; 12 B per plastic update (index +
+    weight read, weight write; traces are per neuron in the reference layout, per synapse here)."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import Event
+    t0 = time.perf_counter()
+    wl = W.stdp_izh(steps=steps + 200)
+    log(f"stdp: built {wl.spec.n} neurons / {wl.syn.count} synapses in {time.perf_counter() - t0:.1f} s")
+    net = wl.make_net()
+    engine = net.pick_engine()
+    net.run(200, engine)
+    net.record(net.time, steps, counts=True, raster=True)
+    ev0 = net.events()
+    e0, e1 = Event(), Event()
+    net.sync()
+    e0.record(net.stream)
+    net.run_async(steps, engine)
+    e1.record(net.stream)
+    e1.synchronize()
+    net.sync()
+    ms = e0.elapsed_ms(e1)
+    n = wl.spec.n
+    bits = np.unpackbits(net.raster().view(np.uint8), axis=1, bitorder="little")[:, :n]
+    fired = bits.sum(axis=0).astype(np.int64)
+    s = wl.syn
+    pre = np.repeat(np.arange(n), np.diff(s.row_ptr))
+    plastic = s.plastic.astype(bool)
+    out_plastic = np.bincount(pre[plastic], minlength=n)
+    in_plastic = np.bincount(s.post[plastic], minlength=n)
+    updates = int((fired * out_plastic).sum() + (fired * in_plastic).sum())
+    secs = ms * 1e-3
+    w_after = net.weights()
+    syn_events = net.events() - ev0
+    net.close()
+    return {"workload": f"STDP Izhikevich {n} neurons, in-degree 500, {int(plastic.sum())} plastic synapses, "
+                        f"delays U{{1..10}}, dt 0.5 ms, {steps} steps",
+            "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
+            "ms_per_step": ms / steps, "neuron_steps_per_s": n * steps / secs,
+            "syn_events_per_s": syn_events / secs,
+            "plastic_updates_per_s": updates / secs,
+            "plastic_update_gbs": round(updates * 12 / secs / 1e9, 1),
+            "spike_rate_hz": float(fired.sum()) / n / (steps * 0.5e-3),
+            "weights_in_range": bool(np.all(w_after[plastic] >= 0.0) and np.all(w_after[plastic] <= 1.0))}
+
+
+def dense_bench(steps: int) -> dict:
+    """Config 1: dense all-to-all current-based Izhikevich, 1,000 neurons, dt 0.1 ms, Gaussian input currents."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import Event
+    wl = W.dense_izh(steps=steps + 100)
+    net = wl.make_net()
+    engine = net.pick_engine()
+    net.run(100, engine)
+    net.record(net.time, steps, counts=True, raster=False)
+    ev0 = net.events()
+    e0, e1 = Event(), Event()
+    net.sync()
+    e0.record(net.stream)
+    net.run_async(steps, engine)
+    e1.record(net.stream)
+    e1.synchronize()
+    net.sync()
+    ms = e0.elapsed_ms(e1)
+    ev = net.events() - ev0
+    spikes = int(net.counts().sum())
+    net.close()
+    return {"workload": f"dense Izhikevich {wl.spec.n} neurons all-to-all, Gaussian currents (std 5 exc / 2 inh "
+                        f"per step), dt 0.1 ms, {steps} steps",
+            "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
+            "ms_per_step": ms / steps, "neuron_steps_per_s": wl.spec.n * steps / (ms * 1e-3),
+            "syn_events_per_s": ev / (ms * 1e-3),
+            "spike_rate_hz": spikes / wl.spec.n / (steps * 0.1e-3)}
+
+
 def adjoint_bench(reps: int) -> dict:
     """Config 5: forward + surrogate-gradient backward (checkpoint 50) of the 2048-neuron recurrent
     LIF network, 64 trials x 500 steps; GEMM flop count over the timed training step."""
@@ -203,6 +281,10 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=5000)
     ap.add_argument("--batched-sample", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-stdp", action="store_true")
+    ap.add_argument("--stdp-steps", type=int, default=2000)
+    ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--dense-steps", type=int, default=10000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -285,6 +367,12 @@ def main() -> int:
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(3)
         log(f"adjoint: {out['adjoint']}")
+    if not args.no_stdp:
+        out["stdp"] = stdp_bench(args.stdp_steps)
+        log(f"stdp: {out['stdp']}")
+    if not args.no_dense:
+        out["dense"] = dense_bench(args.dense_steps)
+        log(f"dense: {out['dense']}")
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)

@@ -216,6 +216,22 @@ int32_t nbx_nbnet_metadata(const char* path, char* buf, int64_t cap, int64_t* le
  * synapse_rule >= 0 marks a synapse plastic (STDP via nbx_net_set_stdp, whose w_max applies). */
 int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checksum);
 
+/* ---------------- monitor export (DebugMonitor directory schema) ----------------
+ * Writes meta.json and the eight CSV tables of the reference's DebugMonitorWriter
+ * (src/native/gpu/src/debug_monitor/DebugMonitorWriter.cpp:193-235) into output_dir:
+ * spikes.csv from the recorded raster (steps that are multiples of sample_interval_steps),
+ * and, when requested, one snapshot of neuron state ("v", "state_1", ...), pending arrivals
+ * of the next step and weights at export time. neuron_ids = NULL selects every neuron. */
+typedef struct {
+    const char* output_dir;
+    const char* component_name; /* NULL = "main" */
+    int32_t sample_interval_steps, flush_interval_steps;
+    int32_t record_spikes, record_state, record_weights, record_pending_channels;
+    const int32_t* neuron_ids;
+    int32_t neuron_count;
+} nbx_monitor_desc;
+int32_t nbx_net_export_monitor(nbx_net* net, const nbx_monitor_desc* desc);
+
 /* Which replay kernel nbx_replay_deliver uses for this network (launch-shape overrides
  * in the environment included): NBX_REPLAY_GLOBAL (global atomics; STDP networks and
  * layouts that do not fit), NBX_REPLAY_OFFSETS (post-blocked offset layout, LDS ring) or

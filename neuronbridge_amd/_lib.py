@@ -93,6 +93,13 @@ class NbnetHeader(C.Structure):
                 ("checksum_verified", C.c_int32)]
 
 
+class MonitorDesc(C.Structure):
+    _fields_ = [("output_dir", C.c_char_p), ("component_name", C.c_char_p),
+                ("sample_interval_steps", C.c_int32), ("flush_interval_steps", C.c_int32),
+                ("record_spikes", C.c_int32), ("record_state", C.c_int32), ("record_weights", C.c_int32),
+                ("record_pending_channels", C.c_int32), ("neuron_ids", _i32p), ("neuron_count", C.c_int32)]
+
+
 _NET = _vp
 # name -> argtypes (all return int32 status unless listed in _RESTYPES)
 SIGNATURES = {
@@ -140,6 +147,7 @@ SIGNATURES = {
     "nbx_nbnet_info": [C.c_char_p, C.c_int32, C.POINTER(NbnetHeader)],
     "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],
+    "nbx_net_export_monitor": [_NET, C.POINTER(MonitorDesc)],
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],

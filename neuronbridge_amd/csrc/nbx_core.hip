@@ -258,6 +258,7 @@ int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn) {
     const int32_t n = net->n;
     NBX_CHECK(syn->row_ptr[0] == 0 && syn->row_ptr[n] == syn->count, "row_ptr must start at 0 and end at count");
     net->h_row_ptr.assign(syn->row_ptr, syn->row_ptr + n + 1);
+    net->h_src_index.clear();
     net->h_syn.resize((size_t)syn->count);
     net->h_w.assign(syn->weight, syn->weight + syn->count);
     if (syn->plastic) net->h_plastic.assign(syn->plastic, syn->plastic + syn->count);

@@ -488,7 +488,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
     NBX_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt64, row, n + 1, st));
 
     std::vector<int64_t> h_row((size_t)n + 1);
-    std::vector<uint32_t> h_syn((size_t)S);
+    std::vector<uint32_t> h_syn((size_t)S), h_src((size_t)S);
     std::vector<float> h_w((size_t)S);
     std::vector<uint8_t> h_pl((size_t)S);
     NBX_HIP_TRY(hipMemcpyAsync(h_row.data(), row, sizeof(int64_t) * h_row.size(), hipMemcpyDeviceToHost, st));
@@ -496,6 +496,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         NBX_HIP_TRY(hipMemcpyAsync(h_syn.data(), syn, sizeof(uint32_t) * (size_t)S, hipMemcpyDeviceToHost, st));
         NBX_HIP_TRY(hipMemcpyAsync(h_w.data(), w_out, sizeof(float) * (size_t)S, hipMemcpyDeviceToHost, st));
         NBX_HIP_TRY(hipMemcpyAsync(h_pl.data(), pl_out, (size_t)S, hipMemcpyDeviceToHost, st));
+        NBX_HIP_TRY(hipMemcpyAsync(h_src.data(), vb.Current(), sizeof(uint32_t) * (size_t)S, hipMemcpyDeviceToHost, st));
     }
     NBX_HIP_TRY(hipStreamSynchronize(st));
     bool any_plastic = false;
@@ -503,6 +504,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
     net->h_row_ptr = std::move(h_row);
     net->h_syn = std::move(h_syn);
     net->h_w = std::move(h_w);
+    net->h_src_index = std::move(h_src);
     if (any_plastic) net->h_plastic = std::move(h_pl);
     else net->h_plastic.clear();
     net->S = S;

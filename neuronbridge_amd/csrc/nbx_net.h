@@ -62,6 +62,7 @@ struct nbx_net {
     std::vector<uint32_t> h_syn;
     std::vector<float> h_w;
     std::vector<uint8_t> h_plastic;
+    std::vector<uint32_t> h_src_index;  // file record index of each CSR synapse (networks loaded from .nbnet)
     // dense
     bool has_dense = false;
     std::vector<float> h_wt;

@@ -289,6 +289,19 @@ class Net:
         L.call("nbx_replay_deliver", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), K, _p(out, C.c_int32))
         return out
 
+    def export_monitor(self, output_dir: str, *, sample_interval: int = 1, flush_interval: int = 100,
+                       spikes: bool = True, state: bool = True, weights: bool = False, pending: bool = True,
+                       neuron_ids=None, name: str = "main") -> None:
+        """Writes the DebugMonitor directory (meta.json + CSV tables) the reference's results.py reads."""
+        ids = None if neuron_ids is None else np.ascontiguousarray(neuron_ids, dtype=np.int32)
+        d = L.MonitorDesc(output_dir=os.fsencode(output_dir), component_name=name.encode(),
+                          sample_interval_steps=sample_interval, flush_interval_steps=flush_interval,
+                          record_spikes=int(spikes), record_state=int(state), record_weights=int(weights),
+                          record_pending_channels=int(pending),
+                          neuron_ids=_p(ids, C.c_int32) if ids is not None else None,
+                          neuron_count=0 if ids is None else int(ids.size))
+        L.call("nbx_net_export_monitor", self.h, C.byref(d))
+
     def replay_path(self) -> int:
         """Replay kernel this network uses: REPLAY_GLOBAL, REPLAY_OFFSETS or REPLAY_SLOTS."""
         p = C.c_int32()
