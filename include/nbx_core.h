@@ -48,7 +48,12 @@ enum {
     NBX_MODEL_LIF_VJ = 0,  /* TimeDrivenLIF_Voltage_jump (TimeDrivenLIF_Voltage_jump.cpp:49-115) */
     NBX_MODEL_LIF_EXP = 1, /* TimeDrivenLIF_Exponential_Decay (.cpp:62-121) */
     NBX_MODEL_IZH = 2,     /* TimeDrivenIzhikevic_Exponential_Decay, AMPA/GABA conductances (.cpp:59-131) */
-    NBX_MODEL_IZH_CUR = 3  /* Izhikevich with current-based synapses (signed weights, one-tick current) */
+    NBX_MODEL_IZH_CUR = 3, /* Izhikevich with current-based synapses (signed weights, one-tick current) */
+    NBX_MODEL_LIF_EXP2 = 4, /* TimeDrivenLIF_Exponential_double: excitatory + inhibitory conductances (.cpp:61-123) */
+    NBX_MODEL_LIF_EXP3 = 5, /* TimeDrivenLIF_Exponential_triple: AMPA, GABA, NMDA (Mg block) conductances (.cpp:79-155) */
+    NBX_MODEL_POISSON = 6,  /* PoissonRate: rate-driven hash spikes (PoissonRate.cpp:7-88) */
+    NBX_MODEL_TRIGGER_RELAY = 7 /* DenseTriggerRelay: fires when the step's excitatory input exceeds v_th
+                                   (dense_subnetwork/model/DenseTriggerRelayDeviceUpdate.cuh:10-32) */
 };
 enum { NBX_EULER = 0, NBX_RK4 = 1 };
 
@@ -62,6 +67,10 @@ typedef struct nbx_model_desc {
     float v_rest, tau, v_th, r, v_reset, g_tau, e_rev;             /* LIF, model-wide */
     float izh_v_th, izh_r, e_ampa, e_gaba, ampa_tau, gaba_tau;     /* Izhikevich, model-wide */
     const float *izh_a, *izh_b, *izh_c, *izh_d;                    /* Izhikevich, per neuron (host) */
+    /* LIF_EXP2 uses e_ampa / ampa_tau for its excitatory and e_gaba / gaba_tau for its inhibitory
+     * conductance (the reference's Eexc, gexc_tau, Einh, ginh_tau); LIF_EXP3 uses all three. */
+    float nmda_tau;                        /* LIF_EXP3 */
+    float rate_bias_hz, rate_gain_hz;      /* POISSON: rate = bias + arrivals + gain * I_ext */
 } nbx_model_desc;
 
 /* Synapses as CSR by presynaptic neuron, each row sorted by (delay, post) —
@@ -118,7 +127,8 @@ int32_t nbx_net_get_fixed_shift(const nbx_net* net, int32_t* shift);
 /* Upload and build device layouts; required before stepping. */
 int32_t nbx_net_finalize(nbx_net* net);
 
-/* State in the reference's AoS order x[i * nstate + j] (nstate 2 / 3 / 6 per model)
+/* State in the reference's AoS order x[i * nstate + j] (nstate per model: LIF_VJ 2, LIF_EXP 3,
+ * IZH / IZH_CUR 6, LIF_EXP2 4, LIF_EXP3 5, POISSON 2, TRIGGER_RELAY 1)
  * plus LastSpike. */
 int32_t nbx_net_nstate(const nbx_net* net);
 int32_t nbx_net_set_state(nbx_net* net, const float* x, const int32_t* last);

@@ -17,7 +17,7 @@ NBX_ERR_HIP = -2
 NBX_ERR_UNSUPPORTED = -3
 NBX_ERR_STATE = -4
 
-LIF_VJ, LIF_EXP, IZH, IZH_CUR = 0, 1, 2, 3
+LIF_VJ, LIF_EXP, IZH, IZH_CUR, LIF_EXP2, LIF_EXP3, POISSON, TRIGGER_RELAY = 0, 1, 2, 3, 4, 5, 6, 7
 EULER, RK4 = 0, 1
 ENGINE_KERNELS, ENGINE_GRAPH, ENGINE_PERSISTENT, ENGINE_AUTO = 0, 1, 2, 3
 REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS = 0, 1, 2
@@ -48,7 +48,18 @@ class ModelDesc(C.Structure):
         ("izh_v_th", C.c_float), ("izh_r", C.c_float), ("e_ampa", C.c_float), ("e_gaba", C.c_float),
         ("ampa_tau", C.c_float), ("gaba_tau", C.c_float),
         ("izh_a", _f32p), ("izh_b", _f32p), ("izh_c", _f32p), ("izh_d", _f32p),
+        ("nmda_tau", C.c_float), ("rate_bias_hz", C.c_float), ("rate_gain_hz", C.c_float),
     ]
+
+
+def model_channels(model: int) -> int:
+    """Arrival channels of a model's delay ring (libnbx's model_channels)."""
+    return 3 if model == LIF_EXP3 else (2 if model in (IZH, LIF_EXP2, TRIGGER_RELAY) else 1)
+
+
+def model_nstate(model: int) -> int:
+    """Columns of the reference's AoS state per model."""
+    return {LIF_VJ: 2, LIF_EXP: 3, LIF_EXP2: 4, LIF_EXP3: 5, POISSON: 2, TRIGGER_RELAY: 1}.get(model, 6)
 
 
 class SparseDesc(C.Structure):

@@ -41,6 +41,35 @@ float decay_factor(float dt, float tau) {
     return std::exp(-dt * inv);
 }
 
+// TimeDrivenLIF_Exponential_triple.cpp:56-60 divides instead of multiplying by 1/tau
+float decay_factor_div(float dt, float tau) { return std::exp(-dt / tau); }
+
+int32_t model_nstate(int32_t model) {
+    switch (model) {
+        case NBX_MODEL_LIF_VJ: return 2;
+        case NBX_MODEL_LIF_EXP: return 3;
+        case NBX_MODEL_LIF_EXP2: return 4;
+        case NBX_MODEL_LIF_EXP3: return 5;
+        case NBX_MODEL_POISSON: return 2;
+        case NBX_MODEL_TRIGGER_RELAY: return 1;
+        default: return 6;
+    }
+}
+
+// Device array behind each column of the reference's AoS state (null: a column this build does
+// not keep, read as 0; the Izhikevich NMDA conductance)
+std::vector<float*> state_columns(nbx_net* net) {
+    switch (net->md.model) {
+        case NBX_MODEL_LIF_VJ: return {net->d_v, net->d_iext};
+        case NBX_MODEL_LIF_EXP: return {net->d_v, net->d_s1, net->d_iext};
+        case NBX_MODEL_LIF_EXP2: return {net->d_v, net->d_ga, net->d_gg, net->d_iext};
+        case NBX_MODEL_LIF_EXP3: return {net->d_v, net->d_ga, net->d_gg, net->d_s1, net->d_iext};
+        case NBX_MODEL_POISSON: return {net->d_v, net->d_iext};
+        case NBX_MODEL_TRIGGER_RELAY: return {net->d_v};
+        default: return {net->d_v, net->d_s1, net->d_ga, net->d_gg, nullptr, net->d_iext};
+    }
+}
+
 int32_t poisson_table(double lambda, std::vector<uint32_t>* tab) {
     tab->clear();
     double p = std::exp(-lambda), cdf = 0.0;
@@ -189,13 +218,18 @@ int32_t nbx_device_count(void) {
 int32_t nbx_net_create(const nbx_model_desc* md, int32_t device, nbx_net** out) {
     NBX_CHECK(md && out, "null argument");
     *out = nullptr;
-    NBX_CHECK(md->model >= NBX_MODEL_LIF_VJ && md->model <= NBX_MODEL_IZH_CUR, "unknown model id");
+    NBX_CHECK(md->model >= NBX_MODEL_LIF_VJ && md->model <= NBX_MODEL_TRIGGER_RELAY, "unknown model id");
     NBX_CHECK(md->integrator == NBX_EULER || md->integrator == NBX_RK4, "unknown integrator");
     NBX_CHECK(md->n > 0 && md->n < kMaxNeurons, "neuron count must be in [1, 2^24)");
     NBX_CHECK(md->dt > 0.0f, "dt must be positive");
     const bool izh = md->model == NBX_MODEL_IZH || md->model == NBX_MODEL_IZH_CUR;
     NBX_CHECK(!izh || (md->izh_a && md->izh_b && md->izh_c && md->izh_d), "Izhikevich models need per-neuron a,b,c,d");
-    NBX_CHECK(izh || md->tau != 0.0f, "LIF tau must be non-zero");
+    const bool lif = md->model == NBX_MODEL_LIF_VJ || md->model == NBX_MODEL_LIF_EXP ||
+                     md->model == NBX_MODEL_LIF_EXP2 || md->model == NBX_MODEL_LIF_EXP3;
+    NBX_CHECK(!lif || md->tau != 0.0f, "LIF tau must be non-zero");
+    NBX_CHECK(md->model != NBX_MODEL_LIF_EXP3 ||
+                  (md->ampa_tau != 0.0f && md->gaba_tau != 0.0f && md->nmda_tau != 0.0f),
+              "LIF_EXP3 conductance time constants must be non-zero");
     int ndev = 0;
     NBX_HIP_TRY(hipGetDeviceCount(&ndev));
     NBX_CHECK(device >= 0 && device < ndev, "device index out of range");
@@ -204,8 +238,8 @@ int32_t nbx_net_create(const nbx_model_desc* md, int32_t device, nbx_net** out) 
     net->md = *md;
     net->md.izh_a = net->md.izh_b = net->md.izh_c = net->md.izh_d = nullptr;
     net->n = md->n;
-    net->ns = md->model == NBX_MODEL_LIF_VJ ? 2 : (md->model == NBX_MODEL_LIF_EXP ? 3 : 6);
-    net->nch = md->model == NBX_MODEL_IZH ? 2 : 1;
+    net->ns = model_nstate(md->model);
+    net->nch = model_channels(md->model);
     if (izh) {
         net->h_a.assign(md->izh_a, md->izh_a + md->n);
         net->h_b.assign(md->izh_b, md->izh_b + md->n);
@@ -233,6 +267,15 @@ int32_t nbx_net_create(const nbx_model_desc* md, int32_t device, nbx_net** out) 
     P.g_decay = decay_factor(md->dt, md->g_tau);
     P.ampa_decay = decay_factor(md->dt, md->ampa_tau);
     P.gaba_decay = decay_factor(md->dt, md->gaba_tau);
+    P.nmda_decay = 1.0f;
+    if (md->model == NBX_MODEL_LIF_EXP3) {
+        P.ampa_decay = decay_factor_div(md->dt, md->ampa_tau);
+        P.gaba_decay = decay_factor_div(md->dt, md->gaba_tau);
+        P.nmda_decay = decay_factor_div(md->dt, md->nmda_tau);
+    }
+    P.rate_bias = md->rate_bias_hz;
+    P.rate_gain = md->rate_gain_hz;
+    P.cond = 0;
     NBX_HIP_TRY(hipSetDevice(device));
     NBX_HIP_TRY(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
     *out = net.release();
@@ -271,7 +314,10 @@ int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn) {
             const int32_t d = syn->delay[j], post = syn->post[j], ty = syn->type[j];
             NBX_CHECK(d >= 1 && d <= syn->max_delay, "synapse delay outside [1, max_delay]");
             NBX_CHECK(post >= 0 && post < n, "synapse post index out of range");
-            NBX_CHECK(ty == 0 || ty == 1, "synapse type must be 0 (excitatory) or 1 (inhibitory)");
+            NBX_CHECK(ty >= 0 && ty <= model_max_type(net->md.model),
+                      net->md.model == NBX_MODEL_LIF_EXP3
+                          ? "synapse type must be 0 (AMPA), 1 (GABA) or 2 (NMDA)"
+                          : "synapse type must be 0 (excitatory) or 1 (inhibitory)");
             NBX_CHECK(d >= prev_d, "each row must be sorted by delay");
             prev_d = d;
             net->h_syn[(size_t)j] = pack_syn(post, d, ty);
@@ -373,12 +419,25 @@ int32_t nbx_net_finalize(nbx_net* net) {
     NBX_CHECK(!(net->stdp && !net->has_sparse), "STDP needs sparse synapses");
     net->shift = net->shift_req ? net->shift_req : auto_shift(net);
     net->words = (n + 31) / 32;
+    // conductance inputs present (the reference models' CheckType flags)
+    {
+        int32_t cond = net->pois ? kCondExc : 0;
+        for (int64_t j = 0; j < net->S; ++j) cond |= 1 << syn_type(net->h_syn[(size_t)j]);
+        if (net->has_dense)
+            for (float w : net->h_wt) cond |= (w < 0.0f) ? kCondInh : kCondExc;
+        net->P.cond = cond & (kCondExc | kCondInh | kCondNmda);
+    }
     // state (SoA) initialised like Neuron_State_Vector::InitNeuronState with sigma 0
     std::vector<float> v((size_t)n), s1((size_t)n, 0.0f), zero((size_t)n, 0.0f);
     std::vector<int32_t> last((size_t)n, 10000);
     for (int32_t i = 0; i < n; ++i) {
-        if (net->md.model == NBX_MODEL_LIF_VJ) v[(size_t)i] = net->md.v_reset;
-        else v[(size_t)i] = net->md.v_rest;
+        switch (net->md.model) {
+            case NBX_MODEL_LIF_VJ:  // TimeDrivenLIF_Voltage_jump.cpp:37
+            case NBX_MODEL_LIF_EXP3: v[(size_t)i] = net->md.v_reset; break;  // ..._triple.cpp:71
+            case NBX_MODEL_POISSON:  // PoissonRate.cpp:45
+            case NBX_MODEL_TRIGGER_RELAY: v[(size_t)i] = 0.0f; break;
+            default: v[(size_t)i] = net->md.v_rest;
+        }
         if (net->md.model == NBX_MODEL_IZH || net->md.model == NBX_MODEL_IZH_CUR)
             s1[(size_t)i] = net->h_b[(size_t)i] * net->md.v_rest;
     }
@@ -478,25 +537,12 @@ int32_t nbx_net_set_state(nbx_net* net, const float* x, const int32_t* last) {
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     const int32_t n = net->n, ns = net->ns;
-    std::vector<float> cols[6];
+    const std::vector<float*> dst = state_columns(net);
+    std::vector<float> col((size_t)n);
     for (int j = 0; j < ns; ++j) {
-        cols[j].resize((size_t)n);
-        for (int32_t i = 0; i < n; ++i) cols[j][(size_t)i] = x[(size_t)i * ns + j];
-    }
-    auto put = [&](float* d, int j) -> hipError_t {
-        return hipMemcpy(d, cols[j].data(), sizeof(float) * n, hipMemcpyHostToDevice);
-    };
-    NBX_HIP_TRY(put(net->d_v, 0));
-    if (net->md.model == NBX_MODEL_LIF_VJ) {
-        NBX_HIP_TRY(put(net->d_iext, 1));
-    } else if (net->md.model == NBX_MODEL_LIF_EXP) {
-        NBX_HIP_TRY(put(net->d_s1, 1));
-        NBX_HIP_TRY(put(net->d_iext, 2));
-    } else {
-        NBX_HIP_TRY(put(net->d_s1, 1));
-        NBX_HIP_TRY(put(net->d_ga, 2));
-        NBX_HIP_TRY(put(net->d_gg, 3));
-        NBX_HIP_TRY(put(net->d_iext, 5));
+        if (!dst[(size_t)j]) continue;
+        for (int32_t i = 0; i < n; ++i) col[(size_t)i] = x[(size_t)i * ns + j];
+        NBX_HIP_TRY(hipMemcpy(dst[(size_t)j], col.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     }
     NBX_HIP_TRY(hipMemcpy(net->d_last, last, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     return NBX_OK;
@@ -516,18 +562,9 @@ int32_t nbx_net_get_state(nbx_net* net, float* x, int32_t* last) {
         return e;
     };
     for (size_t k = 0; k < (size_t)n * ns; ++k) x[k] = 0.0f;
-    NBX_HIP_TRY(get(net->d_v, 0));
-    if (net->md.model == NBX_MODEL_LIF_VJ) {
-        NBX_HIP_TRY(get(net->d_iext, 1));
-    } else if (net->md.model == NBX_MODEL_LIF_EXP) {
-        NBX_HIP_TRY(get(net->d_s1, 1));
-        NBX_HIP_TRY(get(net->d_iext, 2));
-    } else {
-        NBX_HIP_TRY(get(net->d_s1, 1));
-        NBX_HIP_TRY(get(net->d_ga, 2));
-        NBX_HIP_TRY(get(net->d_gg, 3));
-        NBX_HIP_TRY(get(net->d_iext, 5));
-    }
+    const std::vector<float*> src = state_columns(net);
+    for (int j = 0; j < ns; ++j)
+        if (src[(size_t)j]) NBX_HIP_TRY(get(src[(size_t)j], j));
     NBX_HIP_TRY(hipMemcpy(last, net->d_last, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     return NBX_OK;
 }

@@ -54,8 +54,44 @@ struct ModelParams {
     float dt, hdt, dt6;
     float v_rest, tau, v_th, r, v_reset, e_rev;
     float izh_v_th, izh_r, e_ampa, e_gaba;
-    float g_decay, ampa_decay, gaba_decay;
+    float g_decay, ampa_decay, gaba_decay, nmda_decay;
+    float rate_bias, rate_gain;
+    int32_t cond;  // kCond* bits: which conductance inputs the network has (the reference's CheckType flags)
 };
+
+// Conductance inputs present in a network (set at finalize from synapse types, dense signs and the
+// Poisson background); the double/triple-exponential LIF models gate terms and decay on them.
+constexpr int32_t kCondExc = 1, kCondInh = 2, kCondNmda = 4;
+
+// ----------------------------------------------------------------- arrival channels
+// Ring channel of an arrival through a synapse of `type` (0 exc, 1 inh, 2 NMDA) and whether its
+// fixed-point value enters negated, per model (the reference models' ProcessSpike):
+//   LIF_VJ, LIF_EXP, POISSON: one signed channel (V += w / g += w / rate += w, minus for inhibitory)
+//   IZH, LIF_EXP2, TRIGGER_RELAY: channel 0 excitatory, channel 1 inhibitory, both positive
+//   LIF_EXP3: channel = type (AMPA, GABA, NMDA), positive
+//   IZH_CUR: one channel, weights already signed
+__host__ __device__ inline int32_t model_channels(int32_t model) {
+    return model == NBX_MODEL_LIF_EXP3 ? 3
+         : (model == NBX_MODEL_IZH || model == NBX_MODEL_LIF_EXP2 || model == NBX_MODEL_TRIGGER_RELAY) ? 2 : 1;
+}
+__host__ __device__ inline int32_t arrival_channel(int32_t model, int32_t type) {
+    if (model == NBX_MODEL_LIF_EXP3) return type;
+    if (model_channels(model) == 2) return type == 1 ? 1 : 0;
+    return 0;
+}
+__host__ __device__ inline bool arrival_negated(int32_t model, int32_t type) {
+    return type == 1 && (model == NBX_MODEL_LIF_VJ || model == NBX_MODEL_LIF_EXP || model == NBX_MODEL_POISSON);
+}
+// Largest synapse type a model accepts.
+__host__ __device__ inline int32_t model_max_type(int32_t model) { return model == NBX_MODEL_LIF_EXP3 ? 2 : 1; }
+
+// Which SoA state arrays a model keeps besides v, I_ext and LastSpike.
+__host__ __device__ constexpr bool model_uses_s1(int32_t m) {
+    return m == NBX_MODEL_LIF_EXP || m == NBX_MODEL_IZH || m == NBX_MODEL_IZH_CUR || m == NBX_MODEL_LIF_EXP3;
+}
+__host__ __device__ constexpr bool model_uses_g2(int32_t m) {
+    return m == NBX_MODEL_IZH || m == NBX_MODEL_LIF_EXP2 || m == NBX_MODEL_LIF_EXP3;
+}
 
 // Device pointers and scalars the step kernels need; passed by value.
 struct StepArgs {
@@ -237,15 +273,123 @@ __device__ inline bool izh_step(const ModelParams& P, float a, float b, float c,
     return false;
 }
 
-// Applies one neuron's arrivals (a0/a1 fixed point) and external inputs for step t,
+// ---- LIF with excitatory and inhibitory conductances (TimeDrivenLIF_Exponential_double.cpp:61-74):
+// R*I/tau + gexc*(Eexc-V)/tau + ginh*(Einh-V)/tau + (V_rest-V)/tau, left to right
+__device__ inline float lif2_rhs(const ModelParams& P, float V, float I, float ge, float gi) {
+    float cur = P.r * I / P.tau;
+    cur += ge * (P.e_ampa - V) / P.tau;
+    cur += gi * (P.e_gaba - V) / P.tau;
+    return cur + (P.v_rest - V) / P.tau;
+}
+
+// ---- NMDA magnesium block 1 / (1 + exp(-0.062 V) * (1.2 / 3.57)). The exponential is evaluated in
+// double and rounded to float (the oracle does the same); the reference calls float std::exp.
+__device__ inline float nmda_block(float V) {
+    const float e = (float)exp((double)(-0.062f * V));
+    return 1.0f / (1.0f + e * (1.2f / 3.57f));
+}
+
+// ---- triple-exponential LIF (TimeDrivenLIF_Exponential_triple.cpp:79-105):
+// (R * (gampa(Eampa-V) + ggaba(Egaba-V) + gnmda*B(V)*(Eampa-V) + I) + V_rest - V) / tau
+__device__ inline float lif3_rhs(const ModelParams& P, float V, float I, float ga, float gg, float gn) {
+    float cur = 0.0f;
+    if (P.cond & kCondExc) cur += ga * (P.e_ampa - V);
+    if (P.cond & kCondInh) cur += gg * (P.e_gaba - V);
+    if ((P.cond & kCondNmda) && gn != 0.0f) cur += gn * nmda_block(V) * (P.e_ampa - V);  // gn == 0 adds a zero
+    cur += I;
+    return (P.r * cur + P.v_rest - V) / P.tau;
+}
+
+__device__ inline float decay_or_zero(float g, float k) { return g < 1.0e-9f ? 0.0f : g * k; }
+
+template <int NG>
+__device__ inline bool lif_cond_step(const ModelParams& P, float& V, float& ga, float& gg, float& gn, float I,
+                                     int32_t& last) {
+    auto rhs = [&](float v) { return NG == 2 ? lif2_rhs(P, v, I, ga, gg) : lif3_rhs(P, v, I, ga, gg, gn); };
+    if (last > P.t_ref) {
+        if (P.integrator == NBX_EULER) {
+            V += rhs(V) * P.dt;
+        } else {
+            const float k1 = rhs(V);
+            const float k2 = rhs(V + k1 * P.hdt);
+            const float k3 = rhs(V + k2 * P.hdt);
+            const float k4 = rhs(V + k3 * P.dt);
+            float acc = k1 + 2.0f * k2;
+            acc = acc + 2.0f * k3;
+            acc = acc + k4;
+            V += acc * P.dt6;
+        }
+    }
+    if (NG == 2) {  // TimeDrivenLIF_Exponential_double.cpp:76-86: both decay once either input exists
+        if (P.cond & (kCondExc | kCondInh)) {
+            ga *= P.ampa_decay;
+            gg *= P.gaba_decay;
+        }
+    } else {  // TimeDrivenLIF_Exponential_triple.cpp:107-128
+        ga = decay_or_zero(ga, P.ampa_decay);
+        gg = decay_or_zero(gg, P.gaba_decay);
+        gn = decay_or_zero(gn, P.nmda_decay);
+    }
+    last += 1;
+    if (V > P.v_th) {
+        V = P.v_reset;
+        last = 0;
+        return true;
+    }
+    return false;
+}
+
+// ---- PoissonRate hash (PoissonRate.cpp:11-22)
+__host__ __device__ inline float hash_uniform01(int32_t neuron, int32_t t) {
+    uint32_t x = (uint32_t)neuron * 747796405u + (uint32_t)(t + 1) * 2891336453u + 0x9e3779b9u;
+    x ^= x >> 16;
+    x *= 2246822519u;
+    x ^= x >> 13;
+    x *= 3266489917u;
+    x ^= x >> 16;
+    return (float)(x & 0x00FFFFFFu) / (float)0x01000000u;
+}
+
+// Applies one neuron's arrivals (a0/a1/a2 fixed point) and external inputs for step t,
 // then advances it one step. Shared by every engine so their arithmetic is identical.
 template <int MODEL>
-__device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, float& V,
-                                   float& S1, float& GA, float& GG, float& I, int32_t& last) {
+__device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, int32_t a2,
+                                   float& V, float& S1, float& GA, float& GG, float& I, int32_t& last) {
     if (A.cur && t >= A.cur_t0 && t < A.cur_t0 + A.cur_n) I = A.cur[(int64_t)(t - A.cur_t0) * A.n + i];
     if (A.pois) a0 += poisson_count(A.seed_lo, A.seed_hi, i, t, A.pois_tab, A.kmax) * A.q_bg;
     const float f0 = (float)a0 * A.inv_scale;
-    if (MODEL == NBX_MODEL_LIF_VJ) {
+    if (MODEL == NBX_MODEL_LIF_EXP2) {
+        GA += f0;
+        GG += (float)a1 * A.inv_scale;
+        float unused = 0.0f;
+        return lif_cond_step<2>(A.P, V, GA, GG, unused, I, last);
+    } else if (MODEL == NBX_MODEL_LIF_EXP3) {
+        GA += f0;
+        GG += (float)a1 * A.inv_scale;
+        S1 += (float)a2 * A.inv_scale;
+        return lif_cond_step<3>(A.P, V, GA, GG, S1, I, last);
+    } else if (MODEL == NBX_MODEL_POISSON) {  // PoissonRate.cpp:55-77; V holds the transient rate input
+        V += f0;
+        float rate = A.P.rate_bias + V + A.P.rate_gain * I;
+        rate = (0.0f < rate) ? rate : 0.0f;
+        float p = rate * A.P.dt * 0.001f;
+        p = (p < 1.0f) ? p : 1.0f;
+        V = 0.0f;
+        if (hash_uniform01(i, t) < p) {
+            last = 0;
+            return true;
+        }
+        last += 1;
+        return false;
+    } else if (MODEL == NBX_MODEL_TRIGGER_RELAY) {  // V holds last_exc_input
+        V = f0;
+        if (f0 > A.P.v_th) {
+            last = 0;
+            return true;
+        }
+        last += 1;
+        return false;
+    } else if (MODEL == NBX_MODEL_LIF_VJ) {
         if (last > A.P.t_ref) V += f0;
         float g = 0.0f;
         return lif_step<false>(A.P, V, g, I, last);

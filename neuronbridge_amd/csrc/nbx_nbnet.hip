@@ -249,7 +249,7 @@ enum : uint32_t {
 __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t count, int64_t base, int32_t n,
                                   uint64_t* __restrict__ key, uint32_t* __restrict__ idx, uint8_t* __restrict__ type,
                                   float* __restrict__ w, uint8_t* __restrict__ plastic, int32_t* __restrict__ cnt,
-                                  uint32_t* __restrict__ bad, int32_t* __restrict__ max_delay) {
+                                  uint32_t* __restrict__ bad, int32_t* __restrict__ max_delay, int32_t max_type) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t* r = rec + j * 8;
         const uint32_t src = r[0], dst = r[1], dly = r[5];
@@ -259,7 +259,7 @@ __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t coun
         if (src >= (uint32_t)n || dst >= (uint32_t)n) b |= kBadId;
         if (!isfinite(wt) || !isfinite(mw)) b |= kBadWeight;
         if (dly < 1u || dly > (uint32_t)kMaxDelay) b |= kBadDelay;
-        if (ty != 0 && ty != 1) b |= kBadType;
+        if (ty < 0 || ty > max_type) b |= kBadType;
         if (trig >= 0) b |= kTrigger;
         if (b) {
             atomicOr(bad, b);
@@ -429,7 +429,8 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         // rec is reused in stream order: the previous chunk's extraction precedes this copy
         NBX_HIP_TRY(hipMemcpyAsync(rec, sc.pinned[cur], (size_t)c * kRecordSize, hipMemcpyHostToDevice, st));
         NBX_HIP_TRY(hipEventRecord(copied[cur], st));
-        nbx_nbnet_extract<<<blocks_for(c), 256, 0, st>>>(rec, c, base, n, key, idx, type, w, pl, cnt, bad, maxd);
+        nbx_nbnet_extract<<<blocks_for(c), 256, 0, st>>>(rec, c, base, n, key, idx, type, w, pl, cnt, bad, maxd,
+                                                          model_max_type(net->md.model));
         NBX_HIP_TRY(hipGetLastError());
         cur ^= 1;
     }
@@ -451,7 +452,9 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         return NBX_ERR_UNSUPPORTED;
     }
     if (bad_h & kBadType) {
-        set_error("nbnet synapse type must be 0 (excitatory) or 1 (inhibitory)");
+        set_error(net->md.model == NBX_MODEL_LIF_EXP3
+                      ? "nbnet synapse type must be 0 (AMPA), 1 (GABA) or 2 (NMDA)"
+                      : "nbnet synapse type must be 0 (excitatory) or 1 (inhibitory)");
         return NBX_ERR_UNSUPPORTED;
     }
     if (bad_h & kTrigger) {

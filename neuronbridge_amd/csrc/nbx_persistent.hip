@@ -149,8 +149,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         last[u] = 0;
         if (pl < np) {
             V[u] = A.v[i];
-            if (MODEL != NBX_MODEL_LIF_VJ) S1[u] = A.s1[i];
-            if (MODEL == NBX_MODEL_IZH) {
+            if (model_uses_s1(MODEL)) S1[u] = A.s1[i];
+            if (model_uses_g2(MODEL)) {
                 GA[u] = A.ga[i];
                 GG[u] = A.gg[i];
             }
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                     a1 = s_ring[(slot * nch + 1) * P + pl];
                     s_ring[(slot * nch + 1) * P + pl] = 0;
                 }
-                fired = neuron_tick<MODEL>(A, i, t, a0, a1, V[u], S1[u], GA[u], GG[u], I[u], last[u]);
+                fired = neuron_tick<MODEL>(A, i, t, a0, a1, 0, V[u], S1[u], GA[u], GG[u], I[u], last[u]);
             }
             const uint64_t mask = __ballot(fired);
             if (lane == 0) s_wcnt[wave] = __popcll(mask);
@@ -260,8 +260,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         const int32_t i = p0 + pl;
         if (pl < np) {
             A.v[i] = V[u];
-            if (MODEL != NBX_MODEL_LIF_VJ) A.s1[i] = S1[u];
-            if (MODEL == NBX_MODEL_IZH) {
+            if (model_uses_s1(MODEL)) A.s1[i] = S1[u];
+            if (model_uses_g2(MODEL)) {
                 A.ga[i] = GA[u];
                 A.gg[i] = GG[u];
             }
@@ -721,9 +721,9 @@ __global__ __launch_bounds__(256) void nbx_replay_global(StepArgs A, const int32
             const uint32_t en = A.syn[j];
             const int32_t ko = lo + syn_delay(en) - 1;
             if (ko >= K) continue;
-            int32_t q = quantize(A.w[j], A.scale), ch = 0;
-            if (A.P.model == NBX_MODEL_IZH) ch = syn_type(en) == 1 ? 1 : 0;
-            else if (A.P.model != NBX_MODEL_IZH_CUR && syn_type(en) == 1) q = -q;
+            int32_t q = quantize(A.w[j], A.scale);
+            const int32_t ch = arrival_channel(A.P.model, syn_type(en));
+            if (arrival_negated(A.P.model, syn_type(en))) q = -q;
             atomicAdd(&out[((int64_t)ko * A.nch + ch) * A.n + syn_post(en)], q);
         }
         if (threadIdx.x == 0) atomicAdd(A.events, (unsigned long long)(re - rb));
@@ -749,6 +749,9 @@ void* persistent_kernel_for(int model, int npt) {
         case NBX_MODEL_LIF_VJ: return persistent_kernel<NBX_MODEL_LIF_VJ>(npt);
         case NBX_MODEL_LIF_EXP: return persistent_kernel<NBX_MODEL_LIF_EXP>(npt);
         case NBX_MODEL_IZH: return persistent_kernel<NBX_MODEL_IZH>(npt);
+        case NBX_MODEL_LIF_EXP2: return persistent_kernel<NBX_MODEL_LIF_EXP2>(npt);
+        case NBX_MODEL_POISSON: return persistent_kernel<NBX_MODEL_POISSON>(npt);
+        case NBX_MODEL_TRIGGER_RELAY: return persistent_kernel<NBX_MODEL_TRIGGER_RELAY>(npt);
         default: return persistent_kernel<NBX_MODEL_IZH_CUR>(npt);
     }
 }
@@ -800,6 +803,7 @@ std::string persistent_reason(const nbx_net* net) {
     if (net->has_dense) return "dense delivery runs on the graph engine";
     if (net->stdp) return "STDP networks use arrival-time delivery";
     if (net->S >= ((int64_t)1 << 31)) return "more than 2^31 synapses";
+    if (net->nch > 2) return "more than two arrival channels";
     Geometry g = geometry(net);
     return g.why;
 }
@@ -810,6 +814,7 @@ int32_t build_persistent_layout(nbx_net* net) {
     PersistentLayout& L = net->pl;
     if (L.built) return NBX_OK;
     Geometry g = geometry(net);
+    if (net->nch > 2) g.why = "more than two arrival channels (entries carry one channel bit)";
     if (!g.why.empty()) {
         set_error("persistent layout: " + g.why);
         return NBX_ERR_UNSUPPORTED;
@@ -854,9 +859,8 @@ int32_t build_persistent_layout(nbx_net* net) {
             const uint32_t e = net->h_syn[(size_t)j];
             const int32_t post = syn_post(e), ty = syn_type(e), blk = post / P;
             int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
-            int32_t ch = 0;
-            if (net->md.model == NBX_MODEL_IZH) ch = (ty == 1) ? 1 : 0;
-            else if (net->md.model != NBX_MODEL_IZH_CUR && ty == 1) qv = -qv;
+            const int32_t ch = arrival_channel(net->md.model, ty);
+            if (arrival_negated(net->md.model, ty)) qv = -qv;
             const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 24) |
                                   (((uint32_t)blk & 0x7Fu) << 25);
             seg[(size_t)cnt[(size_t)blk]++] = make_uint2(meta, (uint32_t)qv);
@@ -936,9 +940,8 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
             const uint32_t e = net->h_syn[(size_t)j];
             const int32_t post = syn_post(e), ty = syn_type(e), qb = post / Pr;
             int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
-            int32_t ch = 0;
-            if (net->md.model == NBX_MODEL_IZH) ch = (ty == 1) ? 1 : 0;
-            else if (net->md.model != NBX_MODEL_IZH_CUR && ty == 1) qv = -qv;
+            const int32_t ch = arrival_channel(net->md.model, ty);
+            if (arrival_negated(net->md.model, ty)) qv = -qv;
             const uint2 en = make_uint2((uint32_t)(ch * Pr + post - qb * Pr) | ((uint32_t)(syn_delay(e) - 1) << 16),
                                         (uint32_t)qv);
             const int32_t k = fill[(size_t)qb]++;
@@ -1048,7 +1051,7 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
         // a 16-entry slot suits mean segments of ~3..10 entries (longer ones overflow often,
         // shorter ones waste most of the 128 bytes); otherwise the offset layout is used
         const bool slots = slots_req >= 0 ? slots_req != 0 : (mean_seg >= 3.0 && mean_seg <= 10.5);
-        if (!slots && L.rb % m != 0) continue;
+        if (!slots && (L.rb % m != 0 || net->nch > 2)) continue;  // offset entries carry one channel bit
         const int64_t words = (int64_t)net->nch * m * L.P;
         if (slots && words > 65536) continue;  // a slot entry holds its LDS word in 16 bits
         const int64_t stage_bytes = slots ? 8 : 16;  // per staged spike, two buffers

@@ -28,13 +28,9 @@ __device__ inline int32_t step_of(const int32_t* tbase, int32_t k) { return tbas
 // Signed fixed-point contribution of one synapse and its ring channel.
 __device__ inline void syn_effect(const StepArgs& A, uint32_t e, float w, int32_t& ch, int32_t& q) {
     q = quantize(w, A.scale);
-    ch = 0;
     const int32_t type = syn_type(e);
-    if (A.P.model == NBX_MODEL_IZH) {
-        ch = (type == 1) ? 1 : 0;
-    } else if (A.P.model != NBX_MODEL_IZH_CUR) {
-        if (type == 1) q = -q;
-    }
+    ch = arrival_channel(A.P.model, type);
+    if (arrival_negated(A.P.model, type)) q = -q;
 }
 
 // Fan out one presynaptic row [b, e) into the delay ring of step t. Loads are issued
@@ -73,21 +69,25 @@ __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const i
         int32_t* r = A.ring + (int64_t)slot * A.nch * A.n;
         const int32_t a0 = r[i];
         r[i] = 0;
-        int32_t a1 = 0;
+        int32_t a1 = 0, a2 = 0;
         if (A.nch > 1) {
             a1 = r[A.n + i];
             r[A.n + i] = 0;
         }
+        if (A.nch > 2) {
+            a2 = r[2 * A.n + i];
+            r[2 * A.n + i] = 0;
+        }
         float V = A.v[i];
-        float S1 = (MODEL != NBX_MODEL_LIF_VJ) ? A.s1[i] : 0.0f;
-        float GA = (MODEL == NBX_MODEL_IZH) ? A.ga[i] : 0.0f;
-        float GG = (MODEL == NBX_MODEL_IZH) ? A.gg[i] : 0.0f;
+        float S1 = model_uses_s1(MODEL) ? A.s1[i] : 0.0f;
+        float GA = model_uses_g2(MODEL) ? A.ga[i] : 0.0f;
+        float GG = model_uses_g2(MODEL) ? A.gg[i] : 0.0f;
         float I = A.iext[i];
         int32_t last = A.last[i];
-        fired = neuron_tick<MODEL>(A, i, t, a0, a1, V, S1, GA, GG, I, last);
+        fired = neuron_tick<MODEL>(A, i, t, a0, a1, a2, V, S1, GA, GG, I, last);
         A.v[i] = V;
-        if (MODEL != NBX_MODEL_LIF_VJ) A.s1[i] = S1;
-        if (MODEL == NBX_MODEL_IZH) {
+        if (model_uses_s1(MODEL)) A.s1[i] = S1;
+        if (model_uses_g2(MODEL)) {
             A.ga[i] = GA;
             A.gg[i] = GG;
         }
@@ -161,7 +161,9 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_dense(StepArgs A,
     int32_t pos = 0, neg = 0;
     for (int32_t s = 0; s < cnt; ++s) {
         const float w = A.wt[(int64_t)ids[s] * A.n + i];
-        if (A.P.model == NBX_MODEL_IZH && w < 0.0f) neg += quantize(-w, A.scale);
+        // a negative dense weight is an inhibitory synapse of magnitude -w; models with one signed
+        // channel take the signed value directly (rint is symmetric, so quantize(w) == -quantize(-w))
+        if (A.nch > 1 && w < 0.0f) neg += quantize(-w, A.scale);
         else pos += quantize(w, A.scale);
     }
     const int32_t dslot = (t + A.dense_delay) % A.D;
@@ -293,6 +295,16 @@ hipError_t launch_update(const StepArgs& A, const int32_t* tbase, int32_t k, hip
         case NBX_MODEL_LIF_VJ: nbx_update<NBX_MODEL_LIF_VJ><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
         case NBX_MODEL_LIF_EXP: nbx_update<NBX_MODEL_LIF_EXP><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
         case NBX_MODEL_IZH: nbx_update<NBX_MODEL_IZH><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+        case NBX_MODEL_LIF_EXP2:
+            nbx_update<NBX_MODEL_LIF_EXP2><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k);
+            break;
+        case NBX_MODEL_LIF_EXP3:
+            nbx_update<NBX_MODEL_LIF_EXP3><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k);
+            break;
+        case NBX_MODEL_POISSON: nbx_update<NBX_MODEL_POISSON><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
+        case NBX_MODEL_TRIGGER_RELAY:
+            nbx_update<NBX_MODEL_TRIGGER_RELAY><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k);
+            break;
         default: nbx_update<NBX_MODEL_IZH_CUR><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
     }
     return hipGetLastError();

@@ -50,6 +50,11 @@ class ModelSpec:
     e_gaba: float = -80.0
     ampa_tau: float = 5.0
     gaba_tau: float = 10.0
+    # triple-exponential LIF NMDA conductance (TimeDrivenLIF_Exponential_triple.h:17)
+    nmda_tau: float = 10.0
+    # PoissonRate (PoissonRate.h:12-13)
+    rate_bias_hz: float = 0.0
+    rate_gain_hz: float = 1.0
     a: np.ndarray | None = None
     b: np.ndarray | None = None
     c: np.ndarray | None = None
@@ -67,11 +72,16 @@ class ModelSpec:
             g_tau=self.g_tau, e_rev=self.e_rev, izh_v_th=self.izh_v_th, izh_r=self.izh_r,
             e_ampa=self.e_ampa, e_gaba=self.e_gaba, ampa_tau=self.ampa_tau, gaba_tau=self.gaba_tau,
             izh_a=_p(arrs[0], C.c_float), izh_b=_p(arrs[1], C.c_float),
-            izh_c=_p(arrs[2], C.c_float), izh_d=_p(arrs[3], C.c_float))
+            izh_c=_p(arrs[2], C.c_float), izh_d=_p(arrs[3], C.c_float),
+            nmda_tau=self.nmda_tau, rate_bias_hz=self.rate_bias_hz, rate_gain_hz=self.rate_gain_hz)
 
     @property
     def nstate(self) -> int:
-        return {L.LIF_VJ: 2, L.LIF_EXP: 3}.get(self.model, 6)
+        return L.model_nstate(self.model)
+
+    @property
+    def channels(self) -> int:
+        return L.model_channels(self.model)
 
 
 @dataclass
@@ -284,8 +294,7 @@ class Net:
         tp[1:] = np.cumsum([len(s) for s in spikes_per_step])
         ids = np.concatenate([np.asarray(s, dtype=np.int32) for s in spikes_per_step]) if tp[-1] else np.zeros(1, np.int32)
         ids = np.ascontiguousarray(ids, dtype=np.int32)
-        ch = 2 if self.spec.model == L.IZH else 1
-        out = np.zeros((K, ch, self.n), dtype=np.int32)
+        out = np.zeros((K, self.spec.channels, self.n), dtype=np.int32)
         L.call("nbx_replay_deliver", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), K, _p(out, C.c_int32))
         return out
 

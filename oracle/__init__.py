@@ -17,7 +17,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 
-LIF_VJ, LIF_EXP, IZH, IZH_CUR = 0, 1, 2, 3
+LIF_VJ, LIF_EXP, IZH, IZH_CUR, LIF_EXP2, LIF_EXP3, POISSON, TRIGGER_RELAY = 0, 1, 2, 3, 4, 5, 6, 7
+COND_EXC, COND_INH, COND_NMDA = 1, 2, 4
 EULER, RK4 = 0, 1
 ACC_FIXED, ACC_FLOAT_EVENTS = 0, 1
 
@@ -39,6 +40,8 @@ class OrcModel(C.Structure):
         ("ampa_tau", C.c_float), ("gaba_tau", C.c_float),
         ("a", _f32p), ("b", _f32p), ("c", _f32p), ("d", _f32p),
         ("g_decay", C.c_float), ("ampa_decay", C.c_float), ("gaba_decay", C.c_float),
+        ("nmda_tau", C.c_float), ("rate_bias", C.c_float), ("rate_gain", C.c_float),
+        ("nmda_decay", C.c_float), ("cond", C.c_int32), ("step", C.c_int32),
     ]
 
 

@@ -28,9 +28,26 @@
 
 int orc_nstate(int model) {
     switch (model) {
-        case ORC_LIF_VJ: return 2;  /* V, I_ext            (TimeDrivenLIF_Voltage_jump.h:34) */
-        case ORC_LIF_EXP: return 3; /* V, g, I_ext         (TimeDrivenLIF_Exponential_Decay.h) */
-        default: return 6;          /* v, u, g_ampa, g_gaba, g_nmda, I_ext (Izhikevic .h) */
+        case ORC_LIF_VJ: return 2;        /* V, I_ext            (TimeDrivenLIF_Voltage_jump.h:34) */
+        case ORC_LIF_EXP: return 3;       /* V, g, I_ext         (TimeDrivenLIF_Exponential_Decay.h) */
+        case ORC_LIF_EXP2: return 4;      /* V, gexc, ginh, I_ext (TimeDrivenLIF_Exponential_double.h:27-33) */
+        case ORC_LIF_EXP3: return 5;      /* V, ampa, gaba, nmda, I_ext (..._triple.h:19-25) */
+        case ORC_POISSON: return 2;       /* rate_hz, I_ext      (PoissonRate.h:14-16) */
+        case ORC_TRIGGER_RELAY: return 1; /* last excitatory input (DenseTriggerRelayModel.h) */
+        default: return 6;                /* v, u, g_ampa, g_gaba, g_nmda, I_ext (Izhikevic .h) */
+    }
+}
+
+/* column of I_ext in the AoS state, -1 when the model has none */
+static int orc_iext_index(int model) {
+    switch (model) {
+        case ORC_LIF_VJ: return 1;
+        case ORC_LIF_EXP: return 2;
+        case ORC_LIF_EXP2: return 3;
+        case ORC_LIF_EXP3: return 4;
+        case ORC_POISSON: return 1;
+        case ORC_TRIGGER_RELAY: return -1;
+        default: return 5;
     }
 }
 
@@ -45,6 +62,25 @@ void orc_model_prepare(orc_model* m) {
     m->g_decay = ref_decay(m->dt, m->g_tau);
     m->ampa_decay = ref_decay(m->dt, m->ampa_tau);
     m->gaba_decay = ref_decay(m->dt, m->gaba_tau);
+    m->nmda_decay = 1.0f;
+    if (m->model == ORC_LIF_EXP3) { /* TimeDrivenLIF_Exponential_triple.cpp:56-60: exp(-dt / tau) */
+        m->ampa_decay = expf(-m->dt / m->ampa_tau);
+        m->gaba_decay = expf(-m->dt / m->gaba_tau);
+        m->nmda_decay = expf(-m->dt / m->nmda_tau);
+    }
+}
+
+/* Ring channel of an arrival and whether it enters negated (each model's ProcessSpike):
+ *   LIF_VJ V += w / -= w; LIF_EXP g += w / -= w; PoissonRate rate += w / -= w (PoissonRate.cpp:80-88)
+ *   IZH g_ampa / g_gaba; LIF_EXP2 gexc / ginh (..._double.cpp:105-113); TRIGGER_RELAY exc / (unused) inh
+ *   LIF_EXP3 ampa / gaba / nmda by type (..._triple.cpp:142-155); IZH_CUR signed current */
+static int orc_channel(int model, int type) {
+    if (model == ORC_LIF_EXP3) return type;
+    if (model == ORC_IZH || model == ORC_LIF_EXP2 || model == ORC_TRIGGER_RELAY) return type == 1 ? 1 : 0;
+    return 0;
+}
+static int orc_negated(int model, int type) {
+    return type == 1 && (model == ORC_LIF_VJ || model == ORC_LIF_EXP || model == ORC_POISSON);
 }
 
 #define REAL float
@@ -141,6 +177,12 @@ void orc_init_state(const orc_model* m, float* x, int32_t* last) {
             s[0] = m->v_reset; /* TimeDrivenLIF_Voltage_jump.cpp:37 */
         } else if (m->model == ORC_LIF_EXP) {
             s[0] = m->v_rest; /* TimeDrivenLIF_Exponential_Decay.cpp:51 */
+        } else if (m->model == ORC_LIF_EXP2) {
+            s[0] = m->v_rest; /* TimeDrivenLIF_Exponential_double.cpp:51 */
+        } else if (m->model == ORC_LIF_EXP3) {
+            s[0] = m->v_reset; /* TimeDrivenLIF_Exponential_triple.cpp:71 */
+        } else if (m->model == ORC_POISSON || m->model == ORC_TRIGGER_RELAY) {
+            /* all zero (PoissonRate.cpp:45) */
         } else {
             s[0] = m->v_rest;            /* V_rest + init[0], Izhikevic .cpp:49 */
             s[1] = m->b[i] * m->v_rest;  /* b * V_rest + init[1] */
@@ -158,6 +200,7 @@ typedef struct {
     int32_t* last;
     int32_t* acc0;
     int32_t* acc1;
+    int32_t* acc2;
     float* isyn;
     float scale;
     int ns;
@@ -172,16 +215,10 @@ static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
     const orc_model* m = &c->net->m;
     float* s = c->x + (size_t)post * c->ns;
     if (c->acc_mode == ORC_ACC_FIXED) {
-        const int32_t q = (int32_t)lrintf(w * c->scale);
-        switch (m->model) {
-            case ORC_LIF_VJ:
-            case ORC_LIF_EXP: c->acc0[post] += (type == 1) ? -q : q; break;
-            case ORC_IZH:
-                if (type == 1) c->acc1[post] += q;
-                else c->acc0[post] += q;
-                break;
-            default: c->acc0[post] += q; break;
-        }
+        int32_t q = (int32_t)lrintf(w * c->scale);
+        if (orc_negated(m->model, type)) q = -q;
+        int32_t* acc[3] = {c->acc0, c->acc1, c->acc2};
+        acc[orc_channel(m->model, type)][post] += q;
         return;
     }
     switch (m->model) {
@@ -192,6 +229,12 @@ static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
         case ORC_IZH:
             if (type == 1) s[3] += w;
             else s[2] += w;
+            break;
+        case ORC_LIF_EXP2: s[1 + (type == 1)] += w; break;  /* ..._double.cpp:105-113 */
+        case ORC_LIF_EXP3: s[1 + type] += w; break;         /* ..._triple.cpp:142-155 */
+        case ORC_POISSON: s[0] += (type == 1) ? -w : w; break;
+        case ORC_TRIGGER_RELAY:
+            if (type != 1) c->isyn[post] += w; /* this step's excitatory input */
             break;
         default: c->isyn[post] += w; break;
     }
@@ -213,6 +256,16 @@ static void apply_fixed(run_ctx* c) {
                 s[2] += a0;
                 s[3] += (float)c->acc1[i] * inv;
                 break;
+            case ORC_LIF_EXP2:
+                s[1] += a0;
+                s[2] += (float)c->acc1[i] * inv;
+                break;
+            case ORC_LIF_EXP3:
+                s[1] += a0;
+                s[2] += (float)c->acc1[i] * inv;
+                s[3] += (float)c->acc2[i] * inv;
+                break;
+            case ORC_POISSON: s[0] += a0; break;
             default: c->isyn[i] = a0; break;
         }
     }
@@ -248,6 +301,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     c.scale = ldexpf(1.0f, net->fixed_shift);
     c.acc0 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
     c.acc1 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    c.acc2 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
     c.isyn = (float*)calloc((size_t)n, sizeof(float));
     int32_t* spikes = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     uint32_t tab[64];
@@ -255,16 +309,24 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     if (net->pois_on) kmax = orc_poisson_thresholds(net->pois_lambda, tab, 64);
     const float inv_ltp = (float)(1.0 / (double)net->ltp_tau); /* STDP_State.cpp:6-7 */
     const float inv_ltd = (float)(1.0 / (double)net->ltd_tau);
-    const int iidx = (m->model == ORC_LIF_VJ) ? 1 : (m->model == ORC_LIF_EXP ? 2 : 5);
+    const int iidx = orc_iext_index(m->model);
     int64_t ev = 0;
     int32_t status = 0;
+    /* conductance inputs present (CheckType): synapse types, dense weight signs, background */
+    m->cond = net->pois_on ? ORC_COND_EXC : 0;
+    if (net->row_ptr)
+        for (int64_t s = 0; s < net->row_ptr[n]; ++s) m->cond |= 1 << net->type[s];
+    if (net->dense_wt)
+        for (size_t k = 0; k < (size_t)n * n; ++k) m->cond |= net->dense_wt[k] < 0.0f ? ORC_COND_INH : ORC_COND_EXC;
+    m->cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
 
     for (int32_t t = t0; t < t0 + nticks; ++t) {
         memset(c.acc0, 0, sizeof(int32_t) * (size_t)n);
         memset(c.acc1, 0, sizeof(int32_t) * (size_t)n);
+        memset(c.acc2, 0, sizeof(int32_t) * (size_t)n);
         memset(c.isyn, 0, sizeof(float) * (size_t)n);
         /* external current replaces I_ext for the tick (stateful current input) */
-        if (net->cur) {
+        if (net->cur && iidx >= 0) {
             const float* cr = net->cur + (size_t)(t - net->cur_t_begin) * n;
             for (int i = 0; i < n; ++i) x[(size_t)i * c.ns + iidx] = cr[i];
         }
@@ -307,7 +369,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 for (int i = 0; i < n; ++i) {
                     float w = col[i];
                     int32_t type = 0;
-                    if (m->model != ORC_IZH_CUR && w < 0.0f) {
+                    if (m->model != ORC_IZH_CUR && w < 0.0f) { /* a negative weight is inhibitory, magnitude -w */
                         type = 1;
                         w = -w;
                     }
@@ -317,11 +379,12 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
         }
         /* push-ring image: must hold exactly the pull result for this tick */
         if (ring && acc_mode == ORC_ACC_FIXED) {
-            int32_t* r = ring + (size_t)(t % D) * 2 * n;
+            int32_t* r = ring + (size_t)(t % D) * 3 * n;
             for (int i = 0; i < n; ++i) {
-                if (r[i] != c.acc0[i] || r[n + i] != c.acc1[i]) status = -2;
+                if (r[i] != c.acc0[i] || r[n + i] != c.acc1[i] || r[2 * n + i] != c.acc2[i]) status = -2;
                 r[i] = 0;
                 r[n + i] = 0;
+                r[2 * n + i] = 0;
             }
         }
         /* Poisson background: count excitatory arrivals of weight w_bg */
@@ -333,7 +396,9 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
         }
         if (acc_mode == ORC_ACC_FIXED) apply_fixed(&c);
 
-        const int32_t nspk = update_f32(m, x, last, m->model == ORC_IZH_CUR ? c.isyn : NULL, spikes);
+        m->step = t;
+        const int32_t nspk = update_f32(
+            m, x, last, (m->model == ORC_IZH_CUR || m->model == ORC_TRIGGER_RELAY) ? c.isyn : NULL, spikes);
 
         /* post-synaptic learning at the spike tick (TimeDrivenInternalSpike.cpp:70-82, STDP.cpp:37-61) */
         if (net->stdp_on && net->in_ptr) {
@@ -354,27 +419,24 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 const int32_t pre = spikes[k];
                 if (net->row_ptr) {
                     for (int64_t s = net->row_ptr[pre]; s < net->row_ptr[pre + 1]; ++s) {
-                        int32_t* r = ring + (size_t)((t + net->delay[s]) % D) * 2 * n;
-                        const int32_t q = (int32_t)lrintf(net->w[s] * c.scale);
-                        const int32_t p = net->post[s];
-                        if (m->model == ORC_IZH) {
-                            if (net->type[s] == 1) r[n + p] += q;
-                            else r[p] += q;
-                        } else if (m->model == ORC_IZH_CUR) {
-                            r[p] += q;
-                        } else {
-                            r[p] += (net->type[s] == 1) ? -q : q;
-                        }
+                        int32_t* r = ring + (size_t)((t + net->delay[s]) % D) * 3 * n;
+                        int32_t q = (int32_t)lrintf(net->w[s] * c.scale);
+                        if (orc_negated(m->model, net->type[s])) q = -q;
+                        r[(size_t)orc_channel(m->model, net->type[s]) * n + net->post[s]] += q;
                     }
                 }
                 if (net->dense_wt) {
-                    int32_t* r = ring + (size_t)((t + net->dense_delay) % D) * 2 * n;
+                    int32_t* r = ring + (size_t)((t + net->dense_delay) % D) * 3 * n;
                     const float* col = net->dense_wt + (size_t)pre * n;
                     for (int i = 0; i < n; ++i) {
-                        float w = col[i];
-                        if (m->model == ORC_IZH && w < 0.0f) r[n + i] += (int32_t)lrintf(-w * c.scale);
-                        else if (m->model == ORC_IZH_CUR || m->model == ORC_IZH) r[i] += (int32_t)lrintf(w * c.scale);
-                        else r[i] += (w < 0.0f) ? -(int32_t)lrintf(-w * c.scale) : (int32_t)lrintf(w * c.scale);
+                        const float w = col[i];
+                        if (m->model != ORC_IZH_CUR && w < 0.0f) {
+                            int32_t q = (int32_t)lrintf(-w * c.scale);
+                            if (orc_negated(m->model, 1)) q = -q;
+                            r[(size_t)orc_channel(m->model, 1) * n + i] += q;
+                        } else {
+                            r[i] += (int32_t)lrintf(w * c.scale);
+                        }
                     }
                 }
             }
@@ -397,6 +459,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     if (events) *events += ev;
     free(c.acc0);
     free(c.acc1);
+    free(c.acc2);
     free(c.isyn);
     free(spikes);
     return status;

@@ -13,7 +13,10 @@
  *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Voltage_jump.cpp:49-115
  *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Exponential_Decay.cpp:37-121
  *   L/NeuralModel/src/TimeDriven/TimeDrivenIzhikevic_Exponential_Decay.cpp:32-131
- *   L/NeuralModel/src/TimeDriven/PoissonRate.cpp:7-21
+ *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Exponential_double.cpp:36-123
+ *   L/NeuralModel/src/TimeDriven/TimeDrivenLIF_Exponential_triple.cpp:56-155
+ *   L/NeuralModel/src/TimeDriven/PoissonRate.cpp:7-88
+ *   src/native/gpu/include/dense_subnetwork/model/DenseTriggerRelayDeviceUpdate.cuh:10-32
  *   L/NeuralModel/src/NeuralStateVector/Neuron_State_Vector.cpp:43-67
  *   L/Event/inc/Event.h:10 (deliveries before updates within a tick)
  *   L/Event/src/Spike/PropogatedSpikeGroup.cpp:33-71 (ProcessSpike, then pre-learning)
@@ -31,7 +34,18 @@
 extern "C" {
 #endif
 
-enum { ORC_LIF_VJ = 0, ORC_LIF_EXP = 1, ORC_IZH = 2, ORC_IZH_CUR = 3 };
+enum {
+    ORC_LIF_VJ = 0,
+    ORC_LIF_EXP = 1,
+    ORC_IZH = 2,
+    ORC_IZH_CUR = 3,
+    ORC_LIF_EXP2 = 4,     /* excitatory + inhibitory conductance LIF (state V, gexc, ginh, I) */
+    ORC_LIF_EXP3 = 5,     /* AMPA + GABA + NMDA conductance LIF (state V, ampa, gaba, nmda, I) */
+    ORC_POISSON = 6,      /* PoissonRate (state rate, I) */
+    ORC_TRIGGER_RELAY = 7 /* fires when the step's excitatory input exceeds v_th (state last input) */
+};
+/* conductance inputs present in a network: the reference models' CheckType flags */
+enum { ORC_COND_EXC = 1, ORC_COND_INH = 2, ORC_COND_NMDA = 4 };
 enum { ORC_EULER = 0, ORC_RK4 = 1 };
 enum { ORC_ACC_FIXED = 0, ORC_ACC_FLOAT_EVENTS = 1 };
 
@@ -49,6 +63,12 @@ typedef struct orc_model {
     const float *a, *b, *c, *d;
     /* filled by orc_model_prepare: exp(-dt/tau) factors computed the reference way */
     float g_decay, ampa_decay, gaba_decay;
+    /* ORC_LIF_EXP2 uses e_ampa / ampa_tau (Eexc, gexc_tau) and e_gaba / gaba_tau (Einh, ginh_tau) */
+    float nmda_tau;               /* ORC_LIF_EXP3 */
+    float rate_bias, rate_gain;   /* ORC_POISSON */
+    float nmda_decay;             /* filled by orc_model_prepare */
+    int32_t cond;                 /* ORC_COND_* bits (orc_run derives them from the network) */
+    int32_t step;                 /* tick being updated (the PoissonRate hash input; orc_run sets it) */
 } orc_model;
 
 typedef struct orc_net {

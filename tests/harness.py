@@ -9,7 +9,8 @@ import numpy as np
 import oracle as O
 from neuronbridge_amd import _lib as L
 
-MODEL_MAP = {L.LIF_VJ: O.LIF_VJ, L.LIF_EXP: O.LIF_EXP, L.IZH: O.IZH, L.IZH_CUR: O.IZH_CUR}
+MODEL_MAP = {L.LIF_VJ: O.LIF_VJ, L.LIF_EXP: O.LIF_EXP, L.IZH: O.IZH, L.IZH_CUR: O.IZH_CUR,
+             L.LIF_EXP2: O.LIF_EXP2, L.LIF_EXP3: O.LIF_EXP3, L.POISSON: O.POISSON, L.TRIGGER_RELAY: O.TRIGGER_RELAY}
 
 
 def auto_shift(wl) -> int:
@@ -42,10 +43,12 @@ def initial_state(wl):
     x = np.zeros((n, ns), dtype=np.float32)
     last = np.full(n, 10000, dtype=np.int32)
     sp = wl.spec
-    if sp.model == L.LIF_VJ:
+    if sp.model in (L.LIF_VJ, L.LIF_EXP3):
         x[:, 0] = sp.v_reset
-    elif sp.model == L.LIF_EXP:
+    elif sp.model in (L.LIF_EXP, L.LIF_EXP2):
         x[:, 0] = sp.v_rest
+    elif sp.model in (L.POISSON, L.TRIGGER_RELAY):
+        pass
     else:
         x[:, 0] = sp.v_rest
         x[:, 1] = (np.asarray(sp.b, np.float32) * np.float32(sp.v_rest)).astype(np.float32)
@@ -63,7 +66,8 @@ class OracleRun:
         m = O.OrcModel(model=MODEL_MAP[sp.model], integrator=sp.integrator, n=sp.n, t_ref=sp.t_ref, dt=sp.dt,
                        v_rest=sp.v_rest, tau=sp.tau, v_th=sp.v_th, r=sp.r, v_reset=sp.v_reset, g_tau=sp.g_tau,
                        e_rev=sp.e_rev, izh_v_th=sp.izh_v_th, izh_r=sp.izh_r, e_ampa=sp.e_ampa, e_gaba=sp.e_gaba,
-                       ampa_tau=sp.ampa_tau, gaba_tau=sp.gaba_tau)
+                       ampa_tau=sp.ampa_tau, gaba_tau=sp.gaba_tau, nmda_tau=sp.nmda_tau,
+                       rate_bias=sp.rate_bias_hz, rate_gain=sp.rate_gain_hz)
         if sp.a is not None:
             for name in "abcd":
                 arr = np.ascontiguousarray(getattr(sp, name), dtype=np.float32)
@@ -126,7 +130,7 @@ class OracleRun:
         D, n = wl.d_slots, sp.n
         self.hist = np.zeros((D, n), dtype=np.int32)
         self.hist_cnt = np.zeros(D, dtype=np.int32)
-        self.ring = np.zeros((D, 2, n), dtype=np.int32)
+        self.ring = np.zeros((D, 3, n), dtype=np.int32)
         self.t = 0
         self.events = np.zeros(1, dtype=np.int64)
 

@@ -1,0 +1,109 @@
+"""GPU parity for the remaining legacy models (double- and triple-exponential conductance LIF,
+PoissonRate, trigger relay) on every engine, in replay delivery, with dense signed weights and
+from .nbnet tables. Bar: bit-exact against the oracle (state bits, LastSpike, rasters, ring)."""
+import numpy as np
+import pytest
+
+from harness import OracleRun
+from nbnet_file import records, write_nbnet
+from neuronbridge_amd import _lib as L
+from neuronbridge_amd import workloads as W
+from test_gpu_parity import ENGINES, assert_bits_equal, compare, random_spikes, replay_expect
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("lif2-euler", L.LIF_EXP2, L.EULER),
+    ("lif2-rk4", L.LIF_EXP2, L.RK4),
+    ("lif3-euler", L.LIF_EXP3, L.EULER),
+    ("lif3-rk4", L.LIF_EXP3, L.RK4),
+    ("poisson", L.POISSON, L.EULER),
+    ("relay", L.TRIGGER_RELAY, L.EULER),
+]
+
+
+def engines_for(model):
+    # the persistent engine's blocked entries carry one channel bit: two channels at most
+    return [e for e in ENGINES if not (e == "persistent" and model == L.LIF_EXP3)]
+
+
+@pytest.mark.parametrize("case,model,integrator", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_model_closed_loop_bitexact(gpu, case, model, integrator, engine):
+    if engine not in engines_for(model):
+        pytest.skip("three arrival channels run on the kernel and graph engines")
+    wl = W.model_sparse(model, integrator=integrator)
+    T = 300
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, ENGINES[engine])
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert 200 < counts_o.sum() < 0.5 * T * wl.spec.n, "workload too quiet or saturated to test delivery"
+    compare(net, run, wl, counts_o, raster_o)
+
+
+def test_three_channel_network_refuses_persistent_engine(gpu):
+    wl = W.model_sparse(L.LIF_EXP3, n=600, k_in=30)
+    net = wl.make_net()
+    with pytest.raises(L.NbxError) as e:
+        net.run(5, L.ENGINE_PERSISTENT)
+    assert e.value.status == L.NBX_ERR_UNSUPPORTED
+    assert net.pick_engine() != L.ENGINE_PERSISTENT
+    net.run(5)  # AUTO picks an engine that runs it
+
+
+@pytest.mark.parametrize("model", [L.LIF_EXP2, L.LIF_EXP3, L.POISSON], ids=["lif2", "lif3", "poisson"])
+def test_model_replay_matches_brute_force(gpu, model):
+    wl = W.model_sparse(model, n=2500, k_in=80, seed=8)
+    net = wl.make_net()
+    rng = np.random.default_rng(11)
+    K = 64
+    spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
+    e0 = net.events()
+    out = net.replay(spikes)
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
+    assert out.shape[1] == wl.spec.channels
+    assert_bits_equal(out.astype(np.int64), expect, "replay arrivals")
+    assert net.events() - e0 == events
+
+
+@pytest.mark.parametrize("model", [L.LIF_EXP2, L.LIF_EXP3], ids=["lif2", "lif3"])
+def test_dense_signed_weights(gpu, model):
+    # negative dense weights are inhibitory conductance inputs on their own channel
+    n, T = 400, 400
+    wl = W.model_sparse(model, n=n, k_in=10, seed=5)
+    rng = np.random.default_rng(2)
+    wt = rng.uniform(0.0, 0.004, (n, n)).astype(np.float32)
+    wt[int(0.8 * n):] = -rng.uniform(0.0, 0.01, (n - int(0.8 * n), n)).astype(np.float32)
+    wl.syn, wl.dense_wt, wl.dense_delay = None, wt, 2
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 100
+    compare(net, run, wl, counts_o, raster_o)
+
+
+def test_nbnet_nmda_synapses(gpu, tmp_path):
+    wl = W.model_sparse(L.LIF_EXP3, n=1500, k_in=60, seed=12)
+    n = wl.spec.n
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(wl.syn.row_ptr))
+    recs = records(pre, wl.syn.post, wl.syn.type, wl.syn.weight, wl.syn.delay)
+    recs = recs[np.random.default_rng(1).permutation(len(recs))]
+    path = str(tmp_path / "nmda.nbnet")
+    write_nbnet(path, {"layers": []}, recs)
+    assert (recs["type"] == 2).sum() > 1000
+    net = wl.make_net(nbnet=path)
+    T = 250
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o)
+    # a two-channel model has no NMDA input
+    wl2 = W.model_sparse(L.LIF_EXP2, n=1500, k_in=60, seed=12)
+    with pytest.raises(L.NbxError) as e:
+        wl2.make_net(nbnet=path)
+    assert e.value.status == L.NBX_ERR_UNSUPPORTED

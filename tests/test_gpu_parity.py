@@ -39,8 +39,7 @@ def compare(net, run, wl, counts_o, raster_o, ring=True):
     assert_bits_equal(net.raster(), raster_o, "raster")
     assert net.events() == int(run.events[0]), "synaptic events"
     if ring:
-        nch = 2 if wl.spec.model == L.IZH else 1
-        assert_bits_equal(net.ring(), run.ring_as_device(nch), "delay ring")
+        assert_bits_equal(net.ring(), run.ring_as_device(wl.spec.channels), "delay ring")
 
 
 def small_sparse(model, integrator, n=2000, k_in=100, seed=21):
@@ -182,16 +181,23 @@ def test_per_phase_delay_known_answer(gpu):
 
 def replay_expect(wl, fixed_shift, spikes):
     """Brute force: every synapse of every injected spike adds its fixed-point weight at step
-    k + delay - 1, on channel 1 for inhibitory synapses of conductance Izhikevich, negated for
-    inhibitory synapses of the LIF models."""
+    k + delay - 1; inhibitory synapses go to channel 1 in the two-channel models (conductance
+    Izhikevich, double-exponential LIF, trigger relay), every type to its own channel in the
+    triple-exponential LIF, and enter negated in the one-channel LIF and Poisson-rate models."""
     s = wl.syn
     K = len(spikes)
     model = wl.spec.model
     q = np.rint(s.weight * np.float32(2.0 ** fixed_shift)).astype(np.int64)
-    if model in (L.LIF_VJ, L.LIF_EXP):
+    if model in (L.LIF_VJ, L.LIF_EXP, L.POISSON):
         q = np.where(s.type == 1, -q, q)
-    ch = s.type.astype(np.int64) if model == L.IZH else np.zeros(s.type.shape, np.int64)
-    expect = np.zeros((K, 2 if model == L.IZH else 1, wl.spec.n), np.int64)
+    nch = wl.spec.channels
+    if nch == 3:
+        ch = s.type.astype(np.int64)
+    elif nch == 2:
+        ch = (s.type == 1).astype(np.int64)
+    else:
+        ch = np.zeros(s.type.shape, np.int64)
+    expect = np.zeros((K, nch, wl.spec.n), np.int64)
     events = 0
     for k, ids in enumerate(spikes):
         if len(ids) == 0:
