@@ -92,6 +92,25 @@ typedef struct nbx_stdp_desc {
     float max_ltp, ltp_tau, max_ltd, ltd_tau, w_max;
 } nbx_stdp_desc;
 
+/* Trigger-driven learning rules of the legacy engine (LearningRule/src/Rule/):
+ *   NBX_RULE_RSTDP     R_STDP.cpp:19-70 — pair traces feed an eligibility (pre arrival:
+ *                      e += -max_ltd * Apost; post spike: e += Apre * max_ltp); a trigger arrival
+ *                      at the target adds factor * e to each plastic input's weight (factor =
+ *                      reward for excitatory, punishment for inhibitory trigger synapses) and
+ *                      clears e when clear_eligibility is set
+ *   NBX_RULE_ADDITIVE  AdditiveKernalChange.cpp:64-82 — pre arrival: w += a1pre, Apre decays with
+ *                      ltp_tau and jumps by 1; a trigger arrival adds a2prepre * Apre to each
+ *                      plastic input of its target
+ * Weights stay in [0, max_weight] of their synapse (Interconnections::UpdateWeight). */
+enum { NBX_RULE_RSTDP = 1, NBX_RULE_ADDITIVE = 2 };
+typedef struct nbx_rule_desc {
+    int32_t kind;                         /* NBX_RULE_* */
+    float max_ltp, ltp_tau, max_ltd, ltd_tau; /* R-STDP; ADDITIVE uses ltp_tau */
+    float reward, punishment;             /* R-STDP trigger factors (R_STDP.h:16-18: 1, -1) */
+    int32_t clear_eligibility;            /* R-STDP (R_STDP.h:20: true) */
+    float a1pre, a2prepre;                /* ADDITIVE (AdditiveKernalChange.h:8-10: 0.002, -0.08) */
+} nbx_rule_desc;
+
 /* Engines for nbx_net_run. */
 enum {
     NBX_ENGINE_KERNELS = 0,    /* one launch per phase per step */
@@ -117,6 +136,18 @@ int32_t nbx_net_set_dense(nbx_net* net, const float* wt, int32_t delay);
 int32_t nbx_net_set_poisson(nbx_net* net, uint32_t seed_lo, uint32_t seed_hi, double lambda, float w_bg);
 /* Enable pair STDP on the synapses flagged plastic. */
 int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* stdp);
+/* Trigger-driven rules (at most 64). In CSR order: rule[j] >= 0 makes synapse j plastic under
+ * rules[rule[j]], trigger[j] >= 0 makes it a trigger of rules[trigger[j]] (the .nbnet record's
+ * synapse_rule / trigger_rule; NbnetReader.cpp:505-509), max_weight[j] bounds its weight. A
+ * synapse is at most one of: STDP-plastic, rule-plastic, trigger. Within a step, every arrival's
+ * own update comes first, then each target applies its trigger arrivals in ascending synapse
+ * order, then the neurons update, then post-side updates run (Event.h:10 orders deliveries
+ * before updates). Before nbx_net_finalize. */
+int32_t nbx_net_set_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count, const int32_t* rule,
+                          const int32_t* trigger, const float* max_weight);
+/* Per-synapse rule state (CSR order; entries of other synapses read 0). */
+int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* eligibility,
+                               int32_t* last_update, int64_t count);
 /* External current per step: cur[(t - t_begin) * n + i] replaces I_ext at step t
  * (reference: ProcessCurrent / InterfaceChannelValue with overwrite). Host array, copied. */
 int32_t nbx_net_set_current(nbx_net* net, const float* cur, int32_t t_begin, int32_t nsteps);

@@ -52,6 +52,15 @@ class ModelDesc(C.Structure):
     ]
 
 
+RULE_RSTDP, RULE_ADDITIVE = 1, 2
+
+
+class RuleDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("max_ltp", C.c_float), ("ltp_tau", C.c_float), ("max_ltd", C.c_float),
+                ("ltd_tau", C.c_float), ("reward", C.c_float), ("punishment", C.c_float),
+                ("clear_eligibility", C.c_int32), ("a1pre", C.c_float), ("a2prepre", C.c_float)]
+
+
 def model_channels(model: int) -> int:
     """Arrival channels of a model's delay ring (libnbx's model_channels)."""
     return 3 if model == LIF_EXP3 else (2 if model in (IZH, LIF_EXP2, TRIGGER_RELAY) else 1)
@@ -159,6 +168,8 @@ SIGNATURES = {
     "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],
     "nbx_net_export_monitor": [_NET, C.POINTER(MonitorDesc)],
+    "nbx_net_set_rules": [_NET, C.POINTER(RuleDesc), C.c_int32, _i32p, _i32p, _f32p],
+    "nbx_net_get_rule_state": [_NET, _f32p, _f32p, _f32p, _i32p, C.c_int64],
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],

@@ -70,6 +70,84 @@ std::vector<float*> state_columns(nbx_net* net) {
     }
 }
 
+// Trace decay tables by integer step delta: exp(-(delta * dt) * (1/tau)) in float, the arithmetic
+// of STDP_State::SetUpdate (STDP_State.cpp:17-26); the last entry is 0 or covers every delta a
+// run can reach.
+void trace_tables(float dt, float tau_a, float tau_b, std::vector<float>* ta, std::vector<float>* tb) {
+    const float inv_a = (float)(1.0 / (double)tau_a);
+    const float inv_b = (float)(1.0 / (double)tau_b);
+    ta->clear();
+    tb->clear();
+    for (int32_t k = 0;; ++k) {
+        const float dT = (float)k * dt;
+        ta->push_back(std::exp(-dT * inv_a));
+        tb->push_back(std::exp(-dT * inv_b));
+        if ((ta->back() == 0.0f && tb->back() == 0.0f) || k >= (1 << 22)) break;
+    }
+}
+
+// Device tables of the trigger-driven rules: rule records with their trace tables, per-synapse
+// rule / trigger ids and weight bounds, rule-plastic inputs per (post, rule) and trigger synapses
+// per post, both in ascending synapse order.
+int32_t finalize_rules(nbx_net* net) {
+    const int32_t n = net->n, nr = (int32_t)net->rules.size();
+    const size_t S = (size_t)net->S;
+    std::vector<RuleDev> rd((size_t)nr);
+    std::vector<float> tabs;
+    for (int32_t r = 0; r < nr; ++r) {
+        const nbx_rule_desc& q = net->rules[(size_t)r];
+        RuleDev& d = rd[(size_t)r];
+        d.kind = q.kind;
+        d.clear = q.clear_eligibility ? 1 : 0;
+        d.max_ltp = q.max_ltp;
+        d.max_ltd = q.max_ltd;
+        d.reward = q.reward;
+        d.punishment = q.punishment;
+        d.a1pre = q.a1pre;
+        d.a2prepre = q.a2prepre;
+        std::vector<float> ta, tb;  // the additive rule's state decays both traces with ltp_tau
+        trace_tables(net->md.dt, q.ltp_tau, q.kind == NBX_RULE_ADDITIVE ? q.ltp_tau : q.ltd_tau, &ta, &tb);
+        d.tab_off = (int32_t)tabs.size();
+        d.tab_len = (int32_t)ta.size();
+        tabs.insert(tabs.end(), ta.begin(), ta.end());
+        tabs.insert(tabs.end(), tb.begin(), tb.end());
+        if (q.kind == NBX_RULE_RSTDP) net->rstdp = true;
+    }
+    NBX_PASS(net->upload(&net->d_rules, rd.data(), rd.size()));
+    NBX_PASS(net->upload(&net->d_rtab, tabs.data(), tabs.size()));
+    NBX_PASS(net->upload(&net->d_syn_rule, net->h_rule.data(), S));
+    NBX_PASS(net->upload(&net->d_syn_trig, net->h_trig.data(), S));
+    NBX_PASS(net->upload(&net->d_maxw, net->h_maxw.data(), S));
+    std::vector<float> zf(S, 0.0f);
+    NBX_PASS(net->upload(&net->d_elig, zf.data(), S));
+    std::vector<int32_t> stamp(std::max(S, (size_t)n), -1);
+    NBX_PASS(net->upload(&net->d_tstamp, stamp.data(), S));
+    NBX_PASS(net->upload(&net->d_tpost_stamp, stamp.data(), (size_t)n));
+    NBX_PASS(net->alloc(&net->d_tposts, (size_t)2 * n));
+    const int32_t zero2[2] = {0, 0};
+    NBX_PASS(net->upload(&net->d_tcount, zero2, 2));
+    std::vector<int64_t> rin_ptr((size_t)n * nr + 1, 0), tin_ptr((size_t)n + 1, 0);
+    for (size_t j = 0; j < S; ++j) {
+        const int32_t post = syn_post(net->h_syn[j]);
+        if (net->h_rule[j] >= 0) rin_ptr[(size_t)post * nr + net->h_rule[j] + 1]++;
+        if (net->h_trig[j] >= 0) tin_ptr[(size_t)post + 1]++;
+    }
+    for (size_t k = 1; k < rin_ptr.size(); ++k) rin_ptr[k] += rin_ptr[k - 1];
+    for (size_t k = 1; k < tin_ptr.size(); ++k) tin_ptr[k] += tin_ptr[k - 1];
+    std::vector<int64_t> rfill(rin_ptr.begin(), rin_ptr.end() - 1), tfill(tin_ptr.begin(), tin_ptr.end() - 1);
+    std::vector<int64_t> rin_syn((size_t)rin_ptr.back() + 1), tin_syn((size_t)tin_ptr.back() + 1);
+    for (size_t j = 0; j < S; ++j) {  // ascending j within every list
+        const int32_t post = syn_post(net->h_syn[j]);
+        if (net->h_rule[j] >= 0) rin_syn[(size_t)rfill[(size_t)post * nr + net->h_rule[j]]++] = (int64_t)j;
+        if (net->h_trig[j] >= 0) tin_syn[(size_t)tfill[(size_t)post]++] = (int64_t)j;
+    }
+    NBX_PASS(net->upload(&net->d_rin_ptr, rin_ptr.data(), rin_ptr.size()));
+    NBX_PASS(net->upload(&net->d_rin_syn, rin_syn.data(), rin_syn.size()));
+    NBX_PASS(net->upload(&net->d_tin_ptr, tin_ptr.data(), tin_ptr.size()));
+    NBX_PASS(net->upload(&net->d_tin_syn, tin_syn.data(), tin_syn.size()));
+    return NBX_OK;
+}
+
 int32_t poisson_table(double lambda, std::vector<uint32_t>* tab) {
     tab->clear();
     double p = std::exp(-lambda), cdf = 0.0;
@@ -118,9 +196,13 @@ hipError_t enqueue_step(nbx_net* net, const StepArgs& A, const int32_t* tbase, i
         e = launch_pull(A, tbase, k, net->stream);
         if (e != hipSuccess) return e;
     }
+    if (!net->rules.empty()) {
+        e = launch_trigger(A, tbase, k, net->stream);
+        if (e != hipSuccess) return e;
+    }
     e = launch_update(A, tbase, k, net->stream);
     if (e != hipSuccess) return e;
-    if (net->stdp && net->d_in_ptr) {
+    if ((net->stdp && net->d_in_ptr) || net->rstdp) {
         e = launch_ltp(A, tbase, k, net->stream);
         if (e != hipSuccess) return e;
     }
@@ -183,6 +265,8 @@ int32_t auto_shift(nbx_net* net) {
         for (int64_t j = 0; j < net->S; ++j) {
             double w = std::fabs((double)net->h_w[(size_t)j]);
             if (net->stdp && !net->h_plastic.empty() && net->h_plastic[(size_t)j]) w = std::max(w, (double)net->sd.w_max);
+            if (!net->h_rule.empty() && net->h_rule[(size_t)j] >= 0)
+                w = std::max(w, std::fabs((double)net->h_maxw[(size_t)j]));
             bound[(size_t)syn_post(net->h_syn[(size_t)j])] += w;
         }
     }
@@ -362,6 +446,44 @@ int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* sd) {
     return NBX_OK;
 }
 
+int32_t nbx_net_set_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count, const int32_t* rule,
+                          const int32_t* trigger, const float* max_weight) {
+    NBX_CHECK(net && (rules || count == 0), "null argument");
+    NBX_CHECK(!net->finalized, "learning rules must be set before nbx_net_finalize");
+    NBX_CHECK(net->has_sparse, "learning rules need sparse synapses (set them first)");
+    NBX_CHECK(count >= 0 && count <= kMaxRules, "rule count must be in [0, 64]");
+    NBX_CHECK(count == 0 || (rule && trigger && max_weight), "missing per-synapse rule arrays");
+    for (int32_t r = 0; r < count; ++r) {
+        const nbx_rule_desc& q = rules[r];
+        NBX_CHECK(q.kind == NBX_RULE_RSTDP || q.kind == NBX_RULE_ADDITIVE, "unknown learning rule kind");
+        NBX_CHECK(q.ltp_tau > 0.0f && (q.kind == NBX_RULE_ADDITIVE || q.ltd_tau > 0.0f),
+                  "learning rule time constants must be positive");
+    }
+    const size_t S = (size_t)net->S;
+    std::vector<int8_t> hr(S, -1), ht(S, -1);
+    std::vector<float> hm(S, 0.0f);
+    for (size_t j = 0; j < S && count > 0; ++j) {
+        NBX_CHECK(rule[j] >= -1 && rule[j] < count && trigger[j] >= -1 && trigger[j] < count,
+                  "synapse refers to an invalid learning rule");
+        NBX_CHECK(!(rule[j] >= 0 && trigger[j] >= 0), "a synapse is either rule-plastic or a trigger");
+        NBX_CHECK(std::isfinite(max_weight[j]), "max_weight must be finite");
+        hr[j] = (int8_t)rule[j];
+        ht[j] = (int8_t)trigger[j];
+        hm[j] = max_weight[j];
+    }
+    net->rules.assign(rules, rules + count);
+    if (count == 0) {
+        net->h_rule.clear();
+        net->h_trig.clear();
+        net->h_maxw.clear();
+    } else {
+        net->h_rule.swap(hr);
+        net->h_trig.swap(ht);
+        net->h_maxw.swap(hm);
+    }
+    return NBX_OK;
+}
+
 int32_t nbx_net_set_current(nbx_net* net, const float* cur, int32_t t_begin, int32_t nsteps) {
     NBX_CHECK(net && (cur || nsteps == 0), "null argument");
     NBX_CHECK(nsteps >= 0, "negative step count");
@@ -415,8 +537,14 @@ int32_t nbx_net_finalize(nbx_net* net) {
     // max_delay + 1 slots (the reference's delay_slot_count, DenseSubnetworkBuildFinalizer.cpp:62),
     // at least 3 so the slot being filled, the one being read and the one being freed differ
     net->D = std::max(net->max_delay + 1, 3);
-    net->pull_mode = net->stdp;
+    net->pull_mode = net->stdp || !net->rules.empty();
     NBX_CHECK(!(net->stdp && !net->has_sparse), "STDP needs sparse synapses");
+    NBX_CHECK(net->rules.empty() || (net->has_sparse && net->h_rule.size() == (size_t)net->S),
+              "learning rules must be set after the synapses they refer to");
+    if (!net->rules.empty() && net->stdp && !net->h_plastic.empty())
+        for (size_t j = 0; j < (size_t)net->S; ++j)
+            NBX_CHECK(!(net->h_plastic[j] && (net->h_rule[j] >= 0 || net->h_trig[j] >= 0)),
+                      "a synapse is either STDP-plastic, rule-plastic or a trigger");
     net->shift = net->shift_req ? net->shift_req : auto_shift(net);
     net->words = (n + 31) / 32;
     // conductance inputs present (the reference models' CheckType flags)
@@ -492,26 +620,20 @@ int32_t nbx_net_finalize(nbx_net* net) {
     if (net->has_dense) NBX_RC(net->upload(&net->d_wt, net->h_wt.data(), net->h_wt.size()));
     if (net->pois) NBX_RC(net->upload(&net->d_pois_tab, net->pois_tab.data(), net->pois_tab.size()));
     if (net->cur_n) NBX_RC(net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size()));
-    if (net->stdp) {
-        NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
+    if (net->stdp || !net->rules.empty()) {  // per-synapse pair traces (STDP_State), shared by both
         const size_t S = (size_t)net->S;
         std::vector<float> zf(S, 0.0f);
         std::vector<int32_t> zi(S, 0);
         NBX_RC(net->upload(&net->d_apre, zf.data(), S));
         NBX_RC(net->upload(&net->d_apost, zf.data(), S));
         NBX_RC(net->upload(&net->d_slast, zi.data(), S));
-        // decay tables by integer step delta: exp(-(delta * dt) * (1/tau)) in float, the
-        // arithmetic of STDP_State::SetUpdate (STDP_State.cpp:17-26); the last entry is 0
-        // or covers every delta a run can reach
-        const float inv_ltp = (float)(1.0 / (double)net->sd.ltp_tau);
-        const float inv_ltd = (float)(1.0 / (double)net->sd.ltd_tau);
+    }
+    if (!net->rules.empty()) NBX_RC(finalize_rules(net));
+    if (net->stdp) {
+        NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
+        const size_t S = (size_t)net->S;
         std::vector<float> tp, td;
-        for (int32_t k = 0;; ++k) {
-            const float dT = (float)k * net->md.dt;
-            tp.push_back(std::exp(-dT * inv_ltp));
-            td.push_back(std::exp(-dT * inv_ltd));
-            if ((tp.back() == 0.0f && td.back() == 0.0f) || k >= (1 << 22)) break;
-        }
+        trace_tables(net->md.dt, net->sd.ltp_tau, net->sd.ltd_tau, &tp, &td);
         net->tab_len = (int32_t)tp.size();
         NBX_RC(net->upload(&net->d_ltp_tab, tp.data(), tp.size()));
         NBX_RC(net->upload(&net->d_ltd_tab, td.data(), td.size()));
@@ -707,6 +829,7 @@ int32_t nbx_net_run_propagation(nbx_net* net) {
     }
     NBX_RC(device_scope(net));
     if (net->pull_mode && net->has_sparse) NBX_HIP_TRY(launch_pull(net->args(), nullptr, net->t, net->stream));
+    if (!net->rules.empty()) NBX_HIP_TRY(launch_trigger(net->args(), nullptr, net->t, net->stream));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     net->phase = 2;
     return NBX_OK;
@@ -764,7 +887,7 @@ int32_t nbx_net_apply_post_learning(nbx_net* net) {
         return NBX_ERR_STATE;
     }
     NBX_RC(device_scope(net));
-    if (net->stdp && net->d_in_ptr) NBX_HIP_TRY(launch_ltp(net->args(), nullptr, net->t, net->stream));
+    if ((net->stdp && net->d_in_ptr) || net->rstdp) NBX_HIP_TRY(launch_ltp(net->args(), nullptr, net->t, net->stream));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     net->phase = 4;
     return NBX_OK;
@@ -779,7 +902,8 @@ int32_t nbx_net_commit_firings(nbx_net* net) {
     }
     NBX_RC(device_scope(net));
     const StepArgs A = net->args();
-    if (net->phase == 3 && net->stdp && net->d_in_ptr) NBX_HIP_TRY(launch_ltp(A, nullptr, net->t, net->stream));
+    if (net->phase == 3 && ((net->stdp && net->d_in_ptr) || net->rstdp))
+        NBX_HIP_TRY(launch_ltp(A, nullptr, net->t, net->stream));
     if (!net->pull_mode && net->has_sparse) NBX_HIP_TRY(launch_deliver_push(A, nullptr, net->t, net->stream));
     if (net->has_dense) NBX_HIP_TRY(launch_deliver_dense(A, nullptr, net->t, net->stream));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
@@ -880,6 +1004,33 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
     if (apost) NBX_HIP_TRY(hipMemcpy(apost, net->d_apost, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
     if (last_update)
         NBX_HIP_TRY(hipMemcpy(last_update, net->d_slast, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* eligibility, int32_t* last_update,
+                               int64_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(!net->rules.empty(), "no learning rules are set");
+    NBX_CHECK(count == net->S, "buffer size must equal the synapse count");
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    const size_t S = (size_t)count;
+    std::vector<float> f(S);
+    std::vector<int32_t> iv(S);
+    auto masked = [&](float* out, const float* d) -> int32_t {
+        if (!out) return NBX_OK;
+        NBX_HIP_TRY(hipMemcpy(f.data(), d, sizeof(float) * S, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < S; ++j) out[j] = net->h_rule[j] >= 0 ? f[j] : 0.0f;
+        return NBX_OK;
+    };
+    NBX_RC(masked(apre, net->d_apre));
+    NBX_RC(masked(apost, net->d_apost));
+    NBX_RC(masked(eligibility, net->d_elig));
+    if (last_update) {
+        NBX_HIP_TRY(hipMemcpy(iv.data(), net->d_slast, sizeof(int32_t) * S, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < S; ++j) last_update[j] = net->h_rule[j] >= 0 ? iv[j] : 0;
+    }
     return NBX_OK;
 }
 

@@ -93,6 +93,15 @@ __host__ __device__ constexpr bool model_uses_g2(int32_t m) {
     return m == NBX_MODEL_IZH || m == NBX_MODEL_LIF_EXP2 || m == NBX_MODEL_LIF_EXP3;
 }
 
+// One trigger-driven learning rule on the device; its trace decay tables sit at
+// rtab[tab_off .. tab_off + tab_len) (Apre, tau ltp) and the next tab_len entries (Apost, tau ltd).
+struct RuleDev {
+    int32_t kind, clear;
+    float max_ltp, max_ltd, reward, punishment, a1pre, a2prepre;
+    int32_t tab_off, tab_len;
+};
+constexpr int32_t kMaxRules = 64;
+
 // Device pointers and scalars the step kernels need; passed by value.
 struct StepArgs {
     ModelParams P;
@@ -139,6 +148,22 @@ struct StepArgs {
     int32_t tab_len;
     const int64_t* in_ptr;
     const int64_t* in_syn;
+    // trigger-driven rules
+    int32_t nrules, rstdp;        // rule count; any R-STDP rule (post-side updates)
+    const RuleDev* rules;
+    const int8_t* syn_rule;       // [S] rule of a rule-plastic synapse, else -1
+    const int8_t* syn_trig;       // [S] rule of a trigger synapse, else -1
+    const float* maxw;            // [S]
+    float* elig;                  // [S]
+    const float* rtab;
+    const int64_t* rin_ptr;       // [n * nrules + 1] rule-plastic inputs of (post, rule)
+    const int64_t* rin_syn;
+    const int64_t* tin_ptr;       // [n + 1] trigger synapses into each post, ascending
+    const int64_t* tin_syn;
+    int32_t* tstamp;              // [S] step of a trigger synapse's last arrival
+    int32_t* tpost_stamp;         // [n] step a post last had a trigger arrival
+    int32_t* tposts;              // [2][n] posts with trigger arrivals, by step parity
+    int32_t* tcount;              // [2]
     // recording
     int32_t* counts;
     uint32_t* raster;
@@ -412,6 +437,7 @@ hipError_t launch_deliver_push(const StepArgs& A, const int32_t* tbase, int32_t 
 hipError_t launch_deliver_dense(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_trigger(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s);
 hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);

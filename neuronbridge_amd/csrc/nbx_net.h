@@ -10,6 +10,7 @@
 
 using nbx::ModelParams;
 using nbx::StepArgs;
+using nbx::RuleDev;
 
 // Post-blocked layout of the persistent engine and of replay delivery: block b owns
 // neurons [b * P, min((b + 1) * P, n)). Blocks are grouped into regions of rb blocks
@@ -77,6 +78,10 @@ struct nbx_net {
     // STDP
     bool stdp = false;
     nbx_stdp_desc sd{};
+    // trigger-driven rules
+    std::vector<nbx_rule_desc> rules;
+    std::vector<int8_t> h_rule, h_trig;
+    std::vector<float> h_maxw;
     // current
     std::vector<float> h_cur;
     int32_t cur_t0 = 0, cur_n = 0;
@@ -106,6 +111,12 @@ struct nbx_net {
     float *d_ltp_tab = nullptr, *d_ltd_tab = nullptr;
     int32_t tab_len = 1;
     int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
+    RuleDev* d_rules = nullptr;
+    int8_t *d_syn_rule = nullptr, *d_syn_trig = nullptr;
+    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr;
+    int64_t *d_rin_ptr = nullptr, *d_rin_syn = nullptr, *d_tin_ptr = nullptr, *d_tin_syn = nullptr;
+    int32_t *d_tstamp = nullptr, *d_tpost_stamp = nullptr, *d_tposts = nullptr, *d_tcount = nullptr;
+    bool rstdp = false;
     unsigned long long* d_events = nullptr;
     int32_t* d_tbase = nullptr;
     int32_t* d_err = nullptr;
@@ -176,6 +187,22 @@ struct nbx_net {
         A.tab_len = tab_len;
         A.in_ptr = d_in_ptr;
         A.in_syn = d_in_syn;
+        A.nrules = (int32_t)rules.size();
+        A.rstdp = rstdp ? 1 : 0;
+        A.rules = d_rules;
+        A.syn_rule = d_syn_rule;
+        A.syn_trig = d_syn_trig;
+        A.maxw = d_maxw;
+        A.elig = d_elig;
+        A.rtab = d_rtab;
+        A.rin_ptr = d_rin_ptr;
+        A.rin_syn = d_rin_syn;
+        A.tin_ptr = d_tin_ptr;
+        A.tin_syn = d_tin_syn;
+        A.tstamp = d_tstamp;
+        A.tpost_stamp = d_tpost_stamp;
+        A.tposts = d_tposts;
+        A.tcount = d_tcount;
         A.counts = d_counts;
         A.raster = d_raster;
         A.rec_t0 = rec_t0;

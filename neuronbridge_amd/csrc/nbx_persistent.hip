@@ -802,6 +802,7 @@ std::string persistent_reason(const nbx_net* net) {
     if (!net->has_sparse) return "needs sparse synapses";
     if (net->has_dense) return "dense delivery runs on the graph engine";
     if (net->stdp) return "STDP networks use arrival-time delivery";
+    if (!net->rules.empty()) return "networks with learning rules use arrival-time delivery";
     if (net->S >= ((int64_t)1 << 31)) return "more than 2^31 synapses";
     if (net->nch > 2) return "more than two arrival channels";
     Geometry g = geometry(net);
@@ -1088,7 +1089,7 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
 }
 
 int32_t replay_path(const nbx_net* net) {
-    if (net->stdp) return NBX_REPLAY_GLOBAL;
+    if (net->stdp || !net->rules.empty()) return NBX_REPLAY_GLOBAL;
     const ReplayGeometry r = replay_geometry(net);
     if (!r.ok) return NBX_REPLAY_GLOBAL;
     if (r.slots) return NBX_REPLAY_SLOTS;
@@ -1098,7 +1099,7 @@ int32_t replay_path(const nbx_net* net) {
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out) {
     const StepArgs A = net->args();
     // the blocked layouts carry pre-quantized weights, so they are not used while STDP changes them
-    const ReplayGeometry r = net->stdp ? ReplayGeometry() : replay_geometry(net);
+    const ReplayGeometry r = (net->stdp || !net->rules.empty()) ? ReplayGeometry() : replay_geometry(net);
     if (r.ok && (r.slots || build_persistent_layout(net) == NBX_OK)) {
         const PersistentLayout& L = net->pl;
         {

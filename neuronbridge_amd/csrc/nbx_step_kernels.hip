@@ -187,6 +187,52 @@ __device__ inline float stdp_clip(const StepArgs& A, float w) {
     return w;
 }
 
+// ---- trigger-driven rules
+// STDP_State::SetUpdate for rule r (STDP_State.cpp:17-26): both traces decay over the integer
+// step delta, looked up in the rule's tables.
+__device__ inline void rule_decay(const StepArgs& A, const RuleDev& R, int64_t s, int32_t t, float& apre,
+                                  float& apost) {
+    int32_t dt = t - A.slast[s];
+    if (dt > R.tab_len - 1) dt = R.tab_len - 1;
+    apre = A.apre[s] * A.rtab[R.tab_off + dt];
+    apost = A.apost[s] * A.rtab[R.tab_off + R.tab_len + dt];
+    A.slast[s] = t;
+}
+
+// Interconnections::UpdateWeight (Interconnections.h:120-128)
+__device__ inline float update_weight(float w, float inc, float wmax) {
+    w += inc;
+    if (w > wmax) w = wmax;
+    else if (w < 0.0f) w = 0.0f;
+    return w;
+}
+
+// Pre-side update of a rule-plastic synapse at its arrival (after its weight was delivered).
+__device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t r) {
+    const RuleDev R = A.rules[r];
+    float apre, apost;
+    if (R.kind == NBX_RULE_RSTDP) {  // R_STDP.cpp:40-48
+        rule_decay(A, R, j, t, apre, apost);
+        apre += 1.0f;
+        A.elig[j] += -R.max_ltd * apost;
+    } else {  // AdditiveKernalChange.cpp:65-70: weight first, then the trace
+        A.w[j] = update_weight(A.w[j], R.a1pre, A.maxw[j]);
+        rule_decay(A, R, j, t, apre, apost);
+        apre += 1.0f;
+    }
+    A.apre[j] = apre;
+    A.apost[j] = apost;
+}
+
+// A trigger synapse arrived: stamp it, and list its target once per step.
+__device__ inline void note_trigger(const StepArgs& A, int64_t j, int32_t post, int32_t t) {
+    A.tstamp[j] = t;
+    if (atomicExch(&A.tpost_stamp[post], t) != t) {
+        const int32_t k = atomicAdd(&A.tcount[t & 1], 1);
+        A.tposts[(int64_t)(t & 1) * A.n + k] = post;
+    }
+}
+
 // Arrival-time delivery: work items are (delay d, spike of step t-d) pairs; each
 // block walks the slice of synapses with that delay in the spiking neuron's row.
 __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const int32_t* tbase, int32_t k) {
@@ -224,8 +270,15 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const in
                 A.apost[j] = apost;
                 A.w[j] = stdp_clip(A, w + -A.max_ltd * apost);
             }
+            if (A.nrules) {
+                const int32_t r = A.syn_rule[j];
+                if (r >= 0) rule_pre(A, j, t, r);
+                if (A.syn_trig[j] >= 0) note_trigger(A, j, syn_post(en), t);
+            }
         }
     }
+    // the trigger list of step t + 1 (same parity as t - 1, consumed by then) starts empty
+    if (A.nrules && blockIdx.x == 0 && threadIdx.x == 0) A.tcount[(t + 1) & 1] = 0;
     // hist_cnt[t % D] belongs to step t-D, no longer read; the update of step t reuses it
     if (blockIdx.x == 0 && threadIdx.x == 0) A.hist_cnt[t % A.D] = 0;
 }
@@ -237,15 +290,67 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
     const int32_t* ids = A.hist + (int64_t)slot * A.n;
     for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
         const int32_t p = ids[s];
-        const int64_t b = A.in_ptr[p], e = A.in_ptr[p + 1];
-        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
-            const int64_t sid = A.in_syn[j];
-            float apre, apost;
-            stdp_decay(A, sid, t, apre, apost);
-            apost += 1.0f;
-            A.apre[sid] = apre;
-            A.apost[sid] = apost;
-            A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre);
+        if (A.stdp) {
+            const int64_t b = A.in_ptr[p], e = A.in_ptr[p + 1];
+            for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+                const int64_t sid = A.in_syn[j];
+                float apre, apost;
+                stdp_decay(A, sid, t, apre, apost);
+                apost += 1.0f;
+                A.apre[sid] = apre;
+                A.apost[sid] = apost;
+                A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre);
+            }
+        }
+        if (A.rstdp) {  // R_STDP.cpp:19-37: decay, Apost += 1, e += Apre * max_ltp
+            for (int32_t r = 0; r < A.nrules; ++r) {
+                const RuleDev R = A.rules[r];
+                if (R.kind != NBX_RULE_RSTDP) continue;
+                const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
+                for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+                    const int64_t sid = A.rin_syn[j];
+                    float apre, apost;
+                    rule_decay(A, R, sid, t, apre, apost);
+                    apost += 1.0f;
+                    A.apre[sid] = apre;
+                    A.apost[sid] = apost;
+                    A.elig[sid] += apre * R.max_ltp;
+                }
+            }
+        }
+    }
+}
+
+// Trigger arrivals of step t: one workgroup per target that had any; its trigger synapses are
+// applied one after another in ascending synapse order, each to all plastic inputs of its rule
+// (R_STDP.cpp:51-69, AdditiveKernalChange.cpp:71-82).
+__global__ __launch_bounds__(kDeliverThreads) void nbx_trigger(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    const int32_t cnt = A.tcount[t & 1];
+    const int32_t* posts = A.tposts + (int64_t)(t & 1) * A.n;
+    for (int32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        const int32_t p = posts[i];
+        for (int64_t q = A.tin_ptr[p]; q < A.tin_ptr[p + 1]; ++q) {
+            const int64_t tj = A.tin_syn[q];
+            if (A.tstamp[tj] != t) continue;
+            const int32_t r = A.syn_trig[tj];
+            const RuleDev R = A.rules[r];
+            const float factor = syn_type(A.syn[tj]) == 1 ? R.punishment : R.reward;
+            const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
+            for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+                const int64_t sid = A.rin_syn[j];
+                float apre, apost;
+                rule_decay(A, R, sid, t, apre, apost);
+                A.apre[sid] = apre;
+                A.apost[sid] = apost;
+                if (R.kind == NBX_RULE_RSTDP) {
+                    A.w[sid] = update_weight(A.w[sid], factor * A.elig[sid], A.maxw[sid]);
+                    if (R.clear) A.elig[sid] = 0.0f;
+                } else {
+                    A.w[sid] = update_weight(A.w[sid], R.a2prepre * apre, A.maxw[sid]);
+                }
+            }
+            __syncthreads();  // the next trigger of this target sees these weights
         }
     }
 }
@@ -327,6 +432,11 @@ hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipSt
 
 hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
     nbx_ltp<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_trigger(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_trigger<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
     return hipGetLastError();
 }
 

@@ -157,6 +157,28 @@ class Net:
         d = L.StdpDesc(max_ltp=max_ltp, ltp_tau=ltp_tau, max_ltd=max_ltd, ltd_tau=ltd_tau, w_max=w_max)
         L.call("nbx_net_set_stdp", self.h, C.byref(d))
 
+    def set_rules(self, rules: list, rule: np.ndarray, trigger: np.ndarray, max_weight: np.ndarray) -> None:
+        """Trigger-driven learning rules: `rules` is a list of dicts with kind (L.RULE_RSTDP /
+        L.RULE_ADDITIVE) and its parameters; rule / trigger / max_weight are per synapse (CSR order)."""
+        arr = (L.RuleDesc * max(1, len(rules)))()
+        for k, r in enumerate(rules):
+            for f, _ in L.RuleDesc._fields_:
+                if f in r:
+                    setattr(arr[k], f, r[f])
+        self._rule_arrays = [np.ascontiguousarray(rule, dtype=np.int32), np.ascontiguousarray(trigger, dtype=np.int32),
+                             np.ascontiguousarray(max_weight, dtype=np.float32)]
+        ru, tr, mw = self._rule_arrays
+        L.call("nbx_net_set_rules", self.h, arr, len(rules), _p(ru, C.c_int32), _p(tr, C.c_int32), _p(mw, C.c_float))
+
+    def rule_state(self):
+        """(Apre, Apost, eligibility, last update) of the rule-plastic synapses (0 elsewhere)."""
+        S = self.syn_count
+        apre, apost, el = (np.zeros(S, np.float32) for _ in range(3))
+        last = np.zeros(S, np.int32)
+        L.call("nbx_net_get_rule_state", self.h, _p(apre, C.c_float), _p(apost, C.c_float), _p(el, C.c_float),
+               _p(last, C.c_int32), S)
+        return apre, apost, el, last
+
     def set_current(self, cur: np.ndarray, t_begin: int = 0) -> None:
         cur = np.ascontiguousarray(cur, dtype=np.float32)
         assert cur.ndim == 2 and cur.shape[1] == self.n

@@ -41,6 +41,7 @@ class Workload:
     w_bg: float = 0.0
     cur: np.ndarray | None = None
     stdp: dict | None = None
+    rules: dict | None = None  # {"rules": [...], "rule", "trigger", "max_weight": per-synapse arrays}
     x0: np.ndarray | None = None
     last0: np.ndarray | None = None
     steps: int = 0
@@ -69,6 +70,8 @@ class Workload:
             net.set_poisson(self.pois_seed, self.pois_lambda, self.w_bg)
         if self.stdp is not None:
             net.set_stdp(**self.stdp)
+        if self.rules is not None:
+            net.set_rules(self.rules["rules"], self.rules["rule"], self.rules["trigger"], self.rules["max_weight"])
         if self.cur is not None:
             net.set_current(self.cur, 0)
         if fixed_shift:
@@ -134,6 +137,38 @@ def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L
     last0 = np.full(n, 10000, dtype=np.int32) if x0 is not None else None
     return Workload(f"sparse_model_{model}", spec, syn=syn, pois_seed=seed, pois_lambda=lam, w_bg=w_bg, x0=x0,
                     last0=last0, steps=0, extra={"n_exc": n_exc, "k_in": k_in})
+
+
+def trigger_rules(n: int = 2000, k_in: int = 100, seed: int = SEED_BASE + 7, trigger_fraction: float = 0.03,
+                  with_stdp: bool = False) -> Workload:
+    """The config-2 LIF network with the legacy trigger-driven rules on its E->E synapses:
+    40% R-STDP (rule 0: Max_LTP 0.92, LTP_tau 16.8, Max_LTD 0.53, LTD_tau 33.1, reward 0.01,
+    punishment -0.01, eligibility cleared after a trigger), 40% additive kernel (rule 1: a1pre 5e-4,
+    a2prepre -0.02, LTP_tau 16.8), weights bounded by 0.3; trigger_fraction of all synapses are
+    triggers (half of each rule; inhibitory trigger synapses punish). with_stdp makes the remaining
+    E->E synapses pair-STDP plastic."""
+    wl = sparse_lif(n=n, k_in=k_in, steps=0, seed=seed)
+    s = wl.syn
+    rng = np.random.default_rng(seed)
+    n_exc = wl.extra["n_exc"]
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row_ptr))
+    ee = (pre < n_exc) & (s.post < n_exc)
+    u = rng.random(s.count)
+    rule = np.full(s.count, -1, np.int32)
+    rule[ee & (u < 0.4)] = 0
+    rule[ee & (u >= 0.4) & (u < 0.8)] = 1
+    trig = np.full(s.count, -1, np.int32)
+    t_sel = (rule < 0) & (rng.random(s.count) < trigger_fraction)
+    trig[t_sel] = rng.integers(0, 2, int(t_sel.sum()))
+    if with_stdp:
+        s.plastic = ((rule < 0) & (trig < 0) & ee).astype(np.uint8)
+        wl.stdp = dict(max_ltp=0.001, ltp_tau=20.0, max_ltd=0.0012, ltd_tau=20.0, w_max=0.3)
+    rules = [dict(kind=L.RULE_RSTDP, max_ltp=0.92, ltp_tau=16.8, max_ltd=0.53, ltd_tau=33.1, reward=0.01,
+                  punishment=-0.01, clear_eligibility=1),
+             dict(kind=L.RULE_ADDITIVE, ltp_tau=16.8, a1pre=5e-4, a2prepre=-0.02)]
+    wl.rules = {"rules": rules, "rule": rule, "trigger": trig, "max_weight": np.full(s.count, 0.3, np.float32)}
+    wl.name = "trigger_rules"
+    return wl
 
 
 def izh_params(n: int, n_exc: int, rng) -> tuple:

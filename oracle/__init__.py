@@ -59,7 +59,18 @@ class OrcNet(C.Structure):
         ("w_max", C.c_float),
         ("apre", _f32p), ("apost", _f32p), ("slast", _i32p),
         ("in_ptr", _i64p), ("in_syn", _i64p),
+        ("nrules", C.c_int32), ("rules", C.c_void_p), ("syn_rule", C.POINTER(C.c_int8)),
+        ("syn_trig", C.POINTER(C.c_int8)), ("maxw", _f32p), ("elig", _f32p),
     ]
+
+
+RULE_RSTDP, RULE_ADDITIVE = 1, 2
+
+
+class OrcRule(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("clear", C.c_int32), ("max_ltp", C.c_float), ("ltp_tau", C.c_float),
+                ("max_ltd", C.c_float), ("ltd_tau", C.c_float), ("reward", C.c_float), ("punishment", C.c_float),
+                ("a1pre", C.c_float), ("a2prepre", C.c_float)]
 
 
 def build() -> None:

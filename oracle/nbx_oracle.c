@@ -284,6 +284,131 @@ static void stdp_clip(orc_net* net, int64_t s) {
     else if (net->w[s] > net->w_max) net->w[s] = net->w_max;
 }
 
+/* ---------------------------------------------------------------- trigger-driven rules */
+
+/* Interconnections::UpdateWeight (Interconnections.h:120-128) */
+static float update_weight(float w, float inc, float wmax) {
+    w += inc;
+    if (w > wmax) w = wmax;
+    else if (w < 0.0f) w = 0.0f;
+    return w;
+}
+
+/* STDP_State::SetUpdate with the rule's time constants (the additive rule's state uses ltp_tau
+ * for both traces, AdditiveKernalChange.cpp:16-20) */
+static void rule_set_update(orc_net* net, const orc_rule* R, int64_t s, int32_t t) {
+    const float inv_a = (float)(1.0 / (double)R->ltp_tau);
+    const float inv_b = (float)(1.0 / (double)(R->kind == ORC_RULE_ADDITIVE ? R->ltp_tau : R->ltd_tau));
+    const float dT = (float)(t - net->slast[s]) * net->m.dt;
+    net->apre[s] *= expf(-dT * inv_a);
+    net->apost[s] *= expf(-dT * inv_b);
+    net->slast[s] = t;
+}
+
+/* Pre-side update at the synapse's own arrival (after its weight was delivered). */
+static void rule_pre(orc_net* net, int64_t s, int32_t t) {
+    const orc_rule* R = &net->rules[net->syn_rule[s]];
+    if (R->kind == ORC_RULE_RSTDP) { /* R_STDP.cpp:40-48 */
+        rule_set_update(net, R, s, t);
+        net->apre[s] += 1.0f;
+        net->elig[s] += -R->max_ltd * net->apost[s];
+    } else { /* AdditiveKernalChange.cpp:65-70 */
+        net->w[s] = update_weight(net->w[s], R->a1pre, net->maxw[s]);
+        rule_set_update(net, R, s, t);
+        net->apre[s] += 1.0f;
+    }
+}
+
+typedef struct {
+    int64_t *rin_ptr, *rin_syn, *tin_ptr, *tin_syn; /* ascending synapse order in every list */
+    int32_t *tstamp, *post_mark, *posts;
+    int32_t nposts;
+} rule_lists;
+
+static void rule_lists_build(const orc_net* net, rule_lists* L) {
+    const int n = net->m.n, nr = net->nrules;
+    const int64_t S = net->row_ptr[n];
+    L->rin_ptr = (int64_t*)calloc((size_t)n * nr + 1, sizeof(int64_t));
+    L->tin_ptr = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < S; ++s) {
+        if (net->syn_rule[s] >= 0) L->rin_ptr[(size_t)net->post[s] * nr + net->syn_rule[s] + 1]++;
+        if (net->syn_trig[s] >= 0) L->tin_ptr[net->post[s] + 1]++;
+    }
+    for (size_t k = 1; k <= (size_t)n * nr; ++k) L->rin_ptr[k] += L->rin_ptr[k - 1];
+    for (int k = 1; k <= n; ++k) L->tin_ptr[k] += L->tin_ptr[k - 1];
+    L->rin_syn = (int64_t*)malloc(sizeof(int64_t) * (size_t)(L->rin_ptr[(size_t)n * nr] + 1));
+    L->tin_syn = (int64_t*)malloc(sizeof(int64_t) * (size_t)(L->tin_ptr[n] + 1));
+    int64_t* rf = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n * nr + 1));
+    int64_t* tf = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    memcpy(rf, L->rin_ptr, sizeof(int64_t) * ((size_t)n * nr + 1));
+    memcpy(tf, L->tin_ptr, sizeof(int64_t) * ((size_t)n + 1));
+    for (int64_t s = 0; s < S; ++s) {
+        if (net->syn_rule[s] >= 0) L->rin_syn[rf[(size_t)net->post[s] * nr + net->syn_rule[s]]++] = s;
+        if (net->syn_trig[s] >= 0) L->tin_syn[tf[net->post[s]]++] = s;
+    }
+    free(rf);
+    free(tf);
+    L->tstamp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(S > 0 ? S : 1));
+    for (int64_t s = 0; s < S; ++s) L->tstamp[s] = -1;
+    L->post_mark = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    for (int i = 0; i < n; ++i) L->post_mark[i] = -1;
+    L->posts = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    L->nposts = 0;
+}
+
+static void rule_lists_free(rule_lists* L) {
+    free(L->rin_ptr);
+    free(L->rin_syn);
+    free(L->tin_ptr);
+    free(L->tin_syn);
+    free(L->tstamp);
+    free(L->post_mark);
+    free(L->posts);
+}
+
+/* Trigger arrivals of tick t: each target applies its arrived trigger synapses in ascending
+ * synapse order to every plastic input of the trigger's rule (R_STDP.cpp:51-69,
+ * AdditiveKernalChange.cpp:71-82). Targets are independent (a synapse has one target). */
+static void rule_triggers(orc_net* net, rule_lists* L, int32_t t) {
+    const int nr = net->nrules;
+    for (int k = 0; k < L->nposts; ++k) {
+        const int32_t p = L->posts[k];
+        for (int64_t q = L->tin_ptr[p]; q < L->tin_ptr[p + 1]; ++q) {
+            const int64_t tj = L->tin_syn[q];
+            if (L->tstamp[tj] != t) continue;
+            const int r = net->syn_trig[tj];
+            const orc_rule* R = &net->rules[r];
+            const float factor = net->type[tj] == 1 ? R->punishment : R->reward;
+            for (int64_t j = L->rin_ptr[(size_t)p * nr + r]; j < L->rin_ptr[(size_t)p * nr + r + 1]; ++j) {
+                const int64_t s = L->rin_syn[j];
+                rule_set_update(net, R, s, t);
+                if (R->kind == ORC_RULE_RSTDP) {
+                    net->w[s] = update_weight(net->w[s], factor * net->elig[s], net->maxw[s]);
+                    if (R->clear) net->elig[s] = 0.0f;
+                } else {
+                    net->w[s] = update_weight(net->w[s], R->a2prepre * net->apre[s], net->maxw[s]);
+                }
+            }
+        }
+    }
+    L->nposts = 0;
+}
+
+/* Post-side R-STDP at the spike tick (R_STDP.cpp:19-37). */
+static void rule_post(orc_net* net, rule_lists* L, int32_t p, int32_t t) {
+    const int nr = net->nrules;
+    for (int r = 0; r < nr; ++r) {
+        const orc_rule* R = &net->rules[r];
+        if (R->kind != ORC_RULE_RSTDP) continue;
+        for (int64_t j = L->rin_ptr[(size_t)p * nr + r]; j < L->rin_ptr[(size_t)p * nr + r + 1]; ++j) {
+            const int64_t s = L->rin_syn[j];
+            rule_set_update(net, R, s, t);
+            net->apost[s] += 1.0f;
+            net->elig[s] += net->apre[s] * R->max_ltp;
+        }
+    }
+}
+
 int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t* hist, int32_t* hist_cnt,
                 int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
                 int64_t* events) {
@@ -319,6 +444,10 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     if (net->dense_wt)
         for (size_t k = 0; k < (size_t)n * n; ++k) m->cond |= net->dense_wt[k] < 0.0f ? ORC_COND_INH : ORC_COND_EXC;
     m->cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
+    rule_lists rl;
+    memset(&rl, 0, sizeof(rl));
+    const int rules_on = net->nrules > 0 && net->row_ptr;
+    if (rules_on) rule_lists_build(net, &rl);
 
     for (int32_t t = t0; t < t0 + nticks; ++t) {
         memset(c.acc0, 0, sizeof(int32_t) * (size_t)n);
@@ -356,10 +485,21 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                             net->w[s] += -net->max_ltd * net->apost[s];
                             stdp_clip(net, s);
                         }
+                        if (rules_on) {
+                            if (net->syn_rule[s] >= 0) rule_pre(net, s, t);
+                            if (net->syn_trig[s] >= 0) {
+                                rl.tstamp[s] = t;
+                                if (rl.post_mark[net->post[s]] != t) {
+                                    rl.post_mark[net->post[s]] = t;
+                                    rl.posts[rl.nposts++] = net->post[s];
+                                }
+                            }
+                        }
                     }
                 }
             }
         }
+        if (rules_on) rule_triggers(net, &rl, t);
         /* dense all-to-all arrivals (column of the fired presynaptic neuron) */
         if (net->dense_wt && t - net->dense_delay >= 0) {
             const int slot = (t - net->dense_delay) % D;
@@ -413,6 +553,8 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 }
             }
         }
+        if (rules_on)
+            for (int k = 0; k < nspk; ++k) rule_post(net, &rl, spikes[k], t);
         /* push-ring image of this tick's spikes */
         if (ring && acc_mode == ORC_ACC_FIXED) {
             for (int k = 0; k < nspk; ++k) {
@@ -457,6 +599,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
         }
     }
     if (events) *events += ev;
+    if (rules_on) rule_lists_free(&rl);
     free(c.acc0);
     free(c.acc1);
     free(c.acc2);

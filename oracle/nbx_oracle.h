@@ -22,6 +22,8 @@
  *   L/Event/src/Spike/PropogatedSpikeGroup.cpp:33-71 (ProcessSpike, then pre-learning)
  *   L/Event/src/Spike/TimeDrivenInternalSpike.cpp:11-144 (group time = t + delay; post-learning)
  *   L/LearningRule/src/Rule/STDP.cpp:19-61, L/LearningRule/src/State/STDP_State.cpp:17-34
+ *   L/LearningRule/src/Rule/R_STDP.cpp:19-70, L/LearningRule/src/Rule/AdditiveKernalChange.cpp:64-82
+ *   L/connections/inc/Interconnections.h:120-128 (UpdateWeight)
  *
  * Parity status of each function is listed in the header comment of nbx_oracle.c.
  */
@@ -101,7 +103,21 @@ typedef struct orc_net {
     int32_t* slast;
     const int64_t* in_ptr;   /* n + 1: plastic incoming synapses of each post */
     const int64_t* in_syn;   /* synapse ids (CSR positions) */
+    /* trigger-driven rules (LearningRule/src/Rule/R_STDP.cpp, AdditiveKernalChange.cpp); traces in
+     * apre / apost / slast like STDP (a synapse is STDP-plastic, rule-plastic or a trigger) */
+    int32_t nrules;
+    const struct orc_rule* rules;
+    const int8_t* syn_rule;  /* per synapse: rule of a rule-plastic synapse, else -1 */
+    const int8_t* syn_trig;  /* per synapse: rule of a trigger synapse, else -1 */
+    const float* maxw;       /* per synapse weight bound (Interconnections::maximum_weight) */
+    float* elig;             /* per synapse R-STDP eligibility */
 } orc_net;
+
+enum { ORC_RULE_RSTDP = 1, ORC_RULE_ADDITIVE = 2 };
+typedef struct orc_rule {
+    int32_t kind, clear;     /* clear: R-STDP ClearEligibilityAfterTrigger */
+    float max_ltp, ltp_tau, max_ltd, ltd_tau, reward, punishment, a1pre, a2prepre;
+} orc_rule;
 
 int orc_nstate(int model);
 void orc_model_prepare(orc_model* m);

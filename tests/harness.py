@@ -21,6 +21,8 @@ def auto_shift(wl) -> int:
         w = np.abs(wl.syn.weight.astype(np.float64))
         if wl.stdp is not None and wl.syn.plastic is not None:
             w = np.where(wl.syn.plastic != 0, np.maximum(w, wl.stdp["w_max"]), w)
+        if wl.rules is not None:
+            w = np.where(wl.rules["rule"] >= 0, np.maximum(w, np.abs(wl.rules["max_weight"])), w)
         bound += np.bincount(wl.syn.post, weights=w, minlength=n)
     if wl.dense_wt is not None:
         bound += np.abs(wl.dense_wt.astype(np.float64)).sum(axis=0)
@@ -125,6 +127,37 @@ class OracleRun:
             net.slast = O.ptr(self.slast, C.c_int32)
             net.in_ptr = O.ptr(in_ptr, C.c_int64)
             net.in_syn = O.ptr(in_syn, C.c_int64)
+        if wl.rules is not None:
+            S = wl.syn.count
+            rl = wl.rules["rules"]
+            arr = (O.OrcRule * len(rl))()
+            for kk, r in enumerate(rl):
+                arr[kk].kind = r["kind"]
+                arr[kk].clear = int(r.get("clear_eligibility", 0))
+                for f in ("max_ltp", "ltp_tau", "max_ltd", "ltd_tau", "reward", "punishment", "a1pre", "a2prepre"):
+                    setattr(arr[kk], f, float(r.get(f, 0.0)))
+            k(arr)
+            self.syn_rule = np.ascontiguousarray(wl.rules["rule"], dtype=np.int8)
+            self.syn_trig = np.ascontiguousarray(wl.rules["trigger"], dtype=np.int8)
+            self.maxw = np.ascontiguousarray(wl.rules["max_weight"], dtype=np.float32)
+            self.elig = np.zeros(S, np.float32)
+            if wl.stdp is None:
+                self.apre = np.zeros(S, np.float32)
+                self.apost = np.zeros(S, np.float32)
+                self.slast = np.zeros(S, np.int32)
+                for arr_ in (self.apre, self.apost, self.slast):
+                    k(arr_)
+                net.apre = O.ptr(self.apre, C.c_float)
+                net.apost = O.ptr(self.apost, C.c_float)
+                net.slast = O.ptr(self.slast, C.c_int32)
+            for arr_ in (self.syn_rule, self.syn_trig, self.maxw, self.elig):
+                k(arr_)
+            net.nrules = len(rl)
+            net.rules = C.cast(arr, C.c_void_p)
+            net.syn_rule = O.ptr(self.syn_rule, C.c_int8)
+            net.syn_trig = O.ptr(self.syn_trig, C.c_int8)
+            net.maxw = O.ptr(self.maxw, C.c_float)
+            net.elig = O.ptr(self.elig, C.c_float)
         self.net = net
         self.x, self.last = initial_state(wl)
         D, n = wl.d_slots, sp.n
@@ -139,7 +172,7 @@ class OracleRun:
         counts = np.zeros(nsteps, dtype=np.int32)
         words = (n + 31) // 32
         ras = np.zeros((nsteps, words), dtype=np.uint32) if raster else None
-        use_ring = ring and acc_mode == O.ACC_FIXED and self.wl.stdp is None
+        use_ring = ring and acc_mode == O.ACC_FIXED and self.wl.stdp is None and self.wl.rules is None
         st = O.lib().orc_run(C.byref(self.net), acc_mode, O.ptr(self.x, C.c_float), O.ptr(self.last, C.c_int32),
                              O.ptr(self.hist, C.c_int32), O.ptr(self.hist_cnt, C.c_int32),
                              O.ptr(self.ring, C.c_int32) if use_ring else None, self.t, nsteps,

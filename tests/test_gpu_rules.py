@@ -1,0 +1,84 @@
+"""GPU parity of the trigger-driven learning rules (R-STDP eligibility with reward / punishment
+triggers, additive kernel) against the oracle: weights, traces, eligibilities and spikes bit-exact."""
+import numpy as np
+import pytest
+
+from harness import OracleRun
+from neuronbridge_amd import _lib as L
+from neuronbridge_amd import workloads as W
+from test_gpu_parity import ENGINES, assert_bits_equal, compare
+
+
+def check_rules(net, run, wl):
+    sel = wl.rules["rule"] >= 0
+    assert_bits_equal(net.weights(), run.w, "weights")
+    apre, apost, elig, last = net.rule_state()
+    assert_bits_equal(apre[sel], run.apre[sel], "Apre")
+    assert_bits_equal(apost[sel], run.apost[sel], "Apost")
+    assert_bits_equal(elig[sel], run.elig[sel], "eligibility")
+    assert_bits_equal(last[sel], run.slast[sel], "last update")
+
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("with_stdp", [False, True], ids=["rules", "rules+stdp"])
+@pytest.mark.parametrize("engine", ["kernels", "graph"])
+def test_rule_network_bitexact(gpu, engine, with_stdp):
+    wl = W.trigger_rules(with_stdp=with_stdp)
+    T = 400
+    net = wl.make_net()
+    w0 = net.weights()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, ENGINES[engine])
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    assert counts_o.sum() > 500
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    check_rules(net, run, wl)
+    rule = wl.rules["rule"]
+    assert np.mean(net.weights()[rule == 0] != w0[rule == 0]) > 0.05, "R-STDP never triggered"
+    assert np.mean(net.weights()[rule == 1] != w0[rule == 1]) > 0.5
+    if with_stdp:
+        assert_bits_equal(net.stdp_state()[0][wl.syn.plastic != 0], run.apre[wl.syn.plastic != 0], "STDP Apre")
+
+
+def test_many_simultaneous_triggers(gpu):
+    # 30% of the synapses are triggers: targets often take several trigger arrivals in one step,
+    # which must apply in ascending synapse order
+    wl = W.trigger_rules(n=1500, k_in=80, seed=3, trigger_fraction=0.3)
+    T = 300
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    check_rules(net, run, wl)
+
+
+def test_rules_through_per_phase_calls(gpu):
+    # the reference's per-step call sequence drives the same trigger and post-side phases
+    wl = W.trigger_rules(n=1200, k_in=60, seed=5)
+    T = 120
+    net = wl.make_net()
+    net.record(0, T, counts=True, raster=True)
+    for t in range(T):
+        net.begin_step(t)
+        net.run_propagation()
+        net.neuron_step()
+        net.apply_post_learning()
+        net.commit_firings()
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    check_rules(net, run, wl)
+
+
+def test_rule_network_refuses_persistent_engine(gpu):
+    wl = W.trigger_rules(n=800, k_in=40)
+    net = wl.make_net()
+    with pytest.raises(L.NbxError) as e:
+        net.run(3, L.ENGINE_PERSISTENT)
+    assert e.value.status == L.NBX_ERR_UNSUPPORTED
+    assert net.replay_path() == L.REPLAY_GLOBAL
