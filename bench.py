@@ -236,6 +236,85 @@ def dense_bench(steps: int) -> dict:
             "spike_rate_hz": spikes / wl.spec.n / (steps * 0.1e-3)}
 
 
+def models_bench(steps: int) -> dict:
+    """The remaining legacy models on the config-2 topology (1e5 neurons, in-degree 1000, delays 1..20):
+    conductance LIF (double / triple exponential, RK4), PoissonRate and the trigger relay, each on the
+    engine the library picks; closed-loop stepping after 100 warm steps."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import Event
+    from neuronbridge_amd.network import build_fixed_indegree
+    n, k_in, seed = 100_000, 1000, W.SEED_BASE + 6
+    t0 = time.perf_counter()
+    topo = build_fixed_indegree(n, k_in, 1, 20, seed, int(round(0.8 * n)))
+    log(f"models: topology built in {time.perf_counter() - t0:.1f} s")
+    names = {L.LIF_EXP2: "lif_exp_double", L.LIF_EXP3: "lif_exp_triple", L.POISSON: "poisson_rate",
+             L.TRIGGER_RELAY: "trigger_relay"}
+    out = {"workload": f"config-2 topology (1e5 neurons, in-degree 1000, delays U{{1..20}}), dt 0.1 ms, "
+                       f"{steps} steps per model"}
+    for model, name in names.items():
+        integ = L.RK4 if model in (L.LIF_EXP2, L.LIF_EXP3) else L.EULER
+        wl = W.model_sparse(model, n=n, k_in=k_in, integrator=integ, seed=seed, topology=topo)
+        net = wl.make_net()
+        engine = net.pick_engine()
+        net.run(100, engine)
+        net.record(net.time, steps, counts=True, raster=False)
+        ev0 = net.events()
+        e0, e1 = Event(), Event()
+        net.sync()
+        e0.record(net.stream)
+        net.run_async(steps, engine)
+        e1.record(net.stream)
+        e1.synchronize()
+        net.sync()
+        ms = e0.elapsed_ms(e1)
+        ev = net.events() - ev0
+        spikes = int(net.counts().sum())
+        net.close()
+        out[name] = {"engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels",
+                                L.ENGINE_PERSISTENT: "persistent"}.get(engine, str(engine)),
+                     "ms_per_step": ms / steps, "neuron_steps_per_s": n * steps / (ms * 1e-3),
+                     "syn_events_per_s": ev / (ms * 1e-3), "spike_rate_hz": spikes / n / (steps * 0.1e-3)}
+        log(f"models: {name}: {out[name]}")
+    return out
+
+
+def rules_bench(steps: int) -> dict:
+    """Trigger-driven learning rules at config-2 size: 1e5 LIF neurons, in-degree 1000, 40% of E->E
+    synapses R-STDP, 40% additive kernel, 3% of all synapses triggers (graph engine)."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import Event
+    wl = W.trigger_rules(n=100_000, k_in=1000)
+    net = wl.make_net()
+    engine = net.pick_engine()
+    w0 = net.weights()
+    net.run(100, engine)
+    net.record(net.time, steps, counts=True, raster=False)
+    ev0 = net.events()
+    e0, e1 = Event(), Event()
+    net.sync()
+    e0.record(net.stream)
+    net.run_async(steps, engine)
+    e1.record(net.stream)
+    e1.synchronize()
+    net.sync()
+    ms = e0.elapsed_ms(e1)
+    ev = net.events() - ev0
+    spikes = int(net.counts().sum())
+    rule = wl.rules["rule"]
+    changed = net.weights() != w0
+    net.close()
+    n = wl.spec.n
+    return {"workload": "config-2 LIF network (1e5 neurons, 1e8 synapses) with R-STDP and additive-kernel rules "
+                        f"on 80% of E->E synapses, 3% triggers, {steps} steps",
+            "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
+            "ms_per_step": ms / steps, "neuron_steps_per_s": n * steps / (ms * 1e-3),
+            "syn_events_per_s": ev / (ms * 1e-3), "spike_rate_hz": spikes / n / (steps * 0.1e-3),
+            "rstdp_weights_changed": float(changed[rule == 0].mean()),
+            "additive_weights_changed": float(changed[rule == 1].mean())}
+
+
 def adjoint_bench(reps: int) -> dict:
     """Config 5: forward + surrogate-gradient backward (checkpoint 50) of the 2048-neuron recurrent
     LIF network, 64 trials x 500 steps; GEMM flop count over the timed training step."""
@@ -284,6 +363,10 @@ def main() -> int:
     ap.add_argument("--no-stdp", action="store_true")
     ap.add_argument("--stdp-steps", type=int, default=2000)
     ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--no-models", action="store_true")
+    ap.add_argument("--models-steps", type=int, default=1000)
+    ap.add_argument("--no-rules", action="store_true")
+    ap.add_argument("--rules-steps", type=int, default=500)
     ap.add_argument("--dense-steps", type=int, default=10000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
@@ -373,6 +456,11 @@ def main() -> int:
     if not args.no_dense:
         out["dense"] = dense_bench(args.dense_steps)
         log(f"dense: {out['dense']}")
+    if not args.no_models:
+        out["models"] = models_bench(args.models_steps)
+    if not args.no_rules:
+        out["rules"] = rules_bench(args.rules_steps)
+        log(f"rules: {out['rules']}")
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)

@@ -98,9 +98,10 @@ def sparse_lif(n: int = 100_000, k_in: int = 1000, integrator: int = L.RK4, step
 
 
 def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L.EULER, seed: int = SEED_BASE + 6,
-                 nmda_fraction: float = 0.3) -> Workload:
+                 nmda_fraction: float = 0.3, topology: Synapses | None = None) -> Workload:
     """The config-2 topology (fixed in-degree, delays U{1..20}, 80% excitatory) driving one of the
-    remaining legacy models, at test sizes:
+    remaining legacy models (synaptic weights as listed for in-degree 100, scaled by 100 / k_in so
+    the total input per neuron does not depend on the in-degree):
       LIF_EXP2  conductance LIF (V_rest = V_reset -60, tau 20, V_th -50, t_ref 2 ms, Eexc 0 / tau 5,
                 Einh -80 / tau 10), g increments 0.02 exc / 0.05 inh, background 0.008
       LIF_EXP3  the same with AMPA / GABA / NMDA (tau 10 each); nmda_fraction of the excitatory
@@ -109,15 +110,20 @@ def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L
                 inhibitory arrival
       TRIGGER_RELAY threshold 0.25; background events of 0.1, excitatory arrivals 0.002"""
     n_exc = int(round(0.8 * n))
-    syn = build_fixed_indegree(n, k_in, 1, 20, seed, n_exc)
+    if topology is None:
+        syn = build_fixed_indegree(n, k_in, 1, 20, seed, n_exc)
+    else:  # the same (n, k_in, seed) topology built once by the caller; types are copied
+        syn = Synapses(topology.row_ptr, topology.post, topology.delay, topology.type.copy(), topology.weight,
+                       topology.max_delay)
     rng = np.random.default_rng(seed)
     lam = 1.0
     x0 = None
+    g = np.float32(100.0 / k_in)
     if model in (L.LIF_EXP2, L.LIF_EXP3):
         spec = ModelSpec(model=model, n=n, dt=0.1, integrator=integrator, t_ref=20, v_rest=-60.0, tau=20.0,
                          v_th=-50.0, r=1.0, v_reset=-60.0, e_ampa=0.0, e_gaba=-80.0,
                          ampa_tau=5.0 if model == L.LIF_EXP2 else 10.0, gaba_tau=10.0, nmda_tau=10.0)
-        syn.weight = np.where(syn.type == 0, np.float32(0.02), np.float32(0.05)).astype(np.float32)
+        syn.weight = (np.where(syn.type == 0, np.float32(0.02), np.float32(0.05)) * g).astype(np.float32)
         if model == L.LIF_EXP3:
             nmda = (syn.type == 0) & (rng.random(syn.count) < nmda_fraction)
             syn.type = np.where(nmda, 2, syn.type).astype(np.uint8)
@@ -126,11 +132,11 @@ def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L
         x0[:, 0] = rng.uniform(-60.0, -50.0, n).astype(np.float32)
     elif model == L.POISSON:
         spec = ModelSpec(model=model, n=n, dt=0.1, rate_bias_hz=30.0, rate_gain_hz=1.0)
-        syn.weight = np.where(syn.type == 0, np.float32(200.0), np.float32(400.0)).astype(np.float32)
+        syn.weight = (np.where(syn.type == 0, np.float32(200.0), np.float32(400.0)) * g).astype(np.float32)
         w_bg = 20.0
     elif model == L.TRIGGER_RELAY:
         spec = ModelSpec(model=model, n=n, dt=0.1, v_th=0.25)
-        syn.weight = np.where(syn.type == 0, np.float32(0.002), np.float32(0.5)).astype(np.float32)
+        syn.weight = (np.where(syn.type == 0, np.float32(0.002), np.float32(0.5)) * g).astype(np.float32)
         w_bg = 0.1
     else:
         raise ValueError("model_sparse covers LIF_EXP2, LIF_EXP3, POISSON and TRIGGER_RELAY")
