@@ -102,7 +102,7 @@ typedef struct nbx_stdp_desc {
  *                      ltp_tau and jumps by 1; a trigger arrival adds a2prepre * Apre to each
  *                      plastic input of its target
  * Weights stay in [0, max_weight] of their synapse (Interconnections::UpdateWeight). */
-enum { NBX_RULE_RSTDP = 1, NBX_RULE_ADDITIVE = 2 };
+enum { NBX_RULE_STDP = 0, NBX_RULE_RSTDP = 1, NBX_RULE_ADDITIVE = 2 }; /* STDP: nbx_net_assign_nbnet_rules only */
 typedef struct nbx_rule_desc {
     int32_t kind;                         /* NBX_RULE_* */
     float max_ltp, ltp_tau, max_ltd, ltd_tau; /* R-STDP; ADDITIVE uses ltp_tau */
@@ -145,6 +145,14 @@ int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* stdp);
  * before updates). Before nbx_net_finalize. */
 int32_t nbx_net_set_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count, const int32_t* rule,
                           const int32_t* trigger, const float* max_weight);
+/* Learning rules of a network whose synapses came from nbx_net_load_nbnet, by the file's rule
+ * index (the order of its metadata's "learning_rules"): a record's synapse_rule / trigger_rule
+ * select rules[...]; NBX_RULE_STDP rules become pair STDP on their synapses (one parameter set per
+ * network), the others as in nbx_net_set_rules. Every rule-driven weight is bounded by its record's
+ * max_weight (STDP.cpp:29-34, Interconnections.h:120-128); a trigger id naming an STDP rule is
+ * ignored, as the reference ignores triggers of rules without a trigger side (Network.cu:438-445).
+ * Replaces earlier nbx_net_set_stdp / nbx_net_set_rules settings. Before nbx_net_finalize. */
+int32_t nbx_net_assign_nbnet_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count);
 /* Per-synapse rule state (CSR order; entries of other synapses read 0). */
 int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* eligibility,
                                int32_t* last_update, int64_t count);

@@ -52,7 +52,7 @@ class ModelDesc(C.Structure):
     ]
 
 
-RULE_RSTDP, RULE_ADDITIVE = 1, 2
+RULE_STDP, RULE_RSTDP, RULE_ADDITIVE = 0, 1, 2
 
 
 class RuleDesc(C.Structure):
@@ -170,6 +170,7 @@ SIGNATURES = {
     "nbx_net_export_monitor": [_NET, C.POINTER(MonitorDesc)],
     "nbx_net_set_rules": [_NET, C.POINTER(RuleDesc), C.c_int32, _i32p, _i32p, _f32p],
     "nbx_net_get_rule_state": [_NET, _f32p, _f32p, _f32p, _i32p, C.c_int64],
+    "nbx_net_assign_nbnet_rules": [_NET, C.POINTER(RuleDesc), C.c_int32],
     "nbx_build_fixed_indegree": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
                                  C.c_int32, C.c_int32, _i64p, _i32p, _u8p, _u8p],
     "nbx_net_stream": [_NET],

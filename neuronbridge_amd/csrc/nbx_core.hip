@@ -264,7 +264,8 @@ int32_t auto_shift(nbx_net* net) {
     if (net->has_sparse) {
         for (int64_t j = 0; j < net->S; ++j) {
             double w = std::fabs((double)net->h_w[(size_t)j]);
-            if (net->stdp && !net->h_plastic.empty() && net->h_plastic[(size_t)j]) w = std::max(w, (double)net->sd.w_max);
+            if (net->stdp && !net->h_plastic.empty() && net->h_plastic[(size_t)j])
+                w = std::max(w, net->h_smaxw.empty() ? (double)net->sd.w_max : std::fabs((double)net->h_smaxw[(size_t)j]));
             if (!net->h_rule.empty() && net->h_rule[(size_t)j] >= 0)
                 w = std::max(w, std::fabs((double)net->h_maxw[(size_t)j]));
             bound[(size_t)syn_post(net->h_syn[(size_t)j])] += w;
@@ -386,6 +387,10 @@ int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn) {
     NBX_CHECK(syn->row_ptr[0] == 0 && syn->row_ptr[n] == syn->count, "row_ptr must start at 0 and end at count");
     net->h_row_ptr.assign(syn->row_ptr, syn->row_ptr + n + 1);
     net->h_src_index.clear();
+    net->h_file_rule.clear();
+    net->h_file_trig.clear();
+    net->h_file_maxw.clear();
+    net->h_smaxw.clear();
     net->h_syn.resize((size_t)syn->count);
     net->h_w.assign(syn->weight, syn->weight + syn->count);
     if (syn->plastic) net->h_plastic.assign(syn->plastic, syn->plastic + syn->count);
@@ -443,6 +448,7 @@ int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* sd) {
     NBX_CHECK(sd->ltp_tau > 0.0f && sd->ltd_tau > 0.0f, "STDP time constants must be positive");
     net->stdp = true;
     net->sd = *sd;
+    net->h_smaxw.clear();
     return NBX_OK;
 }
 
@@ -632,6 +638,7 @@ int32_t nbx_net_finalize(nbx_net* net) {
     if (net->stdp) {
         NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
         const size_t S = (size_t)net->S;
+        if (!net->h_smaxw.empty()) NBX_RC(net->upload(&net->d_smaxw, net->h_smaxw.data(), S));
         std::vector<float> tp, td;
         trace_tables(net->md.dt, net->sd.ltp_tau, net->sd.ltd_tau, &tp, &td);
         net->tab_len = (int32_t)tp.size();

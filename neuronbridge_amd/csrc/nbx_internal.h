@@ -140,6 +140,7 @@ struct StepArgs {
     // STDP
     int32_t stdp;
     float max_ltp, max_ltd, w_max;
+    const float* smaxw;  // [S] per-synapse STDP weight bound, or null (w_max)
     float* apre;
     float* apost;
     int32_t* slast;

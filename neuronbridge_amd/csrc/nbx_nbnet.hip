@@ -241,15 +241,17 @@ enum : uint32_t {
     kBadWeight = 2,
     kBadDelay = 4,
     kBadType = 8,
-    kTrigger = 16,
+    kBadRule = 16,
+    kHasRule = 32,  // not an error: some record names a learning or trigger rule
 };
 
 // Record j of a chunk -> sort key (pre << 30 | delay << 24 | post), file index, type, weight,
-// plastic flag; counts per presynaptic neuron; validation bits.
+// plastic flag, rule / trigger ids and weight bound; counts per presynaptic neuron; validation bits.
 __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t count, int64_t base, int32_t n,
                                   uint64_t* __restrict__ key, uint32_t* __restrict__ idx, uint8_t* __restrict__ type,
-                                  float* __restrict__ w, uint8_t* __restrict__ plastic, int32_t* __restrict__ cnt,
-                                  uint32_t* __restrict__ bad, int32_t* __restrict__ max_delay, int32_t max_type) {
+                                  float* __restrict__ w, uint8_t* __restrict__ plastic, int3* __restrict__ rules,
+                                  int32_t* __restrict__ cnt, uint32_t* __restrict__ bad,
+                                  int32_t* __restrict__ max_delay, int32_t max_type) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t* r = rec + j * 8;
         const uint32_t src = r[0], dst = r[1], dly = r[5];
@@ -260,7 +262,7 @@ __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t coun
         if (!isfinite(wt) || !isfinite(mw)) b |= kBadWeight;
         if (dly < 1u || dly > (uint32_t)kMaxDelay) b |= kBadDelay;
         if (ty < 0 || ty > max_type) b |= kBadType;
-        if (trig >= 0) b |= kTrigger;
+        if (rule < -1 || trig < -1) b |= kBadRule;  // Network.cu:392-399
         if (b) {
             atomicOr(bad, b);
             continue;
@@ -271,6 +273,8 @@ __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t coun
         type[i] = (uint8_t)ty;
         w[i] = wt;
         plastic[i] = rule >= 0 ? 1 : 0;
+        rules[i] = make_int3(rule, trig, (int32_t)r[4]);
+        if (rule >= 0 || trig >= 0) atomicOr(bad, (uint32_t)kHasRule);
         atomicAdd(&cnt[src], 1);
         atomicMax(max_delay, (int32_t)dly);
     }
@@ -278,14 +282,16 @@ __global__ void nbx_nbnet_extract(const uint32_t* __restrict__ rec, int64_t coun
 
 __global__ void nbx_nbnet_gather(const uint64_t* __restrict__ key, const uint32_t* __restrict__ idx, int64_t count,
                                  const uint8_t* __restrict__ type, const float* __restrict__ w_in,
-                                 const uint8_t* __restrict__ pl_in, uint32_t* __restrict__ syn,
-                                 float* __restrict__ w, uint8_t* __restrict__ plastic) {
+                                 const uint8_t* __restrict__ pl_in, const int3* __restrict__ rules_in,
+                                 uint32_t* __restrict__ syn, float* __restrict__ w, uint8_t* __restrict__ plastic,
+                                 int3* __restrict__ rules) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t k = key[i];
         const uint32_t r = idx[i];
         syn[i] = pack_syn((int32_t)(k & 0xFFFFFFu), (int32_t)((k >> 24) & 0x3Fu), type[r]);
         w[i] = w_in[r];
         plastic[i] = pl_in[r];
+        rules[i] = rules_in[r];
     }
 }
 
@@ -395,6 +401,8 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
     NBX_HIP_TRY(sc.alloc(&type, (size_t)S));
     NBX_HIP_TRY(sc.alloc(&pl, (size_t)S));
     NBX_HIP_TRY(sc.alloc(&w, (size_t)S));
+    int3 *rules = nullptr, *rules_out = nullptr;
+    NBX_HIP_TRY(sc.alloc(&rules, (size_t)S));
     NBX_HIP_TRY(sc.alloc(&cnt, (size_t)n));
     NBX_HIP_TRY(sc.alloc(&row, (size_t)n + 1));
     NBX_HIP_TRY(sc.alloc(&bad, 1));
@@ -429,7 +437,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         // rec is reused in stream order: the previous chunk's extraction precedes this copy
         NBX_HIP_TRY(hipMemcpyAsync(rec, sc.pinned[cur], (size_t)c * kRecordSize, hipMemcpyHostToDevice, st));
         NBX_HIP_TRY(hipEventRecord(copied[cur], st));
-        nbx_nbnet_extract<<<blocks_for(c), 256, 0, st>>>(rec, c, base, n, key, idx, type, w, pl, cnt, bad, maxd,
+        nbx_nbnet_extract<<<blocks_for(c), 256, 0, st>>>(rec, c, base, n, key, idx, type, w, pl, rules, cnt, bad, maxd,
                                                           model_max_type(net->md.model));
         NBX_HIP_TRY(hipGetLastError());
         cur ^= 1;
@@ -457,9 +465,9 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
                       : "nbnet synapse type must be 0 (excitatory) or 1 (inhibitory)");
         return NBX_ERR_UNSUPPORTED;
     }
-    if (bad_h & kTrigger) {
-        set_error("nbnet trigger rules are not supported");
-        return NBX_ERR_UNSUPPORTED;
+    if (bad_h & kBadRule) {
+        set_error("nbnet connection references an invalid learning rule");
+        return NBX_ERR_INVALID;
     }
 
     // sort by (pre, delay, post); radix sort is stable, so equal keys keep file order
@@ -475,8 +483,10 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
     NBX_HIP_TRY(sc.alloc(&syn, (size_t)S));
     NBX_HIP_TRY(sc.alloc(&w_out, (size_t)S));
     NBX_HIP_TRY(sc.alloc(&pl_out, (size_t)S));
+    NBX_HIP_TRY(sc.alloc(&rules_out, (size_t)S));
     if (S) {
-        nbx_nbnet_gather<<<blocks_for(S), 256, 0, st>>>(kb.Current(), vb.Current(), S, type, w, pl, syn, w_out, pl_out);
+        nbx_nbnet_gather<<<blocks_for(S), 256, 0, st>>>(kb.Current(), vb.Current(), S, type, w, pl, rules, syn, w_out,
+                                                        pl_out, rules_out);
         NBX_HIP_TRY(hipGetLastError());
     }
     // row starts: exclusive scan of the per-presynaptic counts
@@ -501,7 +511,24 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         NBX_HIP_TRY(hipMemcpyAsync(h_pl.data(), pl_out, (size_t)S, hipMemcpyDeviceToHost, st));
         NBX_HIP_TRY(hipMemcpyAsync(h_src.data(), vb.Current(), sizeof(uint32_t) * (size_t)S, hipMemcpyDeviceToHost, st));
     }
+    const bool has_rules = (bad_h & kHasRule) != 0;
+    std::vector<int3> h_rules(has_rules ? (size_t)S : 0);
+    if (has_rules && S)
+        NBX_HIP_TRY(hipMemcpyAsync(h_rules.data(), rules_out, sizeof(int3) * (size_t)S, hipMemcpyDeviceToHost, st));
     NBX_HIP_TRY(hipStreamSynchronize(st));
+    net->h_file_rule.clear();
+    net->h_file_trig.clear();
+    net->h_file_maxw.clear();
+    if (has_rules) {  // kept in CSR order for nbx_net_assign_nbnet_rules
+        net->h_file_rule.resize((size_t)S);
+        net->h_file_trig.resize((size_t)S);
+        net->h_file_maxw.resize((size_t)S);
+        for (size_t j = 0; j < (size_t)S; ++j) {
+            net->h_file_rule[j] = h_rules[j].x;
+            net->h_file_trig[j] = h_rules[j].y;
+            std::memcpy(&net->h_file_maxw[j], &h_rules[j].z, sizeof(float));
+        }
+    }
     bool any_plastic = false;
     for (uint8_t v : h_pl) any_plastic |= v != 0;
     net->h_row_ptr = std::move(h_row);
@@ -513,6 +540,102 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
     net->S = S;
     net->max_delay = std::max(net->max_delay, std::max(1, maxd_h));
     net->has_sparse = true;
+    return NBX_OK;
+}
+
+int32_t nbx_net_assign_nbnet_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count) {
+    if (!net || (!rules && count > 0)) {
+        set_error("null argument");
+        return NBX_ERR_INVALID;
+    }
+    if (net->finalized) {
+        set_error("learning rules must be assigned before nbx_net_finalize");
+        return NBX_ERR_STATE;
+    }
+    if (!net->has_sparse || net->h_src_index.size() != (size_t)net->S) {
+        set_error("assign .nbnet rules after nbx_net_load_nbnet");
+        return NBX_ERR_STATE;
+    }
+    if (count < 0 || count > kMaxRules) {
+        set_error("rule count must be in [0, 64]");
+        return NBX_ERR_INVALID;
+    }
+    int32_t stdp_rule = -1;
+    for (int32_t r = 0; r < count; ++r) {
+        const nbx_rule_desc& q = rules[r];
+        if (q.kind != NBX_RULE_STDP && q.kind != NBX_RULE_RSTDP && q.kind != NBX_RULE_ADDITIVE) {
+            set_error("unknown learning rule kind");
+            return NBX_ERR_INVALID;
+        }
+        if (!(q.ltp_tau > 0.0f) || (q.kind != NBX_RULE_ADDITIVE && !(q.ltd_tau > 0.0f))) {
+            set_error("learning rule time constants must be positive");
+            return NBX_ERR_INVALID;
+        }
+        if (q.kind != NBX_RULE_STDP) continue;
+        if (stdp_rule >= 0) {
+            const nbx_rule_desc& a = rules[stdp_rule];
+            if (a.max_ltp != q.max_ltp || a.ltp_tau != q.ltp_tau || a.max_ltd != q.max_ltd || a.ltd_tau != q.ltd_tau) {
+                set_error("STDP rules with different parameters in one network");
+                return NBX_ERR_UNSUPPORTED;
+            }
+        }
+        stdp_rule = r;
+    }
+    const size_t S = (size_t)net->S;
+    const bool has = !net->h_file_rule.empty();
+    std::vector<uint8_t> plastic(S, 0);
+    std::vector<int32_t> rule(S, -1), trig(S, -1);
+    bool any_stdp = false, any_rule = false;
+    for (size_t j = 0; j < S && has; ++j) {
+        const int32_t r = net->h_file_rule[j], tr = net->h_file_trig[j];
+        if (r >= count || tr >= count) {
+            set_error("nbnet connection references an invalid learning rule");
+            return NBX_ERR_INVALID;
+        }
+        if (r >= 0 && rules[r].kind == NBX_RULE_STDP) {
+            plastic[j] = 1;
+            any_stdp = true;
+        } else if (r >= 0) {
+            rule[j] = r;
+            any_rule = true;
+        }
+        if (tr >= 0 && rules[tr].kind != NBX_RULE_STDP) {
+            trig[j] = tr;
+            any_rule = true;
+        }
+        if (trig[j] >= 0 && r >= 0) {
+            set_error("a synapse that is both plastic and a trigger is not supported");
+            return NBX_ERR_UNSUPPORTED;
+        }
+    }
+    net->stdp = any_stdp;
+    net->h_smaxw.clear();
+    if (any_stdp) {
+        const nbx_rule_desc& q = rules[stdp_rule];
+        net->sd.max_ltp = q.max_ltp;
+        net->sd.ltp_tau = q.ltp_tau;
+        net->sd.max_ltd = q.max_ltd;
+        net->sd.ltd_tau = q.ltd_tau;
+        net->sd.w_max = 0.0f;  // bounds are per synapse
+        net->h_plastic.swap(plastic);
+        net->h_smaxw = net->h_file_maxw;
+    } else {
+        net->h_plastic.clear();
+    }
+    net->rules.clear();
+    net->h_rule.clear();
+    net->h_trig.clear();
+    net->h_maxw.clear();
+    if (any_rule) {
+        net->rules.assign(rules, rules + count);
+        net->h_rule.resize(S);
+        net->h_trig.resize(S);
+        for (size_t j = 0; j < S; ++j) {
+            net->h_rule[j] = (int8_t)rule[j];
+            net->h_trig[j] = (int8_t)trig[j];
+        }
+        net->h_maxw = net->h_file_maxw;
+    }
     return NBX_OK;
 }
 

@@ -82,6 +82,9 @@ struct nbx_net {
     std::vector<nbx_rule_desc> rules;
     std::vector<int8_t> h_rule, h_trig;
     std::vector<float> h_maxw;
+    std::vector<int32_t> h_file_rule, h_file_trig;  // a loaded .nbnet table's rule ids (CSR order)
+    std::vector<float> h_file_maxw;                 // and its records' max_weight
+    std::vector<float> h_smaxw;                     // per-synapse STDP weight bound (else sd.w_max)
     // current
     std::vector<float> h_cur;
     int32_t cur_t0 = 0, cur_n = 0;
@@ -113,7 +116,7 @@ struct nbx_net {
     int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
     RuleDev* d_rules = nullptr;
     int8_t *d_syn_rule = nullptr, *d_syn_trig = nullptr;
-    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr;
+    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr, *d_smaxw = nullptr;
     int64_t *d_rin_ptr = nullptr, *d_rin_syn = nullptr, *d_tin_ptr = nullptr, *d_tin_syn = nullptr;
     int32_t *d_tstamp = nullptr, *d_tpost_stamp = nullptr, *d_tposts = nullptr, *d_tcount = nullptr;
     bool rstdp = false;
@@ -179,6 +182,7 @@ struct nbx_net {
         A.max_ltp = sd.max_ltp;
         A.max_ltd = sd.max_ltd;
         A.w_max = sd.w_max;
+        A.smaxw = d_smaxw;
         A.apre = d_apre;
         A.apost = d_apost;
         A.slast = d_slast;

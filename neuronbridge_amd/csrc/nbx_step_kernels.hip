@@ -181,9 +181,11 @@ __device__ inline void stdp_decay(const StepArgs& A, int64_t s, int32_t t, float
     A.slast[s] = t;
 }
 
-__device__ inline float stdp_clip(const StepArgs& A, float w) {
+// STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum
+__device__ inline float stdp_clip(const StepArgs& A, float w, int64_t j) {
+    const float wm = A.smaxw ? A.smaxw[j] : A.w_max;
     if (w < 0.0f) w = 0.0f;
-    else if (w > A.w_max) w = A.w_max;
+    else if (w > wm) w = wm;
     return w;
 }
 
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const in
                 apre += 1.0f;
                 A.apre[j] = apre;
                 A.apost[j] = apost;
-                A.w[j] = stdp_clip(A, w + -A.max_ltd * apost);
+                A.w[j] = stdp_clip(A, w + -A.max_ltd * apost, j);
             }
             if (A.nrules) {
                 const int32_t r = A.syn_rule[j];
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
                 apost += 1.0f;
                 A.apre[sid] = apre;
                 A.apost[sid] = apost;
-                A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre);
+                A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre, sid);
             }
         }
         if (A.rstdp) {  // R_STDP.cpp:19-37: decay, Apost += 1, e += Apre * max_ltp

@@ -170,6 +170,16 @@ class Net:
         ru, tr, mw = self._rule_arrays
         L.call("nbx_net_set_rules", self.h, arr, len(rules), _p(ru, C.c_int32), _p(tr, C.c_int32), _p(mw, C.c_float))
 
+    def assign_nbnet_rules(self, rules: list) -> None:
+        """Rules of a .nbnet-loaded network by the file's rule index (dicts as in set_rules; kind
+        L.RULE_STDP maps onto pair STDP with per-record weight bounds)."""
+        arr = (L.RuleDesc * max(1, len(rules)))()
+        for k, r in enumerate(rules):
+            for f, _ in L.RuleDesc._fields_:
+                if f in r:
+                    setattr(arr[k], f, r[f])
+        L.call("nbx_net_assign_nbnet_rules", self.h, arr, len(rules))
+
     def rule_state(self):
         """(Apre, Apost, eligibility, last update) of the rule-plastic synapses (0 elsewhere)."""
         S = self.syn_count

@@ -56,9 +56,11 @@ class Workload:
             md = max(md, self.dense_delay)
         return max(md + 1, 3)
 
-    def make_net(self, device: int = 0, fixed_shift: int = 0, nbnet: str | None = None) -> Net:
+    def make_net(self, device: int = 0, fixed_shift: int = 0, nbnet: str | None = None,
+                 nbnet_rules: list | None = None) -> Net:
         """The workload as a finalized network; `nbnet` loads the synapses from that .nbnet file
-        (the connection table of this workload) instead of the in-memory CSR."""
+        (the connection table of this workload) instead of the in-memory CSR, and `nbnet_rules`
+        then assigns the file's rule ids (nbx_net_assign_nbnet_rules)."""
         net = Net(self.spec, device)
         if nbnet is not None:
             net.load_nbnet(nbnet)
@@ -68,10 +70,14 @@ class Workload:
             net.set_dense(self.dense_wt, self.dense_delay)
         if self.pois_lambda > 0:
             net.set_poisson(self.pois_seed, self.pois_lambda, self.w_bg)
-        if self.stdp is not None:
-            net.set_stdp(**self.stdp)
-        if self.rules is not None:
+        if self.stdp is not None and nbnet is None:
+            net.set_stdp(**{k: v for k, v in self.stdp.items() if k != "max_weight"})
+        if self.rules is not None and nbnet is None:
             net.set_rules(self.rules["rules"], self.rules["rule"], self.rules["trigger"], self.rules["max_weight"])
+        if nbnet_rules is not None:
+            net.assign_nbnet_rules(nbnet_rules)
+        elif self.stdp is not None and nbnet is not None:
+            net.set_stdp(**{k: v for k, v in self.stdp.items() if k != "max_weight"})
         if self.cur is not None:
             net.set_current(self.cur, 0)
         if fixed_shift:

@@ -60,7 +60,7 @@ class OrcNet(C.Structure):
         ("apre", _f32p), ("apost", _f32p), ("slast", _i32p),
         ("in_ptr", _i64p), ("in_syn", _i64p),
         ("nrules", C.c_int32), ("rules", C.c_void_p), ("syn_rule", C.POINTER(C.c_int8)),
-        ("syn_trig", C.POINTER(C.c_int8)), ("maxw", _f32p), ("elig", _f32p),
+        ("syn_trig", C.POINTER(C.c_int8)), ("maxw", _f32p), ("elig", _f32p), ("stdp_maxw", _f32p),
     ]
 
 

@@ -280,8 +280,9 @@ static void stdp_set_update(orc_net* net, int64_t s, int32_t t, float inv_ltp, f
 }
 
 static void stdp_clip(orc_net* net, int64_t s) {
+    const float wm = net->stdp_maxw ? net->stdp_maxw[s] : net->w_max;
     if (net->w[s] < 0.0f) net->w[s] = 0.0f;
-    else if (net->w[s] > net->w_max) net->w[s] = net->w_max;
+    else if (net->w[s] > wm) net->w[s] = wm;
 }
 
 /* ---------------------------------------------------------------- trigger-driven rules */

@@ -111,6 +111,7 @@ typedef struct orc_net {
     const int8_t* syn_trig;  /* per synapse: rule of a trigger synapse, else -1 */
     const float* maxw;       /* per synapse weight bound (Interconnections::maximum_weight) */
     float* elig;             /* per synapse R-STDP eligibility */
+    const float* stdp_maxw;  /* per synapse STDP weight bound (STDP.cpp:29-34), NULL: w_max for all */
 } orc_net;
 
 enum { ORC_RULE_RSTDP = 1, ORC_RULE_ADDITIVE = 2 };

@@ -20,7 +20,8 @@ def auto_shift(wl) -> int:
     if wl.syn is not None:
         w = np.abs(wl.syn.weight.astype(np.float64))
         if wl.stdp is not None and wl.syn.plastic is not None:
-            w = np.where(wl.syn.plastic != 0, np.maximum(w, wl.stdp["w_max"]), w)
+            wm = wl.stdp.get("max_weight")
+            w = np.where(wl.syn.plastic != 0, np.maximum(w, wl.stdp["w_max"] if wm is None else np.abs(wm)), w)
         if wl.rules is not None:
             w = np.where(wl.rules["rule"] >= 0, np.maximum(w, np.abs(wl.rules["max_weight"])), w)
         bound += np.bincount(wl.syn.post, weights=w, minlength=n)
@@ -122,6 +123,10 @@ class OracleRun:
             net.plastic = O.ptr(plastic, C.c_uint8)
             for f in ("max_ltp", "ltp_tau", "max_ltd", "ltd_tau", "w_max"):
                 setattr(net, f, wl.stdp[f])
+            if wl.stdp.get("max_weight") is not None:  # per-synapse bounds (networks from .nbnet)
+                smax = np.ascontiguousarray(wl.stdp["max_weight"], dtype=np.float32)
+                k(smax)
+                net.stdp_maxw = O.ptr(smax, C.c_float)
             net.apre = O.ptr(self.apre, C.c_float)
             net.apost = O.ptr(self.apost, C.c_float)
             net.slast = O.ptr(self.slast, C.c_int32)

@@ -66,7 +66,8 @@ def test_loaded_plastic_network_matches_oracle(gpu, tmp_path):
     ("delay", 0, L.NBX_ERR_UNSUPPORTED),
     ("delay", 64, L.NBX_ERR_UNSUPPORTED),
     ("type", 2, L.NBX_ERR_UNSUPPORTED),
-    ("trigger", 0, L.NBX_ERR_UNSUPPORTED),
+    ("rule", -2, L.NBX_ERR_INVALID),
+    ("trigger", -5, L.NBX_ERR_INVALID),
     ("dst", 10_000, L.NBX_ERR_INVALID),
     ("weight", np.inf, L.NBX_ERR_INVALID),
 ])

@@ -82,3 +82,58 @@ def test_rule_network_refuses_persistent_engine(gpu):
         net.run(3, L.ENGINE_PERSISTENT)
     assert e.value.status == L.NBX_ERR_UNSUPPORTED
     assert net.replay_path() == L.REPLAY_GLOBAL
+
+
+def test_nbnet_learning_rules(gpu, tmp_path):
+    # a .nbnet table whose records name rules 0 (R-STDP), 1 (additive) and 2 (pair STDP) and carry
+    # per-record weight bounds; trigger ids naming the STDP rule are ignored like the reference does
+    from nbnet_file import records, write_nbnet
+    wl = W.trigger_rules(n=1500, k_in=80, seed=9, with_stdp=True)
+    s = wl.syn
+    S = s.count
+    rng = np.random.default_rng(4)
+    maxw = rng.uniform(0.2, 0.4, S).astype(np.float32)
+    wl.rules["max_weight"] = maxw
+    wl.stdp["max_weight"] = maxw
+    rule_id = np.where(s.plastic != 0, 2, wl.rules["rule"]).astype(np.int32)
+    trig_id = wl.rules["trigger"].copy()
+    static = (rule_id < 0) & (trig_id < 0)
+    trig_id[static & (rng.random(S) < 0.05)] = 2  # triggers of a rule without a trigger side
+    n = wl.spec.n
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row_ptr))
+    recs = records(pre, s.post, s.type, s.weight, s.delay, max_weight=maxw, rule=rule_id, trigger=trig_id)
+    recs = recs[rng.permutation(S)]
+    path = str(tmp_path / "rules.nbnet")
+    write_nbnet(path, {"learning_rules": ["R_STDP", "AdditiveKernalChange", "STDP"]}, recs)
+    st = wl.stdp
+    stdp_rule = dict(kind=L.RULE_STDP, max_ltp=st["max_ltp"], ltp_tau=st["ltp_tau"], max_ltd=st["max_ltd"],
+                     ltd_tau=st["ltd_tau"])
+    net = wl.make_net(nbnet=path, nbnet_rules=[wl.rules["rules"][0], wl.rules["rules"][1], stdp_rule])
+    T = 300
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    check_rules(net, run, wl)
+    plastic = s.plastic != 0
+    assert np.any(run.w[plastic] != s.weight[plastic])
+
+
+def test_nbnet_rule_ids_are_checked(gpu, tmp_path):
+    from nbnet_file import records, write_nbnet
+    wl = W.trigger_rules(n=600, k_in=20, seed=2)
+    s = wl.syn
+    n = wl.spec.n
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row_ptr))
+    path = str(tmp_path / "r.nbnet")
+    write_nbnet(path, {}, records(pre, s.post, s.type, s.weight, s.delay, rule=wl.rules["rule"],
+                                  trigger=wl.rules["trigger"]))
+    with pytest.raises(L.NbxError) as e:  # the file names rules 0 and 1; only one is given
+        wl.make_net(nbnet=path, nbnet_rules=[wl.rules["rules"][0]])
+    assert e.value.status == L.NBX_ERR_INVALID
+    bad = records(pre, s.post, s.type, s.weight, s.delay, rule=-2)
+    write_nbnet(path, {}, bad)
+    with pytest.raises(L.NbxError) as e:
+        wl.make_net(nbnet=path)
+    assert e.value.status == L.NBX_ERR_INVALID
