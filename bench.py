@@ -280,12 +280,13 @@ def models_bench(steps: int) -> dict:
 
 
 def rules_bench(steps: int) -> dict:
-    """Trigger-driven learning rules at config-2 size: 1e5 LIF neurons, in-degree 1000, 40% of E->E
-    synapses R-STDP, 40% additive kernel, 3% of all synapses triggers (graph engine)."""
+    """Trigger-driven learning rules at config-2 size: 1e5 LIF neurons, in-degree 1000, 80% of E->E
+    synapses split over R-STDP, additive-kernel and cerebellar rules, 3% of all synapses triggers
+    (graph engine)."""
     from neuronbridge_amd import _lib as L
     from neuronbridge_amd import workloads as W
     from neuronbridge_amd.hiptime import Event
-    wl = W.trigger_rules(n=100_000, k_in=1000)
+    wl = W.trigger_rules(n=100_000, k_in=1000, cerebellar=True)
     net = wl.make_net()
     engine = net.pick_engine()
     w0 = net.weights()
@@ -306,13 +307,14 @@ def rules_bench(steps: int) -> dict:
     changed = net.weights() != w0
     net.close()
     n = wl.spec.n
-    return {"workload": "config-2 LIF network (1e5 neurons, 1e8 synapses) with R-STDP and additive-kernel rules "
-                        f"on 80% of E->E synapses, 3% triggers, {steps} steps",
+    return {"workload": "config-2 LIF network (1e5 neurons, 1e8 synapses) with R-STDP, additive-kernel and "
+                        f"cerebellar rules on 80% of E->E synapses, 3% triggers, {steps} steps",
             "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
             "ms_per_step": ms / steps, "neuron_steps_per_s": n * steps / (ms * 1e-3),
             "syn_events_per_s": ev / (ms * 1e-3), "spike_rate_hz": spikes / n / (steps * 0.1e-3),
             "rstdp_weights_changed": float(changed[rule == 0].mean()),
-            "additive_weights_changed": float(changed[rule == 1].mean())}
+            "additive_weights_changed": float(changed[rule == 1].mean()),
+            "cerebellar_weights_changed": float(changed[rule == 2].mean())}
 
 
 def adjoint_bench(reps: int) -> dict:

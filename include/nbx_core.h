@@ -101,14 +101,27 @@ typedef struct nbx_stdp_desc {
  *   NBX_RULE_ADDITIVE  AdditiveKernalChange.cpp:64-82 — pre arrival: w += a1pre, Apre decays with
  *                      ltp_tau and jumps by 1; a trigger arrival adds a2prepre * Apre to each
  *                      plastic input of its target
+ *   NBX_RULE_CEREBELLAR CerebullarLearningRule.cpp:56-134 — pre arrival: w += a1pre * U(seed, rule,
+ *                      connection index, step); a trigger arrival adds a2prepre * kernel(elapsed)
+ *                      for every arrival of each plastic input within the kernel's window, newest
+ *                      first, weights bounded below by min_weight; the kernel is
+ *                      (t - initpos) / width * exp(1 - (t - initpos) / width) tabulated every
+ *                      kernel_step ms (width = maxpos - initpos). Each synapse keeps its last
+ *                      `history` arrival steps (0: 16); a run that would drop one still inside the
+ *                      window stops with NBX_ERR_STATE.
  * Weights stay in [0, max_weight] of their synapse (Interconnections::UpdateWeight). */
-enum { NBX_RULE_STDP = 0, NBX_RULE_RSTDP = 1, NBX_RULE_ADDITIVE = 2 }; /* STDP: nbx_net_assign_nbnet_rules only */
+enum { NBX_RULE_STDP = 0, NBX_RULE_RSTDP = 1, NBX_RULE_ADDITIVE = 2, NBX_RULE_CEREBELLAR = 3 };
+/* (NBX_RULE_STDP: nbx_net_assign_nbnet_rules only) */
 typedef struct nbx_rule_desc {
     int32_t kind;                         /* NBX_RULE_* */
     float max_ltp, ltp_tau, max_ltd, ltd_tau; /* R-STDP; ADDITIVE uses ltp_tau */
     float reward, punishment;             /* R-STDP trigger factors (R_STDP.h:16-18: 1, -1) */
     int32_t clear_eligibility;            /* R-STDP (R_STDP.h:20: true) */
-    float a1pre, a2prepre;                /* ADDITIVE (AdditiveKernalChange.h:8-10: 0.002, -0.08) */
+    float a1pre, a2prepre;                /* ADDITIVE (AdditiveKernalChange.h:8-10: 0.002, -0.08), CEREBELLAR */
+    float initpos, maxpos, kernel_step, min_weight; /* CEREBELLAR (CerebullarLearningRule.h:9-19: 120, 150,
+                                                       0.1 ms, 0) */
+    uint32_t seed;                        /* CEREBELLAR random_seed (17) */
+    int32_t history;                      /* CEREBELLAR arrivals kept per synapse (0: 16) */
 } nbx_rule_desc;
 
 /* Engines for nbx_net_run. */

@@ -52,13 +52,15 @@ class ModelDesc(C.Structure):
     ]
 
 
-RULE_STDP, RULE_RSTDP, RULE_ADDITIVE = 0, 1, 2
+RULE_STDP, RULE_RSTDP, RULE_ADDITIVE, RULE_CEREBELLAR = 0, 1, 2, 3
 
 
 class RuleDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("max_ltp", C.c_float), ("ltp_tau", C.c_float), ("max_ltd", C.c_float),
                 ("ltd_tau", C.c_float), ("reward", C.c_float), ("punishment", C.c_float),
-                ("clear_eligibility", C.c_int32), ("a1pre", C.c_float), ("a2prepre", C.c_float)]
+                ("clear_eligibility", C.c_int32), ("a1pre", C.c_float), ("a2prepre", C.c_float),
+                ("initpos", C.c_float), ("maxpos", C.c_float), ("kernel_step", C.c_float), ("min_weight", C.c_float),
+                ("seed", C.c_uint32), ("history", C.c_int32)]
 
 
 def model_channels(model: int) -> int:

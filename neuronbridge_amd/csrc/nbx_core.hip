@@ -86,6 +86,34 @@ void trace_tables(float dt, float tau_a, float tau_b, std::vector<float>* ta, st
     }
 }
 
+// Cerebellar kernel (CerebullarLearningRule.cpp:56-88): the window is grown from maxpos in
+// kernel steps until the kernel falls below 1e-2, then shifted by initpos; all in float.
+void cerebellar_kernel(const nbx_rule_desc& q, std::vector<float>* tab, float* maxtime, float* inv_step) {
+    float maxpos = q.maxpos;
+    if (maxpos <= q.initpos) maxpos = q.initpos + 1e-6f;
+    const float width = maxpos - q.initpos;
+    float mt = maxpos;
+    for (;;) {
+        const float value = (1.0f / width) * mt * std::exp(-(mt / width) + 1.0f);
+        if (value < 1e-2f) break;
+        mt += q.kernel_step;
+    }
+    mt += q.initpos;
+    *maxtime = mt;
+    *inv_step = 1.0f / q.kernel_step;
+    const int32_t len = (int32_t)(mt * *inv_step) + 1;
+    tab->resize((size_t)len);
+    for (int32_t i = 0; i < len; ++i) {
+        const float time = (float)i * q.kernel_step;
+        if (time < q.initpos) {
+            (*tab)[(size_t)i] = 0.0f;
+        } else {
+            const float shifted = time - q.initpos;
+            (*tab)[(size_t)i] = (1.0f / width) * shifted * std::exp(-(shifted / width) + 1.0f);
+        }
+    }
+}
+
 // Device tables of the trigger-driven rules: rule records with their trace tables, per-synapse
 // rule / trigger ids and weight bounds, rule-plastic inputs per (post, rule) and trigger synapses
 // per post, both in ascending synapse order.
@@ -112,6 +140,18 @@ int32_t finalize_rules(nbx_net* net) {
         tabs.insert(tabs.end(), ta.begin(), ta.end());
         tabs.insert(tabs.end(), tb.begin(), tb.end());
         if (q.kind == NBX_RULE_RSTDP) net->rstdp = true;
+        d.min_weight = q.min_weight;
+        d.seed = q.seed;
+        d.kern_off = d.kern_len = 0;
+        d.maxtime = d.inv_step = 0.0f;
+        if (q.kind == NBX_RULE_CEREBELLAR) {
+            std::vector<float> kt;
+            cerebellar_kernel(q, &kt, &d.maxtime, &d.inv_step);
+            d.kern_off = (int32_t)tabs.size();
+            d.kern_len = (int32_t)kt.size();
+            tabs.insert(tabs.end(), kt.begin(), kt.end());
+            net->ccap = std::max(net->ccap, q.history > 0 ? q.history : 16);
+        }
     }
     NBX_PASS(net->upload(&net->d_rules, rd.data(), rd.size()));
     NBX_PASS(net->upload(&net->d_rtab, tabs.data(), tabs.size()));
@@ -140,6 +180,19 @@ int32_t finalize_rules(nbx_net* net) {
         const int32_t post = syn_post(net->h_syn[j]);
         if (net->h_rule[j] >= 0) rin_syn[(size_t)rfill[(size_t)post * nr + net->h_rule[j]]++] = (int64_t)j;
         if (net->h_trig[j] >= 0) tin_syn[(size_t)tfill[(size_t)post]++] = (int64_t)j;
+    }
+    if (net->ccap > 0) {  // cerebellar arrival histories, one row per cerebellar synapse
+        std::vector<int32_t> cidx(S, -1), conn(S);
+        int32_t rows = 0;
+        for (size_t j = 0; j < S; ++j) {
+            conn[j] = net->h_src_index.size() == S ? (int32_t)net->h_src_index[j] : (int32_t)j;
+            if (net->h_rule[j] >= 0 && rd[(size_t)net->h_rule[j]].kind == NBX_RULE_CEREBELLAR) cidx[j] = rows++;
+        }
+        NBX_PASS(net->upload(&net->d_cidx, cidx.data(), S));
+        NBX_PASS(net->upload(&net->d_conn_index, conn.data(), S));
+        std::vector<int32_t> zero((size_t)std::max(rows, 1), 0);
+        NBX_PASS(net->upload(&net->d_chead, zero.data(), zero.size()));
+        NBX_PASS(net->alloc(&net->d_chist, (size_t)std::max(rows, 1) * net->ccap));
     }
     NBX_PASS(net->upload(&net->d_rin_ptr, rin_ptr.data(), rin_ptr.size()));
     NBX_PASS(net->upload(&net->d_rin_syn, rin_syn.data(), rin_syn.size()));
@@ -182,6 +235,11 @@ int32_t require_finalized(const nbx_net* net) {
 int32_t check_device_error(nbx_net* net) {
     int32_t e = 0;
     NBX_HIP_TRY(hipMemcpy(&e, net->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e & kErrHistory) {
+        set_error("cerebellar arrival history overflow: an arrival still inside the kernel window was "
+                  "overwritten (raise nbx_rule_desc.history)");
+        return NBX_ERR_STATE;
+    }
     if (e != 0) {
         set_error("device-side error flag " + std::to_string(e) + " (grid barrier time-out)");
         return NBX_ERR_STATE;
@@ -452,6 +510,35 @@ int32_t nbx_net_set_stdp(nbx_net* net, const nbx_stdp_desc* sd) {
     return NBX_OK;
 }
 
+}  // extern "C"
+
+namespace nbx {
+const char* rule_desc_error(const nbx_rule_desc& q, bool allow_stdp) {
+    switch (q.kind) {
+        case NBX_RULE_STDP:
+            if (!allow_stdp) return "pair STDP is set with nbx_net_set_stdp";
+            if (!(q.ltp_tau > 0.0f && q.ltd_tau > 0.0f)) return "learning rule time constants must be positive";
+            return nullptr;
+        case NBX_RULE_RSTDP:
+            if (!(q.ltp_tau > 0.0f && q.ltd_tau > 0.0f)) return "learning rule time constants must be positive";
+            return nullptr;
+        case NBX_RULE_ADDITIVE:
+            if (!(q.ltp_tau > 0.0f)) return "learning rule time constants must be positive";
+            return nullptr;
+        case NBX_RULE_CEREBELLAR:
+            if (!(q.kernel_step > 0.0f) || !std::isfinite(q.initpos) || !std::isfinite(q.maxpos))
+                return "cerebellar kernel needs a positive step and finite positions";
+            if (q.history < 0 || q.history > 4096) return "cerebellar history must be in [0, 4096]";
+            if ((q.maxpos - q.initpos) / q.kernel_step > 1.0e6f)
+                return "cerebellar kernel window too long for its step";
+            return nullptr;
+        default: return "unknown learning rule kind";
+    }
+}
+}  // namespace nbx
+
+extern "C" {
+
 int32_t nbx_net_set_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t count, const int32_t* rule,
                           const int32_t* trigger, const float* max_weight) {
     NBX_CHECK(net && (rules || count == 0), "null argument");
@@ -460,10 +547,8 @@ int32_t nbx_net_set_rules(nbx_net* net, const nbx_rule_desc* rules, int32_t coun
     NBX_CHECK(count >= 0 && count <= kMaxRules, "rule count must be in [0, 64]");
     NBX_CHECK(count == 0 || (rule && trigger && max_weight), "missing per-synapse rule arrays");
     for (int32_t r = 0; r < count; ++r) {
-        const nbx_rule_desc& q = rules[r];
-        NBX_CHECK(q.kind == NBX_RULE_RSTDP || q.kind == NBX_RULE_ADDITIVE, "unknown learning rule kind");
-        NBX_CHECK(q.ltp_tau > 0.0f && (q.kind == NBX_RULE_ADDITIVE || q.ltd_tau > 0.0f),
-                  "learning rule time constants must be positive");
+        const char* why = rule_desc_error(rules[r], false);
+        NBX_CHECK(why == nullptr, why ? why : "");
     }
     const size_t S = (size_t)net->S;
     std::vector<int8_t> hr(S, -1), ht(S, -1);
@@ -838,6 +923,7 @@ int32_t nbx_net_run_propagation(nbx_net* net) {
     if (net->pull_mode && net->has_sparse) NBX_HIP_TRY(launch_pull(net->args(), nullptr, net->t, net->stream));
     if (!net->rules.empty()) NBX_HIP_TRY(launch_trigger(net->args(), nullptr, net->t, net->stream));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    if (net->ccap > 0) NBX_RC(check_device_error(net));
     net->phase = 2;
     return NBX_OK;
 }

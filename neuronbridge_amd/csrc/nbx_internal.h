@@ -99,7 +99,13 @@ struct RuleDev {
     int32_t kind, clear;
     float max_ltp, max_ltd, reward, punishment, a1pre, a2prepre;
     int32_t tab_off, tab_len;
+    // cerebellar: kernel table at rtab[kern_off .. kern_off + kern_len), its window (ms) and 1/step
+    float min_weight, maxtime, inv_step;
+    int32_t kern_off, kern_len;
+    uint32_t seed;
 };
+// Device error flag bits (nbx_net::d_err)
+constexpr int32_t kErrBarrier = 1, kErrHistory = 2;
 constexpr int32_t kMaxRules = 64;
 
 // Device pointers and scalars the step kernels need; passed by value.
@@ -165,6 +171,11 @@ struct StepArgs {
     int32_t* tpost_stamp;         // [n] step a post last had a trigger arrival
     int32_t* tposts;              // [2][n] posts with trigger arrivals, by step parity
     int32_t* tcount;              // [2]
+    const int32_t* conn_index;    // [S] connection index of each synapse (cerebellar random stream)
+    const int32_t* cidx;          // [S] row of a cerebellar synapse in chist / chead, else -1
+    int32_t* chist;               // [rows][ccap] arrival steps, circular
+    int32_t* chead;               // [rows] arrivals so far
+    int32_t ccap;
     // recording
     int32_t* counts;
     uint32_t* raster;

@@ -563,12 +563,8 @@ int32_t nbx_net_assign_nbnet_rules(nbx_net* net, const nbx_rule_desc* rules, int
     int32_t stdp_rule = -1;
     for (int32_t r = 0; r < count; ++r) {
         const nbx_rule_desc& q = rules[r];
-        if (q.kind != NBX_RULE_STDP && q.kind != NBX_RULE_RSTDP && q.kind != NBX_RULE_ADDITIVE) {
-            set_error("unknown learning rule kind");
-            return NBX_ERR_INVALID;
-        }
-        if (!(q.ltp_tau > 0.0f) || (q.kind != NBX_RULE_ADDITIVE && !(q.ltd_tau > 0.0f))) {
-            set_error("learning rule time constants must be positive");
+        if (const char* why = rule_desc_error(q, true)) {
+            set_error(why);
             return NBX_ERR_INVALID;
         }
         if (q.kind != NBX_RULE_STDP) continue;

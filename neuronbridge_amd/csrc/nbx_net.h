@@ -119,6 +119,8 @@ struct nbx_net {
     float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr, *d_smaxw = nullptr;
     int64_t *d_rin_ptr = nullptr, *d_rin_syn = nullptr, *d_tin_ptr = nullptr, *d_tin_syn = nullptr;
     int32_t *d_tstamp = nullptr, *d_tpost_stamp = nullptr, *d_tposts = nullptr, *d_tcount = nullptr;
+    int32_t *d_conn_index = nullptr, *d_cidx = nullptr, *d_chist = nullptr, *d_chead = nullptr;
+    int32_t ccap = 0;
     bool rstdp = false;
     unsigned long long* d_events = nullptr;
     int32_t* d_tbase = nullptr;
@@ -207,6 +209,11 @@ struct nbx_net {
         A.tpost_stamp = d_tpost_stamp;
         A.tposts = d_tposts;
         A.tcount = d_tcount;
+        A.conn_index = d_conn_index;
+        A.cidx = d_cidx;
+        A.chist = d_chist;
+        A.chead = d_chead;
+        A.ccap = ccap;
         A.counts = d_counts;
         A.raster = d_raster;
         A.rec_t0 = rec_t0;
@@ -251,4 +258,5 @@ int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
 int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLOTS
+const char* rule_desc_error(const nbx_rule_desc& q, bool allow_stdp);  // null when valid
 }  // namespace nbx

@@ -209,10 +209,49 @@ __device__ inline float update_weight(float w, float inc, float wmax) {
     return w;
 }
 
+// ApplyBoundedWeightIncrement (CerebullarLearningRule.cpp:27-38)
+__device__ inline float bounded_increment(float w, float inc, float wmax, float wmin) {
+    w += inc;
+    if (w > wmax) w = wmax;
+    else if (w < wmin) w = wmin;
+    return w;
+}
+
+// StableUnitRandom (CerebullarLearningRule.cpp:11-25): 24-bit uniform keyed by
+// (seed, rule, connection, step)
+__device__ inline float stable_unit_random(uint32_t seed, int32_t rule_id, int32_t synapse_id, int32_t step) {
+    uint32_t v = seed;
+    v ^= (uint32_t)rule_id + 0x9e3779b9u;
+    v *= 1664525u;
+    v += 1013904223u;
+    v ^= (uint32_t)synapse_id + 0x85ebca6bu;
+    v *= 2246822519u;
+    v ^= (uint32_t)step + 0xc2b2ae35u;
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return (float)(v & 0x00ffffffu) / (float)0x01000000u;
+}
+
 // Pre-side update of a rule-plastic synapse at its arrival (after its weight was delivered).
 __device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t r) {
     const RuleDev R = A.rules[r];
     float apre, apost;
+    if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:89-103
+        const float u = stable_unit_random(R.seed, r, A.conn_index[j], t);
+        A.w[j] = bounded_increment(A.w[j], R.a1pre * u, A.maxw[j], R.min_weight);
+        // keep the arrival step; dropping one still inside the kernel window is an error
+        const int64_t c = A.cidx[j];
+        const int32_t h = A.chead[c];
+        int32_t* ring = A.chist + c * A.ccap;
+        if (h >= A.ccap && (float)ring[h % A.ccap] * A.P.dt > (float)t * A.P.dt - R.maxtime)
+            atomicOr(A.err, kErrHistory);
+        ring[h % A.ccap] = t;
+        A.chead[c] = h + 1;
+        return;
+    }
     if (R.kind == NBX_RULE_RSTDP) {  // R_STDP.cpp:40-48
         rule_decay(A, R, j, t, apre, apost);
         apre += 1.0f;
@@ -339,6 +378,29 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_trigger(StepArgs A, const
             const RuleDev R = A.rules[r];
             const float factor = syn_type(A.syn[tj]) == 1 ? R.punishment : R.reward;
             const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
+            if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:105-129
+                const float realtime = (float)t * A.P.dt;
+                const float thr = realtime - R.maxtime;
+                for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+                    const int64_t sid = A.rin_syn[j];
+                    const int64_t c = A.cidx[sid];
+                    const int32_t h = A.chead[c];
+                    const int32_t* ring = A.chist + c * A.ccap;
+                    float w = A.w[sid];
+                    const float wmax = A.maxw[sid];
+                    for (int32_t q2 = h - 1; q2 >= 0 && q2 >= h - A.ccap; --q2) {  // newest first
+                        const float et = (float)ring[q2 % A.ccap] * A.P.dt;
+                        if (!(et > thr)) break;
+                        int32_t ti = (int32_t)((realtime - et) * R.inv_step);
+                        if (ti < 0) continue;
+                        if (ti >= R.kern_len) ti = R.kern_len - 1;
+                        w = bounded_increment(w, R.a2prepre * A.rtab[R.kern_off + ti], wmax, R.min_weight);
+                    }
+                    A.w[sid] = w;
+                }
+                __syncthreads();
+                continue;
+            }
             for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
                 const int64_t sid = A.rin_syn[j];
                 float apre, apost;

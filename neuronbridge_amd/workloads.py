@@ -152,13 +152,15 @@ def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L
 
 
 def trigger_rules(n: int = 2000, k_in: int = 100, seed: int = SEED_BASE + 7, trigger_fraction: float = 0.03,
-                  with_stdp: bool = False) -> Workload:
+                  with_stdp: bool = False, cerebellar: bool = False, history: int = 0) -> Workload:
     """The config-2 LIF network with the legacy trigger-driven rules on its E->E synapses:
     40% R-STDP (rule 0: Max_LTP 0.92, LTP_tau 16.8, Max_LTD 0.53, LTD_tau 33.1, reward 0.01,
     punishment -0.01, eligibility cleared after a trigger), 40% additive kernel (rule 1: a1pre 5e-4,
     a2prepre -0.02, LTP_tau 16.8), weights bounded by 0.3; trigger_fraction of all synapses are
     triggers (half of each rule; inhibitory trigger synapses punish). with_stdp makes the remaining
-    E->E synapses pair-STDP plastic."""
+    E->E synapses pair-STDP plastic. cerebellar moves a third of the rule synapses and triggers to a
+    cerebellar rule (rule 2: alpha 2e-4, beta -5e-3, kernel from 2 to 6 ms in 0.1 ms steps, floor
+    0.01, seed 17, `history` arrivals kept per synapse)."""
     wl = sparse_lif(n=n, k_in=k_in, steps=0, seed=seed)
     s = wl.syn
     rng = np.random.default_rng(seed)
@@ -167,17 +169,21 @@ def trigger_rules(n: int = 2000, k_in: int = 100, seed: int = SEED_BASE + 7, tri
     ee = (pre < n_exc) & (s.post < n_exc)
     u = rng.random(s.count)
     rule = np.full(s.count, -1, np.int32)
-    rule[ee & (u < 0.4)] = 0
-    rule[ee & (u >= 0.4) & (u < 0.8)] = 1
+    nr = 3 if cerebellar else 2
+    for r in range(nr):
+        rule[ee & (u >= 0.8 * r / nr) & (u < 0.8 * (r + 1) / nr)] = r
     trig = np.full(s.count, -1, np.int32)
     t_sel = (rule < 0) & (rng.random(s.count) < trigger_fraction)
-    trig[t_sel] = rng.integers(0, 2, int(t_sel.sum()))
+    trig[t_sel] = rng.integers(0, nr, int(t_sel.sum()))
     if with_stdp:
         s.plastic = ((rule < 0) & (trig < 0) & ee).astype(np.uint8)
         wl.stdp = dict(max_ltp=0.001, ltp_tau=20.0, max_ltd=0.0012, ltd_tau=20.0, w_max=0.3)
     rules = [dict(kind=L.RULE_RSTDP, max_ltp=0.92, ltp_tau=16.8, max_ltd=0.53, ltd_tau=33.1, reward=0.01,
                   punishment=-0.01, clear_eligibility=1),
              dict(kind=L.RULE_ADDITIVE, ltp_tau=16.8, a1pre=5e-4, a2prepre=-0.02)]
+    if cerebellar:
+        rules.append(dict(kind=L.RULE_CEREBELLAR, a1pre=2e-4, a2prepre=-5e-3, initpos=2.0, maxpos=6.0,
+                          kernel_step=0.1, min_weight=0.01, seed=17, history=history))
     wl.rules = {"rules": rules, "rule": rule, "trigger": trig, "max_weight": np.full(s.count, 0.3, np.float32)}
     wl.name = "trigger_rules"
     return wl

@@ -61,16 +61,18 @@ class OrcNet(C.Structure):
         ("in_ptr", _i64p), ("in_syn", _i64p),
         ("nrules", C.c_int32), ("rules", C.c_void_p), ("syn_rule", C.POINTER(C.c_int8)),
         ("syn_trig", C.POINTER(C.c_int8)), ("maxw", _f32p), ("elig", _f32p), ("stdp_maxw", _f32p),
+        ("conn_index", _i32p),
     ]
 
 
-RULE_RSTDP, RULE_ADDITIVE = 1, 2
+RULE_RSTDP, RULE_ADDITIVE, RULE_CEREBELLAR = 1, 2, 3
 
 
 class OrcRule(C.Structure):
     _fields_ = [("kind", C.c_int32), ("clear", C.c_int32), ("max_ltp", C.c_float), ("ltp_tau", C.c_float),
                 ("max_ltd", C.c_float), ("ltd_tau", C.c_float), ("reward", C.c_float), ("punishment", C.c_float),
-                ("a1pre", C.c_float), ("a2prepre", C.c_float)]
+                ("a1pre", C.c_float), ("a2prepre", C.c_float), ("initpos", C.c_float), ("maxpos", C.c_float),
+                ("kernel_step", C.c_float), ("min_weight", C.c_float), ("seed", C.c_uint32)]
 
 
 def build() -> None:
@@ -106,6 +108,10 @@ def lib():
         L.orc_run.argtypes = [C.POINTER(OrcNet), C.c_int32, _f32p, _i32p, _i32p, _i32p, _i32p,
                               C.c_int32, C.c_int32, _i32p, _u32p, _i64p]
         L.orc_stdp_replay.argtypes = [C.c_float] * 7 + [_u8p, _u8p, C.c_int32, _f32p]
+        L.orc_stable_unit_random.restype = C.c_float
+        L.orc_stable_unit_random.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_cerebellar_kernel.restype = C.c_int32
+        L.orc_cerebellar_kernel.argtypes = [C.POINTER(OrcRule), _f32p, C.c_int32, _f32p, _f32p]
         _lib = L
     return _lib
 

@@ -287,6 +287,49 @@ static void stdp_clip(orc_net* net, int64_t s) {
 
 /* ---------------------------------------------------------------- trigger-driven rules */
 
+float orc_stable_unit_random(uint32_t seed, int32_t rule_id, int32_t synapse_id, int32_t spike_time) {
+    uint32_t value = seed;
+    value ^= (uint32_t)rule_id + 0x9e3779b9u;
+    value *= 1664525u;
+    value += 1013904223u;
+    value ^= (uint32_t)synapse_id + 0x85ebca6bu;
+    value *= 2246822519u;
+    value ^= (uint32_t)spike_time + 0xc2b2ae35u;
+    value ^= value >> 16;
+    value *= 0x7feb352du;
+    value ^= value >> 15;
+    value *= 0x846ca68bu;
+    value ^= value >> 16;
+    return (float)(value & 0x00ffffffu) / (float)0x01000000u;
+}
+
+int32_t orc_cerebellar_kernel(const orc_rule* r, float* tab, int32_t cap, float* maxtime, float* inv_step) {
+    float maxpos = r->maxpos;
+    if (maxpos <= r->initpos) maxpos = r->initpos + 1e-6f;
+    const float width = maxpos - r->initpos;
+    float mt = maxpos;
+    for (;;) {
+        const float value = (1.0f / width) * mt * expf(-(mt / width) + 1.0f);
+        if (value < 1e-2f) break;
+        mt += r->kernel_step;
+    }
+    mt += r->initpos;
+    const float inv = 1.0f / r->kernel_step;
+    const int32_t n = (int32_t)(mt * inv) + 1;
+    if (maxtime) *maxtime = mt;
+    if (inv_step) *inv_step = inv;
+    for (int32_t i = 0; tab && i < n && i < cap; ++i) {
+        const float time = (float)i * r->kernel_step;
+        if (time < r->initpos) {
+            tab[i] = 0.0f;
+        } else {
+            const float shifted = time - r->initpos;
+            tab[i] = (1.0f / width) * shifted * expf(-(shifted / width) + 1.0f);
+        }
+    }
+    return n;
+}
+
 /* Interconnections::UpdateWeight (Interconnections.h:120-128) */
 static float update_weight(float w, float inc, float wmax) {
     w += inc;
@@ -306,9 +349,70 @@ static void rule_set_update(orc_net* net, const orc_rule* R, int64_t s, int32_t 
     net->slast[s] = t;
 }
 
+typedef struct {
+    float time;      /* arrival time in ms */
+    int64_t synapse;
+} cb_entry;
+
+/* per (target, rule) buffer of plastic-input arrival times, oldest first (BufferedActivityTime) */
+typedef struct {
+    cb_entry* e;
+    int64_t first, count, cap;
+} cb_buffer;
+
+typedef struct {
+    float* tab;
+    int32_t len;
+    float maxtime, inv_step;
+} cb_kernel;
+
+typedef struct {
+    int64_t *rin_ptr, *rin_syn, *tin_ptr, *tin_syn; /* ascending synapse order in every list */
+    int32_t *tstamp, *post_mark, *posts;
+    int32_t nposts;
+    cb_buffer* cb;   /* [n * nrules] */
+    cb_kernel* ck;   /* [nrules] */
+    size_t ncb;
+    int nck;
+} rule_lists;
+
+/* ApplyBoundedWeightIncrement (CerebullarLearningRule.cpp:27-38) */
+static float bounded_increment(float w, float inc, float wmax, float wmin) {
+    w += inc;
+    if (w > wmax) w = wmax;
+    else if (w < wmin) w = wmin;
+    return w;
+}
+
 /* Pre-side update at the synapse's own arrival (after its weight was delivered). */
-static void rule_pre(orc_net* net, int64_t s, int32_t t) {
-    const orc_rule* R = &net->rules[net->syn_rule[s]];
+static void rule_pre(orc_net* net, rule_lists* L, int64_t s, int32_t t) {
+    const int r = net->syn_rule[s];
+    const orc_rule* R = &net->rules[r];
+    if (R->kind == ORC_RULE_CEREBELLAR) { /* CerebullarLearningRule.cpp:89-103 */
+        const float realtime = (float)t * net->m.dt;
+        const int32_t cid = net->conn_index ? net->conn_index[s] : (int32_t)s;
+        const float u = orc_stable_unit_random(R->seed, r, cid, t);
+        net->w[s] = bounded_increment(net->w[s], R->a1pre * u, net->maxw[s], R->min_weight);
+        cb_buffer* b = &L->cb[(size_t)net->post[s] * net->nrules + r];
+        const float thr = realtime - L->ck[r].maxtime;
+        while (b->count > 0 && !(b->e[b->first].time > thr)) { /* expired: never read again */
+            b->first++;
+            b->count--;
+        }
+        if (b->first + b->count == b->cap) {
+            if (b->first > 0) {
+                memmove(b->e, b->e + b->first, sizeof(cb_entry) * (size_t)b->count);
+                b->first = 0;
+            } else {
+                b->cap = b->cap ? 2 * b->cap : 16;
+                b->e = (cb_entry*)realloc(b->e, sizeof(cb_entry) * (size_t)b->cap);
+            }
+        }
+        b->e[b->first + b->count].time = realtime;
+        b->e[b->first + b->count].synapse = s;
+        b->count++;
+        return;
+    }
     if (R->kind == ORC_RULE_RSTDP) { /* R_STDP.cpp:40-48 */
         rule_set_update(net, R, s, t);
         net->apre[s] += 1.0f;
@@ -319,12 +423,6 @@ static void rule_pre(orc_net* net, int64_t s, int32_t t) {
         net->apre[s] += 1.0f;
     }
 }
-
-typedef struct {
-    int64_t *rin_ptr, *rin_syn, *tin_ptr, *tin_syn; /* ascending synapse order in every list */
-    int32_t *tstamp, *post_mark, *posts;
-    int32_t nposts;
-} rule_lists;
 
 static void rule_lists_build(const orc_net* net, rule_lists* L) {
     const int n = net->m.n, nr = net->nrules;
@@ -355,6 +453,17 @@ static void rule_lists_build(const orc_net* net, rule_lists* L) {
     for (int i = 0; i < n; ++i) L->post_mark[i] = -1;
     L->posts = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
     L->nposts = 0;
+    L->cb = (cb_buffer*)calloc((size_t)n * nr, sizeof(cb_buffer));
+    L->ck = (cb_kernel*)calloc((size_t)nr, sizeof(cb_kernel));
+    L->ncb = (size_t)n * nr;
+    L->nck = nr;
+    for (int r = 0; r < nr; ++r) {
+        if (net->rules[r].kind != ORC_RULE_CEREBELLAR) continue;
+        cb_kernel* k = &L->ck[r];
+        k->len = orc_cerebellar_kernel(&net->rules[r], NULL, 0, &k->maxtime, &k->inv_step);
+        k->tab = (float*)malloc(sizeof(float) * (size_t)k->len);
+        orc_cerebellar_kernel(&net->rules[r], k->tab, k->len, NULL, NULL);
+    }
 }
 
 static void rule_lists_free(rule_lists* L) {
@@ -365,6 +474,10 @@ static void rule_lists_free(rule_lists* L) {
     free(L->tstamp);
     free(L->post_mark);
     free(L->posts);
+    for (size_t k = 0; k < L->ncb; ++k) free(L->cb[k].e);
+    free(L->cb);
+    for (int r = 0; r < L->nck; ++r) free(L->ck[r].tab);
+    free(L->ck);
 }
 
 /* Trigger arrivals of tick t: each target applies its arrived trigger synapses in ascending
@@ -379,6 +492,27 @@ static void rule_triggers(orc_net* net, rule_lists* L, int32_t t) {
             if (L->tstamp[tj] != t) continue;
             const int r = net->syn_trig[tj];
             const orc_rule* R = &net->rules[r];
+            if (R->kind == ORC_RULE_CEREBELLAR) { /* CerebullarLearningRule.cpp:105-129 */
+                const float realtime = (float)t * net->m.dt;
+                const cb_kernel* k = &L->ck[r];
+                cb_buffer* b = &L->cb[(size_t)p * nr + r];
+                const float thr = realtime - k->maxtime;
+                int64_t keep = 0;  /* newest first; stop at the first arrival outside the window */
+                for (int64_t q2 = b->count - 1; q2 >= 0; --q2) {
+                    const cb_entry* e = &b->e[b->first + q2];
+                    if (!(e->time > thr)) break;
+                    keep++;
+                    const float elapse = realtime - e->time;
+                    int ti = (int)(elapse * k->inv_step);
+                    if (ti < 0) continue;
+                    if (ti >= k->len) ti = k->len - 1;
+                    const int64_t sj = e->synapse;
+                    net->w[sj] = bounded_increment(net->w[sj], R->a2prepre * k->tab[ti], net->maxw[sj], R->min_weight);
+                }
+                b->first += b->count - keep; /* older arrivals are dropped */
+                b->count = keep;
+                continue;
+            }
             const float factor = net->type[tj] == 1 ? R->punishment : R->reward;
             for (int64_t j = L->rin_ptr[(size_t)p * nr + r]; j < L->rin_ptr[(size_t)p * nr + r + 1]; ++j) {
                 const int64_t s = L->rin_syn[j];
@@ -487,7 +621,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                             stdp_clip(net, s);
                         }
                         if (rules_on) {
-                            if (net->syn_rule[s] >= 0) rule_pre(net, s, t);
+                            if (net->syn_rule[s] >= 0) rule_pre(net, &rl, s, t);
                             if (net->syn_trig[s] >= 0) {
                                 rl.tstamp[s] = t;
                                 if (rl.post_mark[net->post[s]] != t) {

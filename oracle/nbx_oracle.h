@@ -23,6 +23,7 @@
  *   L/Event/src/Spike/TimeDrivenInternalSpike.cpp:11-144 (group time = t + delay; post-learning)
  *   L/LearningRule/src/Rule/STDP.cpp:19-61, L/LearningRule/src/State/STDP_State.cpp:17-34
  *   L/LearningRule/src/Rule/R_STDP.cpp:19-70, L/LearningRule/src/Rule/AdditiveKernalChange.cpp:64-82
+ *   L/LearningRule/src/Rule/CerebullarLearningRule.cpp:9-134, L/LearningRule/inc/SpikeBuffer/BufferedActivityTime.h
  *   L/connections/inc/Interconnections.h:120-128 (UpdateWeight)
  *
  * Parity status of each function is listed in the header comment of nbx_oracle.c.
@@ -112,12 +113,15 @@ typedef struct orc_net {
     const float* maxw;       /* per synapse weight bound (Interconnections::maximum_weight) */
     float* elig;             /* per synapse R-STDP eligibility */
     const float* stdp_maxw;  /* per synapse STDP weight bound (STDP.cpp:29-34), NULL: w_max for all */
+    const int32_t* conn_index; /* per synapse connection index (Interconnections::Index), NULL: CSR position */
 } orc_net;
 
-enum { ORC_RULE_RSTDP = 1, ORC_RULE_ADDITIVE = 2 };
+enum { ORC_RULE_RSTDP = 1, ORC_RULE_ADDITIVE = 2, ORC_RULE_CEREBELLAR = 3 };
 typedef struct orc_rule {
     int32_t kind, clear;     /* clear: R-STDP ClearEligibilityAfterTrigger */
     float max_ltp, ltp_tau, max_ltd, ltd_tau, reward, punishment, a1pre, a2prepre;
+    float initpos, maxpos, kernel_step, min_weight; /* cerebellar (CerebullarLearningRule.h:9-21) */
+    uint32_t seed;
 } orc_rule;
 
 int orc_nstate(int model);
@@ -127,6 +131,11 @@ void orc_model_prepare(orc_model* m);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* PoissonRate hash (PoissonRate.cpp:7-21): 24-bit uniform in [0,1). */
 float orc_hash_uniform01(int32_t neuron, int32_t t);
+/* CerebullarLearningRule.cpp:11-25: 24-bit uniform keyed by (seed, rule, connection, step). */
+float orc_stable_unit_random(uint32_t seed, int32_t rule_id, int32_t synapse_id, int32_t spike_time);
+/* Cerebellar kernel table (CerebullarLearningRule.cpp:56-88); returns its length, writes the
+ * window (maxtimemeasured, ms) and 1/kernel_step; tab may be NULL for a size query. */
+int32_t orc_cerebellar_kernel(const orc_rule* r, float* tab, int32_t cap, float* maxtime, float* inv_step);
 /* Inverse-CDF thresholds of Poisson(lambda) in units of 2^-32; returns kmax. */
 int32_t orc_poisson_thresholds(double lambda, uint32_t* tab, int32_t cap);
 int32_t orc_poisson_count(uint32_t seed_lo, uint32_t seed_hi, int32_t neuron, int32_t t,

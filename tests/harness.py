@@ -139,8 +139,10 @@ class OracleRun:
             for kk, r in enumerate(rl):
                 arr[kk].kind = r["kind"]
                 arr[kk].clear = int(r.get("clear_eligibility", 0))
-                for f in ("max_ltp", "ltp_tau", "max_ltd", "ltd_tau", "reward", "punishment", "a1pre", "a2prepre"):
+                for f in ("max_ltp", "ltp_tau", "max_ltd", "ltd_tau", "reward", "punishment", "a1pre", "a2prepre",
+                          "initpos", "maxpos", "kernel_step", "min_weight"):
                     setattr(arr[kk], f, float(r.get(f, 0.0)))
+                arr[kk].seed = int(r.get("seed", 0))
             k(arr)
             self.syn_rule = np.ascontiguousarray(wl.rules["rule"], dtype=np.int8)
             self.syn_trig = np.ascontiguousarray(wl.rules["trigger"], dtype=np.int8)
@@ -157,6 +159,10 @@ class OracleRun:
                 net.slast = O.ptr(self.slast, C.c_int32)
             for arr_ in (self.syn_rule, self.syn_trig, self.maxw, self.elig):
                 k(arr_)
+            if wl.extra.get("conn_index") is not None:  # connection indices of a .nbnet table
+                ci = np.ascontiguousarray(wl.extra["conn_index"], dtype=np.int32)
+                k(ci)
+                net.conn_index = O.ptr(ci, C.c_int32)
             net.nrules = len(rl)
             net.rules = C.cast(arr, C.c_void_p)
             net.syn_rule = O.ptr(self.syn_rule, C.c_int8)

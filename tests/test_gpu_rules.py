@@ -22,10 +22,11 @@ def check_rules(net, run, wl):
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("with_stdp", [False, True], ids=["rules", "rules+stdp"])
+@pytest.mark.parametrize("with_stdp,cerebellar", [(False, False), (True, False), (False, True)],
+                         ids=["rules", "rules+stdp", "rules+cerebellar"])
 @pytest.mark.parametrize("engine", ["kernels", "graph"])
-def test_rule_network_bitexact(gpu, engine, with_stdp):
-    wl = W.trigger_rules(with_stdp=with_stdp)
+def test_rule_network_bitexact(gpu, engine, with_stdp, cerebellar):
+    wl = W.trigger_rules(with_stdp=with_stdp, cerebellar=cerebellar)
     T = 400
     net = wl.make_net()
     w0 = net.weights()
@@ -39,6 +40,8 @@ def test_rule_network_bitexact(gpu, engine, with_stdp):
     rule = wl.rules["rule"]
     assert np.mean(net.weights()[rule == 0] != w0[rule == 0]) > 0.05, "R-STDP never triggered"
     assert np.mean(net.weights()[rule == 1] != w0[rule == 1]) > 0.5
+    if cerebellar:
+        assert np.mean(net.weights()[rule == 2] != w0[rule == 2]) > 0.5
     if with_stdp:
         assert_bits_equal(net.stdp_state()[0][wl.syn.plastic != 0], run.apre[wl.syn.plastic != 0], "STDP Apre")
 
@@ -137,3 +140,38 @@ def test_nbnet_rule_ids_are_checked(gpu, tmp_path):
     with pytest.raises(L.NbxError) as e:
         wl.make_net(nbnet=path)
     assert e.value.status == L.NBX_ERR_INVALID
+
+
+def test_cerebellar_history_overflow_is_reported(gpu):
+    # one arrival kept per synapse while the kernel window spans many arrivals: the run stops with
+    # an error instead of silently dropping arrivals the next trigger would need
+    wl = W.trigger_rules(n=1200, k_in=60, seed=6, cerebellar=True, history=1)
+    net = wl.make_net()
+    with pytest.raises(L.NbxError) as e:
+        net.run(600, L.ENGINE_GRAPH)
+    assert e.value.status == L.NBX_ERR_STATE and "history" in str(e.value)
+
+
+def test_cerebellar_rule_from_nbnet_uses_record_order(gpu, tmp_path):
+    # the cerebellar random stream is keyed by the connection's index in the file
+    from nbnet_file import records, write_nbnet
+    wl = W.trigger_rules(n=1200, k_in=60, seed=8, cerebellar=True)
+    s = wl.syn
+    n = wl.spec.n
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row_ptr))
+    recs = records(pre, s.post, s.type, s.weight, s.delay, max_weight=wl.rules["max_weight"],
+                   rule=wl.rules["rule"], trigger=wl.rules["trigger"])
+    perm = np.random.default_rng(3).permutation(s.count)
+    path = str(tmp_path / "cb.nbnet")
+    write_nbnet(path, {}, recs[perm])
+    conn = np.empty(s.count, np.int32)
+    conn[perm] = np.arange(s.count, dtype=np.int32)  # CSR synapse j is record conn[j]
+    wl.extra["conn_index"] = conn
+    net = wl.make_net(nbnet=path, nbnet_rules=wl.rules["rules"])
+    T = 300
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o, ring=False)
+    check_rules(net, run, wl)

@@ -153,3 +153,96 @@ def test_trigger_reaches_only_its_rule_and_target():
     assert r.elig[by[(0, 3, 0)]] != 0.0 and r.w[by[(0, 3, 0)]] == np.float32(0.3)
     want = 0.3 + ADD["a1pre"] + ADD["a2prepre"] * decay(10 - 3, ADD["ltp_tau"])
     assert abs(r.w[by[(4, 1, 1)]] - want) < 1e-6
+
+
+# ------------------------------------------------------------------ cerebellar (L/CerebullarLearningRule.cpp)
+
+import ctypes as C  # noqa: E402
+
+import oracle as O  # noqa: E402
+
+CB = dict(kind=L.RULE_CEREBELLAR, a1pre=0.01, a2prepre=-0.05, initpos=5.0, maxpos=10.0, kernel_step=1.0,
+          min_weight=0.05, seed=17)
+
+
+def orc_rule(d):
+    r = O.OrcRule()
+    r.kind = d["kind"]
+    for f in ("a1pre", "a2prepre", "initpos", "maxpos", "kernel_step", "min_weight"):
+        setattr(r, f, d.get(f, 0.0))
+    r.seed = d.get("seed", 0)
+    return r
+
+
+def kernel(d):
+    r = orc_rule(d)
+    mt, inv = C.c_float(), C.c_float()
+    n = O.lib().orc_cerebellar_kernel(C.byref(r), None, 0, C.byref(mt), C.byref(inv))
+    tab = np.zeros(n, np.float32)
+    O.lib().orc_cerebellar_kernel(C.byref(r), O.ptr(tab, C.c_float), n, None, None)
+    return tab, mt.value, inv.value
+
+
+def test_cerebellar_kernel_shape_and_window():
+    # alpha function of (t - initpos) / width peaking at 1 exactly at maxpos, zero before initpos;
+    # the window ends where x e^(1-x) first falls below 0.01 (x in widths past maxpos' start)
+    d = dict(CB, initpos=120.0, maxpos=150.0, kernel_step=0.1)
+    tab, mt, inv = kernel(d)
+    assert abs(inv - 10.0) < 1e-5
+    assert np.all(tab[:1200] == 0.0)
+    assert abs(tab[1500] - 1.0) < 1e-5 and tab.argmax() in (1499, 1500, 1501)
+    from scipy.optimize import brentq
+    x = brentq(lambda v: v * math.exp(1.0 - v) - 0.01, 1.0, 20.0)
+    assert abs(mt - (120.0 + 30.0 * x)) < 0.11
+    assert len(tab) == int(mt * 10.0) + 1
+
+
+def test_stable_unit_random():
+    f = O.lib().orc_stable_unit_random
+    u = np.array([f(17, 2, s, t) for s in range(200) for t in range(100)], np.float64)
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert np.all(np.abs(u * 2 ** 24 - np.rint(u * 2 ** 24)) == 0)  # 24-bit values
+    hist = np.histogram(u, bins=20, range=(0, 1))[0]
+    assert ((hist - 1000.0) ** 2 / 1000.0).sum() < 50  # chi-square, 19 dof
+    base = f(17, 2, 5, 9)
+    assert len({base, f(18, 2, 5, 9), f(17, 3, 5, 9), f(17, 2, 6, 9), f(17, 2, 5, 10)}) == 5
+
+
+def test_cerebellar_pre_and_trigger_closed_form():
+    # dt 1 ms; arrivals at 3 and 8 each add a1pre * U(seed, rule, connection, step); a trigger
+    # arriving at 20 adds a2prepre * kernel(20 - t) for both, newest first, bounded below
+    global DT
+    old = DT
+    DT = 1.0
+    try:
+        w0 = 0.5
+        edges = [(0, 1, 1, 0, w0, 0, -1), (2, 1, 1, 0, 0.0, -1, 0)]
+        wl, edges = forced(3, {0: [2, 7], 2: [19]}, edges, [CB], 30)
+        r, _ = run(wl, 30)
+    finally:
+        DT = old
+    j = next(k for k, x in enumerate(edges) if x[5] == 0)
+    f = O.lib().orc_stable_unit_random
+    u = f(17, 0, j, 3) + f(17, 0, j, 8)
+    k = lambda e: (1 / 5.0) * (e - 5.0) * math.exp(-(e - 5.0) / 5.0 + 1.0)
+    want = w0 + CB["a1pre"] * u + CB["a2prepre"] * (k(12.0) + k(17.0))
+    assert abs(r.w[j] - want) < 1e-5
+
+
+def test_cerebellar_window_and_floor():
+    global DT
+    old = DT
+    DT = 1.0
+    try:
+        # arrivals far older than the window: the trigger leaves the weight alone
+        edges = [(0, 1, 1, 0, 0.5, 0, -1), (2, 1, 1, 0, 0.0, -1, 0)]
+        wl, edges = forced(3, {0: [2], 2: [90]}, edges, [CB], 95)
+        r, _ = run(wl, 95)
+        j = next(k for k, x in enumerate(edges) if x[5] == 0)
+        assert abs(r.w[j] - (0.5 + CB["a1pre"] * O.lib().orc_stable_unit_random(17, 0, j, 3))) < 1e-6
+        # a large depression stops at min_weight
+        wl, edges = forced(3, {0: [2], 2: [12]}, edges, [dict(CB, a2prepre=-100.0)], 16)
+        r, _ = run(wl, 16)
+        assert r.w[j] == np.float32(CB["min_weight"])
+    finally:
+        DT = old
