@@ -206,12 +206,13 @@ def stdp_bench(steps: int) -> dict:
             "weights_in_range": bool(np.all(w_after[plastic] >= 0.0) and np.all(w_after[plastic] <= 1.0))}
 
 
-def dense_bench(steps: int) -> dict:
-    """Config 1: dense all-to-all current-based Izhikevich, 1,000 neurons, dt 0.1 ms, Gaussian input currents."""
+def dense_bench(steps: int, noise_scale: float = 1.0) -> dict:
+    """Config 1: dense all-to-all current-based Izhikevich, 1,000 neurons, dt 0.1 ms, Gaussian input currents
+    (noise_scale multiplies them; sqrt(10) restores the per-millisecond variance of the 1 ms original)."""
     from neuronbridge_amd import _lib as L
     from neuronbridge_amd import workloads as W
     from neuronbridge_amd.hiptime import Event
-    wl = W.dense_izh(steps=steps + 100)
+    wl = W.dense_izh(steps=steps + 100, noise_scale=noise_scale)
     net = wl.make_net()
     engine = net.pick_engine()
     net.run(100, engine)
@@ -229,7 +230,7 @@ def dense_bench(steps: int) -> dict:
     spikes = int(net.counts().sum())
     net.close()
     return {"workload": f"dense Izhikevich {wl.spec.n} neurons all-to-all, Gaussian currents (std 5 exc / 2 inh "
-                        f"per step), dt 0.1 ms, {steps} steps",
+                        f"per step x {noise_scale:.3g}), dt 0.1 ms, {steps} steps",
             "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
             "ms_per_step": ms / steps, "neuron_steps_per_s": wl.spec.n * steps / (ms * 1e-3),
             "syn_events_per_s": ev / (ms * 1e-3),
@@ -458,6 +459,8 @@ def main() -> int:
     if not args.no_dense:
         out["dense"] = dense_bench(args.dense_steps)
         log(f"dense: {out['dense']}")
+        out["dense_active"] = dense_bench(args.dense_steps, noise_scale=10 ** 0.5)
+        log(f"dense (active): {out['dense_active']}")
     if not args.no_models:
         out["models"] = models_bench(args.models_steps)
     if not args.no_rules:
