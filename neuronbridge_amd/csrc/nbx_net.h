@@ -29,7 +29,7 @@ struct PersistentLayout {
     int64_t seg_count = 0;           // entries in seg, region padding included
     int64_t max_row = 0;             // longest presynaptic row
     int32_t* d_off = nullptr;        // [n][nr * gs]
-    uint2* d_seg = nullptr;          // {post_local | delay << 16 | channel << 24 | (block & 127) << 25, q}
+    uint2* d_seg = nullptr;          // {post_local | delay << 16 (6 bits) | channel << 22 (2 bits) | (block & 127) << 25, q}
     int32_t* d_bar = nullptr;        // grid barrier counter
 };
 

@@ -193,8 +193,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                         e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
                     }
                 auto deposit = [&](const uint2 x) {
-                    const int32_t d = (int32_t)((x.x >> 16) & 0xFFu);
-                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)((x.x >> 24) & 1u)) * P +
+                    const int32_t d = (int32_t)((x.x >> 16) & 0x3Fu);
+                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)((x.x >> 22) & 3u)) * P +
                                       (int32_t)(x.x & 0xFFFFu)],
                               (int32_t)x.y);
                 };
@@ -220,12 +220,16 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             if (pl < np) {
                 const int32_t a0 = s_ring[(slot * nch) * P + pl];
                 s_ring[(slot * nch) * P + pl] = 0;
-                int32_t a1 = 0;
+                int32_t a1 = 0, a2 = 0;
                 if (nch > 1) {
                     a1 = s_ring[(slot * nch + 1) * P + pl];
                     s_ring[(slot * nch + 1) * P + pl] = 0;
                 }
-                fired = neuron_tick<MODEL>(A, i, t, a0, a1, 0, V[u], S1[u], GA[u], GG[u], I[u], last[u]);
+                if (nch > 2) {
+                    a2 = s_ring[(slot * nch + 2) * P + pl];
+                    s_ring[(slot * nch + 2) * P + pl] = 0;
+                }
+                fired = neuron_tick<MODEL>(A, i, t, a0, a1, a2, V[u], S1[u], GA[u], GG[u], I[u], last[u]);
             }
             const uint64_t mask = __ballot(fired);
             if (lane == 0) s_wcnt[wave] = __popcll(mask);
@@ -417,13 +421,13 @@ __global__ __launch_bounds__(1024) void nbx_replay_ring(ReplayArgs X) {
             load_ids(w + 2, idn2);
 
             auto deposit = [&](const uint2 x, int32_t tkr) {
-                const int32_t rel = tkr + (int32_t)((x.x >> 16) & 0xFFu) - 1;  // output step - k0
+                const int32_t rel = tkr + (int32_t)((x.x >> 16) & 0x3Fu) - 1;  // output step - k0
                 const int32_t ko = k0 + rel;
                 if (ko >= o_begin && ko < o_end) {
                     int32_t slot = phys + rel;
                     if (slot >= R) slot -= R;
                     const int32_t sub = (int32_t)(((x.x >> 25) - b0lo) & 0x7Fu);
-                    atomicAdd(&s_ring[slot * words + (int32_t)((x.x >> 24) & 1u) * Pr + sub * X.P +
+                    atomicAdd(&s_ring[slot * words + (int32_t)((x.x >> 22) & 3u) * Pr + sub * X.P +
                                       (int32_t)(x.x & 0xFFFFu)],
                               (int32_t)x.y);
                 }
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_ring(ReplayArgs X) {
                 int32_t* row = s_ring + slot * words;
                 const bool own = k0 + kk >= o_begin;
                 for (int32_t j = tid; j < words; j += T) {
-                    const int32_t ch = (j >= Pr) ? 1 : 0, pl = j - ch * Pr;
+                    const int32_t ch = j / Pr, pl = j - ch * Pr;
                     if (own && pl < np) X.out[((int64_t)(k0 + kk) * X.nch + ch) * X.n + p0 + pl] = row[j];
                     row[j] = 0;
                 }
@@ -689,7 +693,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
                 int32_t* row = s_ring + (((uint32_t)phys + (uint32_t)kk) & rmask) * (uint32_t)words;
                 const bool own = k0 + kk >= o_begin;
                 for (int32_t j = tid; j < words; j += T) {
-                    const int32_t ch = (j >= Pr) ? 1 : 0, pl = j - ch * Pr;
+                    const int32_t ch = j / Pr, pl = j - ch * Pr;
                     if (own && pl < np) X.out[((int64_t)(k0 + kk) * X.nch + ch) * X.n + p0 + pl] = row[j];
                     row[j] = 0;
                 }
@@ -750,6 +754,7 @@ void* persistent_kernel_for(int model, int npt) {
         case NBX_MODEL_LIF_EXP: return persistent_kernel<NBX_MODEL_LIF_EXP>(npt);
         case NBX_MODEL_IZH: return persistent_kernel<NBX_MODEL_IZH>(npt);
         case NBX_MODEL_LIF_EXP2: return persistent_kernel<NBX_MODEL_LIF_EXP2>(npt);
+        case NBX_MODEL_LIF_EXP3: return persistent_kernel<NBX_MODEL_LIF_EXP3>(npt);
         case NBX_MODEL_POISSON: return persistent_kernel<NBX_MODEL_POISSON>(npt);
         case NBX_MODEL_TRIGGER_RELAY: return persistent_kernel<NBX_MODEL_TRIGGER_RELAY>(npt);
         default: return persistent_kernel<NBX_MODEL_IZH_CUR>(npt);
@@ -804,7 +809,6 @@ std::string persistent_reason(const nbx_net* net) {
     if (net->stdp) return "STDP networks use arrival-time delivery";
     if (!net->rules.empty()) return "networks with learning rules use arrival-time delivery";
     if (net->S >= ((int64_t)1 << 31)) return "more than 2^31 synapses";
-    if (net->nch > 2) return "more than two arrival channels";
     Geometry g = geometry(net);
     return g.why;
 }
@@ -815,7 +819,6 @@ int32_t build_persistent_layout(nbx_net* net) {
     PersistentLayout& L = net->pl;
     if (L.built) return NBX_OK;
     Geometry g = geometry(net);
-    if (net->nch > 2) g.why = "more than two arrival channels (entries carry one channel bit)";
     if (!g.why.empty()) {
         set_error("persistent layout: " + g.why);
         return NBX_ERR_UNSUPPORTED;
@@ -862,7 +865,7 @@ int32_t build_persistent_layout(nbx_net* net) {
             int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
             const int32_t ch = arrival_channel(net->md.model, ty);
             if (arrival_negated(net->md.model, ty)) qv = -qv;
-            const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 24) |
+            const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 22) |
                                   (((uint32_t)blk & 0x7Fu) << 25);
             seg[(size_t)cnt[(size_t)blk]++] = make_uint2(meta, (uint32_t)qv);
         }
@@ -1052,7 +1055,7 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
         // a 16-entry slot suits mean segments of ~3..10 entries (longer ones overflow often,
         // shorter ones waste most of the 128 bytes); otherwise the offset layout is used
         const bool slots = slots_req >= 0 ? slots_req != 0 : (mean_seg >= 3.0 && mean_seg <= 10.5);
-        if (!slots && (L.rb % m != 0 || net->nch > 2)) continue;  // offset entries carry one channel bit
+        if (!slots && L.rb % m != 0) continue;
         const int64_t words = (int64_t)net->nch * m * L.P;
         if (slots && words > 65536) continue;  // a slot entry holds its LDS word in 16 bits
         const int64_t stage_bytes = slots ? 8 : 16;  // per staged spike, two buffers

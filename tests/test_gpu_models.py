@@ -22,16 +22,9 @@ CASES = [
 ]
 
 
-def engines_for(model):
-    # the persistent engine's blocked entries carry one channel bit: two channels at most
-    return [e for e in ENGINES if not (e == "persistent" and model == L.LIF_EXP3)]
-
-
 @pytest.mark.parametrize("case,model,integrator", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("engine", list(ENGINES))
 def test_model_closed_loop_bitexact(gpu, case, model, integrator, engine):
-    if engine not in engines_for(model):
-        pytest.skip("three arrival channels run on the kernel and graph engines")
     wl = W.model_sparse(model, integrator=integrator)
     T = 300
     net = wl.make_net()
@@ -43,24 +36,36 @@ def test_model_closed_loop_bitexact(gpu, case, model, integrator, engine):
     compare(net, run, wl, counts_o, raster_o)
 
 
-def test_three_channel_network_refuses_persistent_engine(gpu):
-    wl = W.model_sparse(L.LIF_EXP3, n=600, k_in=30)
+def test_three_channel_engines_interoperate(gpu):
+    # NMDA arrivals (channel 2) carry over between the persistent engine's LDS ring and the device ring
+    wl = W.model_sparse(L.LIF_EXP3, n=3000, k_in=120, integrator=L.RK4, seed=10)
+    assert wl.spec.channels == 3
     net = wl.make_net()
-    with pytest.raises(L.NbxError) as e:
-        net.run(5, L.ENGINE_PERSISTENT)
-    assert e.value.status == L.NBX_ERR_UNSUPPORTED
-    assert net.pick_engine() != L.ENGINE_PERSISTENT
-    net.run(5)  # AUTO picks an engine that runs it
+    assert net.pick_engine() == L.ENGINE_PERSISTENT
+    net.record(0, 260, counts=True, raster=True)
+    net.run(90, L.ENGINE_PERSISTENT)
+    net.run(70, L.ENGINE_GRAPH)
+    net.run(100, L.ENGINE_PERSISTENT)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(260)
+    compare(net, run, wl, counts_o, raster_o)
 
 
-@pytest.mark.parametrize("model", [L.LIF_EXP2, L.LIF_EXP3, L.POISSON], ids=["lif2", "lif3", "poisson"])
-def test_model_replay_matches_brute_force(gpu, model):
+@pytest.mark.parametrize("model,slots", [(L.LIF_EXP2, None), (L.LIF_EXP3, None), (L.POISSON, None),
+                                         (L.LIF_EXP2, "1"), (L.LIF_EXP3, "1"), (L.LIF_EXP3, "0")],
+                         ids=["lif2", "lif3", "poisson", "lif2-slots", "lif3-slots", "lif3-offsets"])
+def test_model_replay_matches_brute_force(gpu, model, slots, monkeypatch):
+    if slots is not None:  # force the fixed-slot or the offset-layout kernel
+        monkeypatch.setenv("NBX_REPLAY_SLOTS", slots)
+        monkeypatch.setenv("NBX_REPLAY_M", "1")
     wl = W.model_sparse(model, n=2500, k_in=80, seed=8)
     net = wl.make_net()
     rng = np.random.default_rng(11)
     K = 64
     spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
     e0 = net.events()
+    if slots is not None:
+        assert net.replay_path() == (L.REPLAY_SLOTS if slots == "1" else L.REPLAY_OFFSETS)
     out = net.replay(spikes)
     expect, events = replay_expect(wl, net.fixed_shift, spikes)
     assert out.shape[1] == wl.spec.channels
