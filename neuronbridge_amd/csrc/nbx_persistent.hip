@@ -522,7 +522,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_ring(ReplayArgs X) {
 // spike ids (with their window step) are staged in LDS, those of the next window are in
 // flight while the current one is delivered; synaptic events are counted once per spike from
 // its row length by the workgroup of post block 0.
-constexpr int kSR = 6;  // spikes per lane group per batch
+constexpr int kSR = 4;  // spikes per lane group per batch (more spills registers with the prefetched batch)
 
 __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
     constexpr int LANES = 8;
@@ -545,7 +545,10 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
     const int32_t o_begin = g * Kg, o_end = min(X.K, o_begin + Kg);
     if (np == 0 || o_begin >= o_end) return;  // uniform over the workgroup
     const int32_t k_begin = max(0, o_begin - (X.D - 2));
-    const bool counter = (q == 0);  // this workgroup counts the time group's synaptic events
+    // synaptic events are counted once per time group: the staged spike at window index f is
+    // counted by the workgroup with q == f % Ba (Ba = blocks that hold neurons; the others have
+    // returned above), which spreads the extra row reads evenly
+    const int32_t Ba = min(X.Bq, (X.n + Pr - 1) / Pr);
 
     for (int32_t j = tid; j < R * words; j += T) s_ring[j] = 0;
     unsigned long long ev = 0;
@@ -571,7 +574,8 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
         auto load_deg = [&](const int32_t (&id)[kStageU], int32_t (&deg)[kStageU]) {
 #pragma unroll
             for (int u = 0; u < kStageU; ++u)
-                deg[u] = (counter && id[u] >= 0) ? (int32_t)(X.row_ptr[id[u] + 1] - X.row_ptr[id[u]]) : 0;
+                deg[u] = (id[u] >= 0 && (tid + u * T) % Ba == q)
+                             ? (int32_t)(X.row_ptr[id[u] + 1] - X.row_ptr[id[u]]) : 0;
         };
         auto step_of = [&](int32_t w, int32_t fa) {
             const int32_t kw0 = w * W, wl = min(W, sl - kw0);
@@ -592,10 +596,28 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
                 if (id[u] >= 0) {
                     const int32_t tk = step_of(w, lo + f);
                     s_sid[buf * C + f] = id[u] | (tk << 24);
-                    if (kw + tk >= o_begin) ev += (unsigned long long)deg[u];  // deg is 0 unless counter
+                    if (kw + tk >= o_begin) ev += (unsigned long long)deg[u];  // 0 unless this workgroup counts f
                 }
             }
         };
+
+        // one slot batch of the staged spikes [base, base + kSR * ngrp) of staging buffer sb
+        auto fetch = [&](int32_t sb, int32_t cc, int32_t base, int32_t (&sid)[kSR], uint4 (&v)[kSR]) {
+#pragma unroll
+            for (int r = 0; r < kSR; ++r) {
+                const int32_t f = base + r * ngrp;
+                sid[r] = (f < cc) ? s_sid[sb * C + f] : -1;
+            }
+#pragma unroll
+            for (int r = 0; r < kSR; ++r)
+                v[r] = (sid[r] >= 0) ? X.slots[((int64_t)(sid[r] & 0xFFFFFF) * X.Bq + q) * (kSlotCap / 2) + gl]
+                                     : make_uint4(kSlotEmpty, 0u, kSlotEmpty, 0u);
+        };
+        // the first batch of the next window, fetched before this window's flush so its loads
+        // are in flight while the completed steps are written out
+        int32_t sidP[kSR];
+        uint4 vP[kSR];
+        bool pre = false;
 
         int32_t idn[kStageU], degn[kStageU];
         {
@@ -625,23 +647,12 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
                         const int32_t fa = lo + c0 + f;
                         const int32_t pre = X.ids[fa], tk = step_of(w, fa);
                         s_sid[buf * C + f] = pre | (tk << 24);
-                        if (counter && k0 + tk >= o_begin)
+                        if ((c0 + f) % Ba == q && k0 + tk >= o_begin)
                             ev += (unsigned long long)(X.row_ptr[pre + 1] - X.row_ptr[pre]);
                     }
                     __syncthreads();
                 }
                 const int32_t step = ngrp * kSR;
-                auto fetch = [&](int32_t base, int32_t (&sid)[kSR], uint4 (&v)[kSR]) {
-#pragma unroll
-                    for (int r = 0; r < kSR; ++r) {
-                        const int32_t f = base + r * ngrp;
-                        sid[r] = (f < cc) ? s_sid[buf * C + f] : -1;
-                    }
-#pragma unroll
-                    for (int r = 0; r < kSR; ++r)
-                        v[r] = (sid[r] >= 0) ? X.slots[((int64_t)(sid[r] & 0xFFFFFF) * X.Bq + q) * (kSlotCap / 2) + gl]
-                                             : make_uint4(kSlotEmpty, 0u, kSlotEmpty, 0u);
-                };
                 auto consume = [&](const int32_t (&sid)[kSR], const uint4 (&v)[kSR], auto chk) {
                     constexpr bool CHK = decltype(chk)::value;
 #pragma unroll
@@ -668,18 +679,29 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
                         }
                     }
                 };
+                const bool use_pre = pre && c0 == 0;
                 auto run = [&](auto chk) {
                     int32_t sidA[kSR], sidB[kSR];
                     uint4 vA[kSR], vB[kSR];
                     int32_t base = grp;
-                    if (base < cc) fetch(base, sidA, vA);
+                    if (base < cc) {
+                        if (use_pre) {
+#pragma unroll
+                            for (int r = 0; r < kSR; ++r) {
+                                sidA[r] = sidP[r];
+                                vA[r] = vP[r];
+                            }
+                        } else {
+                            fetch(buf, cc, base, sidA, vA);
+                        }
+                    }
                     while (base < cc) {
                         const int32_t b2 = base + step;
-                        if (b2 < cc) fetch(b2, sidB, vB);
+                        if (b2 < cc) fetch(buf, cc, b2, sidB, vB);
                         consume(sidA, vA, chk);
                         if (b2 >= cc) break;
                         const int32_t b3 = b2 + step;
-                        if (b3 < cc) fetch(b3, sidA, vA);
+                        if (b3 < cc) fetch(buf, cc, b3, sidA, vA);
                         consume(sidB, vB, chk);
                         base = b3;
                     }
@@ -689,6 +711,11 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
             }
             commit(w + 1, idn, degn, buf ^ 1);
             __syncthreads();
+            pre = w + 1 < nwin;
+            if (pre) {
+                const int32_t cc1 = min(C, win_cnt(w + 1));
+                if (grp < cc1) fetch(buf ^ 1, cc1, grp, sidP, vP);
+            }
             for (int32_t kk = 0; kk < wl; ++kk) {
                 int32_t* row = s_ring + (((uint32_t)phys + (uint32_t)kk) & rmask) * (uint32_t)words;
                 const bool own = k0 + kk >= o_begin;
