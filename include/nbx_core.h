@@ -383,6 +383,16 @@ int32_t nbx_lif_fwd_step(const nbx_lif_adj_params* p, const float* d_I, float* d
                          void* stream);
 int32_t nbx_lif_bwd_step(const nbx_lif_adj_params* p, const float* d_G, const float* d_R, const float* d_Vm,
                          float* d_dV, float* d_dI, void* stream);
+/* Fused recurrent steps (the product with W_rec on FP32 matrix cores inside the step kernel), over
+ * a segment of nsteps consecutive steps, all [n][batch] device arrays on `stream`:
+ *   forward:  for k: s_k = step(V, IN_k + W s_{k-1}) with s_{-1} = s_init (null: zero); S and Vm
+ *             (optional) get nsteps slices;
+ *   backward: for k = nsteps-1 .. 0: dI_k = adjoint(dV, R_k (optional) + W^T dI_{k+1}, Vm_k) with
+ *             dI_nsteps = dI_after (null: no recurrent term), WT = W_rec transposed. */
+int32_t nbx_lif_fwd_segment(const nbx_lif_adj_params* p, const float* d_W, const float* d_IN, const float* d_s_init,
+                            float* d_V, float* d_S, float* d_Vm, int32_t nsteps, void* stream);
+int32_t nbx_lif_bwd_segment(const nbx_lif_adj_params* p, const float* d_WT, const float* d_dI_after, const float* d_R,
+                            const float* d_Vm, float* d_dV, float* d_dI, int32_t nsteps, void* stream);
 /* Leaky readout y_t = beta*y_{t-1} + (1-beta)*Z_t over [T][m] (state carried in d_state if given),
  * and its adjoint dZ_t = (1-beta)*(gy_t + beta*(...)) as a reverse scan. */
 int32_t nbx_readout_fwd(int32_t T, int64_t m, float beta, const float* d_Z, float* d_y, float* d_state, void* stream);

@@ -1,18 +1,21 @@
 """Surrogate-gradient adjoint of a recurrent LIF network, behind torch.autograd.Function.
 
 Forward, per step t (state [N, B] float32, B trials):
-    I_t  = W_in x_t + W_rec s_{t-1}             (GEMMs; the input term for all t in one GEMM)
+    I_t  = W_in x_t + W_rec s_{t-1}             (the input term for all t in one GEMM; the recurrent
+                                                  term on the matrix cores inside the step kernel)
     V_t, s_t = LIF step                          (hand-written HIP kernel: RK4-exact affine update,
                                                   strict threshold, subtractive reset)
     y_t  = beta y_{t-1} + (1 - beta) W_out s_t   (GEMM for all t + HIP scan kernel)
 Backward walks the steps in reverse with the hand-written adjoint step kernel
-(fast-sigmoid surrogate, slope 10 by default) and one GEMM W_rec^T dI_{t+1} per step.
+(fast-sigmoid surrogate, slope 10 by default) fused with the product W_rec^T dI_{t+1}.
 Only the membrane potential at every `checkpoint`-th step is kept from the forward pass;
 each segment is recomputed before its backward sweep (checkpoint=0 keeps every step).
 Weight gradients are accumulated per segment as large GEMMs over (steps x trials).
 
-The GEMMs run through torch (hipBLASLt) on torch's current stream; the step kernels are
-libnbx's nbx_lif_fwd_step / nbx_lif_bwd_step / nbx_readout_* on the same stream.
+The per-step recurrent products run inside libnbx's fused step kernels (FP32 MFMA tiles,
+nbx_lif_fwd_segment / nbx_lif_bwd_segment, one launch per step from a C loop); the large
+GEMMs over all steps (input currents, readout, weight gradients) run through torch (hipBLASLt),
+all on torch's current stream.
 """
 from __future__ import annotations
 
@@ -63,13 +66,11 @@ def _stream() -> C.c_void_p:
 
 def _fwd_segment(cp, IN, W_rec, s_init, V, t0, t1, S_out, Vm_out=None):
     """Forward steps [t0, t1) from state V (updated in place) and the previous step's spikes
-    s_init; spikes of step t go to S_out[t - t0], membrane potentials to Vm_out[t - t0]."""
-    s_prev = s_init
-    for t in range(t0, t1):
-        I = torch.addmm(IN[t], W_rec, s_prev)
-        L.call("nbx_lif_fwd_step", C.byref(cp), _ptr(I), _ptr(V), _ptr(Vm_out[t - t0] if Vm_out is not None else None),
-               _ptr(S_out[t - t0]), _stream())
-        s_prev = S_out[t - t0]
+    s_init; spikes of step t go to S_out[t - t0], membrane potentials to Vm_out[t - t0].
+    One fused kernel per step: W_rec s_{t-1} on the matrix cores, then the LIF step."""
+    assert S_out.is_contiguous() and (Vm_out is None or Vm_out.is_contiguous()) and s_init.is_contiguous()
+    L.call("nbx_lif_fwd_segment", C.byref(cp), _ptr(W_rec), _ptr(IN[t0]), _ptr(s_init), _ptr(V), _ptr(S_out),
+           _ptr(Vm_out), t1 - t0, _stream())
 
 
 class LifAdjoint(torch.autograd.Function):
@@ -124,7 +125,7 @@ class LifAdjoint(torch.autograd.Function):
         dW_rec = torch.zeros_like(W_rec)
         dW_in = torch.zeros_like(W_in)
         dV = torch.zeros((N, B), device=x.device, dtype=torch.float32)
-        G = None
+        dI_after = None
         zeros_s = torch.zeros((1, N, B), device=x.device, dtype=torch.float32)
         for seg in reversed(range(len(ckpts))):
             t0, t1 = seg * K, min(T, (seg + 1) * K)
@@ -134,10 +135,11 @@ class LifAdjoint(torch.autograd.Function):
             S_scr = torch.empty((k, N, B), device=x.device, dtype=torch.float32)
             _fwd_segment(cp, IN, W_rec, S[t0 - 1] if t0 > 0 else zeros_s[0], V, t0, t1, S_scr, Vm_out=Vm)
             dI = torch.empty((k, N, B), device=x.device, dtype=torch.float32)
-            for t in reversed(range(t0, t1)):
-                L.call("nbx_lif_bwd_step", C.byref(cp), _ptr(G), _ptr(R[t]), _ptr(Vm[t - t0]), _ptr(dV),
-                       _ptr(dI[t - t0]), _stream())
-                G = torch.matmul(W_rec_t, dI[t - t0])
+            # steps t1-1 .. t0, each fusing G = W_rec^T dI_{t+1} with the adjoint step; dI_{t1} is
+            # the first step of the later segment (none after the last step)
+            L.call("nbx_lif_bwd_segment", C.byref(cp), _ptr(W_rec_t), _ptr(dI_after), _ptr(R[t0]), _ptr(Vm), _ptr(dV),
+                   _ptr(dI), k, _stream())
+            dI_after = dI[0]
             # spikes feeding each step of the segment: s_{t-1}
             S_prev = S[t0 - 1:t1 - 1] if t0 > 0 else torch.cat([zeros_s, S[0:t1 - 1]], dim=0)
             dI2 = dI.permute(1, 0, 2).reshape(N, k * B)

@@ -16,7 +16,8 @@
 //   s = 1/2 + (1/2) u/(1 + |u|),  u = slope*(Vm - V_th),  with its exact derivative
 //   sigma'(x) = (slope/2)/(1 + |u|)^2,
 // so finite differences can check the whole recurrence.
-// The products with W_rec, W_in, W_out are GEMMs run by the caller between these steps.
+// The step kernels take the recurrent term from the caller (a library GEMM between steps); the fused
+// segment entry points compute it on the matrix cores inside the step kernel instead.
 #include <hip/hip_runtime.h>
 
 #include "nbx_internal.h"
@@ -94,6 +95,141 @@ __global__ __launch_bounds__(kThreads) void nbx_readout_bwd_kernel(int32_t T, in
     if (carry) carry[j] = acc;
 }
 
+// ---------------------------------------------------------------- fused recurrent steps
+//
+// One step of the recurrence with its product on the matrix cores: out[i][b] = sum_j M[i][j] X[j][b]
+// (M = W_rec forward, W_rec^T backward; X = the previous step's spikes / the next step's dI, both
+// [n][batch]) in FP32 MFMA 16x16x4 tiles, then the step's elementwise update on the tile.
+// Workgroup = one 16 x 16 output tile, eight waves splitting the reduction over j in 64-wide
+// chunks (double-buffered loads), partial tiles summed in a fixed order through LDS (so
+// recomputation is bit-identical).
+// Within a chunk, lane (r, kb) holds M[i0 + r][k0 + 16 kb .. k0 + 16 kb + 16) and lane (c, kb)
+// holds X[k0 + 16 kb .. + 16][b0 + c]; MFMA s (s = 0..15) reduces k = k0 + 16 kb + s over the four
+// lane groups kb. Tiles of one row block are placed on one XCD so its M rows are read into one L2.
+constexpr int kRecWaves = 8;
+constexpr int kRecThreads = 64 * kRecWaves;
+
+struct RecArgs {
+    nbx_lif_adj_params p;
+    const float* M;   // [n][n]
+    const float* X;   // [n][batch] or null (no recurrent term)
+    const float* add; // forward: IN_t [n][batch]; backward: R_t or null
+    const float* Vm;  // backward: Vm_t
+    float* V;         // forward: V (in/out); backward: dV (in/out)
+    float* out;       // forward: s_t; backward: dI_t
+    float* Vm_out;    // forward: Vm_t or null
+    int32_t tiles_c;
+};
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void load16(const float* __restrict__ p, int32_t valid, float (&v)[16]) {
+    if (valid >= 16 && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const float4 x = q[h];
+            v[4 * h] = x.x;
+            v[4 * h + 1] = x.y;
+            v[4 * h + 2] = x.z;
+            v[4 * h + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) v[s] = s < valid ? p[s] : 0.0f;
+    }
+}
+
+template <bool BWD>
+__global__ __launch_bounds__(kRecThreads) void nbx_lif_rec_kernel(RecArgs a) {
+    __shared__ float red[kRecWaves][256];
+    const nbx_lif_adj_params& p = a.p;
+    const int32_t n = p.n, B = p.batch;
+    const int32_t G = gridDim.x;
+    const int32_t lin = (G % 8 == 0) ? (int32_t)(blockIdx.x % 8) * (G / 8) + (int32_t)blockIdx.x / 8
+                                     : (int32_t)blockIdx.x;
+    const int32_t i0 = (lin / a.tiles_c) * 16, b0 = (lin % a.tiles_c) * 16;
+    const int32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int32_t r = lane & 15, kb = lane >> 4;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (a.X) {
+        // reduction range per wave, in chunks of 64; the next chunk's operands are loaded while
+        // the current one runs through the matrix cores
+        const int32_t kw = (n + kRecWaves * 64 - 1) / (kRecWaves * 64) * 64;
+        const int32_t kbeg = wave * kw, kend = min(n, kbeg + kw);
+        const bool row_ok = i0 + r < n, col_ok = b0 + r < B;
+        const float* mrow = a.M + (int64_t)(row_ok ? i0 + r : 0) * n;
+        auto load = [&](int32_t k0, float (&av)[16], float (&xv)[16]) {
+            const int32_t kk = k0 + 16 * kb;
+            const int32_t valid = min(16, max(0, kend - kk));
+            load16(mrow + kk, row_ok ? valid : 0, av);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) xv[s] = (col_ok && s < valid) ? a.X[(int64_t)(kk + s) * B + b0 + r] : 0.0f;
+        };
+        float a0[16], x0[16], a1[16], x1[16];
+        int32_t k0 = kbeg;
+        if (k0 < kend) load(k0, a0, x0);
+        while (k0 < kend) {
+            const int32_t k1 = k0 + 64;
+            if (k1 < kend) load(k1, a1, x1);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[s], x0[s], acc, 0, 0, 0);
+            if (k1 >= kend) break;
+            const int32_t k2 = k1 + 64;
+            if (k2 < kend) load(k2, a0, x0);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[s], x1[s], acc, 0, 0, 0);
+            k0 = k2;
+        }
+    }
+    // lane holds rows 4 kb .. 4 kb + 3 of column r of this wave's partial tile
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[wave][(4 * kb + e) * 16 + r] = acc[e];
+    __syncthreads();
+    if (tid >= 256) return;
+    const int32_t row = tid >> 4, col = tid & 15;
+    const int32_t i = i0 + row, b = b0 + col;
+    if (i >= n || b >= B) return;
+    float prod = red[0][tid];
+#pragma unroll
+    for (int w = 1; w < kRecWaves; ++w) prod += red[w][tid];  // fixed order: recomputation is bit-identical
+    const int64_t j = (int64_t)i * B + b;
+    if (!BWD) {  // the forward step of nbx_lif_fwd_kernel on I = IN_t + W_rec s_{t-1}
+        const float I = a.add[j] + prod;
+        const float vm = p.g * a.V[j] + p.one_minus_g * (p.v_rest + p.r * I);
+        float sp;
+        if (p.mode == 2) {
+            const float u = p.slope * (vm - p.v_th);
+            sp = 0.5f + 0.5f * u / (1.0f + fabsf(u));
+        } else {
+            sp = vm > p.v_th ? 1.0f : 0.0f;
+        }
+        a.V[j] = vm - p.delta_reset * sp;
+        if (a.Vm_out) a.Vm_out[j] = vm;
+        a.out[j] = sp;
+    } else {  // the adjoint step of nbx_lif_bwd_kernel with G = W_rec^T dI_{t+1}
+        const float dv = a.V[j];
+        float ds = -p.delta_reset * dv;
+        if (a.add) ds += a.add[j];
+        if (a.X) ds += prod;
+        const float den = 1.0f + p.slope * fabsf(a.Vm[j] - p.v_th);
+        float sg = 0.0f;
+        if (p.mode == 1) sg = 1.0f / (den * den);
+        else if (p.mode == 2) sg = 0.5f * p.slope / (den * den);
+        const float dvm = dv + sg * ds;
+        a.out[j] = p.one_minus_g * p.r * dvm;
+        a.V[j] = p.g * dvm;
+    }
+}
+
+hipError_t launch_rec(bool bwd, const RecArgs& a, hipStream_t st) {
+    const int32_t tr = (a.p.n + 15) / 16;
+    const int32_t G = tr * a.tiles_c;
+    if (bwd) nbx_lif_rec_kernel<true><<<G, kRecThreads, 0, st>>>(a);
+    else nbx_lif_rec_kernel<false><<<G, kRecThreads, 0, st>>>(a);
+    return hipGetLastError();
+}
+
 int blocks_for(int64_t count) { return (int)((count + kThreads - 1) / kThreads); }
 
 int32_t check(const nbx_lif_adj_params* p) {
@@ -133,6 +269,54 @@ int32_t nbx_lif_bwd_step(const nbx_lif_adj_params* p, const float* d_G, const fl
     const int64_t count = (int64_t)p->n * p->batch;
     nbx_lif_bwd_kernel<<<blocks_for(count), kThreads, 0, (hipStream_t)stream>>>(*p, d_G, d_R, d_Vm, d_dV, d_dI, count);
     NBX_HIP_TRY(hipGetLastError());
+    return NBX_OK;
+}
+
+int32_t nbx_lif_fwd_segment(const nbx_lif_adj_params* p, const float* d_W, const float* d_IN, const float* d_s_init,
+                            float* d_V, float* d_S, float* d_Vm, int32_t nsteps, void* stream) {
+    int32_t rc = check(p);
+    if (rc) return rc;
+    if (!d_W || !d_IN || !d_V || !d_S || nsteps < 0) {
+        set_error("nbx_lif_fwd_segment: bad arguments");
+        return NBX_ERR_INVALID;
+    }
+    const int64_t nb = (int64_t)p->n * p->batch;
+    RecArgs a{};
+    a.p = *p;
+    a.M = d_W;
+    a.tiles_c = (p->batch + 15) / 16;
+    for (int32_t k = 0; k < nsteps; ++k) {
+        a.X = k == 0 ? d_s_init : d_S + (k - 1) * nb;
+        a.add = d_IN + k * nb;
+        a.V = d_V;
+        a.out = d_S + k * nb;
+        a.Vm_out = d_Vm ? d_Vm + k * nb : nullptr;
+        NBX_HIP_TRY(launch_rec(false, a, (hipStream_t)stream));
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_lif_bwd_segment(const nbx_lif_adj_params* p, const float* d_WT, const float* d_dI_after, const float* d_R,
+                            const float* d_Vm, float* d_dV, float* d_dI, int32_t nsteps, void* stream) {
+    int32_t rc = check(p);
+    if (rc) return rc;
+    if (!d_WT || !d_Vm || !d_dV || !d_dI || nsteps < 0) {
+        set_error("nbx_lif_bwd_segment: bad arguments");
+        return NBX_ERR_INVALID;
+    }
+    const int64_t nb = (int64_t)p->n * p->batch;
+    RecArgs a{};
+    a.p = *p;
+    a.M = d_WT;
+    a.tiles_c = (p->batch + 15) / 16;
+    for (int32_t k = nsteps - 1; k >= 0; --k) {
+        a.X = k == nsteps - 1 ? d_dI_after : d_dI + (k + 1) * nb;
+        a.add = d_R ? d_R + k * nb : nullptr;
+        a.Vm = d_Vm + k * nb;
+        a.V = d_dV;
+        a.out = d_dI + k * nb;
+        NBX_HIP_TRY(launch_rec(true, a, (hipStream_t)stream));
+    }
     return NBX_OK;
 }
 

@@ -81,3 +81,49 @@ def test_full_size_configuration(gpu):
     # N=2048, 64 trials, T=500, checkpoint every 50 steps (the configuration's sizes)
     errs, flips, rate = check_against_fp64(2048, 64, 500, 50)
     print(f"full size: relative gradient errors {errs}, spike decisions differing {flips}, rate {rate * 1000:.1f} Hz")
+
+
+@pytest.mark.parametrize("n,b", [(2048, 64), (100, 5), (96, 4), (300, 33)])
+def test_fused_recurrent_steps_against_fp64(gpu, n, b):
+    # the fused step kernels (W s on FP32 matrix cores + the step) against the same step in FP64,
+    # at the configuration's shape and at shapes that exercise partial tiles
+    import ctypes as C
+    from neuronbridge_amd import _lib as L
+    rng = np.random.default_rng(n + b)
+    p = LifParams()
+    cp = p.c_params(n, b)
+    W = rng.standard_normal((n, n)) / np.sqrt(n)
+    s_prev = (rng.random((n, b)) < 0.05).astype(np.float64)
+    IN = rng.standard_normal((n, b)) * 0.05
+    V0 = rng.uniform(-0.2, 0.9, (n, b))
+    Wd, sd, INd, Vd = to_dev(W.astype(np.float32), s_prev.astype(np.float32), IN.astype(np.float32),
+                             V0.astype(np.float32))
+    S = torch.empty((1, n, b), device=DEV)
+    Vm = torch.empty((1, n, b), device=DEV)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    L.call("nbx_lif_fwd_segment", C.byref(cp), ptr(Wd), ptr(INd), ptr(sd), ptr(Vd), ptr(S), ptr(Vm), 1, stream)
+    torch.cuda.synchronize()
+    Wf, sf, INf, V0f = (a.astype(np.float32).astype(np.float64) for a in (W, s_prev, IN, V0))
+    vm64 = p.g * V0f + (1 - p.g) * (p.v_rest + p.r * (INf + Wf @ sf))
+    vm = Vm[0].double().cpu().numpy()
+    assert np.max(np.abs(vm - vm64)) <= 1e-5 * max(1.0, np.max(np.abs(vm64)))
+    near = np.abs(vm64 - p.v_th) < 1e-4  # threshold decisions away from rounding distance agree
+    assert np.array_equal((S[0].cpu().numpy() > 0)[~near], (vm64 > p.v_th)[~near])
+    # one adjoint step: dI = (1-g) r (dV + sigma'(Vm - V_th) (R + W^T dI_next - (V_th - V_reset) dV))
+    dI_next = rng.standard_normal((n, b)) * 0.1
+    Rr = rng.standard_normal((n, b)) * 0.1
+    dV = rng.standard_normal((n, b)) * 0.1
+    WTd, dInd, Rd, dVd = to_dev(np.ascontiguousarray(W.T).astype(np.float32), dI_next.astype(np.float32),
+                                Rr.astype(np.float32), dV.astype(np.float32))
+    dI = torch.empty((1, n, b), device=DEV)
+    L.call("nbx_lif_bwd_segment", C.byref(cp), ptr(WTd), ptr(dInd), ptr(Rd), ptr(Vm), ptr(dVd), ptr(dI), 1, stream)
+    torch.cuda.synchronize()
+    dIn_f, R_f, dV_f = (a.astype(np.float32).astype(np.float64) for a in (dI_next, Rr, dV))
+    ds = R_f + Wf.T @ dIn_f - (p.v_th - p.v_reset) * dV_f
+    sg = 1.0 / (1.0 + p.slope * np.abs(vm - p.v_th)) ** 2
+    dvm = dV_f + sg * ds
+    want = (1 - p.g) * p.r * dvm
+    got = dI[0].double().cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.max(np.abs(want)))
+    assert np.max(np.abs(dVd.double().cpu().numpy() - p.g * dvm)) <= 1e-5 * max(1.0, np.max(np.abs(dvm)))
