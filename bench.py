@@ -116,6 +116,9 @@ def batched_bench(steps: int, sample: int) -> dict:
         f"{sum(i.n * i.n for i in inst) / 1e9:.2f} G BF16 weights) in {time.perf_counter() - t0:.1f} s")
     b = Batch(inst, **shared)
     thr = b.tune(100)
+    parts = b.parts()
+    log(f"batched: tuned to {'split' if parts.any() else 'one workgroup per instance'}, {thr} threads"
+        + (f", {int(parts.sum())} parts (max {int(parts.max())} per instance)" if parts.any() else ""))
     b.record(0, steps)
     e0, e1 = Event(), Event()
     e0.record(b.stream)
@@ -148,6 +151,8 @@ def batched_bench(steps: int, sample: int) -> dict:
     out = {"workload": f"{len(inst)} LIF instances, N~U{{512..8192}}, dense BF16 W, RK4 dt 0.1 ms, {steps} steps",
            "value": batched_rate, "unit": "instance-neuron-steps/s", "ms": round(ms_b, 2),
            "syn_events_per_s": ev_b / (ms_b * 1e-3), "workgroup_threads": thr,
+           "schedule": ({"kind": "split", "parts": int(parts.sum()), "split_instances": int((parts > 1).sum()),
+                         "max_parts": int(parts.max())} if parts.any() else {"kind": "one workgroup per instance"}),
            # BF16 weight-row bytes the gather reads (2 B per synaptic event) per second
            "weight_row_gbs": round(ev_b * 2 / (ms_b * 1e-3) / 1e9, 1)}
     if plain_n:
@@ -354,7 +359,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "graph", "kernels"])
     ap.add_argument("--integrator", default="rk4", choices=["rk4", "euler"])
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--replay-reps", type=int, default=5)
     ap.add_argument("--replay-steps", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")

@@ -326,10 +326,13 @@ int32_t nbx_build_dense_normal_bf16(int32_t n, float sigma, uint32_t seed_lo, ui
 /* ---------------- batched multi-instance engine (no reference counterpart) ----------------
  * Hundreds of independent LIF voltage-jump networks of uneven size and parameters, each with
  * dense all-to-all BF16 weights (delay 1 step), its own tau, Poisson background and
- * fixed-point shift, advanced together: one workgroup per instance, state in registers, the
- * fired rows of W gathered per step, chunks of steps replayed from a captured hipGraph.
- * Arrivals are the same int32 fixed-point sums the single-network engines compute from the
- * same (BF16-valued) weights, so each instance's raster equals the per-instance FP32 run. */
+ * fixed-point shift, advanced together, state in registers, the fired rows of W gathered per
+ * step. Two schedules: one workgroup per instance (chunks of steps replayed from a captured
+ * hipGraph), or a split schedule in which an instance runs as several column parts that
+ * exchange their spike words every step (cooperative launches; the autotuner splits the
+ * instances with the most synaptic events). Arrivals are the same int32 fixed-point sums the
+ * single-network engines compute from the same (BF16-valued) weights, so each instance's
+ * raster equals the per-instance FP32 run under either schedule. */
 typedef struct nbx_batch nbx_batch;
 typedef struct nbx_batch_instance {
     int32_t n;               /* neurons, 1 .. 16384 */
@@ -350,10 +353,19 @@ typedef struct nbx_batch_desc {
 } nbx_batch_desc;
 int32_t nbx_batch_create(const nbx_batch_desc* desc, int32_t device, nbx_batch** out);
 int32_t nbx_batch_destroy(nbx_batch* b);
-/* Autotuner: times `steps` steps per candidate workgroup size (state restored), keeps the fastest. */
+/* Autotuner: times `steps` steps per candidate workgroup size and the split schedule planned
+ * from the measured per-instance events (state restored), keeps the fastest; threads_out is
+ * 256 when the split schedule won. */
 int32_t nbx_batch_tune(nbx_batch* b, int32_t steps, int32_t* threads_out);
+/* One workgroup per instance with `threads` threads (256, 512, 1024); clears a split. */
 int32_t nbx_batch_set_schedule(nbx_batch* b, int32_t threads, int32_t chunk);
 int32_t nbx_batch_get_schedule(const nbx_batch* b, int32_t* threads, int32_t* chunk);
+/* Split schedule: parts[i] >= 1 column parts for instance i (256-thread workgroups of at most
+ * 2048 columns, widths rounded up to 32 columns, so an instance may get fewer parts than asked);
+ * NBX_ERR_INVALID if an instance needs more parts, NBX_ERR_UNSUPPORTED if all parts cannot be
+ * co-resident. get_parts reports the parts in use (0 for every instance when not split). */
+int32_t nbx_batch_set_parts(nbx_batch* b, const int32_t* parts, int32_t count);
+int32_t nbx_batch_get_parts(const nbx_batch* b, int32_t* parts, int32_t count);
 int32_t nbx_batch_record(nbx_batch* b, int32_t t_begin, int32_t nsteps);
 int32_t nbx_batch_run(nbx_batch* b, int32_t nsteps);
 int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps);

@@ -182,6 +182,8 @@ SIGNATURES = {
     "nbx_batch_tune": [_vp, C.c_int32, _i32p],
     "nbx_batch_set_schedule": [_vp, C.c_int32, C.c_int32],
     "nbx_batch_get_schedule": [_vp, _i32p, _i32p],
+    "nbx_batch_set_parts": [_vp, _i32p, C.c_int32],
+    "nbx_batch_get_parts": [_vp, _i32p, C.c_int32],
     "nbx_batch_record": [_vp, C.c_int32, C.c_int32],
     "nbx_batch_run": [_vp, C.c_int32],
     "nbx_batch_run_async": [_vp, C.c_int32],

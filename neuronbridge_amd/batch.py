@@ -85,6 +85,19 @@ class Batch:
         L.call("nbx_batch_get_schedule", self.h, C.byref(t), C.byref(c))
         return t.value, c.value
 
+    def set_parts(self, parts) -> None:
+        """Split schedule: parts[i] column parts for instance i (see nbx_batch_set_parts)."""
+        arr = np.ascontiguousarray(parts, dtype=np.int32)
+        if arr.shape != (len(self.instances),):
+            raise ValueError("one part count per instance")
+        L.call("nbx_batch_set_parts", self.h, arr.ctypes.data_as(C.POINTER(C.c_int32)), arr.size)
+
+    def parts(self) -> np.ndarray:
+        """Parts per instance in use (all 0 under the one-workgroup-per-instance schedule)."""
+        out = np.zeros(len(self.instances), np.int32)
+        L.call("nbx_batch_get_parts", self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size)
+        return out
+
     def record(self, t_begin: int, nsteps: int) -> None:
         L.call("nbx_batch_record", self.h, int(t_begin), int(nsteps))
         self._rec = (t_begin, nsteps)

@@ -10,7 +10,15 @@
 // weights, so rasters are identical). It then integrates and appends its new spikes to the
 // next list (one LDS atomic per thread that fired). Instances never communicate, so a step
 // needs only workgroup barriers, never a grid barrier. Chunks of steps are launched from a
-// captured hipGraph; the workgroup size is picked by a timing autotuner.
+// captured hipGraph.
+//
+// Such a launch lasts as long as its busiest instance (a near-saturated 8192-neuron instance
+// streams ~400 fired 16 KB rows per step through one CU while most CUs idle), so the split
+// schedule (nbx_batch_parts) runs an instance as several 256-thread workgroups, each owning a
+// range of columns and exchanging spike words with the instance's other parts every step.
+// The autotuner times both schedules (and the workgroup sizes of the first) and picks the
+// split from the measured per-instance synaptic events. Between launches the last step's
+// spikes are kept as the same stamped exchange words under both schedules.
 //
 // Weight rows are padded to a multiple of 16 elements (32 bytes), so every thread's
 // 16-byte loads are aligned and stay inside its row.
@@ -23,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -38,6 +47,7 @@ constexpr int kMaxPoisTab = 32;
 constexpr int kMaxBatchN = 16384;
 
 constexpr int kRowPad = 16;  // weight row stride: a multiple of 16 elements
+constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) polls before giving up
 
 struct InstDev {
     int32_t n;
@@ -46,6 +56,8 @@ struct InstDev {
     int64_t w_off;       // element offset of W[pre][post] (bf16), row stride ld
     int64_t s_off;       // offset into the state arrays
     int64_t r_off;       // offset of this instance's raster rows (words)
+    int64_t b_off;       // offset of this instance's spike exchange words
+    int32_t fast;        // every |fixed-point weight| below 2^22: single-fma quantization
     float tau;
     float scale, inv_scale;
     int32_t q_bg;
@@ -61,22 +73,163 @@ struct BatchArgs {
     const uint16_t* w;
     float* v;
     int32_t* last;
-    uint16_t* spk;        // [sum n] previous-step spike ids per instance (persisted between launches)
-    int32_t* spk_cnt;     // [instances]
+    // spike exchange words, [2][total words]: {step stamp << 32 | 32 spike bits}; step t's spikes
+    // are in half t & 1 with stamp t + 1 (both halves start as stamp 0, no spikes)
+    uint64_t* xb;
+    int64_t xb_half;
     uint32_t* raster;
     int32_t rec_t0, rec_n;
     unsigned long long* events;
+    unsigned long long* inst_events;  // [instances] per-instance event counts (autotuner)
+    int32_t* err;
     int32_t t0, nsteps;
-    int32_t max_n;        // spike-list stride in LDS
+    int32_t max_n;  // spike-list length in LDS
+};
+
+// One part of an instance: columns [lo, lo + w) (lo a multiple of 32), gathered by cg column
+// groups of NPT neurons times rg row groups that split the fired rows between them.
+struct PartDev {
+    int32_t inst, lo, w, cg, rg, parts;
 };
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// A value the compiler must treat as changed here: keeps the step-invariant part of the Philox
+// counter (the neuron index) from being hoisted out of the step loop, where the products it
+// would keep alive for every neuron of the thread cost more registers than they save.
+__device__ inline int32_t opaque(int32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+__device__ inline uint64_t ld_x(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_x(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- weight rows: NPT consecutive elements of a fired row per thread
+template <int NPT>
+__device__ inline void load_piece(const uint16_t* p, bool ok, uint32_t (&q)[NPT / 2]) {
+    if constexpr (NPT == 4) {
+        const uint2 v = ok ? *reinterpret_cast<const uint2*>(p) : make_uint2(0u, 0u);
+        q[0] = v.x;
+        q[1] = v.y;
+    } else {
+#pragma unroll
+        for (int h = 0; h < NPT / 8; ++h) {
+            const uint4 v = ok ? reinterpret_cast<const uint4*>(p)[h] : make_uint4(0u, 0u, 0u, 0u);
+            q[4 * h] = v.x;
+            q[4 * h + 1] = v.y;
+            q[4 * h + 2] = v.z;
+            q[4 * h + 3] = v.w;
+        }
+    }
+}
+
+// Adds one piece to the int32 fixed-point sums quantize(w, scale) = round-half-even(w * scale).
+// FAST (every |w * scale| of the instance below 2^22): one fma(w, scale, 1.5 * 2^23) rounds
+// w * scale to an integer exactly as quantize() does (the sum lies in [2^23, 2^24), where floats
+// are the integers, and the multiply by a power of two is exact), and its bit pattern is that
+// integer plus kMagicBits.
+constexpr float kMagic = 12582912.0f;  // 1.5 * 2^23
+constexpr int32_t kMagicBits = 0x4B400000;
+constexpr int32_t kFastLimit = 1 << 22;
+
+template <int NPT, bool FAST>
+__device__ inline void add_piece(int32_t (&acc)[NPT], const uint32_t (&q)[NPT / 2], float scale) {
+#pragma unroll
+    for (int e = 0; e < NPT / 2; ++e) {
+        const float lo = __uint_as_float(q[e] << 16), hi = __uint_as_float(q[e] & 0xFFFF0000u);
+        if constexpr (FAST) {
+            acc[2 * e] += (int32_t)__float_as_uint(__fmaf_rn(lo, scale, kMagic)) - kMagicBits;
+            acc[2 * e + 1] += (int32_t)__float_as_uint(__fmaf_rn(hi, scale, kMagic)) - kMagicBits;
+        } else {
+            acc[2 * e] += quantize(lo, scale);
+            acc[2 * e + 1] += quantize(hi, scale);
+        }
+    }
+}
+
+// Sums this thread's pieces of the rows list[s] over batches of kRows consecutive entries
+// starting at s = first, first + step, ... (kRows loads in flight per batch).
+template <int NPT, bool FAST, int kRows>
+__device__ inline void gather_rows(int32_t (&acc)[NPT], const uint16_t* wbase, int32_t ld, const uint16_t* list,
+                                   int32_t cnt, int32_t first, int32_t step, float scale) {
+    for (int32_t s0 = first; s0 < cnt; s0 += step) {
+        uint32_t q[kRows][NPT / 2];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const bool ok = s0 + r < cnt;
+            load_piece<NPT>(wbase + (int64_t)(ok ? list[s0 + r] : 0) * ld, ok, q[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) add_piece<NPT, FAST>(acc, q[r], scale);
+    }
+}
+
+// Largest |fixed-point weight| of one instance (it selects the FAST gather).
+__global__ void nbx_batch_qmax(const uint16_t* __restrict__ w, int64_t count, float scale, int32_t* qmax) {
+    int64_t m = 0;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = quantize(bf16_to_f32(w[j]), scale);
+        m = max(m, q < 0 ? -q : q);
+    }
+    if (m) atomicMax(qmax, (int32_t)min(m, (int64_t)0x7FFFFFFF));
+}
+
+// Waits until exchange word p carries `stamp` and returns its spike bits. The spin is bounded:
+// on time-out (or when another workgroup has raised the error flag) it returns false.
+__device__ inline bool poll_word(const uint64_t* p, uint32_t stamp, uint32_t& bits, int32_t* err) {
+    for (uint32_t spins = 0;; ++spins) {
+        const uint64_t x = ld_x(p);
+        if ((uint32_t)(x >> 32) == stamp) {
+            bits = (uint32_t)x;
+            return true;
+        }
+        if ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+        if (spins > kSpinLimit) {
+            atomicExch(err, kErrExchange);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Writes the neuron ids of the set bits of this thread's words b0 (word tid) and b1 (word
+// tid + blockDim) to list[] in one workgroup-wide exclusive scan; returns the list length.
+// Ids are listed per thread (word tid, then word tid + blockDim): the order is fixed and the
+// arrival sums are integer, so it does not affect any result.
+__device__ inline int32_t compact_words(uint32_t b0, uint32_t b1, uint16_t* list, int32_t* s_wsum) {
+    const int32_t tid = threadIdx.x, T = blockDim.x, lane = tid & 63, wv = tid >> 6, nwv = T >> 6;
+    const int32_t c = __popc(b0) + __popc(b1);
+    int32_t x = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wv] = x;
+    __syncthreads();
+    int32_t pos = x - c, total = 0;
+    for (int32_t q = 0; q < nwv; ++q) {
+        const int32_t s = s_wsum[q];
+        if (q < wv) pos += s;
+        total += s;
+    }
+    for (uint32_t m = b0; m; m &= m - 1u) list[pos++] = (uint16_t)(tid * 32 + __builtin_ctz(m));
+    for (uint32_t m = b1; m; m &= m - 1u) list[pos++] = (uint16_t)((tid + T) * 32 + __builtin_ctz(m));
+    return total;
+}
+
 template <int NPT>
 __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
     static_assert(NPT % 8 == 0, "a thread owns whole 16-byte pieces of a row");
-    extern __shared__ uint16_t s_ids[];  // [2][max_n] spike lists, then [blockDim] raster bits
+    extern __shared__ uint16_t s_ids[];  // [2][max_n] spike lists, then [blockDim] spike bits
     __shared__ int32_t s_cnt[2];
+    __shared__ int32_t s_wsum[16];
     const int32_t inst_id = B.order[blockIdx.x];
     const InstDev& I = B.inst[inst_id];
     const int32_t n = I.n;
@@ -84,6 +237,7 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
     const int32_t c0 = tid * NPT;  // first neuron of this thread
     const bool own = c0 < n;
     uint16_t* s_rb = s_ids + 2 * B.max_n;
+    constexpr int kPer = 32 / NPT;  // threads per spike word
     ModelParams P = B.P;
     P.tau = I.tau;
     float V[NPT];
@@ -98,13 +252,31 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
             last[j] = B.last[I.s_off + i];
         }
     }
-    const int32_t cnt0 = B.spk_cnt[inst_id];
-    for (int32_t j = tid; j < cnt0; j += T) s_ids[j] = B.spk[I.s_off + j];
-    if (tid == 0) s_cnt[0] = cnt0;
+    {  // the previous step's spikes, as published at the end of the previous launch
+        const uint64_t* xw = B.xb + (int64_t)((B.t0 - 1) & 1) * B.xb_half + I.b_off;
+        uint32_t b0 = 0, b1 = 0;
+        bool bad = false;
+        if (tid < I.words) {
+            const uint64_t x = ld_x(xw + tid);
+            bad |= (uint32_t)(x >> 32) != (uint32_t)B.t0;
+            b0 = (uint32_t)x;
+        }
+        if (tid + T < I.words) {
+            const uint64_t x = ld_x(xw + tid + T);
+            bad |= (uint32_t)(x >> 32) != (uint32_t)B.t0;
+            b1 = (uint32_t)x;
+        }
+        if (bad) atomicExch(B.err, kErrExchange);
+        const int32_t c = compact_words(b0, b1, s_ids, s_wsum);
+        if (tid == 0) s_cnt[0] = c;
+    }
     __syncthreads();
     unsigned long long ev = 0;
     int cur = 0;
+    uint32_t fired_bits = 0;
     const uint16_t* wbase = B.w + I.w_off + c0;
+    constexpr int kRows = (NPT == 8) ? 16 : 4;
+    constexpr int kRowsFast = kRows / 2;  // the fma path holds both halves of a piece: fewer rows in flight
     for (int32_t k = 0; k < B.nsteps; ++k) {
         const int32_t t = B.t0 + k;
         const int32_t cnt = s_cnt[cur];
@@ -113,36 +285,18 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
         int32_t acc[NPT];
 #pragma unroll
         for (int j = 0; j < NPT; ++j) acc[j] = 0;
-        constexpr int kRows = (NPT == 8) ? 16 : 4;
-        for (int32_t s0 = 0; s0 < cnt; s0 += kRows) {
-            uint4 wv[kRows][NPT / 8];
-#pragma unroll
-            for (int r = 0; r < kRows; ++r) {
-                const bool ok = own && s0 + r < cnt;
-                const uint4* row = reinterpret_cast<const uint4*>(wbase + (int64_t)(ok ? ids[s0 + r] : 0) * I.ld);
-#pragma unroll
-                for (int h = 0; h < NPT / 8; ++h) wv[r][h] = ok ? row[h] : make_uint4(0u, 0u, 0u, 0u);
-            }
-#pragma unroll
-            for (int r = 0; r < kRows; ++r)
-#pragma unroll
-                for (int h = 0; h < NPT / 8; ++h) {
-                    const uint32_t q4[4] = {wv[r][h].x, wv[r][h].y, wv[r][h].z, wv[r][h].w};
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        acc[h * 8 + 2 * e] += quantize(bf16_to_f32((uint16_t)(q4[e] & 0xFFFFu)), I.scale);
-                        acc[h * 8 + 2 * e + 1] += quantize(bf16_to_f32((uint16_t)(q4[e] >> 16)), I.scale);
-                    }
-                }
+        if (own) {
+            if (I.fast) gather_rows<NPT, true, kRowsFast>(acc, wbase, I.ld, ids, cnt, 0, kRowsFast, I.scale);
+            else gather_rows<NPT, false, kRows>(acc, wbase, I.ld, ids, cnt, 0, kRows, I.scale);
         }
         if (tid == 0) s_cnt[cur ^ 1] = 0;
         __syncthreads();
-        uint32_t fired_bits = 0;
+        fired_bits = 0;
 #pragma unroll
         for (int j = 0; j < NPT; ++j) {
             const int32_t i = c0 + j;
             if (i < n) {
-                const int32_t a0 = acc[j] + poisson_count(I.seed_lo, I.seed_hi, i, t, I.tab, I.kmax) * I.q_bg;
+                const int32_t a0 = acc[j] + poisson_count(I.seed_lo, I.seed_hi, opaque(i), t, I.tab, I.kmax) * I.q_bg;
                 if (last[j] > P.t_ref) V[j] += (float)a0 * I.inv_scale;
                 float g = 0.0f;
                 if (lif_step<false>(P, V[j], g, 0.0f, last[j])) fired_bits |= 1u << j;
@@ -159,7 +313,6 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
         if (rec) {
             // raster word w holds the bits of threads [w * 32/NPT, (w + 1) * 32/NPT)
             uint32_t* row = B.raster + I.r_off + (int64_t)(t - B.rec_t0) * I.words;
-            constexpr int kPer = 32 / NPT;
             for (int32_t w = tid; w < I.words; w += T) {
                 uint32_t word = 0;
 #pragma unroll
@@ -173,7 +326,7 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
         ev += (unsigned long long)s_cnt[cur ^ 1] * (unsigned long long)n;
         cur ^= 1;  // s_rb and the old list are rewritten only after the next step's first barrier
     }
-    // persist state and the last step's spikes for the next launch
+    // persist state and publish the last step's spikes for the next launch
 #pragma unroll
     for (int j = 0; j < NPT; ++j) {
         const int32_t i = c0 + j;
@@ -182,15 +335,192 @@ __global__ __launch_bounds__(1024) void nbx_batch_kernel(BatchArgs B) {
             B.last[I.s_off + i] = last[j];
         }
     }
-    const int32_t cl = s_cnt[cur];
-    for (int32_t j = tid; j < cl; j += T) B.spk[I.s_off + j] = s_ids[cur * B.max_n + j];
-    if (tid == 0) {
-        B.spk_cnt[inst_id] = cl;
-        if (ev) atomicAdd(B.events, ev);
+    if (B.nsteps > 0) {
+        const int32_t tl = B.t0 + B.nsteps - 1;
+        __syncthreads();
+        s_rb[tid] = (uint16_t)fired_bits;
+        __syncthreads();
+        uint64_t* xw = B.xb + (int64_t)(tl & 1) * B.xb_half + I.b_off;
+        for (int32_t w = tid; w < I.words; w += T) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const int32_t th = w * kPer + q;
+                if (th * NPT < n) word |= (uint32_t)s_rb[th] << (q * NPT);
+            }
+            st_x(xw + w, ((uint64_t)(uint32_t)(tl + 1) << 32) | word);
+        }
+    }
+    if (tid == 0 && ev) {
+        atomicAdd(B.events, ev);
+        atomicAdd(B.inst_events + inst_id, ev);
     }
 }
 
 void* batch_kernel(int npt) { return npt == 8 ? (void*)nbx_batch_kernel<8> : (void*)nbx_batch_kernel<16>; }
+
+// Split instances. Workgroup b runs part parts[b] of an instance: its column groups own NPT
+// neurons each (state in registers), its row groups split the previous step's fired rows and
+// add their partial sums through LDS in a fixed order. After the update each part publishes
+// its spike words (stamp t + 1) with device-scope stores; then every part of the instance
+// polls the instance's words until all carry the stamp and rebuilds the full spike list.
+// There is no barrier: a word is written once per step and carries its own step, and the
+// half-per-parity layout means a word is rewritten (two steps later) only after every part
+// has read it (a part cannot publish step t + 1 before it has read all of step t, which every
+// part publishes only after it has read all of step t - 1). All parts of a launch must be
+// co-resident (cooperative launch); every wait is bounded by kSpinLimit and the error flag.
+// An instance of one part keeps its exchange in LDS and publishes only its last step.
+constexpr int kPartThreads = 256;
+
+template <int NPT>
+__global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(NPT == 4 ? 5 : 4)))
+void nbx_batch_parts(BatchArgs B, const PartDev* __restrict__ parts) {
+    static_assert(NPT == 4 || NPT == 8, "pieces of 8 or 16 bytes");
+    constexpr int kGroupsPerWord = 32 / NPT;
+    constexpr int kRows = NPT == 4 ? 16 : 12;  // loads in flight per thread (more spill registers at NPT 8)
+    constexpr int kRowsFast = NPT == 4 ? 12 : 6;  // the fma path holds both halves of a piece
+    extern __shared__ __align__(16) uint16_t s_list[];                          // [max_n] spike ids
+    int32_t* s_red = reinterpret_cast<int32_t*>(s_list + ((B.max_n + 7) & ~7));  // [rg][NPT cg] partial sums
+    __shared__ int32_t s_cnt, s_abort;
+    __shared__ int32_t s_wsum[kPartThreads / 64];
+    __shared__ uint8_t s_rb[kPartThreads];  // this step's spike bits per column group
+    const PartDev pd = parts[blockIdx.x];
+    const InstDev& I = B.inst[pd.inst];
+    const int32_t n = I.n, tid = threadIdx.x;
+    const int32_t cgi = tid % pd.cg, rgi = tid / pd.cg;
+    const int32_t c0 = pd.lo + cgi * NPT;
+    const bool act = rgi < pd.rg && cgi * NPT < pd.w;  // gathers columns c0 .. c0 + NPT - 1 (c0 < n)
+    const bool owner = act && rgi == 0;                // and keeps their state
+    const int32_t wl = (pd.w + 31) / 32;               // spike words of this part
+    const int32_t wg0 = pd.lo / 32;                    // its first word in the instance
+    const int32_t pitch = NPT * pd.cg;                 // s_red row-group stride
+    ModelParams P = B.P;
+    P.tau = I.tau;
+    float V[NPT];
+    int32_t last[NPT];
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+        const int32_t i = c0 + j;
+        V[j] = 0.0f;
+        last[j] = 0;
+        if (owner && i < n) {
+            V[j] = B.v[I.s_off + i];
+            last[j] = B.last[I.s_off + i];
+        }
+    }
+    if (tid == 0) s_abort = 0;
+    {  // the previous step's spikes, as published at the end of the previous launch
+        const uint64_t* xw = B.xb + (int64_t)((B.t0 - 1) & 1) * B.xb_half + I.b_off;
+        uint32_t b0 = 0, b1 = 0;
+        bool bad = false;
+        if (tid < I.words) {
+            const uint64_t x = ld_x(xw + tid);
+            bad |= (uint32_t)(x >> 32) != (uint32_t)B.t0;
+            b0 = (uint32_t)x;
+        }
+        if (tid + kPartThreads < I.words) {
+            const uint64_t x = ld_x(xw + tid + kPartThreads);
+            bad |= (uint32_t)(x >> 32) != (uint32_t)B.t0;
+            b1 = (uint32_t)x;
+        }
+        if (bad) atomicExch(B.err, kErrExchange);
+        const int32_t c = compact_words(b0, b1, s_list, s_wsum);
+        if (tid == 0) s_cnt = c;
+    }
+    __syncthreads();
+    uint32_t nspk = 0;  // spikes of this thread's words
+    const uint16_t* wbase = B.w + I.w_off + c0;
+    for (int32_t k = 0; k < B.nsteps; ++k) {
+        const int32_t t = B.t0 + k;
+        const int32_t cnt = s_cnt;
+        int32_t acc[NPT];
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) acc[j] = 0;
+        // row group g takes the batches of kRows consecutive list entries g, g + rg, g + 2 rg, ...
+        if (act) {
+            if (I.fast)
+                gather_rows<NPT, true, kRowsFast>(acc, wbase, I.ld, s_list, cnt, rgi * kRowsFast, kRowsFast * pd.rg, I.scale);
+            else gather_rows<NPT, false, kRows>(acc, wbase, I.ld, s_list, cnt, rgi * kRows, kRows * pd.rg, I.scale);
+        }
+        if (pd.rg > 1) {  // row groups 1.. hand their partial sums to row group 0 through LDS
+            if (act && rgi > 0) {
+                int4* dst = reinterpret_cast<int4*>(s_red + rgi * pitch + cgi * NPT);
+#pragma unroll
+                for (int h = 0; h < NPT / 4; ++h)
+                    dst[h] = make_int4(acc[4 * h], acc[4 * h + 1], acc[4 * h + 2], acc[4 * h + 3]);
+            }
+            __syncthreads();
+            if (owner)
+                for (int32_t g = 1; g < pd.rg; ++g) {
+                    const int4* src = reinterpret_cast<const int4*>(s_red + g * pitch + cgi * NPT);
+#pragma unroll
+                    for (int h = 0; h < NPT / 4; ++h) {
+                        const int4 a = src[h];
+                        acc[4 * h] += a.x;
+                        acc[4 * h + 1] += a.y;
+                        acc[4 * h + 2] += a.z;
+                        acc[4 * h + 3] += a.w;
+                    }
+                }
+        }
+        uint32_t fired = 0;
+        if (owner) {
+#pragma unroll
+            for (int j = 0; j < NPT; ++j) {
+                const int32_t i = c0 + j;
+                if (i < n) {
+                    const int32_t a0 = acc[j] + poisson_count(I.seed_lo, I.seed_hi, opaque(i), t, I.tab, I.kmax) * I.q_bg;
+                    if (last[j] > P.t_ref) V[j] += (float)a0 * I.inv_scale;
+                    float g = 0.0f;
+                    if (lif_step<false>(P, V[j], g, 0.0f, last[j])) fired |= 1u << j;
+                }
+            }
+        }
+        if (rgi == 0) s_rb[cgi] = (uint8_t)fired;
+        __syncthreads();  // every gather read of s_list and s_red is done
+        // this part's spike words: publish, record, count
+        uint32_t word = 0;
+        if (tid < wl) {
+#pragma unroll
+            for (int q = 0; q < kGroupsPerWord; ++q) word |= (uint32_t)s_rb[kGroupsPerWord * tid + q] << (q * NPT);
+            if (pd.parts > 1 || k == B.nsteps - 1)
+                st_x(B.xb + (int64_t)(t & 1) * B.xb_half + I.b_off + wg0 + tid, ((uint64_t)(uint32_t)(t + 1) << 32) | word);
+            if (B.raster && t >= B.rec_t0 && t < B.rec_t0 + B.rec_n)
+                B.raster[I.r_off + (int64_t)(t - B.rec_t0) * I.words + wg0 + tid] = word;
+            nspk += (uint32_t)__popc(word);
+        }
+        // the instance's spike words of step t
+        uint32_t b0 = 0, b1 = 0;
+        if (pd.parts > 1) {
+            const uint64_t* xw = B.xb + (int64_t)(t & 1) * B.xb_half + I.b_off;
+            bool ok = true;
+            if (tid < I.words) ok &= poll_word(xw + tid, (uint32_t)(t + 1), b0, B.err);
+            if (tid + kPartThreads < I.words) ok &= poll_word(xw + tid + kPartThreads, (uint32_t)(t + 1), b1, B.err);
+            if (!ok) s_abort = 1;
+        } else {
+            b0 = tid < wl ? word : 0u;  // one part: tid < wl covers every word of the instance
+        }
+        const int32_t c = compact_words(b0, b1, s_list, s_wsum);  // its barrier also publishes s_abort
+        if (s_abort) break;
+        if (tid == 0) s_cnt = c;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+        const int32_t i = c0 + j;
+        if (owner && i < n) {
+            B.v[I.s_off + i] = V[j];
+            B.last[I.s_off + i] = last[j];
+        }
+    }
+    if (nspk) {
+        const unsigned long long ev = (unsigned long long)nspk * (unsigned long long)n;
+        atomicAdd(B.events, ev);
+        atomicAdd(B.inst_events + pd.inst, ev);
+    }
+}
+
+const void* parts_kernel(int npt) { return npt == 4 ? (const void*)nbx_batch_parts<4> : (const void*)nbx_batch_parts<8>; }
 
 }  // namespace
 
@@ -208,16 +538,23 @@ struct nbx_batch {
     uint16_t* d_w = nullptr;
     float* d_v = nullptr;
     int32_t* d_last = nullptr;
-    uint16_t* d_spk = nullptr;
-    int32_t* d_spk_cnt = nullptr;
+    uint64_t* d_xb = nullptr;
     uint32_t* d_raster = nullptr;
     unsigned long long* d_events = nullptr;
+    unsigned long long* d_inst_ev = nullptr;
+    int32_t* d_err = nullptr;
     int32_t rec_t0 = 0, rec_n = 0;
     int32_t t = 0;
     int32_t threads = 0;  // 0 = not tuned yet
     int32_t chunk = 500;
     hipGraphExec_t gexec = nullptr;
-    int32_t g_steps = 0;
+    // split schedule: parts per instance (empty = one workgroup per instance, nbx_batch_kernel)
+    std::vector<int32_t> h_nparts;
+    std::vector<PartDev> h_parts;
+    PartDev* d_parts = nullptr;
+    size_t parts_cap = 0;
+    int32_t part_npt = 8;        // neurons per thread of the split schedule's kernel (4 or 8)
+    std::vector<double> h_work;  // per-instance work estimate (synaptic events), heaviest parts first
 
     template <typename T>
     int32_t alloc(T** p, size_t count_) {
@@ -235,12 +572,19 @@ struct nbx_batch {
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
     // neurons per thread: 8 or 16 (whole 16-byte row pieces); 32 means the size does not fit
     int npt_for(int32_t thr) const {
         const int need = (max_n + thr - 1) / thr;
         return need <= 8 ? 8 : (need <= 16 ? 16 : 32);
     }
-    hipError_t launch(int32_t t0, int32_t nsteps, int32_t thr) {
+    size_t part_lds(int32_t npt) const {
+        return (size_t)((max_n + 7) & ~7) * sizeof(uint16_t) + (size_t)kPartThreads * npt * 4;
+    }
+    BatchArgs args(int32_t t0, int32_t nsteps) const {
         BatchArgs A{};
         A.P = P;
         A.inst = d_inst;
@@ -248,19 +592,148 @@ struct nbx_batch {
         A.w = d_w;
         A.v = d_v;
         A.last = d_last;
-        A.spk = d_spk;
-        A.spk_cnt = d_spk_cnt;
+        A.xb = d_xb;
+        A.xb_half = total_words;
         A.raster = d_raster;
         A.rec_t0 = rec_t0;
         A.rec_n = rec_n;
         A.events = d_events;
+        A.inst_events = d_inst_ev;
+        A.err = d_err;
         A.t0 = t0;
         A.nsteps = nsteps;
         A.max_n = max_n;
+        return A;
+    }
+    hipError_t launch(int32_t t0, int32_t nsteps, int32_t thr) {
+        BatchArgs A = args(t0, nsteps);
         void* fn = batch_kernel(npt_for(thr));
         const size_t lds = ((size_t)2 * max_n + (size_t)thr) * sizeof(uint16_t);
-        void* args[] = {&A};
-        return hipLaunchKernel(fn, dim3(count), dim3(thr), args, lds, stream);
+        void* kargs[] = {&A};
+        return hipLaunchKernel(fn, dim3(count), dim3(thr), kargs, lds, stream);
+    }
+    // Every part of a launch must be co-resident: the cooperative launch has the runtime check it
+    // (capacity is also checked when the split is set). NBX_COOPERATIVE=0 selects a regular
+    // launch for tools that cannot trace cooperative dispatches; the polls are bounded either way.
+    hipError_t launch_parts(int32_t t0, int32_t nsteps) {
+        BatchArgs A = args(t0, nsteps);
+        const PartDev* parts = d_parts;
+        void* kargs[] = {&A, &parts};
+        const void* fn = parts_kernel(part_npt);
+        const char* coop = std::getenv("NBX_COOPERATIVE");
+        if (coop && coop[0] == '0')
+            return hipLaunchKernel(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs, part_lds(part_npt),
+                                   stream);
+        return hipLaunchCooperativeKernel(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs,
+                                          (unsigned)part_lds(part_npt), stream);
+    }
+    int32_t part_capacity(int32_t npt, int32_t* cap) const {
+        int per_cu = 0, cus = 0;
+        NBX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, parts_kernel(npt), kPartThreads,
+                                                                 part_lds(npt)));
+        NBX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        *cap = per_cu * cus;
+        return NBX_OK;
+    }
+    // Builds the parts of a split schedule (parts[i] requested for instance i; the part width is
+    // rounded up to 32 columns, so an instance may get fewer) and uploads them, heaviest first.
+    // Parts of at most 1024 columns run 4 neurons per thread, wider ones (up to 2048) 8.
+    int32_t set_parts(const int32_t* req) {
+        std::vector<int32_t> np((size_t)count), pwv((size_t)count);
+        std::vector<PartDev> parts;
+        std::vector<double> load;
+        int32_t max_pw = 0;
+        for (int32_t i = 0; i < count; ++i) {
+            const int32_t n = h_inst[(size_t)i].n;
+            const int32_t S = std::max(1, std::min(req[i], (n + 31) / 32));
+            const int32_t pw = ((n + S - 1) / S + 31) / 32 * 32;
+            if (pw > kPartThreads * 8) {
+                set_error("nbx_batch_set_parts: instance " + std::to_string(i) + " (n = " + std::to_string(n) +
+                          ") needs at least " + std::to_string((n + kPartThreads * 8 - 1) / (kPartThreads * 8)) +
+                          " parts");
+                return NBX_ERR_INVALID;
+            }
+            pwv[(size_t)i] = pw;
+            max_pw = std::max(max_pw, pw);
+        }
+        const int32_t npt = max_pw <= kPartThreads * 4 ? 4 : 8;
+        for (int32_t i = 0; i < count; ++i) {
+            const int32_t n = h_inst[(size_t)i].n, pw = pwv[(size_t)i];
+            np[(size_t)i] = (n + pw - 1) / pw;
+            const int32_t cg = pw / npt;
+            for (int32_t j = 0; j < np[(size_t)i]; ++j) {
+                PartDev d{};
+                d.inst = i;
+                d.lo = j * pw;
+                d.w = std::min(pw, n - d.lo);
+                d.cg = cg;
+                d.rg = kPartThreads / cg;
+                d.parts = np[(size_t)i];
+                parts.push_back(d);
+                load.push_back(h_work[(size_t)i] / np[(size_t)i]);
+            }
+        }
+        int32_t cap = 0;
+        NBX_PASS(part_capacity(npt, &cap));
+        if ((int64_t)parts.size() > cap) {
+            set_error("nbx_batch_set_parts: " + std::to_string(parts.size()) + " parts exceed the " +
+                      std::to_string(cap) + " co-resident workgroups");
+            return NBX_ERR_UNSUPPORTED;
+        }
+        std::vector<int32_t> idx(parts.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return load[(size_t)x] > load[(size_t)y]; });
+        h_parts.clear();
+        for (int32_t k : idx) h_parts.push_back(parts[(size_t)k]);
+        if (h_parts.size() > parts_cap) {
+            NBX_PASS(alloc(&d_parts, h_parts.size()));
+            parts_cap = h_parts.size();
+        }
+        NBX_HIP_TRY(hipStreamSynchronize(stream));
+        NBX_HIP_TRY(hipMemcpy(d_parts, h_parts.data(), sizeof(PartDev) * h_parts.size(), hipMemcpyHostToDevice));
+        h_nparts = np;
+        part_npt = npt;
+        drop_graph();
+        return NBX_OK;
+    }
+    // Split plan for the autotuner: the fewest parts of at most 256 * npt columns, then one more
+    // part at a time for the instance with the most work per part while that exceeds twice the
+    // mean work per part, parts keep at least 256 columns and the co-resident capacity allows.
+    // Parts of similar work spread evenly over the CUs whatever the dispatcher's placement.
+    std::vector<int32_t> plan_parts(int32_t npt, int32_t cap) const {
+        const int32_t maxw = kPartThreads * npt;
+        std::vector<int32_t> S((size_t)count);
+        double total = 0.0;
+        int64_t used = 0;
+        for (int32_t i = 0; i < count; ++i) {
+            S[(size_t)i] = (h_inst[(size_t)i].n + maxw - 1) / maxw;
+            total += h_work[(size_t)i];
+            used += S[(size_t)i];
+        }
+        std::vector<bool> done((size_t)count, false);
+        while (used < cap) {
+            int32_t best = -1;
+            double bw = 0.0;
+            for (int32_t i = 0; i < count; ++i) {
+                const double w = h_work[(size_t)i] / S[(size_t)i];
+                if (!done[(size_t)i] && w > bw) {
+                    bw = w;
+                    best = i;
+                }
+            }
+            if (best < 0 || bw <= 2.0 * total / (double)used) break;
+            const int32_t n = h_inst[(size_t)best].n, s2 = S[(size_t)best] + 1;
+            if ((n + s2 - 1) / s2 < 256) {
+                done[(size_t)best] = true;
+                continue;
+            }
+            S[(size_t)best] = s2;
+            ++used;
+        }
+        return S;
+    }
+    hipError_t launch_any(int32_t t0, int32_t nsteps) {
+        return h_nparts.empty() ? launch(t0, nsteps, threads) : launch_parts(t0, nsteps);
     }
 };
 
@@ -277,6 +750,51 @@ int32_t poisson_tab(double lambda, uint32_t* tab, int32_t cap) {
         p = p * lambda / (double)(k + 1);
     }
     return cap - 1;
+}
+
+// Snapshot of the mutable state, for timing runs that must leave no trace.
+struct BatchSnapshot {
+    std::vector<float> v;
+    std::vector<int32_t> last;
+    std::vector<uint64_t> xb;
+    std::vector<unsigned long long> inst_ev;
+    unsigned long long ev = 0;
+    int32_t save(nbx_batch* b) {
+        const size_t n = (size_t)b->total_n, nw = 2 * (size_t)b->total_words;
+        v.resize(n);
+        last.resize(n);
+        xb.resize(nw);
+        inst_ev.resize((size_t)b->count);
+        NBX_HIP_TRY(hipStreamSynchronize(b->stream));
+        NBX_HIP_TRY(hipMemcpy(v.data(), b->d_v, sizeof(float) * n, hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(last.data(), b->d_last, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(xb.data(), b->d_xb, sizeof(uint64_t) * nw, hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(inst_ev.data(), b->d_inst_ev, sizeof(unsigned long long) * inst_ev.size(),
+                              hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(&ev, b->d_events, sizeof(ev), hipMemcpyDeviceToHost));
+        return NBX_OK;
+    }
+    int32_t restore(nbx_batch* b) const {
+        NBX_HIP_TRY(hipStreamSynchronize(b->stream));
+        NBX_HIP_TRY(hipMemcpy(b->d_v, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+        NBX_HIP_TRY(hipMemcpy(b->d_last, last.data(), sizeof(int32_t) * last.size(), hipMemcpyHostToDevice));
+        NBX_HIP_TRY(hipMemcpy(b->d_xb, xb.data(), sizeof(uint64_t) * xb.size(), hipMemcpyHostToDevice));
+        NBX_HIP_TRY(hipMemcpy(b->d_inst_ev, inst_ev.data(), sizeof(unsigned long long) * inst_ev.size(),
+                              hipMemcpyHostToDevice));
+        NBX_HIP_TRY(hipMemcpy(b->d_events, &ev, sizeof(ev), hipMemcpyHostToDevice));
+        return NBX_OK;
+    }
+};
+
+int32_t check_batch_error(nbx_batch* b) {
+    int32_t e = 0;
+    NBX_HIP_TRY(hipMemcpy(&e, b->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e == 0) return NBX_OK;
+    const int32_t z = 0;
+    NBX_HIP_TRY(hipMemcpy(b->d_err, &z, sizeof(int32_t), hipMemcpyHostToDevice));
+    set_error(e == kErrExchange ? "batched engine: spike exchange out of step or timed out (parts not co-resident?)"
+                                : "batched engine: device error flag " + std::to_string(e));
+    return NBX_ERR_STATE;
 }
 
 }  // namespace
@@ -323,6 +841,7 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
         I.w_off = b->total_w;
         I.s_off = b->total_n;
         I.r_off = 0;
+        I.b_off = b->total_words;
         I.tau = s.tau;
         I.scale = std::ldexp(1.0f, s.fixed_shift);
         I.inv_scale = 1.0f / I.scale;
@@ -331,6 +850,7 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
         I.seed_hi = s.seed_hi;
         I.kmax = s.pois_lambda > 0.0 ? poisson_tab(s.pois_lambda, I.tab, kMaxPoisTab) : 0;
         b->h_inst.push_back(I);
+        b->h_work.push_back((double)s.n * (double)s.n);  // until the autotuner has measured
         b->total_w += (int64_t)s.n * I.ld;
         b->total_n += s.n;
         b->total_words += I.words;
@@ -350,18 +870,43 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
                                 hipMemcpyHostToDevice));
         for (int32_t i = 0; i < s.n; ++i) v[(size_t)(I.s_off + i)] = s.v0 ? s.v0[i] : d->v_reset;
     }
+    // instances whose fixed-point weights all stay below 2^22 take the single-fma quantization;
+    // NBX_BATCH_EXACT=1 keeps every instance on the scale-and-round path (for tests)
+    {
+        const char* keep = std::getenv("NBX_BATCH_EXACT");
+        int32_t* d_qmax = nullptr;
+        if ((rc = b->alloc(&d_qmax, (size_t)b->count))) return rc;
+        NBX_HIP_TRY(hipMemset(d_qmax, 0, sizeof(int32_t) * (size_t)b->count));
+        auto grid = [](int64_t cnt) { return (unsigned)std::min<int64_t>((cnt + 255) / 256, 2048); };
+        for (int32_t k = 0; k < d->count; ++k) {
+            const InstDev& I = b->h_inst[(size_t)k];
+            const int64_t cnt = (int64_t)I.n * I.ld;
+            nbx_batch_qmax<<<grid(cnt), 256, 0, b->stream>>>(b->d_w + I.w_off, cnt, I.scale, d_qmax + k);
+        }
+        NBX_HIP_TRY(hipGetLastError());
+        std::vector<int32_t> qmax((size_t)b->count);
+        NBX_HIP_TRY(hipStreamSynchronize(b->stream));
+        NBX_HIP_TRY(hipMemcpy(qmax.data(), d_qmax, sizeof(int32_t) * qmax.size(), hipMemcpyDeviceToHost));
+        for (int32_t k = 0; k < d->count; ++k) {
+            InstDev& I = b->h_inst[(size_t)k];
+            I.fast = !(keep && keep[0] == '1') && qmax[(size_t)k] < kFastLimit;
+        }
+    }
     std::vector<int32_t> last((size_t)b->total_n, 10000);
     if ((rc = b->alloc(&b->d_v, (size_t)b->total_n))) return rc;
     if ((rc = b->alloc(&b->d_last, (size_t)b->total_n))) return rc;
-    if ((rc = b->alloc(&b->d_spk, (size_t)b->total_n))) return rc;
-    if ((rc = b->alloc(&b->d_spk_cnt, (size_t)b->count))) return rc;
+    if ((rc = b->alloc(&b->d_xb, 2 * (size_t)b->total_words))) return rc;
     if ((rc = b->alloc(&b->d_events, 1))) return rc;
+    if ((rc = b->alloc(&b->d_inst_ev, (size_t)b->count))) return rc;
+    if ((rc = b->alloc(&b->d_err, 1))) return rc;
     if ((rc = b->alloc(&b->d_inst, b->h_inst.size()))) return rc;
     if ((rc = b->alloc(&b->d_order, (size_t)b->count))) return rc;
     NBX_HIP_TRY(hipMemcpy(b->d_v, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
     NBX_HIP_TRY(hipMemcpy(b->d_last, last.data(), sizeof(int32_t) * last.size(), hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemset(b->d_spk_cnt, 0, sizeof(int32_t) * b->count));
+    NBX_HIP_TRY(hipMemset(b->d_xb, 0, sizeof(uint64_t) * 2 * (size_t)b->total_words));  // stamp 0: no spikes
     NBX_HIP_TRY(hipMemset(b->d_events, 0, sizeof(unsigned long long)));
+    NBX_HIP_TRY(hipMemset(b->d_inst_ev, 0, sizeof(unsigned long long) * (size_t)b->count));
+    NBX_HIP_TRY(hipMemset(b->d_err, 0, sizeof(int32_t)));
     NBX_HIP_TRY(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(InstDev) * b->h_inst.size(), hipMemcpyHostToDevice));
     // largest instances first: they bound the launch, so they start on the first wave of CUs
     std::vector<int32_t> order((size_t)b->count);
@@ -408,10 +953,7 @@ int32_t nbx_batch_record(nbx_batch* b, int32_t t_begin, int32_t nsteps) {
         if (rc) return rc;
         NBX_HIP_TRY(hipMemset(b->d_raster, 0, sizeof(uint32_t) * (size_t)off));
     }
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        b->gexec = nullptr;
-    }
+    b->drop_graph();
     return NBX_OK;
 }
 
@@ -426,10 +968,8 @@ int32_t nbx_batch_set_schedule(nbx_batch* b, int32_t threads, int32_t chunk) {
     }
     b->threads = threads;
     b->chunk = chunk;
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        b->gexec = nullptr;
-    }
+    b->h_nparts.clear();  // one workgroup per instance
+    b->drop_graph();
     return NBX_OK;
 }
 
@@ -438,32 +978,42 @@ int32_t nbx_batch_get_schedule(const nbx_batch* b, int32_t* threads, int32_t* ch
         set_error("null argument");
         return NBX_ERR_INVALID;
     }
-    *threads = b->threads;
+    *threads = b->h_nparts.empty() ? b->threads : kPartThreads;
     *chunk = b->chunk;
+    return NBX_OK;
+}
+
+int32_t nbx_batch_set_parts(nbx_batch* b, const int32_t* parts, int32_t count) {
+    if (!b || !parts || count != b->count) {
+        set_error("nbx_batch_set_parts: one entry per instance required");
+        return NBX_ERR_INVALID;
+    }
+    NBX_HIP_TRY(hipSetDevice(b->device));
+    return b->set_parts(parts);
+}
+
+int32_t nbx_batch_get_parts(const nbx_batch* b, int32_t* parts, int32_t count) {
+    if (!b || !parts || count != b->count) {
+        set_error("nbx_batch_get_parts: one entry per instance required");
+        return NBX_ERR_INVALID;
+    }
+    for (int32_t i = 0; i < count; ++i) parts[i] = b->h_nparts.empty() ? 0 : b->h_nparts[(size_t)i];
     return NBX_OK;
 }
 
 int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps);
 
-// Autotuner: times `steps` steps for each workgroup size on a scratch copy of the state
-// (restored afterwards) and keeps the fastest.
+// Autotuner: times `steps` steps for each workgroup size of the one-workgroup-per-instance
+// kernel, measures each instance's synaptic events, then times the split schedule planned
+// from them, all on a snapshot of the state (restored afterwards), and keeps the fastest.
 int32_t nbx_batch_tune(nbx_batch* b, int32_t steps, int32_t* chosen) {
     if (!b || steps <= 0) {
         set_error("nbx_batch_tune: bad arguments");
         return NBX_ERR_INVALID;
     }
     NBX_HIP_TRY(hipSetDevice(b->device));
-    NBX_HIP_TRY(hipStreamSynchronize(b->stream));
-    const size_t n = (size_t)b->total_n;
-    std::vector<float> v(n);
-    std::vector<int32_t> last(n), cnt((size_t)b->count);
-    std::vector<uint16_t> spk(n);
-    unsigned long long ev = 0;
-    NBX_HIP_TRY(hipMemcpy(v.data(), b->d_v, sizeof(float) * n, hipMemcpyDeviceToHost));
-    NBX_HIP_TRY(hipMemcpy(last.data(), b->d_last, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    NBX_HIP_TRY(hipMemcpy(spk.data(), b->d_spk, sizeof(uint16_t) * n, hipMemcpyDeviceToHost));
-    NBX_HIP_TRY(hipMemcpy(cnt.data(), b->d_spk_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
-    NBX_HIP_TRY(hipMemcpy(&ev, b->d_events, sizeof(ev), hipMemcpyDeviceToHost));
+    BatchSnapshot snap;
+    NBX_PASS(snap.save(b));
     const int32_t saved_rec = b->rec_n;
     b->rec_n = 0;  // no raster writes while tuning
     float best = 1e30f;
@@ -471,43 +1021,85 @@ int32_t nbx_batch_tune(nbx_batch* b, int32_t steps, int32_t* chosen) {
     hipEvent_t e0, e1;
     NBX_HIP_TRY(hipEventCreate(&e0));
     NBX_HIP_TRY(hipEventCreate(&e1));
-    for (int32_t thr : {256, 512, 1024}) {
-        if (b->npt_for(thr) > 16) continue;
-        NBX_HIP_TRY(b->launch(b->t, std::min(steps, 20), thr));  // warm
+    auto timed = [&](auto&& go, float* ms) -> int32_t {
+        NBX_PASS(snap.restore(b));
+        NBX_HIP_TRY(go(b->t, std::min(steps, 20)));  // warm
         NBX_HIP_TRY(hipEventRecord(e0, b->stream));
-        NBX_HIP_TRY(b->launch(b->t, steps, thr));
+        NBX_HIP_TRY(go(b->t + std::min(steps, 20), steps));
         NBX_HIP_TRY(hipEventRecord(e1, b->stream));
         NBX_HIP_TRY(hipEventSynchronize(e1));
+        NBX_HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+        return check_batch_error(b);
+    };
+    int32_t rc = NBX_OK;
+    for (int32_t thr : {256, 512, 1024}) {
+        if (b->npt_for(thr) > 16 || rc) continue;
         float ms = 0.0f;
-        NBX_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
+        rc = timed([&](int32_t t0, int32_t k) { return b->launch(t0, k, thr); }, &ms);
+        if (!rc && ms < best) {
             best = ms;
             best_thr = thr;
+        }
+    }
+    // per-instance work of the last timed run, then the split schedule planned from it
+    std::vector<unsigned long long> iev((size_t)b->count);
+    bool split = false;
+    if (!rc) {
+        rc = [&]() -> int32_t {
+            NBX_HIP_TRY(hipMemcpy(iev.data(), b->d_inst_ev, sizeof(unsigned long long) * iev.size(),
+                                  hipMemcpyDeviceToHost));
+            return NBX_OK;
+        }();
+    }
+    if (!rc) {
+        for (int32_t i = 0; i < b->count; ++i) {
+            const double ev = (double)(iev[(size_t)i] - snap.inst_ev[(size_t)i]);
+            // a neuron update costs about as much as gathering 8 synaptic events
+            b->h_work[(size_t)i] = ev + 8.0 * (double)b->h_inst[(size_t)i].n * (steps + std::min(steps, 20));
+        }
+        int32_t cap4 = 0, cap8 = 0;
+        rc = b->part_capacity(4, &cap4);
+        if (!rc) rc = b->part_capacity(8, &cap8);
+        if (!rc) {
+            int64_t min4 = 0;
+            for (const InstDev& I : b->h_inst) min4 += (I.n + kPartThreads * 4 - 1) / (kPartThreads * 4);
+            const int32_t npt = min4 <= cap4 ? 4 : 8;
+            const int32_t cap = npt == 4 ? cap4 : cap8;
+            const std::vector<int32_t> S = b->plan_parts(npt, cap);
+            int64_t total = 0;
+            for (int32_t s : S) total += s;
+            if (total <= cap && (rc = b->set_parts(S.data())) == NBX_OK) {
+                float ms = 0.0f;
+                rc = timed([&](int32_t t0, int32_t k) { return b->launch_parts(t0, k); }, &ms);
+                if (!rc && ms < best) {
+                    best = ms;
+                    split = true;
+                }
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->rec_n = saved_rec;
-    NBX_HIP_TRY(hipMemcpy(b->d_v, v.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemcpy(b->d_last, last.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemcpy(b->d_spk, spk.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemcpy(b->d_spk_cnt, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemcpy(b->d_events, &ev, sizeof(ev), hipMemcpyHostToDevice));
-    if (best_thr == 0) {
+    const int32_t rs = snap.restore(b);
+    if (rc) return rc;
+    if (rs) return rs;
+    if (best_thr == 0 && !split) {
         set_error("nbx_batch_tune: no workgroup size fits the largest instance");
         return NBX_ERR_UNSUPPORTED;
     }
-    b->threads = best_thr;
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        b->gexec = nullptr;
+    if (!split) {
+        b->h_nparts.clear();
+        b->threads = best_thr;
     }
-    if (chosen) *chosen = best_thr;
+    b->drop_graph();
+    if (chosen) *chosen = split ? kPartThreads : best_thr;
     return NBX_OK;
 }
 
-// Runs nsteps as a captured graph of `chunk`-step launches (the graph is rebuilt when the
-// number of steps, the schedule or the recording window changes).
+// Runs nsteps: one workgroup per instance as a captured graph of `chunk`-step launches (rebuilt
+// for every call, as the step numbers are baked into the launches), or the split schedule as
+// direct cooperative launches of `chunk` steps.
 int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps) {
     if (!b || nsteps < 0) {
         set_error("nbx_batch_run: bad arguments");
@@ -515,16 +1107,18 @@ int32_t nbx_batch_run_async(nbx_batch* b, int32_t nsteps) {
     }
     if (nsteps == 0) return NBX_OK;
     NBX_HIP_TRY(hipSetDevice(b->device));
+    if (!b->h_nparts.empty()) {
+        for (int32_t done = 0; done < nsteps; done += b->chunk)
+            NBX_HIP_TRY(b->launch_parts(b->t + done, std::min(b->chunk, nsteps - done)));
+        b->t += nsteps;
+        return NBX_OK;
+    }
     if (b->threads == 0) {  // untuned: the smallest workgroup that gives each thread at most 8 neurons
         int32_t thr = 1024;
         while (thr > 256 && b->max_n <= thr / 2 * 8) thr /= 2;
         b->threads = thr;
     }
-    // step numbers are baked into the launches, so the graph is specific to (t, nsteps)
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        b->gexec = nullptr;
-    }
+    b->drop_graph();
     hipGraph_t g = nullptr;
     NBX_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
@@ -551,7 +1145,7 @@ int32_t nbx_batch_sync(nbx_batch* b) {
     }
     NBX_HIP_TRY(hipSetDevice(b->device));
     NBX_HIP_TRY(hipStreamSynchronize(b->stream));
-    return NBX_OK;
+    return check_batch_error(b);
 }
 
 int32_t nbx_batch_run(nbx_batch* b, int32_t nsteps) {

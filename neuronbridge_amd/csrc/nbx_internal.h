@@ -104,8 +104,8 @@ struct RuleDev {
     int32_t kern_off, kern_len;
     uint32_t seed;
 };
-// Device error flag bits (nbx_net::d_err)
-constexpr int32_t kErrBarrier = 1, kErrHistory = 2;
+// Device error codes (nbx_net::d_err; the batched engine raises kErrExchange)
+constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4;
 constexpr int32_t kMaxRules = 64;
 
 // Device pointers and scalars the step kernels need; passed by value.
@@ -195,10 +195,11 @@ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        // each 32x32 -> 64-bit product is one v_mad_u64_u32 (not a mul_lo / mul_hi pair)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
