@@ -713,17 +713,15 @@ int32_t nbx_net_finalize(nbx_net* net) {
     if (net->cur_n) NBX_RC(net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size()));
     if (net->stdp || !net->rules.empty()) {  // per-synapse pair traces (STDP_State), shared by both
         const size_t S = (size_t)net->S;
-        std::vector<float> zf(S, 0.0f);
-        std::vector<int32_t> zi(S, 0);
-        NBX_RC(net->upload(&net->d_apre, zf.data(), S));
-        NBX_RC(net->upload(&net->d_apost, zf.data(), S));
-        NBX_RC(net->upload(&net->d_slast, zi.data(), S));
+        std::vector<SynTrace> tr(S);
+        for (size_t j = 0; j < S; ++j)
+            tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j]};
+        NBX_RC(net->upload(&net->d_trace, tr.data(), S));
     }
     if (!net->rules.empty()) NBX_RC(finalize_rules(net));
     if (net->stdp) {
         NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
         const size_t S = (size_t)net->S;
-        if (!net->h_smaxw.empty()) NBX_RC(net->upload(&net->d_smaxw, net->h_smaxw.data(), S));
         std::vector<float> tp, td;
         trace_tables(net->md.dt, net->sd.ltp_tau, net->sd.ltd_tau, &tp, &td);
         net->tab_len = (int32_t)tp.size();
@@ -1093,10 +1091,13 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
     NBX_CHECK(count == net->S, "buffer size must equal the synapse count");
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-    if (apre) NBX_HIP_TRY(hipMemcpy(apre, net->d_apre, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
-    if (apost) NBX_HIP_TRY(hipMemcpy(apost, net->d_apost, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
-    if (last_update)
-        NBX_HIP_TRY(hipMemcpy(last_update, net->d_slast, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToHost));
+    std::vector<SynTrace> tr((size_t)count);
+    NBX_HIP_TRY(hipMemcpy(tr.data(), net->d_trace, sizeof(SynTrace) * tr.size(), hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < tr.size(); ++j) {
+        if (apre) apre[j] = tr[j].apre;
+        if (apost) apost[j] = tr[j].apost;
+        if (last_update) last_update[j] = tr[j].slast;
+    }
     return NBX_OK;
 }
 
@@ -1109,20 +1110,19 @@ int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* e
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     const size_t S = (size_t)count;
-    std::vector<float> f(S);
-    std::vector<int32_t> iv(S);
-    auto masked = [&](float* out, const float* d) -> int32_t {
-        if (!out) return NBX_OK;
-        NBX_HIP_TRY(hipMemcpy(f.data(), d, sizeof(float) * S, hipMemcpyDeviceToHost));
-        for (size_t j = 0; j < S; ++j) out[j] = net->h_rule[j] >= 0 ? f[j] : 0.0f;
-        return NBX_OK;
-    };
-    NBX_RC(masked(apre, net->d_apre));
-    NBX_RC(masked(apost, net->d_apost));
-    NBX_RC(masked(eligibility, net->d_elig));
-    if (last_update) {
-        NBX_HIP_TRY(hipMemcpy(iv.data(), net->d_slast, sizeof(int32_t) * S, hipMemcpyDeviceToHost));
-        for (size_t j = 0; j < S; ++j) last_update[j] = net->h_rule[j] >= 0 ? iv[j] : 0;
+    std::vector<SynTrace> tr(S);
+    NBX_HIP_TRY(hipMemcpy(tr.data(), net->d_trace, sizeof(SynTrace) * S, hipMemcpyDeviceToHost));
+    std::vector<float> el;
+    if (eligibility) {
+        el.resize(S);
+        NBX_HIP_TRY(hipMemcpy(el.data(), net->d_elig, sizeof(float) * S, hipMemcpyDeviceToHost));
+    }
+    for (size_t j = 0; j < S; ++j) {
+        const bool on = net->h_rule[j] >= 0;
+        if (apre) apre[j] = on ? tr[j].apre : 0.0f;
+        if (apost) apost[j] = on ? tr[j].apost : 0.0f;
+        if (eligibility) eligibility[j] = on ? el[j] : 0.0f;
+        if (last_update) last_update[j] = on ? tr[j].slast : 0;
     }
     return NBX_OK;
 }

@@ -109,6 +109,14 @@ constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4;
 constexpr int32_t kMaxRules = 64;
 
 // Device pointers and scalars the step kernels need; passed by value.
+// Pair-trace state of one plastic synapse (STDP_State: Apre, Apost, time of the last update) and
+// its STDP weight bound, in one 16-byte record: an update reads and writes one line.
+struct SynTrace {
+    float apre, apost;
+    int32_t slast;
+    float wmax;
+};
+
 struct StepArgs {
     ModelParams P;
     int32_t n, nch, D, words;
@@ -146,10 +154,7 @@ struct StepArgs {
     // STDP
     int32_t stdp;
     float max_ltp, max_ltd, w_max;
-    const float* smaxw;  // [S] per-synapse STDP weight bound, or null (w_max)
-    float* apre;
-    float* apost;
-    int32_t* slast;
+    SynTrace* trace;  // [S] pair traces of STDP and rule-plastic synapses
     const float* ltp_tab;
     const float* ltd_tab;
     int32_t tab_len;

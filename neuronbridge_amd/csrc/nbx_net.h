@@ -11,6 +11,7 @@
 using nbx::ModelParams;
 using nbx::StepArgs;
 using nbx::RuleDev;
+using nbx::SynTrace;
 
 // Post-blocked layout of the persistent engine and of replay delivery: block b owns
 // neurons [b * P, min((b + 1) * P, n)). Blocks are grouped into regions of rb blocks
@@ -109,14 +110,13 @@ struct nbx_net {
     float* d_wt = nullptr;
     uint32_t* d_pois_tab = nullptr;
     float* d_cur = nullptr;
-    float *d_apre = nullptr, *d_apost = nullptr;
-    int32_t* d_slast = nullptr;
+    SynTrace* d_trace = nullptr;
     float *d_ltp_tab = nullptr, *d_ltd_tab = nullptr;
     int32_t tab_len = 1;
     int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
     RuleDev* d_rules = nullptr;
     int8_t *d_syn_rule = nullptr, *d_syn_trig = nullptr;
-    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr, *d_smaxw = nullptr;
+    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr;
     int64_t *d_rin_ptr = nullptr, *d_rin_syn = nullptr, *d_tin_ptr = nullptr, *d_tin_syn = nullptr;
     int32_t *d_tstamp = nullptr, *d_tpost_stamp = nullptr, *d_tposts = nullptr, *d_tcount = nullptr;
     int32_t *d_conn_index = nullptr, *d_cidx = nullptr, *d_chist = nullptr, *d_chead = nullptr;
@@ -184,10 +184,7 @@ struct nbx_net {
         A.max_ltp = sd.max_ltp;
         A.max_ltd = sd.max_ltd;
         A.w_max = sd.w_max;
-        A.smaxw = d_smaxw;
-        A.apre = d_apre;
-        A.apost = d_apost;
-        A.slast = d_slast;
+        A.trace = d_trace;
         A.ltp_tab = d_ltp_tab;
         A.ltd_tab = d_ltd_tab;
         A.tab_len = tab_len;

@@ -172,18 +172,18 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_dense(StepArgs A,
     if (A.nch > 1 && neg) atomicAdd(&r[A.n + i], neg);
 }
 
-// STDP_State::SetUpdate with the exponential looked up by the integer step delta.
-__device__ inline void stdp_decay(const StepArgs& A, int64_t s, int32_t t, float& apre, float& apost) {
-    int32_t dt = t - A.slast[s];
+// STDP_State::SetUpdate with the exponentials looked up by the integer step delta; the record's
+// last-update step becomes t.
+__device__ inline void stdp_decay(const StepArgs& A, SynTrace& r, int32_t t) {
+    int32_t dt = t - r.slast;
     if (dt > A.tab_len - 1) dt = A.tab_len - 1;
-    apre = A.apre[s] * A.ltp_tab[dt];
-    apost = A.apost[s] * A.ltd_tab[dt];
-    A.slast[s] = t;
+    r.apre = r.apre * A.ltp_tab[dt];
+    r.apost = r.apost * A.ltd_tab[dt];
+    r.slast = t;
 }
 
 // STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum
-__device__ inline float stdp_clip(const StepArgs& A, float w, int64_t j) {
-    const float wm = A.smaxw ? A.smaxw[j] : A.w_max;
+__device__ inline float stdp_clip(float w, float wm) {
     if (w < 0.0f) w = 0.0f;
     else if (w > wm) w = wm;
     return w;
@@ -192,13 +192,12 @@ __device__ inline float stdp_clip(const StepArgs& A, float w, int64_t j) {
 // ---- trigger-driven rules
 // STDP_State::SetUpdate for rule r (STDP_State.cpp:17-26): both traces decay over the integer
 // step delta, looked up in the rule's tables.
-__device__ inline void rule_decay(const StepArgs& A, const RuleDev& R, int64_t s, int32_t t, float& apre,
-                                  float& apost) {
-    int32_t dt = t - A.slast[s];
+__device__ inline void rule_decay(const StepArgs& A, const RuleDev& R, SynTrace& r, int32_t t) {
+    int32_t dt = t - r.slast;
     if (dt > R.tab_len - 1) dt = R.tab_len - 1;
-    apre = A.apre[s] * A.rtab[R.tab_off + dt];
-    apost = A.apost[s] * A.rtab[R.tab_off + R.tab_len + dt];
-    A.slast[s] = t;
+    r.apre = r.apre * A.rtab[R.tab_off + dt];
+    r.apost = r.apost * A.rtab[R.tab_off + R.tab_len + dt];
+    r.slast = t;
 }
 
 // Interconnections::UpdateWeight (Interconnections.h:120-128)
@@ -238,7 +237,6 @@ __device__ inline float stable_unit_random(uint32_t seed, int32_t rule_id, int32
 // Pre-side update of a rule-plastic synapse at its arrival (after its weight was delivered).
 __device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t r) {
     const RuleDev R = A.rules[r];
-    float apre, apost;
     if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:89-103
         const float u = stable_unit_random(R.seed, r, A.conn_index[j], t);
         A.w[j] = bounded_increment(A.w[j], R.a1pre * u, A.maxw[j], R.min_weight);
@@ -252,17 +250,17 @@ __device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t
         A.chead[c] = h + 1;
         return;
     }
+    SynTrace tr = A.trace[j];
     if (R.kind == NBX_RULE_RSTDP) {  // R_STDP.cpp:40-48
-        rule_decay(A, R, j, t, apre, apost);
-        apre += 1.0f;
-        A.elig[j] += -R.max_ltd * apost;
+        rule_decay(A, R, tr, t);
+        tr.apre += 1.0f;
+        A.elig[j] += -R.max_ltd * tr.apost;
     } else {  // AdditiveKernalChange.cpp:65-70: weight first, then the trace
         A.w[j] = update_weight(A.w[j], R.a1pre, A.maxw[j]);
-        rule_decay(A, R, j, t, apre, apost);
-        apre += 1.0f;
+        rule_decay(A, R, tr, t);
+        tr.apre += 1.0f;
     }
-    A.apre[j] = apre;
-    A.apost[j] = apost;
+    A.trace[j] = tr;
 }
 
 // A trigger synapse arrived: stamp it, and list its target once per step.
@@ -274,8 +272,9 @@ __device__ inline void note_trigger(const StepArgs& A, int64_t j, int32_t post, 
     }
 }
 
-// Arrival-time delivery: work items are (delay d, spike of step t-d) pairs; each
-// block walks the slice of synapses with that delay in the spiking neuron's row.
+// Arrival-time delivery: work items are (delay d, spike of step t-d) pairs; each wave walks
+// the slice of synapses with that delay in the spiking neuron's row (slices are short, so a
+// wave per item keeps four items per workgroup in flight).
 __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const int32_t* tbase, int32_t k) {
     const int32_t t = step_of(tbase, k);
     __shared__ int32_t s_pref[kMaxDelay + 2];
@@ -291,25 +290,27 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const in
     __syncthreads();
     const int32_t total = s_pref[A.D];
     int32_t* racc = A.ring + (int64_t)(t % A.D) * A.nch * A.n;
-    for (int32_t p = blockIdx.x; p < total; p += gridDim.x) {
+    constexpr int32_t kWaves = kDeliverThreads / 64;
+    const int32_t lane = threadIdx.x & 63;
+    for (int32_t p = blockIdx.x * kWaves + (int32_t)(threadIdx.x >> 6); p < total; p += gridDim.x * kWaves) {
         int d = 1;
         while (d + 1 < A.D && s_pref[d + 1] <= p) ++d;
         const int32_t pre = A.hist[(int64_t)((t - d) % A.D) * A.n + (p - s_pref[d])];
         const int64_t b = A.dptr[(int64_t)pre * (A.D + 1) + d];
         const int64_t e = A.dptr[(int64_t)pre * (A.D + 1) + d + 1];
-        for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
+        for (int64_t j = b + lane; j < e; j += 64) {
             const uint32_t en = A.syn[j];
             const float w = A.w[j];
+            // every synapse has a trace record: load it with the flag rather than after it
+            SynTrace tr = A.stdp ? A.trace[j] : SynTrace{};
             int32_t ch, q;
             syn_effect(A, en, w, ch, q);
             atomicAdd(&racc[(int64_t)ch * A.n + syn_post(en)], q);
             if (A.stdp && A.plastic[j]) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
-                float apre, apost;
-                stdp_decay(A, j, t, apre, apost);
-                apre += 1.0f;
-                A.apre[j] = apre;
-                A.apost[j] = apost;
-                A.w[j] = stdp_clip(A, w + -A.max_ltd * apost, j);
+                stdp_decay(A, tr, t);
+                tr.apre += 1.0f;
+                A.trace[j] = tr;
+                A.w[j] = stdp_clip(w + -A.max_ltd * tr.apost, tr.wmax);
             }
             if (A.nrules) {
                 const int32_t r = A.syn_rule[j];
@@ -331,16 +332,27 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
     const int32_t* ids = A.hist + (int64_t)slot * A.n;
     for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
         const int32_t p = ids[s];
-        if (A.stdp) {
+        if (A.stdp) {  // STDP.cpp:37-59: decay, Apost += 1, w += A+ * Apre, clip
+            // two synapses per thread, every load issued before the dependent arithmetic (the
+            // synapses of one target are distinct, so the stores cannot alias the other's loads)
             const int64_t b = A.in_ptr[p], e = A.in_ptr[p + 1];
-            for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
-                const int64_t sid = A.in_syn[j];
-                float apre, apost;
-                stdp_decay(A, sid, t, apre, apost);
-                apost += 1.0f;
-                A.apre[sid] = apre;
-                A.apost[sid] = apost;
-                A.w[sid] = stdp_clip(A, A.w[sid] + A.max_ltp * apre, sid);
+            for (int64_t j0 = b + threadIdx.x; j0 < e; j0 += 2 * kDeliverThreads) {
+                const int64_t j1 = j0 + kDeliverThreads;
+                const bool two = j1 < e;
+                const int64_t s0 = A.in_syn[j0];
+                const int64_t s1 = two ? A.in_syn[j1] : s0;
+                SynTrace r0 = A.trace[s0], r1 = A.trace[s1];
+                const float w0 = A.w[s0], w1 = A.w[s1];
+                stdp_decay(A, r0, t);
+                stdp_decay(A, r1, t);
+                r0.apost += 1.0f;
+                r1.apost += 1.0f;
+                A.trace[s0] = r0;
+                A.w[s0] = stdp_clip(w0 + A.max_ltp * r0.apre, r0.wmax);
+                if (two) {
+                    A.trace[s1] = r1;
+                    A.w[s1] = stdp_clip(w1 + A.max_ltp * r1.apre, r1.wmax);
+                }
             }
         }
         if (A.rstdp) {  // R_STDP.cpp:19-37: decay, Apost += 1, e += Apre * max_ltp
@@ -350,12 +362,11 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
                 const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
                 for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
                     const int64_t sid = A.rin_syn[j];
-                    float apre, apost;
-                    rule_decay(A, R, sid, t, apre, apost);
-                    apost += 1.0f;
-                    A.apre[sid] = apre;
-                    A.apost[sid] = apost;
-                    A.elig[sid] += apre * R.max_ltp;
+                    SynTrace tr = A.trace[sid];
+                    rule_decay(A, R, tr, t);
+                    tr.apost += 1.0f;
+                    A.trace[sid] = tr;
+                    A.elig[sid] += tr.apre * R.max_ltp;
                 }
             }
         }
@@ -403,15 +414,14 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_trigger(StepArgs A, const
             }
             for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
                 const int64_t sid = A.rin_syn[j];
-                float apre, apost;
-                rule_decay(A, R, sid, t, apre, apost);
-                A.apre[sid] = apre;
-                A.apost[sid] = apost;
+                SynTrace tr = A.trace[sid];
+                rule_decay(A, R, tr, t);
+                A.trace[sid] = tr;
                 if (R.kind == NBX_RULE_RSTDP) {
                     A.w[sid] = update_weight(A.w[sid], factor * A.elig[sid], A.maxw[sid]);
                     if (R.clear) A.elig[sid] = 0.0f;
                 } else {
-                    A.w[sid] = update_weight(A.w[sid], R.a2prepre * apre, A.maxw[sid]);
+                    A.w[sid] = update_weight(A.w[sid], R.a2prepre * tr.apre, A.maxw[sid]);
                 }
             }
             __syncthreads();  // the next trigger of this target sees these weights
