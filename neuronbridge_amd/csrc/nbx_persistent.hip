@@ -610,7 +610,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
             }
 #pragma unroll
             for (int r = 0; r < kSR; ++r)
-                v[r] = (sid[r] >= 0) ? X.slots[((int64_t)(sid[r] & 0xFFFFFF) * X.Bq + q) * (kSlotCap / 2) + gl]
+                v[r] = (sid[r] >= 0) ? X.slots[((int64_t)q * X.n + (sid[r] & 0xFFFFFF)) * (kSlotCap / 2) + gl]
                                      : make_uint4(kSlotEmpty, 0u, kSlotEmpty, 0u);
         };
         // the first batch of the next window, fetched before this window's flush so its loads
@@ -960,10 +960,11 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
         std::fill(cnt.begin(), cnt.end(), 0);
         std::fill(fill.begin(), fill.end(), 0);
         for (int64_t j = rbeg; j < rend; ++j) cnt[(size_t)(syn_post(net->h_syn[(size_t)j]) / Pr)]++;
-        uint2* row = slots.data() + (size_t)p * Bq * kSlotCap;
+        // block-major: the slots one workgroup reads (one post block) are contiguous
+        auto slot = [&](int32_t qb) { return slots.data() + ((size_t)qb * n + (size_t)p) * kSlotCap; };
         for (int32_t qb = 0; qb < Bq; ++qb)
             if (cnt[(size_t)qb] > kSlotCap) {  // 15 entries in the slot, the rest behind a link
-                row[(size_t)qb * kSlotCap + kSlotCap - 1] =
+                slot(qb)[kSlotCap - 1] =
                     make_uint2(kSlotLink | (uint32_t)(cnt[(size_t)qb] - (kSlotCap - 1)), (uint32_t)ovf.size());
                 ovf.resize(ovf.size() + (size_t)(cnt[(size_t)qb] - (kSlotCap - 1)));
             }
@@ -977,8 +978,8 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
                                         (uint32_t)qv);
             const int32_t k = fill[(size_t)qb]++;
             const bool linked = cnt[(size_t)qb] > kSlotCap;
-            if (!linked || k < kSlotCap - 1) row[(size_t)qb * kSlotCap + k] = en;
-            else ovf[(size_t)row[(size_t)qb * kSlotCap + kSlotCap - 1].y + (size_t)(k - (kSlotCap - 1))] = en;
+            if (!linked || k < kSlotCap - 1) slot(qb)[k] = en;
+            else ovf[(size_t)slot(qb)[kSlotCap - 1].y + (size_t)(k - (kSlotCap - 1))] = en;
         }
     }
     if ((int64_t)ovf.size() >= ((int64_t)1 << 31)) {
