@@ -876,7 +876,7 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
         const char* keep = std::getenv("NBX_BATCH_EXACT");
         int32_t* d_qmax = nullptr;
         if ((rc = b->alloc(&d_qmax, (size_t)b->count))) return rc;
-        NBX_HIP_TRY(hipMemset(d_qmax, 0, sizeof(int32_t) * (size_t)b->count));
+        NBX_HIP_TRY(hipMemsetAsync(d_qmax, 0, sizeof(int32_t) * (size_t)b->count, b->stream));
         auto grid = [](int64_t cnt) { return (unsigned)std::min<int64_t>((cnt + 255) / 256, 2048); };
         for (int32_t k = 0; k < d->count; ++k) {
             const InstDev& I = b->h_inst[(size_t)k];
@@ -903,10 +903,12 @@ int32_t nbx_batch_create(const nbx_batch_desc* d, int32_t device, nbx_batch** ou
     if ((rc = b->alloc(&b->d_order, (size_t)b->count))) return rc;
     NBX_HIP_TRY(hipMemcpy(b->d_v, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
     NBX_HIP_TRY(hipMemcpy(b->d_last, last.data(), sizeof(int32_t) * last.size(), hipMemcpyHostToDevice));
-    NBX_HIP_TRY(hipMemset(b->d_xb, 0, sizeof(uint64_t) * 2 * (size_t)b->total_words));  // stamp 0: no spikes
-    NBX_HIP_TRY(hipMemset(b->d_events, 0, sizeof(unsigned long long)));
-    NBX_HIP_TRY(hipMemset(b->d_inst_ev, 0, sizeof(unsigned long long) * (size_t)b->count));
-    NBX_HIP_TRY(hipMemset(b->d_err, 0, sizeof(int32_t)));
+    // clears ordered with the batch's (non-blocking) stream, finished before the call returns
+    NBX_HIP_TRY(hipMemsetAsync(b->d_xb, 0, sizeof(uint64_t) * 2 * (size_t)b->total_words, b->stream));  // stamp 0
+    NBX_HIP_TRY(hipMemsetAsync(b->d_events, 0, sizeof(unsigned long long), b->stream));
+    NBX_HIP_TRY(hipMemsetAsync(b->d_inst_ev, 0, sizeof(unsigned long long) * (size_t)b->count, b->stream));
+    NBX_HIP_TRY(hipMemsetAsync(b->d_err, 0, sizeof(int32_t), b->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(b->stream));
     NBX_HIP_TRY(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(InstDev) * b->h_inst.size(), hipMemcpyHostToDevice));
     // largest instances first: they bound the launch, so they start on the first wave of CUs
     std::vector<int32_t> order((size_t)b->count);
@@ -951,7 +953,8 @@ int32_t nbx_batch_record(nbx_batch* b, int32_t t_begin, int32_t nsteps) {
     if (nsteps) {
         int32_t rc = b->alloc(&b->d_raster, (size_t)off);
         if (rc) return rc;
-        NBX_HIP_TRY(hipMemset(b->d_raster, 0, sizeof(uint32_t) * (size_t)off));
+        NBX_HIP_TRY(hipMemsetAsync(b->d_raster, 0, sizeof(uint32_t) * (size_t)off, b->stream));
+        NBX_HIP_TRY(hipStreamSynchronize(b->stream));  // the stream is non-blocking
     }
     b->drop_graph();
     return NBX_OK;

@@ -678,17 +678,19 @@ int32_t nbx_net_finalize(nbx_net* net) {
         NBX_RC(net->upload(&net->d_d, net->h_d.data(), (size_t)n));
     }
     NBX_RC(net->alloc(&net->d_ring, (size_t)net->D * net->nch * n));
-    NBX_HIP_TRY(hipMemset(net->d_ring, 0, sizeof(int32_t) * (size_t)net->D * net->nch * n));
+    NBX_HIP_TRY(hipMemsetAsync(net->d_ring, 0, sizeof(int32_t) * (size_t)net->D * net->nch * n, net->stream));
     NBX_RC(net->alloc(&net->d_hist, (size_t)net->D * n));
     NBX_RC(net->alloc(&net->d_hist_cnt, (size_t)net->D));
-    NBX_HIP_TRY(hipMemset(net->d_hist_cnt, 0, sizeof(int32_t) * net->D));
+    NBX_HIP_TRY(hipMemsetAsync(net->d_hist_cnt, 0, sizeof(int32_t) * net->D, net->stream));
     NBX_RC(net->alloc(&net->d_did_fire, (size_t)n));
-    NBX_HIP_TRY(hipMemset(net->d_did_fire, 0, (size_t)n));
+    NBX_HIP_TRY(hipMemsetAsync(net->d_did_fire, 0, (size_t)n, net->stream));
     NBX_RC(net->alloc(&net->d_events, 1));
-    NBX_HIP_TRY(hipMemset(net->d_events, 0, sizeof(unsigned long long)));
+    NBX_HIP_TRY(hipMemsetAsync(net->d_events, 0, sizeof(unsigned long long), net->stream));
     NBX_RC(net->alloc(&net->d_tbase, 1));
     NBX_RC(net->alloc(&net->d_err, 1));
-    NBX_HIP_TRY(hipMemset(net->d_err, 0, sizeof(int32_t)));
+    NBX_HIP_TRY(hipMemsetAsync(net->d_err, 0, sizeof(int32_t), net->stream));
+    // the stream is non-blocking: finish the clears before anything else touches these buffers
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     if (net->has_sparse) {
         NBX_RC(net->upload(&net->d_row_ptr, net->h_row_ptr.data(), net->h_row_ptr.size()));
         NBX_RC(net->upload(&net->d_syn, net->h_syn.data(), net->h_syn.size()));
@@ -799,13 +801,15 @@ int32_t nbx_net_record(nbx_net* net, int32_t t_begin, int32_t nsteps, int32_t fl
     net->rec_n = nsteps;
     if (nsteps && (flags & NBX_RECORD_COUNTS)) {
         NBX_RC(net->alloc(&net->d_counts, (size_t)nsteps));
-        NBX_HIP_TRY(hipMemset(net->d_counts, 0, sizeof(int32_t) * (size_t)nsteps));
+        NBX_HIP_TRY(hipMemsetAsync(net->d_counts, 0, sizeof(int32_t) * (size_t)nsteps, net->stream));
     }
     if (nsteps && (flags & NBX_RECORD_RASTER)) {
         const size_t nw = (size_t)nsteps * net->words;
         NBX_RC(net->alloc(&net->d_raster, nw));
-        NBX_HIP_TRY(hipMemset(net->d_raster, 0, sizeof(uint32_t) * nw));
+        NBX_HIP_TRY(hipMemsetAsync(net->d_raster, 0, sizeof(uint32_t) * nw, net->stream));
     }
+    // ordered with the network's (non-blocking) stream, and done before the call returns
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     if (net->gexec) {  // captured kernels hold the old recording pointers
         hipGraphExecDestroy(net->gexec);
         net->gexec = nullptr;

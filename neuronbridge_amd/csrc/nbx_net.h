@@ -35,15 +35,15 @@ struct PersistentLayout {
 };
 
 // Fixed-slot layout of replay delivery: the synapses of presynaptic neuron i onto the m*P
-// neurons of post block q sit in one 128-byte slot of 16 entries, slots[(i * Bq + q) * 16 ..],
-// addressable without an offset lookup. Unused entries hold kSlotEmpty; a segment longer than
+// neurons of post block q sit in one 128-byte slot of 16 entries, slots[(q * n + i) * 16 ..]
+// (block-major: one workgroup's slots are contiguous), addressable without an offset lookup. Unused entries hold kSlotEmpty; a segment longer than
 // 16 keeps 15 entries in the slot and a link in the last one ({kSlotLink | count, start}) to the
 // rest, stored contiguously in ovf. Entry: {channel*m*P + post - q*m*P | (delay - 1) << 16, q}:
 // the entry's LDS word within one ring step, and its output step offset.
 struct ReplaySlots {
     bool built = false;
     int32_t m = 0, Bq = 0;
-    uint2* d_slots = nullptr;  // [n][Bq][16]
+    uint2* d_slots = nullptr;  // [Bq][n][16]
     uint2* d_ovf = nullptr;
     int64_t ovf_count = 0;
 };
