@@ -12,9 +12,10 @@
 // needs only workgroup barriers, never a grid barrier. Chunks of steps are launched from a
 // captured hipGraph.
 //
-// Such a launch lasts as long as its busiest instance (a near-saturated 8192-neuron instance
-// streams ~400 fired 16 KB rows per step through one CU while most CUs idle), so the split
-// schedule (nbx_batch_parts) runs an instance as several 256-thread workgroups, each owning a
+// Such a launch lasts as long as its busiest workgroup: an 8192-neuron instance puts 8192
+// RK4 + Philox updates and the conversion of every fired row's 8192 weights on one CU while
+// CUs holding small instances idle. The split schedule (nbx_batch_parts) therefore runs an
+// instance as several 256-thread workgroups of at most 1024 (or 2048) columns, each owning a
 // range of columns and exchanging spike words with the instance's other parts every step.
 // The autotuner times both schedules (and the workgroup sizes of the first) and picks the
 // split from the measured per-instance synaptic events. Between launches the last step's
@@ -731,9 +732,6 @@ struct nbx_batch {
             ++used;
         }
         return S;
-    }
-    hipError_t launch_any(int32_t t0, int32_t nsteps) {
-        return h_nparts.empty() ? launch(t0, nsteps, threads) : launch_parts(t0, nsteps);
     }
 };
 
