@@ -247,11 +247,20 @@ int32_t check_device_error(nbx_net* net) {
     return NBX_OK;
 }
 
-// Launch the kernels of one step; tbase == nullptr means k is the absolute step.
-hipError_t enqueue_step(nbx_net* net, const StepArgs& A, const int32_t* tbase, int32_t k) {
+// Plain STDP (no rules): the post-side updates of step k run inside the pull of step k + 1,
+// so a sequence of steps ends with one post-side launch of its own.
+bool merged_stdp(const nbx_net* net) {
+    return net->stdp && net->d_in_ptr && net->d_fmask && net->rules.empty() && net->pull_mode && net->has_sparse;
+}
+
+// Launch the kernels of one step; tbase == nullptr means k is the absolute step. first / last:
+// the step opens / closes a sequence (the post-side updates of the step before the sequence are
+// done, those of its last step are launched at its end).
+hipError_t enqueue_step(nbx_net* net, const StepArgs& A, const int32_t* tbase, int32_t k, bool first, bool last) {
     hipError_t e = hipSuccess;
+    const bool merged = merged_stdp(net);
     if (net->pull_mode && net->has_sparse) {
-        e = launch_pull(A, tbase, k, net->stream);
+        e = (merged && !first) ? launch_pull_ltp(A, tbase, k, net->stream) : launch_pull(A, tbase, k, net->stream);
         if (e != hipSuccess) return e;
     }
     if (!net->rules.empty()) {
@@ -260,7 +269,7 @@ hipError_t enqueue_step(nbx_net* net, const StepArgs& A, const int32_t* tbase, i
     }
     e = launch_update(A, tbase, k, net->stream);
     if (e != hipSuccess) return e;
-    if ((net->stdp && net->d_in_ptr) || net->rstdp) {
+    if (((net->stdp && net->d_in_ptr) || net->rstdp) && (!merged || last)) {
         e = launch_ltp(A, tbase, k, net->stream);
         if (e != hipSuccess) return e;
     }
@@ -282,7 +291,8 @@ int32_t run_graph(nbx_net* net, int32_t nsteps) {
             hipGraph_t g = nullptr;
             NBX_HIP_TRY(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = hipSuccess;
-            for (int32_t k = 0; k < kGraphChunk && e == hipSuccess; ++k) e = enqueue_step(net, A, net->d_tbase, k);
+            for (int32_t k = 0; k < kGraphChunk && e == hipSuccess; ++k)
+                e = enqueue_step(net, A, net->d_tbase, k, k == 0, k == kGraphChunk - 1);
             if (e == hipSuccess) e = launch_advance(net->d_tbase, kGraphChunk, net->stream);
             hipError_t ec = hipStreamEndCapture(net->stream, &g);
             if (e != hipSuccess || ec != hipSuccess) {
@@ -303,14 +313,15 @@ int32_t run_graph(nbx_net* net, int32_t nsteps) {
             done += net->gchunk;
         }
     }
-    for (; done < nsteps; ++done) NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + done));
+    for (const int32_t rest = done; done < nsteps; ++done)
+        NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + done, done == rest, done == nsteps - 1));
     net->t += nsteps;
     return NBX_OK;
 }
 
 int32_t run_kernels(nbx_net* net, int32_t nsteps) {
     const StepArgs A = net->args();
-    for (int32_t k = 0; k < nsteps; ++k) NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + k));
+    for (int32_t k = 0; k < nsteps; ++k) NBX_HIP_TRY(enqueue_step(net, A, nullptr, net->t + k, k == 0, k == nsteps - 1));
     net->t += nsteps;
     return NBX_OK;
 }
@@ -682,8 +693,10 @@ int32_t nbx_net_finalize(nbx_net* net) {
     NBX_RC(net->alloc(&net->d_hist, (size_t)net->D * n));
     NBX_RC(net->alloc(&net->d_hist_cnt, (size_t)net->D));
     NBX_HIP_TRY(hipMemsetAsync(net->d_hist_cnt, 0, sizeof(int32_t) * net->D, net->stream));
-    NBX_RC(net->alloc(&net->d_did_fire, (size_t)n));
-    NBX_HIP_TRY(hipMemsetAsync(net->d_did_fire, 0, (size_t)n, net->stream));
+    if (net->stdp) {  // firing history of the merged pre/post STDP kernel
+        NBX_RC(net->alloc(&net->d_fmask, (size_t)n));
+        NBX_HIP_TRY(hipMemsetAsync(net->d_fmask, 0, sizeof(uint64_t) * (size_t)n, net->stream));
+    }
     NBX_RC(net->alloc(&net->d_events, 1));
     NBX_HIP_TRY(hipMemsetAsync(net->d_events, 0, sizeof(unsigned long long), net->stream));
     NBX_RC(net->alloc(&net->d_tbase, 1));
@@ -735,10 +748,18 @@ int32_t nbx_net_finalize(nbx_net* net) {
             if (net->h_plastic[j]) in_ptr[(size_t)syn_post(net->h_syn[j]) + 1]++;
         for (int32_t i = 0; i < n; ++i) in_ptr[(size_t)i + 1] += in_ptr[(size_t)i];
         std::vector<int64_t> fill(in_ptr.begin(), in_ptr.end() - 1), in_syn((size_t)in_ptr[(size_t)n]);
-        for (size_t j = 0; j < S; ++j)
-            if (net->h_plastic[j]) in_syn[(size_t)fill[(size_t)syn_post(net->h_syn[j])]++] = (int64_t)j;
+        std::vector<int32_t> in_pd(in_syn.size());
+        for (int32_t p = 0; p < n; ++p)
+            for (int64_t j = net->h_row_ptr[(size_t)p]; j < net->h_row_ptr[(size_t)p + 1]; ++j) {
+                if (!net->h_plastic[(size_t)j]) continue;
+                const uint32_t e = net->h_syn[(size_t)j];
+                const int64_t k = fill[(size_t)syn_post(e)]++;
+                in_syn[(size_t)k] = j;
+                in_pd[(size_t)k] = p | (syn_delay(e) << 24);
+            }
         NBX_RC(net->upload(&net->d_in_ptr, in_ptr.data(), in_ptr.size()));
         NBX_RC(net->upload(&net->d_in_syn, in_syn.data(), in_syn.size()));
+        NBX_RC(net->upload(&net->d_in_pd, in_pd.data(), in_pd.size()));
     }
     net->finalized = true;
     return NBX_OK;

@@ -132,7 +132,7 @@ struct StepArgs {
     int32_t* ring;      // [D][nch][n]
     int32_t* hist;      // [D][n]
     int32_t* hist_cnt;  // [D]
-    uint8_t* did_fire;  // [n] or null
+    uint64_t* fmask;    // [n] firing history, bit k: fired k steps before the last update (STDP), or null
     const int64_t* row_ptr;
     const uint32_t* syn;
     float* w;
@@ -160,6 +160,7 @@ struct StepArgs {
     int32_t tab_len;
     const int64_t* in_ptr;
     const int64_t* in_syn;
+    const int32_t* in_pd;  // [P] presynaptic neuron | delay << 24 of each in_syn entry
     // trigger-driven rules
     int32_t nrules, rstdp;        // rule count; any R-STDP rule (post-side updates)
     const RuleDev* rules;
@@ -455,6 +456,7 @@ hipError_t launch_deliver_push(const StepArgs& A, const int32_t* tbase, int32_t 
 hipError_t launch_deliver_dense(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
+hipError_t launch_pull_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_trigger(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s);
 hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s);
 hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s);

@@ -101,7 +101,7 @@ struct nbx_net {
     int32_t* d_last = nullptr;
     float *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_d = nullptr;
     int32_t *d_ring = nullptr, *d_hist = nullptr, *d_hist_cnt = nullptr;
-    uint8_t* d_did_fire = nullptr;
+    uint64_t* d_fmask = nullptr;
     int64_t* d_row_ptr = nullptr;
     uint32_t* d_syn = nullptr;
     float* d_w = nullptr;
@@ -114,6 +114,7 @@ struct nbx_net {
     float *d_ltp_tab = nullptr, *d_ltd_tab = nullptr;
     int32_t tab_len = 1;
     int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
+    int32_t* d_in_pd = nullptr;
     RuleDev* d_rules = nullptr;
     int8_t *d_syn_rule = nullptr, *d_syn_trig = nullptr;
     float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr;
@@ -160,7 +161,7 @@ struct nbx_net {
         A.ring = d_ring;
         A.hist = d_hist;
         A.hist_cnt = d_hist_cnt;
-        A.did_fire = d_did_fire;
+        A.fmask = d_fmask;
         A.row_ptr = d_row_ptr;
         A.syn = d_syn;
         A.w = d_w;
@@ -190,6 +191,7 @@ struct nbx_net {
         A.tab_len = tab_len;
         A.in_ptr = d_in_ptr;
         A.in_syn = d_in_syn;
+        A.in_pd = d_in_pd;
         A.nrules = (int32_t)rules.size();
         A.rstdp = rstdp ? 1 : 0;
         A.rules = d_rules;

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const i
         }
         A.iext[i] = I;
         A.last[i] = last;
-        if (A.did_fire) A.did_fire[i] = fired ? 1 : 0;
+        if (A.fmask) A.fmask[i] = (A.fmask[i] << 1) | (fired ? 1ull : 0ull);
     }
     // ---- spike compaction: ballot per wavefront, prefix over the block's wavefronts
     const uint64_t mask = __ballot(fired);
@@ -275,24 +275,19 @@ __device__ inline void note_trigger(const StepArgs& A, int64_t j, int32_t post, 
 // Arrival-time delivery: work items are (delay d, spike of step t-d) pairs; each wave walks
 // the slice of synapses with that delay in the spiking neuron's row (slices are short, so a
 // wave per item keeps four items per workgroup in flight).
-__global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const int32_t* tbase, int32_t k) {
-    const int32_t t = step_of(tbase, k);
-    __shared__ int32_t s_pref[kMaxDelay + 2];
-    if (threadIdx.x == 0) {
-        int32_t tot = 0;
-        s_pref[0] = 0;
-        for (int d = 1; d < A.D; ++d) {
-            s_pref[d] = tot;
-            if (t - d >= 0) tot += A.hist_cnt[(t - d) % A.D];
-        }
-        s_pref[A.D] = tot;
-    }
-    __syncthreads();
+//
+// MERGED (plain STDP, no rules): the post-side updates of step t - 1 run in the same launch
+// (ltp_items below) instead of a kernel of their own after the update of step t - 1. A synapse
+// whose target fired at t - 1 and whose presynaptic spike arrives at t gets both updates here,
+// in the reference's order (post-side of t - 1, delivery of the new weight, pre-side of t); the
+// post-side pass skips it. fmask tells both sides which case applies.
+template <bool MERGED>
+__device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s_pref, int32_t bi, int32_t nb) {
     const int32_t total = s_pref[A.D];
     int32_t* racc = A.ring + (int64_t)(t % A.D) * A.nch * A.n;
     constexpr int32_t kWaves = kDeliverThreads / 64;
     const int32_t lane = threadIdx.x & 63;
-    for (int32_t p = blockIdx.x * kWaves + (int32_t)(threadIdx.x >> 6); p < total; p += gridDim.x * kWaves) {
+    for (int32_t p = bi * kWaves + (int32_t)(threadIdx.x >> 6); p < total; p += nb * kWaves) {
         int d = 1;
         while (d + 1 < A.D && s_pref[d + 1] <= p) ++d;
         const int32_t pre = A.hist[(int64_t)((t - d) % A.D) * A.n + (p - s_pref[d])];
@@ -300,13 +295,20 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const in
         const int64_t e = A.dptr[(int64_t)pre * (A.D + 1) + d + 1];
         for (int64_t j = b + lane; j < e; j += 64) {
             const uint32_t en = A.syn[j];
-            const float w = A.w[j];
+            float w = A.w[j];
             // every synapse has a trace record: load it with the flag rather than after it
             SynTrace tr = A.stdp ? A.trace[j] : SynTrace{};
+            const bool plastic = A.stdp && A.plastic[j];
+            if (MERGED && plastic && (A.fmask[syn_post(en)] & 1ull)) {
+                // STDP.cpp:37-59 for step t - 1 first: the target fired then
+                stdp_decay(A, tr, t - 1);
+                tr.apost += 1.0f;
+                w = stdp_clip(w + A.max_ltp * tr.apre, tr.wmax);  // stored below with the pre-side result
+            }
             int32_t ch, q;
             syn_effect(A, en, w, ch, q);
             atomicAdd(&racc[(int64_t)ch * A.n + syn_post(en)], q);
-            if (A.stdp && A.plastic[j]) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
+            if (plastic) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
                 stdp_decay(A, tr, t);
                 tr.apre += 1.0f;
                 A.trace[j] = tr;
@@ -319,18 +321,43 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const in
             }
         }
     }
+}
+
+__device__ inline void pull_prefix(const StepArgs& A, int32_t t, int32_t* s_pref) {
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        s_pref[0] = 0;
+        for (int d = 1; d < A.D; ++d) {
+            s_pref[d] = tot;
+            if (t - d >= 0) tot += A.hist_cnt[(t - d) % A.D];
+        }
+        s_pref[A.D] = tot;
+    }
+    __syncthreads();
+}
+
+__device__ inline void pull_reset(const StepArgs& A, int32_t t) {
     // the trigger list of step t + 1 (same parity as t - 1, consumed by then) starts empty
     if (A.nrules && blockIdx.x == 0 && threadIdx.x == 0) A.tcount[(t + 1) & 1] = 0;
     // hist_cnt[t % D] belongs to step t-D, no longer read; the update of step t reuses it
     if (blockIdx.x == 0 && threadIdx.x == 0) A.hist_cnt[t % A.D] = 0;
 }
 
-__global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int32_t* tbase, int32_t k) {
+__global__ __launch_bounds__(kDeliverThreads) void nbx_pull(StepArgs A, const int32_t* tbase, int32_t k) {
     const int32_t t = step_of(tbase, k);
+    __shared__ int32_t s_pref[kMaxDelay + 2];
+    pull_prefix(A, t, s_pref);
+    pull_items<false>(A, t, s_pref, (int32_t)blockIdx.x, (int32_t)gridDim.x);
+    pull_reset(A, t);
+}
+
+// Post-side updates of the neurons that fired at step t.
+template <bool MERGED>
+__device__ inline void ltp_items(const StepArgs& A, int32_t t, int32_t bi, int32_t nb) {
     const int32_t slot = t % A.D;
     const int32_t cnt = A.hist_cnt[slot];
     const int32_t* ids = A.hist + (int64_t)slot * A.n;
-    for (int32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+    for (int32_t s = bi; s < cnt; s += nb) {
         const int32_t p = ids[s];
         if (A.stdp) {  // STDP.cpp:37-59: decay, Apost += 1, w += A+ * Apre, clip
             // two synapses per thread, every load issued before the dependent arithmetic (the
@@ -338,17 +365,24 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
             const int64_t b = A.in_ptr[p], e = A.in_ptr[p + 1];
             for (int64_t j0 = b + threadIdx.x; j0 < e; j0 += 2 * kDeliverThreads) {
                 const int64_t j1 = j0 + kDeliverThreads;
-                const bool two = j1 < e;
+                bool two = j1 < e, one = true;
                 const int64_t s0 = A.in_syn[j0];
                 const int64_t s1 = two ? A.in_syn[j1] : s0;
+                if (MERGED) {  // skip a synapse whose presynaptic spike arrives at t + 1: pull_items has it
+                    const int32_t pd0 = A.in_pd[j0], pd1 = two ? A.in_pd[j1] : pd0;
+                    one = !((A.fmask[pd0 & 0xFFFFFF] >> ((pd0 >> 24) - 1)) & 1ull);
+                    two = two && !((A.fmask[pd1 & 0xFFFFFF] >> ((pd1 >> 24) - 1)) & 1ull);
+                }
                 SynTrace r0 = A.trace[s0], r1 = A.trace[s1];
                 const float w0 = A.w[s0], w1 = A.w[s1];
                 stdp_decay(A, r0, t);
                 stdp_decay(A, r1, t);
                 r0.apost += 1.0f;
                 r1.apost += 1.0f;
-                A.trace[s0] = r0;
-                A.w[s0] = stdp_clip(w0 + A.max_ltp * r0.apre, r0.wmax);
+                if (one) {
+                    A.trace[s0] = r0;
+                    A.w[s0] = stdp_clip(w0 + A.max_ltp * r0.apre, r0.wmax);
+                }
                 if (two) {
                     A.trace[s1] = r1;
                     A.w[s1] = stdp_clip(w1 + A.max_ltp * r1.apre, r1.wmax);
@@ -371,6 +405,27 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int
             }
         }
     }
+}
+
+__global__ __launch_bounds__(kDeliverThreads) void nbx_ltp(StepArgs A, const int32_t* tbase, int32_t k) {
+    ltp_items<false>(A, step_of(tbase, k), (int32_t)blockIdx.x, (int32_t)gridDim.x);
+}
+
+// Plain STDP: the pull of step t and the post-side updates of step t - 1 in one launch, on
+// disjoint workgroups (the first kLtpBlocks take the fired neurons of t - 1) so that the two
+// chains of dependent loads overlap instead of adding up.
+constexpr int32_t kLtpBlocks = 256;
+
+__global__ __launch_bounds__(kDeliverThreads) void nbx_pull_ltp(StepArgs A, const int32_t* tbase, int32_t k) {
+    const int32_t t = step_of(tbase, k);
+    __shared__ int32_t s_pref[kMaxDelay + 2];
+    if ((int32_t)blockIdx.x < kLtpBlocks) {
+        ltp_items<true>(A, t - 1, (int32_t)blockIdx.x, kLtpBlocks);
+    } else {
+        pull_prefix(A, t, s_pref);
+        pull_items<true>(A, t, s_pref, (int32_t)blockIdx.x - kLtpBlocks, (int32_t)gridDim.x - kLtpBlocks);
+    }
+    pull_reset(A, t);
 }
 
 // Trigger arrivals of step t: one workgroup per target that had any; its trigger synapses are
@@ -443,7 +498,7 @@ __global__ void nbx_inject(StepArgs A, const int32_t* ids, int32_t count, int32_
     for (int32_t j = threadIdx.x; j < count; j += blockDim.x) {
         const int32_t id = ids[j];
         A.hist[(int64_t)slot * A.n + s_base + j] = id;
-        if (A.did_fire) A.did_fire[id] = 1;
+        if (A.fmask) atomicOr(&A.fmask[id], 1ull);  // after this step's update: bit 0 is step t
         if (A.raster && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n)
             atomicOr(&A.raster[(int64_t)(t - A.rec_t0) * A.words + (id >> 5)], 1u << (id & 31));
     }
@@ -506,6 +561,11 @@ hipError_t launch_pull(const StepArgs& A, const int32_t* tbase, int32_t k, hipSt
 
 hipError_t launch_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
     nbx_ltp<<<deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_pull_ltp(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
+    nbx_pull_ltp<<<kLtpBlocks + deliver_grid(), kDeliverThreads, 0, s>>>(A, tbase, k);
     return hipGetLastError();
 }
 
