@@ -133,16 +133,39 @@ def test_dense_izhikevich_current_config(gpu):
     compare(net, run, wl, counts_o, raster_o)
 
 
+def coincident_updates(wl, raster):
+    """Plastic synapses whose target fired at t - 1 while a presynaptic spike arrives at t: the
+    pairs for which one launch applies the post-side update of t - 1 and the pre-side update of t."""
+    fired = np.unpackbits(raster.view(np.uint8), axis=1, bitorder="little")[:, :wl.spec.n].astype(bool)
+    syn = wl.syn
+    pre = np.repeat(np.arange(wl.spec.n), np.diff(syn.row_ptr))
+    plastic = syn.plastic == 1
+    total = 0
+    for d in np.unique(syn.delay[plastic]):
+        m = plastic & (syn.delay == d)
+        a, b = pre[m], syn.post[m]
+        # target fired at t - 1 (rows 0 .. T-3) and the source at t - d (rows 1 - d .. T-1-d)
+        for t in range(max(1, int(d)), fired.shape[0]):
+            total += int(np.count_nonzero(fired[t - 1, b] & fired[t - int(d), a]))
+    return total
+
+
+@pytest.mark.parametrize("split", [False, True], ids=["one-run", "three-runs"])
 @pytest.mark.parametrize("engine", ["kernels", "graph"])
-def test_stdp_weights_bitexact(gpu, engine):
+def test_stdp_weights_bitexact(gpu, engine, split):
     wl = W.stdp_izh(n=1500, k_in=60, steps=1200, w_bg=2.0)
     T = 1200
     net = wl.make_net()
     net.record(0, T, counts=True, raster=True)
-    net.run(T, ENGINES[engine])
+    if split:  # each run opens with a plain pull and closes with its own post-side launch
+        for k in (333, 64, T - 397):
+            net.run(k, ENGINES[engine])
+    else:
+        net.run(T, ENGINES[engine])
     run = OracleRun(wl, fixed_shift=net.fixed_shift)
     counts_o, raster_o = run.run(T)
     assert counts_o.sum() > 200, "too few spikes for plasticity"
+    assert coincident_updates(wl, raster_o) > 50, "no synapse takes both updates in one step"
     compare(net, run, wl, counts_o, raster_o, ring=False)
     w = net.weights()
     assert_bits_equal(w, run.w, "STDP weights")
