@@ -157,9 +157,6 @@ int32_t finalize_rules(nbx_net* net) {
     NBX_PASS(net->upload(&net->d_rtab, tabs.data(), tabs.size()));
     NBX_PASS(net->upload(&net->d_syn_rule, net->h_rule.data(), S));
     NBX_PASS(net->upload(&net->d_syn_trig, net->h_trig.data(), S));
-    NBX_PASS(net->upload(&net->d_maxw, net->h_maxw.data(), S));
-    std::vector<float> zf(S, 0.0f);
-    NBX_PASS(net->upload(&net->d_elig, zf.data(), S));
     std::vector<int32_t> stamp(std::max(S, (size_t)n), -1);
     NBX_PASS(net->upload(&net->d_tstamp, stamp.data(), S));
     NBX_PASS(net->upload(&net->d_tpost_stamp, stamp.data(), (size_t)n));
@@ -730,7 +727,8 @@ int32_t nbx_net_finalize(nbx_net* net) {
         const size_t S = (size_t)net->S;
         std::vector<SynTrace> tr(S);
         for (size_t j = 0; j < S; ++j)
-            tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j]};
+            tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j], 0.0f,
+                             net->h_maxw.empty() ? 0.0f : net->h_maxw[j], 0, 0};
         NBX_RC(net->upload(&net->d_trace, tr.data(), S));
     }
     if (!net->rules.empty()) NBX_RC(finalize_rules(net));
@@ -1137,16 +1135,11 @@ int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* e
     const size_t S = (size_t)count;
     std::vector<SynTrace> tr(S);
     NBX_HIP_TRY(hipMemcpy(tr.data(), net->d_trace, sizeof(SynTrace) * S, hipMemcpyDeviceToHost));
-    std::vector<float> el;
-    if (eligibility) {
-        el.resize(S);
-        NBX_HIP_TRY(hipMemcpy(el.data(), net->d_elig, sizeof(float) * S, hipMemcpyDeviceToHost));
-    }
     for (size_t j = 0; j < S; ++j) {
         const bool on = net->h_rule[j] >= 0;
         if (apre) apre[j] = on ? tr[j].apre : 0.0f;
         if (apost) apost[j] = on ? tr[j].apost : 0.0f;
-        if (eligibility) eligibility[j] = on ? el[j] : 0.0f;
+        if (eligibility) eligibility[j] = on ? tr[j].elig : 0.0f;
         if (last_update) last_update[j] = on ? tr[j].slast : 0;
     }
     return NBX_OK;

@@ -109,13 +109,27 @@ constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4;
 constexpr int32_t kMaxRules = 64;
 
 // Device pointers and scalars the step kernels need; passed by value.
-// Pair-trace state of one plastic synapse (STDP_State: Apre, Apost, time of the last update) and
-// its STDP weight bound, in one 16-byte record: an update reads and writes one line.
-struct SynTrace {
+// Per-synapse learning state in one 32-byte record, so an update touches this line and the
+// weight's: the pair traces (STDP_State: Apre, Apost, time of the last update), the STDP weight
+// bound, the R-STDP eligibility and the learning rules' weight bound (the connection's maximum).
+struct alignas(32) SynTrace {
+    float apre, apost;
+    int32_t slast;
+    float wmax;  // STDP bound
+    float elig;  // R-STDP eligibility
+    float rmax;  // learning-rule bound
+    int32_t pad0, pad1;
+};
+static_assert(sizeof(SynTrace) == 32, "one 32-byte record per synapse");
+
+// The first 16 bytes of a SynTrace: all that pair STDP reads and writes.
+struct alignas(16) TraceHead {
     float apre, apost;
     int32_t slast;
     float wmax;
 };
+__device__ inline TraceHead load_head(const SynTrace* r) { return *reinterpret_cast<const TraceHead*>(r); }
+__device__ inline void store_head(SynTrace* r, const TraceHead& h) { *reinterpret_cast<TraceHead*>(r) = h; }
 
 struct StepArgs {
     ModelParams P;
@@ -166,8 +180,6 @@ struct StepArgs {
     const RuleDev* rules;
     const int8_t* syn_rule;       // [S] rule of a rule-plastic synapse, else -1
     const int8_t* syn_trig;       // [S] rule of a trigger synapse, else -1
-    const float* maxw;            // [S]
-    float* elig;                  // [S]
     const float* rtab;
     const int64_t* rin_ptr;       // [n * nrules + 1] rule-plastic inputs of (post, rule)
     const int64_t* rin_syn;

@@ -117,7 +117,7 @@ struct nbx_net {
     int32_t* d_in_pd = nullptr;
     RuleDev* d_rules = nullptr;
     int8_t *d_syn_rule = nullptr, *d_syn_trig = nullptr;
-    float *d_maxw = nullptr, *d_elig = nullptr, *d_rtab = nullptr;
+    float* d_rtab = nullptr;
     int64_t *d_rin_ptr = nullptr, *d_rin_syn = nullptr, *d_tin_ptr = nullptr, *d_tin_syn = nullptr;
     int32_t *d_tstamp = nullptr, *d_tpost_stamp = nullptr, *d_tposts = nullptr, *d_tcount = nullptr;
     int32_t *d_conn_index = nullptr, *d_cidx = nullptr, *d_chist = nullptr, *d_chead = nullptr;
@@ -197,8 +197,6 @@ struct nbx_net {
         A.rules = d_rules;
         A.syn_rule = d_syn_rule;
         A.syn_trig = d_syn_trig;
-        A.maxw = d_maxw;
-        A.elig = d_elig;
         A.rtab = d_rtab;
         A.rin_ptr = d_rin_ptr;
         A.rin_syn = d_rin_syn;

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_dense(StepArgs A,
 
 // STDP_State::SetUpdate with the exponentials looked up by the integer step delta; the record's
 // last-update step becomes t.
-__device__ inline void stdp_decay(const StepArgs& A, SynTrace& r, int32_t t) {
+__device__ inline void stdp_decay(const StepArgs& A, TraceHead& r, int32_t t) {
     int32_t dt = t - r.slast;
     if (dt > A.tab_len - 1) dt = A.tab_len - 1;
     r.apre = r.apre * A.ltp_tab[dt];
@@ -239,7 +239,7 @@ __device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t
     const RuleDev R = A.rules[r];
     if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:89-103
         const float u = stable_unit_random(R.seed, r, A.conn_index[j], t);
-        A.w[j] = bounded_increment(A.w[j], R.a1pre * u, A.maxw[j], R.min_weight);
+        A.w[j] = bounded_increment(A.w[j], R.a1pre * u, A.trace[j].rmax, R.min_weight);
         // keep the arrival step; dropping one still inside the kernel window is an error
         const int64_t c = A.cidx[j];
         const int32_t h = A.chead[c];
@@ -254,9 +254,9 @@ __device__ inline void rule_pre(const StepArgs& A, int64_t j, int32_t t, int32_t
     if (R.kind == NBX_RULE_RSTDP) {  // R_STDP.cpp:40-48
         rule_decay(A, R, tr, t);
         tr.apre += 1.0f;
-        A.elig[j] += -R.max_ltd * tr.apost;
+        tr.elig += -R.max_ltd * tr.apost;
     } else {  // AdditiveKernalChange.cpp:65-70: weight first, then the trace
-        A.w[j] = update_weight(A.w[j], R.a1pre, A.maxw[j]);
+        A.w[j] = update_weight(A.w[j], R.a1pre, tr.rmax);
         rule_decay(A, R, tr, t);
         tr.apre += 1.0f;
     }
@@ -297,7 +297,7 @@ __device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s
             const uint32_t en = A.syn[j];
             float w = A.w[j];
             // every synapse has a trace record: load it with the flag rather than after it
-            SynTrace tr = A.stdp ? A.trace[j] : SynTrace{};
+            TraceHead tr = A.stdp ? load_head(A.trace + j) : TraceHead{};
             const bool plastic = A.stdp && A.plastic[j];
             if (MERGED && plastic && (A.fmask[syn_post(en)] & 1ull)) {
                 // STDP.cpp:37-59 for step t - 1 first: the target fired then
@@ -311,7 +311,7 @@ __device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s
             if (plastic) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
                 stdp_decay(A, tr, t);
                 tr.apre += 1.0f;
-                A.trace[j] = tr;
+                store_head(A.trace + j, tr);
                 A.w[j] = stdp_clip(w + -A.max_ltd * tr.apost, tr.wmax);
             }
             if (A.nrules) {
@@ -373,18 +373,18 @@ __device__ inline void ltp_items(const StepArgs& A, int32_t t, int32_t bi, int32
                     one = !((A.fmask[pd0 & 0xFFFFFF] >> ((pd0 >> 24) - 1)) & 1ull);
                     two = two && !((A.fmask[pd1 & 0xFFFFFF] >> ((pd1 >> 24) - 1)) & 1ull);
                 }
-                SynTrace r0 = A.trace[s0], r1 = A.trace[s1];
+                TraceHead r0 = load_head(A.trace + s0), r1 = load_head(A.trace + s1);
                 const float w0 = A.w[s0], w1 = A.w[s1];
                 stdp_decay(A, r0, t);
                 stdp_decay(A, r1, t);
                 r0.apost += 1.0f;
                 r1.apost += 1.0f;
                 if (one) {
-                    A.trace[s0] = r0;
+                    store_head(A.trace + s0, r0);
                     A.w[s0] = stdp_clip(w0 + A.max_ltp * r0.apre, r0.wmax);
                 }
                 if (two) {
-                    A.trace[s1] = r1;
+                    store_head(A.trace + s1, r1);
                     A.w[s1] = stdp_clip(w1 + A.max_ltp * r1.apre, r1.wmax);
                 }
             }
@@ -399,8 +399,8 @@ __device__ inline void ltp_items(const StepArgs& A, int32_t t, int32_t bi, int32
                     SynTrace tr = A.trace[sid];
                     rule_decay(A, R, tr, t);
                     tr.apost += 1.0f;
+                    tr.elig += tr.apre * R.max_ltp;
                     A.trace[sid] = tr;
-                    A.elig[sid] += tr.apre * R.max_ltp;
                 }
             }
         }
@@ -428,58 +428,69 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_pull_ltp(StepArgs A, cons
     pull_reset(A, t);
 }
 
-// Trigger arrivals of step t: one workgroup per target that had any; its trigger synapses are
+// Trigger arrivals of step t: one wave per target that had any; its trigger synapses are
 // applied one after another in ascending synapse order, each to all plastic inputs of its rule
 // (R_STDP.cpp:51-69, AdditiveKernalChange.cpp:71-82).
 __global__ __launch_bounds__(kDeliverThreads) void nbx_trigger(StepArgs A, const int32_t* tbase, int32_t k) {
     const int32_t t = step_of(tbase, k);
     const int32_t cnt = A.tcount[t & 1];
     const int32_t* posts = A.tposts + (int64_t)(t & 1) * A.n;
-    for (int32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    constexpr int32_t kWaves = kDeliverThreads / 64;
+    const int32_t lane = threadIdx.x & 63;
+    // one wave per target. Its trigger inputs are tested 64 at a time (stamp loads in parallel)
+    // and the arrived ones applied in ascending synapse order (ballot bits, lowest first). A
+    // synapse sits at a fixed position of its rule's input list and is always handled by the
+    // same lane, so each lane's program order is the order between the target's triggers.
+    for (int32_t i = blockIdx.x * kWaves + (int32_t)(threadIdx.x >> 6); i < cnt; i += gridDim.x * kWaves) {
         const int32_t p = posts[i];
-        for (int64_t q = A.tin_ptr[p]; q < A.tin_ptr[p + 1]; ++q) {
-            const int64_t tj = A.tin_syn[q];
-            if (A.tstamp[tj] != t) continue;
-            const int32_t r = A.syn_trig[tj];
-            const RuleDev R = A.rules[r];
-            const float factor = syn_type(A.syn[tj]) == 1 ? R.punishment : R.reward;
-            const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
-            if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:105-129
-                const float realtime = (float)t * A.P.dt;
-                const float thr = realtime - R.maxtime;
-                for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
-                    const int64_t sid = A.rin_syn[j];
-                    const int64_t c = A.cidx[sid];
-                    const int32_t h = A.chead[c];
-                    const int32_t* ring = A.chist + c * A.ccap;
-                    float w = A.w[sid];
-                    const float wmax = A.maxw[sid];
-                    for (int32_t q2 = h - 1; q2 >= 0 && q2 >= h - A.ccap; --q2) {  // newest first
-                        const float et = (float)ring[q2 % A.ccap] * A.P.dt;
-                        if (!(et > thr)) break;
-                        int32_t ti = (int32_t)((realtime - et) * R.inv_step);
-                        if (ti < 0) continue;
-                        if (ti >= R.kern_len) ti = R.kern_len - 1;
-                        w = bounded_increment(w, R.a2prepre * A.rtab[R.kern_off + ti], wmax, R.min_weight);
+        const int64_t qb = A.tin_ptr[p], qe = A.tin_ptr[p + 1];
+        for (int64_t c0 = qb; c0 < qe; c0 += 64) {
+            const int64_t q = c0 + lane;
+            const int64_t tq = q < qe ? A.tin_syn[q] : -1;
+            uint64_t m = __ballot(tq >= 0 && A.tstamp[tq] == t);
+            while (m) {
+                const int32_t src = __builtin_ctzll(m);
+                m &= m - 1ull;
+                const int64_t tj = __shfl(tq, src, 64);
+                const int32_t r = A.syn_trig[tj];
+                const RuleDev R = A.rules[r];
+                const float factor = syn_type(A.syn[tj]) == 1 ? R.punishment : R.reward;
+                const int64_t b = A.rin_ptr[(int64_t)p * A.nrules + r], e = A.rin_ptr[(int64_t)p * A.nrules + r + 1];
+                if (R.kind == NBX_RULE_CEREBELLAR) {  // CerebullarLearningRule.cpp:105-129
+                    const float realtime = (float)t * A.P.dt;
+                    const float thr = realtime - R.maxtime;
+                    for (int64_t j = b + lane; j < e; j += 64) {
+                        const int64_t sid = A.rin_syn[j];
+                        const int64_t c = A.cidx[sid];
+                        const int32_t h = A.chead[c];
+                        const int32_t* ring = A.chist + c * A.ccap;
+                        float w = A.w[sid];
+                        const float wmax = A.trace[sid].rmax;
+                        for (int32_t q2 = h - 1; q2 >= 0 && q2 >= h - A.ccap; --q2) {  // newest first
+                            const float et = (float)ring[q2 % A.ccap] * A.P.dt;
+                            if (!(et > thr)) break;
+                            int32_t ti = (int32_t)((realtime - et) * R.inv_step);
+                            if (ti < 0) continue;
+                            if (ti >= R.kern_len) ti = R.kern_len - 1;
+                            w = bounded_increment(w, R.a2prepre * A.rtab[R.kern_off + ti], wmax, R.min_weight);
+                        }
+                        A.w[sid] = w;
                     }
-                    A.w[sid] = w;
-                }
-                __syncthreads();
-                continue;
-            }
-            for (int64_t j = b + threadIdx.x; j < e; j += kDeliverThreads) {
-                const int64_t sid = A.rin_syn[j];
-                SynTrace tr = A.trace[sid];
-                rule_decay(A, R, tr, t);
-                A.trace[sid] = tr;
-                if (R.kind == NBX_RULE_RSTDP) {
-                    A.w[sid] = update_weight(A.w[sid], factor * A.elig[sid], A.maxw[sid]);
-                    if (R.clear) A.elig[sid] = 0.0f;
                 } else {
-                    A.w[sid] = update_weight(A.w[sid], R.a2prepre * tr.apre, A.maxw[sid]);
+                    for (int64_t j = b + lane; j < e; j += 64) {
+                        const int64_t sid = A.rin_syn[j];
+                        SynTrace tr = A.trace[sid];
+                        rule_decay(A, R, tr, t);
+                        if (R.kind == NBX_RULE_RSTDP) {
+                            A.w[sid] = update_weight(A.w[sid], factor * tr.elig, tr.rmax);
+                            if (R.clear) tr.elig = 0.0f;
+                        } else {
+                            A.w[sid] = update_weight(A.w[sid], R.a2prepre * tr.apre, tr.rmax);
+                        }
+                        A.trace[sid] = tr;
+                    }
                 }
             }
-            __syncthreads();  // the next trigger of this target sees these weights
         }
     }
 }
