@@ -156,3 +156,52 @@ def loss_and_grads(W_in, W_rec, W_out, x, target, p: LifParams, checkpoint: int 
     loss = ((y - target) ** 2).mean()
     loss.backward()
     return loss.detach(), [w.grad for w in ws], S
+
+
+class _Spike(torch.autograd.Function):
+    """s = Vm > V_th (or the forced spikes), with the fast-sigmoid surrogate derivative."""
+
+    @staticmethod
+    def forward(ctx, vm, v_th: float, slope: float, forced):
+        ctx.save_for_backward(vm)
+        ctx.v_th, ctx.slope = v_th, slope
+        return forced if forced is not None else (vm > v_th).to(vm.dtype)
+
+    @staticmethod
+    def backward(ctx, gs):
+        (vm,) = ctx.saved_tensors
+        den = 1.0 + ctx.slope * (vm - ctx.v_th).abs()
+        return gs / (den * den), None, None, None
+
+
+def autograd_loss_and_grads(W_in, W_rec, W_out, x, target, p: LifParams, spikes=None):
+    """The plain path the adjoint replaces: the same network and loss unrolled step by step with
+    torch operations (FP32 on the GPU, matmul through the library BLAS) and differentiated by
+    torch autograd, holding every step's tensors. Surrogate mode only (p.mode == 1). `spikes`
+    ([T, N, B]) forces the spike train, so gradients can be compared on the same trajectory.
+    Returns (loss, [dW_in, dW_rec, dW_out], spikes)."""
+    assert p.mode == 1, "the plain unroll implements the surrogate-gradient mode"
+    ws = [w.detach().clone().requires_grad_(True) for w in (W_in, W_rec, W_out)]
+    Wi, Wr, Wo = ws
+    T, cin, B = x.shape
+    N = Wr.shape[0]
+    g = p.g
+    IN = torch.matmul(Wi, x)  # [T, N, B]
+    V = torch.full((N, B), p.v_rest, device=x.device, dtype=torch.float32)
+    s = torch.zeros((N, B), device=x.device, dtype=torch.float32)
+    y = None
+    beta = p.beta
+    loss = 0.0
+    out = []
+    for t in range(T):
+        I = IN[t] + torch.matmul(Wr, s)
+        vm = g * V + (1.0 - g) * (p.v_rest + p.r * I)
+        s = _Spike.apply(vm, p.v_th, p.slope, spikes[t] if spikes is not None else None)
+        V = vm - (p.v_th - p.v_reset) * s
+        z = torch.matmul(Wo, s)
+        y = (1.0 - beta) * z if y is None else beta * y + (1.0 - beta) * z
+        loss = loss + ((y - target[t]) ** 2).sum()
+        out.append(s.detach())
+    loss = loss / target.numel()
+    loss.backward()
+    return loss.detach(), [w.grad for w in ws], torch.stack(out)

@@ -72,3 +72,26 @@ def test_surrogate_off_is_exact_gradient():
     fd, stable = R.finite_difference(W_in, W_rec, W_out, x, p, tgt, 2, idx, eps=1e-7)
     assert stable
     assert torch.allclose(fd, grads[2].reshape(-1)[idx], rtol=1e-6, atol=1e-9)
+
+
+def test_plain_autograd_unroll_matches_fp64_oracle():
+    # the plain FP32 unroll timed beside the adjoint is the same network: on the oracle's spike
+    # train its gradients agree with the FP64 autograd gradients to FP32 accuracy
+    from neuronbridge_amd.adjoint import LifParams, autograd_loss_and_grads
+    rng = np.random.default_rng(4)
+    N, cin, M, B, T = 48, 12, 3, 5, 80
+    Wi = rng.normal(0, 1 / np.sqrt(cin), (N, cin)).astype(np.float32)
+    Wr = rng.normal(0, 1 / np.sqrt(N), (N, N)).astype(np.float32)
+    Wo = rng.normal(0, 1 / np.sqrt(N), (M, N)).astype(np.float32)
+    x = (rng.random((T, cin, B)) < 0.3).astype(np.float32)
+    tg = rng.normal(0, 1, (T, M, B)).astype(np.float32)
+    p = LifParams()
+    args = [torch.tensor(a) for a in (Wi, Wr, Wo, x, tg)]
+    g64, loss64, raster = R.gradients(args[0], args[1], args[2], args[3], p, args[4])
+    assert 0.02 < float(raster.mean()) < 0.5
+    loss32, g32, s32 = autograd_loss_and_grads(*args, p, spikes=raster.to(torch.float32))
+    assert torch.equal(s32, raster.to(torch.float32))
+    assert abs(float(loss32) - loss64) <= 1e-5 * abs(loss64)
+    for a, b in zip(g32, g64):
+        err = float((a.double() - b).norm() / b.norm())
+        assert err < 1e-4, err
