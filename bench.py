@@ -12,7 +12,9 @@ HBM before the timed region. Prints one JSON line.
   roofline   the dominant kernel (the persistent step kernel) timed with HIP events
              on its own stream; algorithmic bytes = 24 B per neuron-step + 8 B per
              synaptic event. A second entry covers replay delivery (the recorded spikes of
-             --replay-steps steps injected in one launch; 8 B per event + 4 B per output arrival).
+             --replay-steps steps injected in one launch; 8 B per event + 4 B per output arrival), and
+             a third replay STDP (the raster of --stdp-steps steps of the STDP network in one pass over
+             its synapses; 36 B per plastic synapse).
   cpu_baseline  the CPU oracle (reference-faithful per-event float delivery), one core,
              on the first --cpu-steps steps of the same network
 """
@@ -209,6 +211,77 @@ def stdp_bench(steps: int) -> dict:
             "plastic_update_gbs": round(updates * 12 / secs / 1e9, 1),
             "spike_rate_hz": float(fired.sum()) / n / (steps * 0.5e-3),
             "weights_in_range": bool(np.all(w_after[plastic] >= 0.0) and np.all(w_after[plastic] <= 1.0))}
+
+
+REPLAY_STDP_BYTES = 36  # per plastic synapse and launch: index+delay, weight, Apre/Apost/last in; weight, traces out
+
+
+def replay_stdp_bench(steps: int, reps: int) -> dict:
+    """Replay STDP on the STDP configuration's network: the raster of `steps` closed-loop steps (from
+    step 0) replayed in one launch sequence (spike lists per neuron, then one pass over the synapses).
+    The first replay must leave the closed loop's weights and trace records bit for bit; the timed
+    launches replay the same raster at later steps (same work). Algorithmic bytes: 36 B per plastic
+    synapse per launch (each plastic synapse's state read and written once, whatever the step count)."""
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.hiptime import DeviceBuffer, Event
+    wl = W.stdp_izh(steps=steps)
+    n = wl.spec.n
+    ref = wl.make_net()
+    engine = ref.pick_engine()
+    ref.record(0, steps, counts=True, raster=True)
+    e0, e1 = Event(), Event()
+    ref.sync()
+    e0.record(ref.stream)
+    ref.run_async(steps, engine)
+    e1.record(ref.stream)
+    e1.synchronize()
+    ref.sync()
+    loop_ms = e0.elapsed_ms(e1)
+    bits = np.unpackbits(ref.raster().view(np.uint8), axis=1, bitorder="little")[:, :n]
+    w_ref = ref.weights()
+    tr_ref = ref.stdp_state()
+    ref.close()
+    spikes = [np.nonzero(b)[0].astype(np.int32) for b in bits]
+    tp = np.zeros(steps + 1, np.int32)
+    tp[1:] = np.cumsum([x.size for x in spikes])
+    ids = np.concatenate(spikes).astype(np.int32) if tp[-1] else np.zeros(1, np.int32)
+    d_ids, d_tp, d_upd = DeviceBuffer(ids.nbytes), DeviceBuffer(tp.nbytes), DeviceBuffer(8)
+    d_ids.upload(ids)
+    d_tp.upload(tp)
+    d_upd.upload(np.zeros(1, np.uint64))
+    net = wl.make_net()
+    L.call("nbx_replay_stdp_device", net.h, d_ids.ptr, d_tp.ptr, 0, 0, steps, int(tp[-1]), d_upd.ptr)
+    net.sync()
+    upd = np.zeros(1, np.uint64)
+    d_upd.download(upd)
+    updates = int(upd[0])
+    same = np.array_equal(net.weights().view(np.uint32), w_ref.view(np.uint32))
+    for a, b in zip(net.stdp_state(), tr_ref):
+        same = same and np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                       b.view(np.uint32) if b.dtype == np.float32 else b)
+    e0.record(net.stream)
+    for r in range(reps):
+        L.call("nbx_replay_stdp_device", net.h, d_ids.ptr, d_tp.ptr, (r + 1) * steps, 0, steps, int(tp[-1]), None)
+    e1.record(net.stream)
+    e1.synchronize()
+    net.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    plastic = int(wl.syn.plastic.sum())
+    net.close()
+    alg = plastic * REPLAY_STDP_BYTES
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"workload": f"replay STDP: raster of {steps} closed-loop steps of the STDP network ({n} Izhikevich "
+                        f"neurons, in-degree 500, {plastic} plastic synapses, delays U{{1..10}}, dt 0.5 ms) "
+                        f"in one launch sequence",
+            "ms_per_launch": round(ms, 4), "plastic_updates_per_launch": updates,
+            "plastic_updates_per_s": updates / (ms * 1e-3),
+            "closed_loop_ms_same_steps": round(loop_ms, 3), "speedup_vs_closed_loop": round(loop_ms / ms, 1),
+            "identical_to_closed_loop": bool(same),
+            "roofline": {"kernel": "nbx_replay_stdp", "bound": "hbm", "achieved": round(ach, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic("nbx_replay_stdp"), "algorithmic_bytes": alg,
+                         "timed": "whole launch sequence (spike lists + synapse pass)"}}
 
 
 def dense_bench(steps: int, noise_scale: float = 1.0) -> dict:
@@ -474,6 +547,8 @@ def main() -> int:
     if not args.no_stdp:
         out["stdp"] = stdp_bench(args.stdp_steps)
         log(f"stdp: {out['stdp']}")
+        out["replay_stdp"] = replay_stdp_bench(args.stdp_steps, args.replay_reps)
+        log(f"replay stdp: {out['replay_stdp']}")
     if not args.no_dense:
         out["dense"] = dense_bench(args.dense_steps)
         log(f"dense: {out['dense']}")

@@ -256,6 +256,29 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
  * ring, given only these spikes. Host arrays. */
 int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t nsteps,
                            int32_t* out);
+
+/* ---------------- replay STDP (multi-step, recorded raster) ----------------
+ * Replaces the per-step STDP::ApplyPreSynaticSpike / ApplyPostSynaticSpike calls
+ * (LearningRule/src/Rule/STDP.cpp:19-61) of a replayed spike schedule
+ * (tests/model_codegen/pair_stdp_gpu_trajectory.cpp:52-66: per tick the presynaptic side,
+ * then the postsynaptic side) for every plastic synapse at once.
+ * ids[tick_ptr[k] .. tick_ptr[k+1]) fired at step t0 - lead + k, k < lead + nsteps (a neuron
+ * at most once per step). Applies, in each synapse's time order, the pre-side update of every
+ * arrival (spike step + delay) at a step in [t0, t0 + nsteps) and the post-side update of every
+ * spike of a step in [t0, t0 + nsteps): what nsteps closed-loop steps whose neurons fire
+ * exactly these spikes leave in the weights and trace records, bit for bit. The lead steps
+ * (lead <= t0; D - 1 of them continue an earlier call exactly) only supply arrivals. Neuron
+ * state, delay ring, firing history and the step counter are not touched. *updates (may be
+ * NULL) receives the number of plastic updates applied. Networks with trigger rules are
+ * rejected. Host arrays. */
+int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,
+                        int32_t nsteps, int64_t* updates);
+/* Device-resident variant for timing: device pointers, enqueued on the network's stream;
+ * nspikes = tick_ptr[lead + nsteps] sizes the workspace. *d_updates (device, may be NULL) is
+ * incremented. An invalid raster changes no synapse and is reported (NBX_ERR_INVALID) by the
+ * next nbx_net_sync (or nbx_net_run). */
+int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
+                               int32_t nsteps, int64_t nspikes, uint64_t* d_updates);
 /* ---------------- .nbnet connection tables ----------------
  * The reference's binary network description (python/src/neuronbridge/nbnet.py:34-38,
  * read by src/native/shared/src/streaming_build/NbnetReader.cpp): header, metadata JSON,

@@ -237,6 +237,12 @@ int32_t check_device_error(nbx_net* net) {
                   "overwritten (raise nbx_rule_desc.history)");
         return NBX_ERR_STATE;
     }
+    if (e & kErrRaster) {  // an argument error of one replay call: reported once
+        NBX_PASS(clear_raster_error(net));
+        set_error("replay STDP: invalid raster (a spike id out of range, a neuron listed twice in one step, "
+                  "tick_ptr not ascending or more spikes than nspikes); no synapse was changed");
+        return NBX_ERR_INVALID;
+    }
     if (e != 0) {
         set_error("device-side error flag " + std::to_string(e) + " (grid barrier time-out)");
         return NBX_ERR_STATE;
@@ -1176,6 +1182,66 @@ int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick
     if (rc) return rc;
     NBX_HIP_TRY(e);
     return NBX_OK;
+}
+
+int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,
+                        int32_t nsteps, int64_t* updates) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->stdp && net->has_sparse, "replay STDP needs sparse synapses with STDP (nbx_net_set_stdp)");
+    NBX_CHECK(net->rules.empty(), "replay STDP does not run trigger-driven learning rules");
+    NBX_CHECK(tick_ptr && nsteps > 0 && lead >= 0 && lead <= t0, "bad replay arguments");
+    const int32_t K = lead + nsteps;
+    NBX_CHECK(tick_ptr[0] == 0, "tick_ptr must start at 0");
+    for (int32_t k = 0; k < K; ++k) NBX_CHECK(tick_ptr[k] <= tick_ptr[k + 1], "tick_ptr must be non-decreasing");
+    const int32_t total = tick_ptr[K];
+    NBX_CHECK(total == 0 || ids, "null spike ids");
+    {  // ids in range and distinct within a step (a neuron fires at most once per step)
+        std::vector<int32_t> seen((size_t)net->n, -1);
+        for (int32_t k = 0; k < K; ++k)
+            for (int32_t f = tick_ptr[k]; f < tick_ptr[k + 1]; ++f) {
+                NBX_CHECK(ids[f] >= 0 && ids[f] < net->n, "spike id out of range");
+                NBX_CHECK(seen[(size_t)ids[f]] != k, "a neuron is listed twice in one step");
+                seen[(size_t)ids[f]] = k;
+            }
+    }
+    NBX_RC(device_scope(net));
+    if (!net->rsw.upd) NBX_RC(net->alloc(&net->rsw.upd, 1));
+    int32_t *d_ids = nullptr, *d_tp = nullptr;
+    hipError_t e = hipMalloc(&d_ids, sizeof(int32_t) * std::max(total, 1));
+    if (e == hipSuccess) e = hipMalloc(&d_tp, sizeof(int32_t) * (K + 1));
+    if (e == hipSuccess && total) e = hipMemcpy(d_ids, ids, sizeof(int32_t) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_tp, tick_ptr, sizeof(int32_t) * (K + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(net->rsw.upd, 0, sizeof(unsigned long long), net->stream);
+    if (e == hipSuccess) {
+        rc = launch_replay_stdp(net, d_ids, d_tp, t0, lead, nsteps, total, net->rsw.upd);
+        if (rc == NBX_OK) e = hipStreamSynchronize(net->stream);
+    }
+    unsigned long long upd = 0;
+    if (e == hipSuccess && rc == NBX_OK)
+        e = hipMemcpy(&upd, net->rsw.upd, sizeof(upd), hipMemcpyDeviceToHost);
+    hipFree(d_ids);
+    hipFree(d_tp);
+    if (rc) return rc;
+    NBX_HIP_TRY(e);
+    NBX_RC(check_device_error(net));
+    if (updates) *updates = (int64_t)upd;
+    return refresh_persistent_weights(net);
+}
+
+int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
+                               int32_t nsteps, int64_t nspikes, uint64_t* d_updates) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->stdp && net->has_sparse, "replay STDP needs sparse synapses with STDP (nbx_net_set_stdp)");
+    NBX_CHECK(net->rules.empty(), "replay STDP does not run trigger-driven learning rules");
+    NBX_CHECK(d_tick_ptr && nsteps > 0 && lead >= 0 && lead <= t0 && nspikes >= 0, "bad replay arguments");
+    NBX_CHECK(nspikes == 0 || d_ids, "null spike ids");
+    NBX_RC(device_scope(net));
+    NBX_RC(refresh_persistent_weights(net));
+    static_assert(sizeof(uint64_t) == sizeof(unsigned long long), "64-bit counter");
+    return launch_replay_stdp(net, d_ids, d_tick_ptr, t0, lead, nsteps, nspikes,
+                              reinterpret_cast<unsigned long long*>(d_updates));
 }
 
 int32_t nbx_replay_path(nbx_net* net, int32_t* path) {

@@ -104,8 +104,8 @@ struct RuleDev {
     int32_t kern_off, kern_len;
     uint32_t seed;
 };
-// Device error codes (nbx_net::d_err; the batched engine raises kErrExchange)
-constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4;
+// Device error codes (nbx_net::d_err; the batched engine raises kErrExchange, replay STDP kErrRaster)
+constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4, kErrRaster = 8;
 constexpr int32_t kMaxRules = 64;
 
 // Device pointers and scalars the step kernels need; passed by value.
@@ -130,6 +130,27 @@ struct alignas(16) TraceHead {
 };
 __device__ inline TraceHead load_head(const SynTrace* r) { return *reinterpret_cast<const TraceHead*>(r); }
 __device__ inline void store_head(SynTrace* r, const TraceHead& h) { *reinterpret_cast<TraceHead*>(r) = h; }
+
+// STDP_State::SetUpdate (STDP_State.cpp:17-26) with the exponentials looked up by the integer
+// step delta (tab[k] = expf(-k dt / tau), k < len; the last entry covers every longer delta);
+// the record's last-update step becomes t. A negative delta (a record already updated past t)
+// decays nothing.
+__device__ inline void stdp_decay_tab(const float* __restrict__ ltp_tab, const float* __restrict__ ltd_tab,
+                                      int32_t len, TraceHead& r, int32_t t) {
+    int32_t dt = t - r.slast;
+    if (dt > len - 1) dt = len - 1;
+    if (dt < 0) dt = 0;
+    r.apre = r.apre * ltp_tab[dt];
+    r.apost = r.apost * ltd_tab[dt];
+    r.slast = t;
+}
+
+// STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum
+__device__ inline float stdp_clip(float w, float wm) {
+    if (w < 0.0f) w = 0.0f;
+    else if (w > wm) w = wm;
+    return w;
+}
 
 struct StepArgs {
     ModelParams P;

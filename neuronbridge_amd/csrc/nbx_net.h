@@ -48,6 +48,17 @@ struct ReplaySlots {
     int64_t ovf_count = 0;
 };
 
+// Replay-STDP workspace: per-neuron spike lists of the replayed raster (grown on demand).
+struct ReplayStdpWork {
+    int32_t* cnt = nullptr;  // [n + 1] spike counts, then fill positions
+    int32_t* ptr = nullptr;  // [n + 1] list starts
+    int32_t* k = nullptr;    // [cap] local steps
+    int64_t cap = 0;
+    uint8_t* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    unsigned long long* upd = nullptr;  // update counter of the host entry point
+};
+
 struct nbx_net {
     int32_t device = 0;
     hipStream_t stream = nullptr;
@@ -140,6 +151,8 @@ struct nbx_net {
     PersistentLayout pl;
     // replay delivery
     ReplaySlots rs;
+    // replay STDP
+    ReplayStdpWork rsw;
 
     StepArgs args() const {
         StepArgs A{};
@@ -255,5 +268,8 @@ int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
 int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLOTS
+int32_t clear_raster_error(nbx_net* net);
+int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
+                           int32_t nsteps, int64_t nspikes, unsigned long long* d_updates);
 const char* rule_desc_error(const nbx_rule_desc& q, bool allow_stdp);  // null when valid
 }  // namespace nbx

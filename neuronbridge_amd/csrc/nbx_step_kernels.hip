@@ -172,21 +172,9 @@ __global__ __launch_bounds__(kDeliverThreads) void nbx_deliver_dense(StepArgs A,
     if (A.nch > 1 && neg) atomicAdd(&r[A.n + i], neg);
 }
 
-// STDP_State::SetUpdate with the exponentials looked up by the integer step delta; the record's
-// last-update step becomes t.
+// STDP_State::SetUpdate on the network's trace tables.
 __device__ inline void stdp_decay(const StepArgs& A, TraceHead& r, int32_t t) {
-    int32_t dt = t - r.slast;
-    if (dt > A.tab_len - 1) dt = A.tab_len - 1;
-    r.apre = r.apre * A.ltp_tab[dt];
-    r.apost = r.apost * A.ltd_tab[dt];
-    r.slast = t;
-}
-
-// STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum
-__device__ inline float stdp_clip(float w, float wm) {
-    if (w < 0.0f) w = 0.0f;
-    else if (w > wm) w = wm;
-    return w;
+    stdp_decay_tab(A.ltp_tab, A.ltd_tab, A.tab_len, r, t);
 }
 
 // ---- trigger-driven rules

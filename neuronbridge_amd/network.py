@@ -330,6 +330,23 @@ class Net:
         L.call("nbx_replay_deliver", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), K, _p(out, C.c_int32))
         return out
 
+    def replay_stdp(self, spikes_per_step: list, t0: int = 0, lead: int = 0) -> int:
+        """Advance the plastic synapses through the steps t0 .. t0+K-1-lead of a recorded raster.
+
+        spikes_per_step[k] lists the neurons that fired at step t0 - lead + k; the first `lead`
+        lists only supply arrivals (their post-side updates were applied before). Weights and trace
+        records change as closed-loop steps with exactly these spikes would change them. Returns
+        the number of plastic updates applied."""
+        K = len(spikes_per_step)
+        tp = np.zeros(K + 1, dtype=np.int32)
+        tp[1:] = np.cumsum([len(s) for s in spikes_per_step])
+        ids = np.concatenate([np.asarray(s, dtype=np.int32) for s in spikes_per_step]) if tp[-1] else np.zeros(1, np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        upd = C.c_int64()
+        L.call("nbx_replay_stdp", self.h, _p(ids, C.c_int32), _p(tp, C.c_int32), int(t0), int(lead), K - int(lead),
+               C.byref(upd))
+        return upd.value
+
     def export_monitor(self, output_dir: str, *, sample_interval: int = 1, flush_interval: int = 100,
                        spikes: bool = True, state: bool = True, weights: bool = False, pending: bool = True,
                        neuron_ids=None, name: str = "main") -> None:

@@ -19,6 +19,9 @@
  *                          (returns -2 on mismatch); delay KAT of tests/native/dense_cuda_smoke.cpp:109-118
  *   STDP                   closed-form pair updates; replay scenario of
  *                          tests/model_codegen/pair_stdp_gpu_trajectory.cpp:52-66
+ *   orc_replay_stdp        orc_run's own final weights and traces when fed orc_run's raster
+ *                          (single and chained calls); per synapse, orc_stdp_replay on that
+ *                          synapse's arrival / spike schedule
  */
 #include "nbx_oracle.h"
 
@@ -741,6 +744,49 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     free(c.isyn);
     free(spikes);
     return status;
+}
+
+/* Replay STDP over a recorded raster: the STDP part of orc_run (STDP.cpp:19-61 at the delivery
+ * of each arrival, then at each spike of the step) with the neuron update replaced by the raster.
+ * ids[tick_ptr[k] .. tick_ptr[k+1]) fired at step t0 - lead + k; updates of steps [t0, t0+nsteps). */
+int64_t orc_replay_stdp(orc_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,
+                        int32_t nsteps) {
+    const int32_t D = net->d_slots, base = t0 - lead, K = lead + nsteps;
+    const float inv_ltp = (float)(1.0 / (double)net->ltp_tau);
+    const float inv_ltd = (float)(1.0 / (double)net->ltd_tau);
+    int64_t updates = 0;
+    for (int32_t t = t0; t < t0 + nsteps; ++t) {
+        /* pre-side: spikes of t - d through their synapses of delay d */
+        for (int d = 1; d < D; ++d) {
+            const int32_t k = t - d - base;
+            if (k < 0 || k >= K) continue;
+            for (int32_t f = tick_ptr[k]; f < tick_ptr[k + 1]; ++f) {
+                const int32_t pre = ids[f];
+                for (int64_t s = net->row_ptr[pre]; s < net->row_ptr[pre + 1]; ++s) {
+                    if (net->delay[s] != d || !net->plastic[s]) continue;
+                    stdp_set_update(net, s, t, inv_ltp, inv_ltd);
+                    net->apre[s] += 1.0f;
+                    net->w[s] += -net->max_ltd * net->apost[s];
+                    stdp_clip(net, s);
+                    ++updates;
+                }
+            }
+        }
+        /* post-side: spikes of t through the plastic inputs of each target */
+        const int32_t k = t - base;
+        for (int32_t f = tick_ptr[k]; f < tick_ptr[k + 1]; ++f) {
+            const int32_t p = ids[f];
+            for (int64_t j = net->in_ptr[p]; j < net->in_ptr[p + 1]; ++j) {
+                const int64_t s = net->in_syn[j];
+                stdp_set_update(net, s, t, inv_ltp, inv_ltd);
+                net->apost[s] += 1.0f;
+                net->w[s] += net->max_ltp * net->apre[s];
+                stdp_clip(net, s);
+                ++updates;
+            }
+        }
+    }
+    return updates;
 }
 
 void orc_stdp_replay(float max_ltp, float ltp_tau, float max_ltd, float ltd_tau, float dt, float w0, float w_max,

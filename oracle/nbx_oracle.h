@@ -164,6 +164,12 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
                 int64_t* events);
 
+/* Replay STDP over a recorded raster: ids[tick_ptr[k] .. tick_ptr[k+1]) fired at step
+ * t0 - lead + k (k < lead + nsteps); applies the pair-STDP updates of steps [t0, t0 + nsteps)
+ * in orc_run's order (arrivals of a step, then its spikes). Returns the number of updates. */
+int64_t orc_replay_stdp(orc_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,
+                        int32_t nsteps);
+
 /* STDP replay on one synapse for a given event schedule (pair_stdp_gpu_trajectory.cpp:52-66). */
 void orc_stdp_replay(float max_ltp, float ltp_tau, float max_ltd, float ltd_tau, float dt, float w0, float w_max,
                      const uint8_t* pre, const uint8_t* post, int32_t nticks, float* w_out);

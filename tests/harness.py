@@ -58,6 +58,21 @@ def initial_state(wl):
     return x, last
 
 
+def raster_lists(raster: np.ndarray, n: int) -> list:
+    """Spike lists per step from a bit raster [steps][ceil(n/32)] (bit i of a word: neuron 32w + i)."""
+    bits = np.unpackbits(np.ascontiguousarray(raster).view(np.uint8), axis=1, bitorder="little")[:, :n]
+    return [np.nonzero(row)[0].astype(np.int32) for row in bits]
+
+
+def raster_csr(spikes_per_step: list):
+    """(ids, tick_ptr) of spike lists, the layout of the replay entry points."""
+    tp = np.zeros(len(spikes_per_step) + 1, dtype=np.int32)
+    tp[1:] = np.cumsum([len(s) for s in spikes_per_step])
+    ids = (np.concatenate([np.asarray(s, dtype=np.int32) for s in spikes_per_step]) if tp[-1]
+           else np.zeros(1, np.int32))
+    return np.ascontiguousarray(ids, dtype=np.int32), tp
+
+
 class OracleRun:
     """Holds an OrcNet built from a workload plus the oracle-side mutable arrays."""
 
@@ -192,6 +207,12 @@ class OracleRun:
         assert st == 0, f"oracle self-check failed (status {st}): push-ring image != pull delivery"
         self.t += nsteps
         return counts, ras
+
+    def replay_stdp(self, spikes_per_step: list, t0: int = 0, lead: int = 0) -> int:
+        """orc_replay_stdp on this run's weights and traces; returns the number of updates."""
+        ids, tp = raster_csr(spikes_per_step)
+        return int(O.lib().orc_replay_stdp(C.byref(self.net), O.ptr(ids, C.c_int32), O.ptr(tp, C.c_int32), int(t0),
+                                           int(lead), len(spikes_per_step) - int(lead)))
 
     def ring_as_device(self, nch: int) -> np.ndarray:
         """Oracle ring image in the device layout [slot][channel][n]."""

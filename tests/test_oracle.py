@@ -359,3 +359,77 @@ def test_stdp_replay_scenario_and_clipping():
     kw3 = dict(kw, w0=0.1)
     lo = _replay([5, 6, 7], [0, 1, 2, 3, 4], 8, **kw3)
     assert lo.min() == 0.0
+
+
+# ---- replay STDP (orc_replay_stdp): pinned to the oracle's own closed loop and to the
+# single-synapse trajectory restatement above
+
+def _stdp_run(T):
+    from neuronbridge_amd import workloads as W
+    wl = W.stdp_izh(n=800, k_in=50, steps=T, w_bg=2.0)
+    run = OracleRun(wl, fixed_shift=20)
+    counts, raster = run.run(T)
+    return wl, run, counts, raster
+
+
+def test_replay_stdp_equals_closed_loop_on_its_raster():
+    from harness import raster_lists
+    T = 600
+    wl, run, counts, raster = _stdp_run(T)
+    assert counts.sum() > 200
+    lists = raster_lists(raster, wl.spec.n)
+    fresh = OracleRun(wl, fixed_shift=20)
+    upd = fresh.replay_stdp(lists)
+    assert upd > 10000
+    for name in ("w", "apre", "apost", "slast"):
+        a, b = getattr(fresh, name), getattr(run, name)
+        assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                              b.view(np.uint32) if b.dtype == np.float32 else b), name
+    # chained: the second call re-reads the last D - 1 steps of the first for their arrivals only
+    D = wl.d_slots
+    chained = OracleRun(wl, fixed_shift=20)
+    t1 = 257
+    u1 = chained.replay_stdp(lists[:t1], t0=0)
+    u2 = chained.replay_stdp(lists[t1 - (D - 1):], t0=t1, lead=D - 1)
+    assert u1 + u2 == upd
+    assert np.array_equal(chained.w.view(np.uint32), run.w.view(np.uint32))
+    assert np.array_equal(chained.slast, run.slast)
+    # a call that leaves out the lead steps misses their arrivals: the pin is sensitive to it
+    short = OracleRun(wl, fixed_shift=20)
+    short.replay_stdp(lists[:t1], t0=0)
+    short.replay_stdp(lists[t1:], t0=t1)
+    assert not np.array_equal(short.w.view(np.uint32), run.w.view(np.uint32))
+
+
+def test_replay_stdp_per_synapse_trajectories():
+    """Each synapse's final weight equals the single-synapse schedule replay (orc_stdp_replay,
+    pair_stdp_gpu_trajectory.cpp:52-66) on its arrival steps (pre spike + delay) and its
+    target's spike steps."""
+    from harness import raster_lists
+    T = 400
+    wl, run, counts, raster = _stdp_run(T)
+    lists = raster_lists(raster, wl.spec.n)
+    fresh = OracleRun(wl, fixed_shift=20)
+    fresh.replay_stdp(lists)
+    fired = np.zeros((T, wl.spec.n), np.uint8)
+    for t, ids in enumerate(lists):
+        fired[t, ids] = 1
+    syn = wl.syn
+    pre_of = np.repeat(np.arange(wl.spec.n), np.diff(syn.row_ptr))
+    plastic = np.nonzero(syn.plastic)[0]
+    rng = np.random.default_rng(5)
+    sd = wl.stdp
+    checked = moved = 0
+    for s in rng.choice(plastic, 300, replace=False):
+        d = int(syn.delay[s])
+        pre_sched = np.zeros(T, np.uint8)
+        pre_sched[d:] = fired[:T - d, pre_of[s]]
+        post_sched = np.ascontiguousarray(fired[:, syn.post[s]])
+        out = np.zeros(T + 1, np.float32)
+        L_.orc_stdp_replay(sd["max_ltp"], sd["ltp_tau"], sd["max_ltd"], sd["ltd_tau"], wl.spec.dt,
+                           float(syn.weight[s]), sd["w_max"], O.ptr(pre_sched, C.c_uint8),
+                           O.ptr(post_sched, C.c_uint8), T, O.ptr(out, C.c_float))
+        assert out[-1].view(np.uint32) == fresh.w[s].view(np.uint32), s
+        checked += 1
+        moved += out[-1] != syn.weight[s]
+    assert checked == 300 and moved > 100
