@@ -1191,6 +1191,7 @@ int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_pt
     NBX_CHECK(net->stdp && net->has_sparse, "replay STDP needs sparse synapses with STDP (nbx_net_set_stdp)");
     NBX_CHECK(net->rules.empty(), "replay STDP does not run trigger-driven learning rules");
     NBX_CHECK(tick_ptr && nsteps > 0 && lead >= 0 && lead <= t0, "bad replay arguments");
+    NBX_CHECK((int64_t)lead + nsteps <= 65535, "replay STDP covers at most 65535 steps per call (chain calls)");
     const int32_t K = lead + nsteps;
     NBX_CHECK(tick_ptr[0] == 0, "tick_ptr must start at 0");
     for (int32_t k = 0; k < K; ++k) NBX_CHECK(tick_ptr[k] <= tick_ptr[k + 1], "tick_ptr must be non-decreasing");
@@ -1236,6 +1237,7 @@ int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t
     NBX_CHECK(net->stdp && net->has_sparse, "replay STDP needs sparse synapses with STDP (nbx_net_set_stdp)");
     NBX_CHECK(net->rules.empty(), "replay STDP does not run trigger-driven learning rules");
     NBX_CHECK(d_tick_ptr && nsteps > 0 && lead >= 0 && lead <= t0 && nspikes >= 0, "bad replay arguments");
+    NBX_CHECK((int64_t)lead + nsteps <= 65535, "replay STDP covers at most 65535 steps per call (chain calls)");
     NBX_CHECK(nspikes == 0 || d_ids, "null spike ids");
     NBX_RC(device_scope(net));
     NBX_RC(refresh_persistent_weights(net));

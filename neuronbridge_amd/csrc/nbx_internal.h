@@ -145,12 +145,9 @@ __device__ inline void stdp_decay_tab(const float* __restrict__ ltp_tab, const f
     r.slast = t;
 }
 
-// STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum
-__device__ inline float stdp_clip(float w, float wm) {
-    if (w < 0.0f) w = 0.0f;
-    else if (w > wm) w = wm;
-    return w;
-}
+// STDP.cpp:29-34: below 0 -> 0, above the synapse's maximum weight -> that maximum (selects,
+// no branches)
+__device__ inline float stdp_clip(float w, float wm) { return w < 0.0f ? 0.0f : (w > wm ? wm : w); }
 
 struct StepArgs {
     ModelParams P;

@@ -50,10 +50,14 @@ struct ReplaySlots {
 
 // Replay-STDP workspace: per-neuron spike lists of the replayed raster (grown on demand).
 struct ReplayStdpWork {
-    int32_t* cnt = nullptr;  // [n + 1] spike counts, then fill positions
+    int32_t* cnt = nullptr;  // [n + 1] spike counts
     int32_t* ptr = nullptr;  // [n + 1] list starts
-    int32_t* k = nullptr;    // [cap] local steps
+    int32_t* k = nullptr;    // [cap] local steps, ascending per neuron
     int64_t cap = 0;
+    unsigned long long* mask = nullptr;  // [n][blocks] spike bits per 64-step block
+    int32_t* blk = nullptr;              // [n][blocks] list position of each block's first spike
+    int64_t mcap = 0;                    // n * blocks allocated
+    float2* tab2 = nullptr;              // [tab_len] {ltp_tab, ltd_tab} interleaved
     uint8_t* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     unsigned long long* upd = nullptr;  // update counter of the host entry point

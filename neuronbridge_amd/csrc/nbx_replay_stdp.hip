@@ -12,11 +12,13 @@
 //
 // Per launch the kernel reads and writes each plastic synapse's weight and trace record once,
 // whatever K is, instead of once per update: the HBM stream is the synapse table itself. The
-// spike-time lists (a few bytes per neuron) stay in L2.
+// spike-step lists (a few bytes per neuron) stay in L2.
 //
-// Steps: per-neuron spike counts (atomics), their exclusive scan, a scatter of the local step
-// of each spike into its neuron's list, a per-neuron sort (which also finds a neuron listed
-// twice in one step), then the synapse pass: one wave per presynaptic row, a lane per synapse.
+// Spike lists per neuron, sorted, without a sort: each spike sets its bit in a 64-bit word per
+// (neuron, 64-step block) (a bit already set is a neuron listed twice in one step); a pass per
+// neuron turns the words' popcounts into the list position of each block's first spike and the
+// neuron's count; a scan gives the list starts; each spike then lands at start + block position +
+// popcount of the lower bits of its word.
 #include <hipcub/hipcub.hpp>
 
 #include "nbx_internal.h"
@@ -26,7 +28,6 @@ namespace nbx {
 
 namespace {
 
-constexpr int kRsThreads = 256;
 constexpr int32_t kNever = 0x7FFFFFFF;
 
 struct ReplayStdpArgs {
@@ -35,8 +36,7 @@ struct ReplayStdpArgs {
     const uint8_t* plastic;
     float* w;
     SynTrace* trace;
-    const float* ltp_tab;
-    const float* ltd_tab;
+    const float2* tab;  // {exp(-k dt / tau+), exp(-k dt / tau-)}, k < tab_len
     int32_t tab_len;
     float max_ltp, max_ltd;
     const int32_t* st_ptr;  // [n + 1] spike lists per neuron
@@ -50,116 +50,235 @@ struct ReplayStdpArgs {
 // Clears error bits (the raster error of an earlier call; other bits stay).
 __global__ void rs_clear_err(int32_t* err, int32_t bits) { atomicAnd(err, ~bits); }
 
-__global__ void rs_zero(int32_t* p, int32_t count) {
-    for (int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); j < count; j += (int32_t)(gridDim.x * blockDim.x))
-        p[j] = 0;
+__global__ void rs_zero64(unsigned long long* p, int64_t count) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = 0ull;
 }
 
-// Spikes per neuron. The raster's length is checked against the workspace here; an id out of
-// range raises kErrRaster and is not counted.
-__global__ void rs_count(const int32_t* __restrict__ ids, const int32_t* __restrict__ tick_ptr, int32_t K, int64_t cap,
-                         int32_t n, int32_t* cnt, int32_t* err) {
+__global__ void rs_interleave(const float* __restrict__ a, const float* __restrict__ b, int32_t len, float2* out) {
+    for (int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < len; k += (int32_t)(gridDim.x * blockDim.x))
+        out[k] = make_float2(a[k], b[k]);
+}
+
+// Workgroup k sets the bits of the spikes of local step k. Entries beyond tick_ptr[K] or beyond
+// the workspace, ids out of range and bits already set (a neuron twice in one step, or step lists
+// that overlap) raise kErrRaster.
+__global__ void rs_mark(const int32_t* __restrict__ ids, const int32_t* __restrict__ tick_ptr, int32_t K, int64_t cap,
+                        int32_t n, int32_t blocks, unsigned long long* mask, int32_t* err) {
     const int32_t total = tick_ptr[K];
     if (tick_ptr[0] != 0 || total < 0 || total > cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrRaster);
         return;
     }
-    for (int32_t f = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); f < total; f += (int32_t)(gridDim.x * blockDim.x)) {
-        const int32_t id = ids[f];
-        if (id < 0 || id >= n) {
-            atomicOr(err, kErrRaster);
+    for (int32_t k = (int32_t)blockIdx.x; k < K; k += (int32_t)gridDim.x) {
+        const int32_t b = tick_ptr[k], e = tick_ptr[k + 1];
+        if (b < 0 || b > e || e > total) {  // step lists must tile [0, total): at most total bits are set
+            if (threadIdx.x == 0) atomicOr(err, kErrRaster);
             continue;
         }
-        atomicAdd(&cnt[id], 1);
+        const unsigned long long bit = 1ull << (k & 63);
+        for (int32_t f = b + (int32_t)threadIdx.x; f < e; f += (int32_t)blockDim.x) {
+            const int32_t id = ids[f];
+            if (id < 0 || id >= n) {
+                atomicOr(err, kErrRaster);
+                continue;
+            }
+            if (atomicOr(&mask[(int64_t)id * blocks + (k >> 6)], bit) & bit) atomicOr(err, kErrRaster);
+        }
     }
 }
 
-// Workgroup k places the spikes of local step k into their neurons' lists (any order within a
-// list; rs_sort orders them).
-__global__ void rs_scatter(const int32_t* __restrict__ ids, const int32_t* __restrict__ tick_ptr, int32_t K, int32_t n,
-                           const int32_t* __restrict__ st_ptr, int32_t* fill, int32_t* st_k, int32_t* err) {
+// Per neuron: list position of each block's first spike, and the neuron's spike count.
+__global__ void rs_blocks(const unsigned long long* __restrict__ mask, int32_t n, int32_t blocks, int32_t* blk,
+                          int32_t* cnt) {
+    for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x)) {
+        int32_t run = 0;
+        for (int32_t q = 0; q < blocks; ++q) {
+            blk[(int64_t)i * blocks + q] = run;
+            run += __popcll(mask[(int64_t)i * blocks + q]);
+        }
+        cnt[i] = run;
+    }
+}
+
+// Workgroup k places the spikes of local step k at their sorted list positions (the step lists
+// were checked by rs_mark).
+__global__ void rs_place(const int32_t* __restrict__ ids, const int32_t* __restrict__ tick_ptr, int32_t K, int32_t n,
+                         int32_t blocks, const unsigned long long* __restrict__ mask, const int32_t* __restrict__ blk,
+                         const int32_t* __restrict__ st_ptr, int32_t* st_k, const int32_t* err) {
     if (*err & kErrRaster) return;
     for (int32_t k = (int32_t)blockIdx.x; k < K; k += (int32_t)gridDim.x) {
         const int32_t b = tick_ptr[k], e = tick_ptr[k + 1];
+        const unsigned long long below = (1ull << (k & 63)) - 1ull;
         for (int32_t f = b + (int32_t)threadIdx.x; f < e; f += (int32_t)blockDim.x) {
             const int32_t id = ids[f];
             if (id < 0 || id >= n) continue;
-            const int32_t pos = st_ptr[id] + atomicAdd(&fill[id], 1);
-            if (pos < st_ptr[id + 1]) st_k[pos] = k;
-            else atomicOr(err, kErrRaster);  // step lists that overlap (tick_ptr not ascending)
+            const int64_t q = (int64_t)id * blocks + (k >> 6);
+            st_k[st_ptr[id] + blk[q] + __popcll(mask[q] & below)] = k;
         }
     }
 }
 
-// Insertion sort of each neuron's list (a few entries); equal steps mean the neuron was listed
-// twice in one step.
-__global__ void rs_sort(const int32_t* __restrict__ st_ptr, int32_t* st_k, int32_t n, int32_t* err) {
-    for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x)) {
-        const int32_t b = st_ptr[i], e = st_ptr[i + 1];
-        bool dup = false;
-        for (int32_t j = b + 1; j < e; ++j) {
-            const int32_t v = st_k[j];
-            int32_t m = j - 1;
-            while (m >= b && st_k[m] > v) {
-                st_k[m + 1] = st_k[m];
-                --m;
-            }
-            st_k[m + 1] = v;
-            dup = dup || (m >= b && st_k[m] == v);
-        }
-        if (dup) atomicOr(err, kErrRaster);
-    }
+// Orders one wave's LDS stores before its later LDS loads (other lanes' data): wavefront-scope
+// release / acquire around a wave barrier.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Arrival step of list entry i (a synapse of delay d), or kNever past the window.
-__device__ inline int32_t arrival_at(const ReplayStdpArgs& X, int32_t i, int32_t e, int32_t d) {
-    if (i >= e) return kNever;
-    const int32_t a = X.st_k[i] + d;
-    return a < X.kend ? a : kNever;
+// The synapse pass: wave per presynaptic row (grid-stride), lane per synapse. Each plastic synapse
+// merges its arrival list (the presynaptic spikes + delay) with its target's spike list and applies
+// STDP.cpp:19-61 per entry; an arrival goes first when both fall on one step.
+//
+// Everything a merge step reads is in LDS, so the chain of dependent steps waits on no memory and
+// runs without branches: the decay tables (FULL_TAB: all of them; else their first entries, longer
+// step deltas reading the table in memory), the row's presynaptic spike steps once per wave and each
+// lane's target spike steps (as [entry][lane], all of a lane's loads in flight together). Steps are
+// 16-bit local steps (a call covers at most 65535 steps); each list ends with the sentinel 65535,
+// past every step of the window, so a step only compares the two heads with the window end. A wave
+// whose row or one of whose targets has a longer list runs the same merge from memory.
+constexpr int32_t kPreCap = 64;
+constexpr int32_t kPostCap = 20;
+constexpr int32_t kTabPart = 2048;   // entries staged when the whole table does not fit
+constexpr int32_t kTabFull = 4352;   // entries that fit (34 KB)
+constexpr uint16_t kEnd = 0xFFFF;
+constexpr int kRsPassThreads = 1024;
+constexpr int32_t kRsPassWaves = kRsPassThreads / 64;
+
+struct MergeState {
+    float w;
+    TraceHead tr;
+    int32_t since;  // last update, local steps
+};
+
+// One merge step on the state: the entry at local step t, an arrival when pre_side.
+__device__ inline void merge_step(const ReplayStdpArgs& X, MergeState& s, bool pre_side, int32_t t, float2 f) {
+    // STDP_State::SetUpdate on the tables (as stdp_decay_tab)
+    s.tr.apre = s.tr.apre * f.x;
+    s.tr.apost = s.tr.apost * f.y;
+    s.since = t;
+    // STDP.cpp:19-35 (arrival): Apre += 1, w -= A- * Apost; STDP.cpp:37-59 (spike): Apost += 1,
+    // w += A+ * Apre; both clipped to [0, w_max]
+    s.tr.apre = pre_side ? s.tr.apre + 1.0f : s.tr.apre;
+    s.tr.apost = pre_side ? s.tr.apost : s.tr.apost + 1.0f;
+    s.w = stdp_clip(s.w + (pre_side ? -X.max_ltd * s.tr.apost : X.max_ltp * s.tr.apre), s.tr.wmax);
 }
 
-// The synapse pass: wave per presynaptic row (grid-stride), lane per synapse.
-__global__ __launch_bounds__(kRsThreads) void nbx_replay_stdp(ReplayStdpArgs X) {
-    if (*X.err & kErrRaster) return;  // uniform: a bad raster changes no synapse
-    constexpr int32_t kWaves = kRsThreads / 64;
-    const int32_t lane = (int32_t)threadIdx.x & 63;
-    const int32_t nw = (int32_t)gridDim.x * kWaves;
+// Two 1024-thread workgroups per CU (LDS 76 KB each with the whole table): 8 waves per SIMD to
+// overlap the dependent loads that open each group of 64 synapses (index, then the target's list
+// bounds, then its spike steps).
+template <bool FULL_TAB>
+__global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(8))) void nbx_replay_stdp(
+    ReplayStdpArgs X) {
+    __shared__ float2 s_tab[FULL_TAB ? kTabFull : kTabPart];
+    __shared__ uint16_t s_pre[kRsPassWaves][kPreCap];
+    __shared__ uint16_t s_post[kRsPassWaves][kPostCap * 64];
+    if (*X.err & kErrRaster) return;  // the same flag for every thread: a bad raster changes no synapse
+    constexpr int32_t kTab = FULL_TAB ? kTabFull : kTabPart;
+    const int32_t ntab = min(X.tab_len, kTab);
+    for (int32_t k = (int32_t)threadIdx.x; k < ntab; k += kRsPassThreads) s_tab[k] = X.tab[k];
+    __syncthreads();
+    const int32_t lane = (int32_t)threadIdx.x & 63, wave = (int32_t)threadIdx.x >> 6;
+    const int32_t nw = (int32_t)gridDim.x * kRsPassWaves;
+    const int32_t last = X.tab_len - 1, kend = X.kend, lead = X.lead;
+    uint16_t* sp = s_pre[wave];
+    uint16_t* sq = s_post[wave] + lane;  // this lane's list: entry i at sq[64 i]
+    auto table = [&](int32_t dt) -> float2 {
+        if (FULL_TAB) return s_tab[dt];
+        float2 f = s_tab[min(dt, kTabPart - 1)];
+        if (dt >= kTabPart) f = X.tab[dt];
+        return f;
+    };
     unsigned long long upd = 0;
-    for (int32_t pre = (int32_t)blockIdx.x * kWaves + ((int32_t)threadIdx.x >> 6); pre < X.n; pre += nw) {
+    for (int32_t pre = (int32_t)blockIdx.x * kRsPassWaves + wave; pre < X.n; pre += nw) {
         const int64_t rb = X.row_ptr[pre], re = X.row_ptr[pre + 1];
-        const int32_t pb = X.st_ptr[pre], pe = X.st_ptr[pre + 1];
-        for (int64_t j = rb + lane; j < re; j += 64) {
-            if (!X.plastic[j]) continue;
-            const uint32_t en = X.syn[j];
-            const int32_t d = syn_delay(en), post = syn_post(en);
-            int32_t ip = pb;
-            while (ip < pe && X.st_k[ip] + d < X.lead) ++ip;  // arrived before the window: applied earlier
-            int32_t iq = X.st_ptr[post];
-            const int32_t qe = X.st_ptr[post + 1];
-            while (iq < qe && X.st_k[iq] < X.lead) ++iq;  // post-sides of the lead steps: applied earlier
-            int32_t a = arrival_at(X, ip, pe, d);
-            int32_t b = iq < qe ? X.st_k[iq] : kNever;
-            if (a == kNever && b == kNever) continue;
-            float w = X.w[j];
-            TraceHead tr = load_head(X.trace + j);
-            do {
-                ++upd;
-                if (a <= b) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
-                    stdp_decay_tab(X.ltp_tab, X.ltd_tab, X.tab_len, tr, X.base + a);
-                    tr.apre += 1.0f;
-                    w = stdp_clip(w + -X.max_ltd * tr.apost, tr.wmax);
-                    a = arrival_at(X, ++ip, pe, d);
-                } else {  // STDP.cpp:37-59: decay, Apost += 1, w += A+ * Apre, clip
-                    stdp_decay_tab(X.ltp_tab, X.ltd_tab, X.tab_len, tr, X.base + b);
-                    tr.apost += 1.0f;
-                    w = stdp_clip(w + X.max_ltp * tr.apre, tr.wmax);
-                    ++iq;
-                    b = iq < qe ? X.st_k[iq] : kNever;
-                }
-            } while (a != kNever || b != kNever);
-            store_head(X.trace + j, tr);
-            X.w[j] = w;
+        const int32_t pb = X.st_ptr[pre], na = X.st_ptr[pre + 1] - pb;
+        const bool pre_lds = na < kPreCap;  // wave-uniform
+        // every lane of the wave is here (the row loop is wave-uniform)
+        if (pre_lds) {
+            for (int32_t e = lane; e < na; e += 64) sp[e] = (uint16_t)X.st_k[pb + e];
+            if (lane == 0) sp[na] = kEnd;
         }
+        wave_sync();
+        for (int64_t j0 = rb; j0 < re; j0 += 64) {  // wave-uniform chunk loop
+            const int64_t j = j0 + lane;
+            const bool pl = j < re && X.plastic[j];
+            uint32_t en = 0;
+            int32_t qb = 0, nb = 0;
+            float w = 0.0f;
+            TraceHead tr{};
+            if (pl) {
+                en = X.syn[j];
+                w = X.w[j];
+                tr = load_head(X.trace + j);
+                qb = X.st_ptr[syn_post(en)];
+                nb = X.st_ptr[syn_post(en) + 1] - qb;
+            }
+            const int32_t d = syn_delay(en);
+            const bool fast = pre_lds && !__any(pl && nb >= kPostCap);  // wave-uniform
+            if (fast) {
+                // stage the target's spike steps (all loads before the first store) and the sentinel
+                if (pl) {
+                    int32_t qv[kPostCap];
+#pragma unroll
+                    for (int e = 0; e < kPostCap; ++e) qv[e] = e < nb ? X.st_k[qb + e] : (int32_t)kEnd;
+#pragma unroll
+                    for (int e = 0; e < kPostCap; ++e)
+                        if (e <= nb) sq[64 * e] = (uint16_t)qv[e];
+                }
+                if (pl) {
+                    int32_t ia = 0, ib = 0;
+                    if (lead > 0) {
+                        while ((int32_t)sp[ia] + d < lead) ++ia;  // arrived before the window: applied earlier
+                        while ((int32_t)sq[64 * ib] < lead) ++ib;  // post-sides of the lead steps: applied earlier
+                    }
+                    const int32_t ia0 = ia, ib0 = ib;
+                    int32_t a = (int32_t)sp[ia] + d, b = sq[64 * ib];
+                    MergeState s{w, tr, tr.slast - X.base};
+                    while (min(a, b) < kend) {
+                        const bool pre_side = a <= b;
+                        const int32_t t = pre_side ? a : b;
+                        merge_step(X, s, pre_side, t, table(min(max(t - s.since, 0), last)));
+                        ia += pre_side ? 1 : 0;
+                        ib += pre_side ? 0 : 1;
+                        const int32_t an = (int32_t)sp[ia] + d, bn = sq[64 * ib];
+                        a = pre_side ? an : a;
+                        b = pre_side ? b : bn;
+                    }
+                    if (ia != ia0 || ib != ib0) {
+                        upd += (unsigned long long)(ia - ia0 + ib - ib0);
+                        s.tr.slast = s.since + X.base;
+                        store_head(X.trace + j, s.tr);
+                        X.w[j] = s.w;
+                    }
+                }
+            } else if (pl) {  // the same merge with the lists read from memory
+                auto arr = [&](int32_t i) { return i < na ? X.st_k[pb + i] + d : kNever; };
+                auto spk = [&](int32_t i) { return i < nb ? X.st_k[qb + i] : kNever; };
+                int32_t ia = 0, ib = 0;
+                while (arr(ia) < lead) ++ia;
+                while (spk(ib) < lead) ++ib;
+                const int32_t ia0 = ia, ib0 = ib;
+                int32_t a = arr(ia), b = spk(ib);
+                MergeState s{w, tr, tr.slast - X.base};
+                while (min(a, b) < kend) {
+                    const bool pre_side = a <= b;
+                    const int32_t t = pre_side ? a : b;
+                    const int32_t dt = min(max(t - s.since, 0), last);
+                    merge_step(X, s, pre_side, t, X.tab[dt]);
+                    if (pre_side) a = arr(++ia);
+                    else b = spk(++ib);
+                }
+                if (ia != ia0 || ib != ib0) {
+                    upd += (unsigned long long)(ia - ia0 + ib - ib0);
+                    s.tr.slast = s.since + X.base;
+                    store_head(X.trace + j, s.tr);
+                    X.w[j] = s.w;
+                }
+            }
+        }
+        wave_sync();  // the next row's staging follows every read of this one
     }
     if (X.updates) {
         for (int off = 32; off > 0; off >>= 1) upd += __shfl_xor(upd, off);
@@ -178,53 +297,70 @@ int32_t clear_raster_error(nbx_net* net) {
     return NBX_OK;
 }
 
-// Workspace sized for n neurons and cap spikes (grown on demand, kept with the network).
-static int32_t replay_stdp_workspace(nbx_net* net, int64_t cap) {
+template <typename T>
+static int32_t regrow(nbx_net* net, T** p, size_t count) {
+    if (*p) {
+        auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)*p);
+        if (it != net->allocs.end()) net->allocs.erase(it);
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        NBX_HIP_TRY(hipFree(*p));
+        *p = nullptr;
+    }
+    return net->alloc(p, count);
+}
+
+// Workspace for n neurons, cap spikes and `blocks` 64-step blocks (grown on demand, kept with the
+// network): n * blocks * 12 bytes of bit words and block positions, 4 bytes per spike.
+static int32_t replay_stdp_workspace(nbx_net* net, int64_t cap, int32_t blocks) {
     ReplayStdpWork& R = net->rsw;
     const int32_t n = net->n;
     if (!R.ptr) {
         NBX_PASS(net->alloc(&R.cnt, (size_t)n + 1));
         NBX_PASS(net->alloc(&R.ptr, (size_t)n + 1));
+        NBX_HIP_TRY(hipMemsetAsync(R.cnt, 0, sizeof(int32_t) * ((size_t)n + 1), net->stream));  // cnt[n] stays 0
         size_t bytes = 0;
         NBX_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, R.cnt, R.ptr, n + 1, net->stream));
         NBX_PASS(net->alloc(&R.scan_tmp, bytes));
         R.scan_bytes = bytes;
+        NBX_PASS(net->alloc(&R.tab2, (size_t)net->tab_len));
+        rs_interleave<<<std::max(1, std::min((net->tab_len + 255) / 256, 256)), 256, 0, net->stream>>>(
+            net->d_ltp_tab, net->d_ltd_tab, net->tab_len, R.tab2);
+        NBX_HIP_TRY(hipGetLastError());
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     }
     if (cap > R.cap) {
-        if (R.k) {
-            auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)R.k);
-            if (it != net->allocs.end()) net->allocs.erase(it);
-            NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-            NBX_HIP_TRY(hipFree(R.k));
-            R.k = nullptr;
-            R.cap = 0;
-        }
-        NBX_PASS(net->alloc(&R.k, (size_t)cap));
+        NBX_PASS(regrow(net, &R.k, (size_t)cap));
         R.cap = cap;
+    }
+    const int64_t m = (int64_t)n * blocks;
+    if (m > R.mcap) {
+        NBX_PASS(regrow(net, &R.mask, (size_t)m));
+        NBX_PASS(regrow(net, &R.blk, (size_t)m));
+        R.mcap = m;
     }
     return NBX_OK;
 }
 
 int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
                            int32_t nsteps, int64_t nspikes, unsigned long long* d_updates) {
-    NBX_PASS(replay_stdp_workspace(net, std::max<int64_t>(nspikes, 1)));
+    const int32_t n = net->n, K = lead + nsteps, blocks = (K + 63) / 64;
+    NBX_PASS(replay_stdp_workspace(net, std::max<int64_t>(nspikes, 1), blocks));
     ReplayStdpWork& R = net->rsw;
     hipStream_t st = net->stream;
-    const int32_t n = net->n, K = lead + nsteps;
+    const int64_t m = (int64_t)n * blocks;
+    const int32_t kgrid = std::max(1, std::min(K, 4096));
+    const int32_t ngrid = std::max(1, std::min((n + 255) / 256, 2048));
     rs_clear_err<<<1, 1, 0, st>>>(net->d_err, kErrRaster);
     NBX_HIP_TRY(hipGetLastError());
-    rs_zero<<<std::max(1, std::min((n + 256) / 256, 1024)), 256, 0, st>>>(R.cnt, n + 1);
+    rs_zero64<<<(int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, 4096)), 256, 0, st>>>(R.mask, m);
     NBX_HIP_TRY(hipGetLastError());
-    rs_count<<<std::max<int64_t>(1, std::min<int64_t>((nspikes + 255) / 256, 2048)), 256, 0, st>>>(
-        d_ids, d_tick_ptr, K, R.cap, n, R.cnt, net->d_err);
+    rs_mark<<<kgrid, 256, 0, st>>>(d_ids, d_tick_ptr, K, R.cap, n, blocks, R.mask, net->d_err);
+    NBX_HIP_TRY(hipGetLastError());
+    rs_blocks<<<ngrid, 256, 0, st>>>(R.mask, n, blocks, R.blk, R.cnt);
     NBX_HIP_TRY(hipGetLastError());
     size_t bytes = R.scan_bytes;
     NBX_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(R.scan_tmp, bytes, R.cnt, R.ptr, n + 1, st));
-    rs_zero<<<std::max(1, std::min((n + 256) / 256, 1024)), 256, 0, st>>>(R.cnt, n + 1);
-    NBX_HIP_TRY(hipGetLastError());
-    rs_scatter<<<std::max(1, std::min(K, 4096)), 256, 0, st>>>(d_ids, d_tick_ptr, K, n, R.ptr, R.cnt, R.k, net->d_err);
-    NBX_HIP_TRY(hipGetLastError());
-    rs_sort<<<std::max(1, std::min((n + 255) / 256, 2048)), 256, 0, st>>>(R.ptr, R.k, n, net->d_err);
+    rs_place<<<kgrid, 256, 0, st>>>(d_ids, d_tick_ptr, K, n, blocks, R.mask, R.blk, R.ptr, R.k, net->d_err);
     NBX_HIP_TRY(hipGetLastError());
 
     ReplayStdpArgs X{};
@@ -233,8 +369,7 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     X.plastic = net->d_plastic;
     X.w = net->d_w;
     X.trace = net->d_trace;
-    X.ltp_tab = net->d_ltp_tab;
-    X.ltd_tab = net->d_ltd_tab;
+    X.tab = R.tab2;
     X.tab_len = net->tab_len;
     X.max_ltp = net->sd.max_ltp;
     X.max_ltd = net->sd.max_ltd;
@@ -246,8 +381,9 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     X.base = t0 - lead;
     X.updates = d_updates;
     X.err = net->d_err;
-    const int32_t grid = std::max(1, std::min((n + 3) / 4, 8192));
-    nbx_replay_stdp<<<grid, kRsThreads, 0, st>>>(X);
+    const int32_t grid = std::max(1, std::min((n + kRsPassWaves - 1) / kRsPassWaves, 4096));
+    if (net->tab_len <= kTabFull) nbx_replay_stdp<true><<<grid, kRsPassThreads, 0, st>>>(X);
+    else nbx_replay_stdp<false><<<grid, kRsPassThreads, 0, st>>>(X);
     NBX_HIP_TRY(hipGetLastError());
     return NBX_OK;
 }

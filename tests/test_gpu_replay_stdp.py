@@ -118,6 +118,8 @@ def test_replay_stdp_edge_cases(gpu):
         net.replay_stdp([[n]], t0=200)
     with pytest.raises(RuntimeError, match="bad replay arguments"):
         net.replay_stdp([[1], [2]], t0=0, lead=1)  # lead steps before step 0
+    with pytest.raises(RuntimeError, match="65535"):
+        net.replay_stdp([[]] * 65536, t0=300)  # local steps are 16-bit: longer rasters are chained
     assert np.array_equal(net.weights(), w1)  # rejected calls change nothing
     plain = W.sparse_lif(n=500, k_in=20, steps=10).make_net()
     with pytest.raises(RuntimeError, match="STDP"):
