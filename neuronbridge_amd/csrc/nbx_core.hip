@@ -729,13 +729,21 @@ int32_t nbx_net_finalize(nbx_net* net) {
     if (net->has_dense) NBX_RC(net->upload(&net->d_wt, net->h_wt.data(), net->h_wt.size()));
     if (net->pois) NBX_RC(net->upload(&net->d_pois_tab, net->pois_tab.data(), net->pois_tab.size()));
     if (net->cur_n) NBX_RC(net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size()));
-    if (net->stdp || !net->rules.empty()) {  // per-synapse pair traces (STDP_State), shared by both
+    if (!net->rules.empty()) {  // per-synapse learning records (STDP_State traces shared with STDP)
         const size_t S = (size_t)net->S;
         std::vector<SynTrace> tr(S);
         for (size_t j = 0; j < S; ++j)
             tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j], 0.0f,
                              net->h_maxw.empty() ? 0.0f : net->h_maxw[j], 0, 0};
         NBX_RC(net->upload(&net->d_trace, tr.data(), S));
+        net->d_heads = reinterpret_cast<TraceHead*>(net->d_trace);
+        net->head_stride = (int32_t)(sizeof(SynTrace) / sizeof(TraceHead));
+    } else if (net->stdp) {  // pair STDP only: 16-byte records
+        const size_t S = (size_t)net->S;
+        std::vector<TraceHead> hd(S);
+        for (size_t j = 0; j < S; ++j) hd[j] = TraceHead{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j]};
+        NBX_RC(net->upload(&net->d_heads, hd.data(), S));
+        net->head_stride = 1;
     }
     if (!net->rules.empty()) NBX_RC(finalize_rules(net));
     if (net->stdp) {
@@ -1120,12 +1128,13 @@ int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t*
     NBX_CHECK(count == net->S, "buffer size must equal the synapse count");
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-    std::vector<SynTrace> tr((size_t)count);
-    NBX_HIP_TRY(hipMemcpy(tr.data(), net->d_trace, sizeof(SynTrace) * tr.size(), hipMemcpyDeviceToHost));
-    for (size_t j = 0; j < tr.size(); ++j) {
-        if (apre) apre[j] = tr[j].apre;
-        if (apost) apost[j] = tr[j].apost;
-        if (last_update) last_update[j] = tr[j].slast;
+    const size_t st = (size_t)net->head_stride;
+    std::vector<TraceHead> hd((size_t)count * st);
+    NBX_HIP_TRY(hipMemcpy(hd.data(), net->d_heads, sizeof(TraceHead) * hd.size(), hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < (size_t)count; ++j) {
+        if (apre) apre[j] = hd[j * st].apre;
+        if (apost) apost[j] = hd[j * st].apost;
+        if (last_update) last_update[j] = hd[j * st].slast;
     }
     return NBX_OK;
 }

@@ -122,14 +122,16 @@ struct alignas(32) SynTrace {
 };
 static_assert(sizeof(SynTrace) == 32, "one 32-byte record per synapse");
 
-// The first 16 bytes of a SynTrace: all that pair STDP reads and writes.
+// All that pair STDP reads and writes of a synapse: a 16-byte record of its own when the network
+// has no learning rules, else the first 16 bytes of its SynTrace (heads at a stride of 2 records).
 struct alignas(16) TraceHead {
     float apre, apost;
     int32_t slast;
     float wmax;
 };
-__device__ inline TraceHead load_head(const SynTrace* r) { return *reinterpret_cast<const TraceHead*>(r); }
-__device__ inline void store_head(SynTrace* r, const TraceHead& h) { *reinterpret_cast<TraceHead*>(r) = h; }
+static_assert(sizeof(TraceHead) == 16, "one 16-byte head per synapse");
+__device__ inline TraceHead load_head(const TraceHead* h, int32_t stride, int64_t j) { return h[j * stride]; }
+__device__ inline void store_head(TraceHead* h, int32_t stride, int64_t j, const TraceHead& v) { h[j * stride] = v; }
 
 // STDP_State::SetUpdate (STDP_State.cpp:17-26) with the exponentials looked up by the integer
 // step delta (tab[k] = expf(-k dt / tau), k < len; the last entry covers every longer delta);
@@ -186,7 +188,9 @@ struct StepArgs {
     // STDP
     int32_t stdp;
     float max_ltp, max_ltd, w_max;
-    SynTrace* trace;  // [S] pair traces of STDP and rule-plastic synapses
+    SynTrace* trace;  // [S] learning records of networks with rules (pair traces, eligibility, bounds), else null
+    TraceHead* heads; // pair-STDP records: heads[j * head_stride]
+    int32_t head_stride;
     const float* ltp_tab;
     const float* ltd_tab;
     int32_t tab_len;

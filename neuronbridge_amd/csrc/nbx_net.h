@@ -12,6 +12,7 @@ using nbx::ModelParams;
 using nbx::StepArgs;
 using nbx::RuleDev;
 using nbx::SynTrace;
+using nbx::TraceHead;
 
 // Post-blocked layout of the persistent engine and of replay delivery: block b owns
 // neurons [b * P, min((b + 1) * P, n)). Blocks are grouped into regions of rb blocks
@@ -54,10 +55,11 @@ struct ReplayStdpWork {
     int32_t* ptr = nullptr;  // [n + 1] list starts
     int32_t* k = nullptr;    // [cap] local steps, ascending per neuron
     int64_t cap = 0;
-    unsigned long long* mask = nullptr;  // [n][blocks] spike bits per 64-step block
-    int32_t* blk = nullptr;              // [n][blocks] list position of each block's first spike
+    unsigned long long* mask = nullptr;  // [blocks][n] spike bits per 64-step block
+    int32_t* blk = nullptr;              // [blocks][n] list position of each block's first spike
     int64_t mcap = 0;                    // n * blocks allocated
     float2* tab2 = nullptr;              // [tab_len] {ltp_tab, ltd_tab} interleaved
+    uint16_t* fix = nullptr;             // [n][20] first spike steps of each neuron, 0xFFFF after the last
     uint8_t* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     unsigned long long* upd = nullptr;  // update counter of the host entry point
@@ -125,7 +127,9 @@ struct nbx_net {
     float* d_wt = nullptr;
     uint32_t* d_pois_tab = nullptr;
     float* d_cur = nullptr;
-    SynTrace* d_trace = nullptr;
+    SynTrace* d_trace = nullptr;   // networks with learning rules
+    TraceHead* d_heads = nullptr;  // pair-STDP records: d_trace's heads (stride 2) or an array of their own
+    int32_t head_stride = 1;
     float *d_ltp_tab = nullptr, *d_ltd_tab = nullptr;
     int32_t tab_len = 1;
     int64_t *d_in_ptr = nullptr, *d_in_syn = nullptr;
@@ -203,6 +207,8 @@ struct nbx_net {
         A.max_ltd = sd.max_ltd;
         A.w_max = sd.w_max;
         A.trace = d_trace;
+        A.heads = d_heads;
+        A.head_stride = head_stride;
         A.ltp_tab = d_ltp_tab;
         A.ltd_tab = d_ltd_tab;
         A.tab_len = tab_len;

@@ -15,7 +15,7 @@
 // spike-step lists (a few bytes per neuron) stay in L2.
 //
 // Spike lists per neuron, sorted, without a sort: each spike sets its bit in a 64-bit word per
-// (neuron, 64-step block) (a bit already set is a neuron listed twice in one step); a pass per
+// (64-step block, neuron) (a bit already set is a neuron listed twice in one step); a pass per
 // neuron turns the words' popcounts into the list position of each block's first spike and the
 // neuron's count; a scan gives the list starts; each spike then lands at start + block position +
 // popcount of the lower bits of its word.
@@ -35,17 +35,22 @@ struct ReplayStdpArgs {
     const uint32_t* syn;
     const uint8_t* plastic;
     float* w;
-    SynTrace* trace;
+    TraceHead* heads;
+    int32_t head_stride;
     const float2* tab;  // {exp(-k dt / tau+), exp(-k dt / tau-)}, k < tab_len
     int32_t tab_len;
     float max_ltp, max_ltd;
     const int32_t* st_ptr;  // [n + 1] spike lists per neuron
     const int32_t* st_k;    // local steps, ascending within a neuron
+    const uint2* fix;       // [n][kFix / 4]: each neuron's first kFix steps (16-bit), 0xFFFF after the last
     int32_t n, lead, kend;  // local step k is step base + k; post-sides from k >= lead, arrivals in [lead, kend)
     int32_t base;
     unsigned long long* updates;  // plastic updates applied (or null)
     int32_t* err;
 };
+
+constexpr int32_t kFix = 20;  // spike steps per neuron in the fixed-stride copy (40 bytes)
+constexpr uint16_t kEnd = 0xFFFF;
 
 // Clears error bits (the raster error of an earlier call; other bits stay).
 __global__ void rs_clear_err(int32_t* err, int32_t bits) { atomicAnd(err, ~bits); }
@@ -53,6 +58,11 @@ __global__ void rs_clear_err(int32_t* err, int32_t bits) { atomicAnd(err, ~bits)
 __global__ void rs_zero64(unsigned long long* p, int64_t count) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
         p[j] = 0ull;
+}
+
+__global__ void rs_fill16(uint16_t* p, int64_t count) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = kEnd;
 }
 
 __global__ void rs_interleave(const float* __restrict__ a, const float* __restrict__ b, int32_t len, float2* out) {
@@ -83,7 +93,7 @@ __global__ void rs_mark(const int32_t* __restrict__ ids, const int32_t* __restri
                 atomicOr(err, kErrRaster);
                 continue;
             }
-            if (atomicOr(&mask[(int64_t)id * blocks + (k >> 6)], bit) & bit) atomicOr(err, kErrRaster);
+            if (atomicOr(&mask[(int64_t)(k >> 6) * n + id], bit) & bit) atomicOr(err, kErrRaster);
         }
     }
 }
@@ -94,8 +104,8 @@ __global__ void rs_blocks(const unsigned long long* __restrict__ mask, int32_t n
     for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x)) {
         int32_t run = 0;
         for (int32_t q = 0; q < blocks; ++q) {
-            blk[(int64_t)i * blocks + q] = run;
-            run += __popcll(mask[(int64_t)i * blocks + q]);
+            blk[(int64_t)q * n + i] = run;
+            run += __popcll(mask[(int64_t)q * n + i]);
         }
         cnt[i] = run;
     }
@@ -105,7 +115,7 @@ __global__ void rs_blocks(const unsigned long long* __restrict__ mask, int32_t n
 // were checked by rs_mark).
 __global__ void rs_place(const int32_t* __restrict__ ids, const int32_t* __restrict__ tick_ptr, int32_t K, int32_t n,
                          int32_t blocks, const unsigned long long* __restrict__ mask, const int32_t* __restrict__ blk,
-                         const int32_t* __restrict__ st_ptr, int32_t* st_k, const int32_t* err) {
+                         const int32_t* __restrict__ st_ptr, int32_t* st_k, uint16_t* fix, const int32_t* err) {
     if (*err & kErrRaster) return;
     for (int32_t k = (int32_t)blockIdx.x; k < K; k += (int32_t)gridDim.x) {
         const int32_t b = tick_ptr[k], e = tick_ptr[k + 1];
@@ -113,8 +123,10 @@ __global__ void rs_place(const int32_t* __restrict__ ids, const int32_t* __restr
         for (int32_t f = b + (int32_t)threadIdx.x; f < e; f += (int32_t)blockDim.x) {
             const int32_t id = ids[f];
             if (id < 0 || id >= n) continue;
-            const int64_t q = (int64_t)id * blocks + (k >> 6);
-            st_k[st_ptr[id] + blk[q] + __popcll(mask[q] & below)] = k;
+            const int64_t q = (int64_t)(k >> 6) * n + id;
+            const int32_t pos = blk[q] + __popcll(mask[q] & below);
+            st_k[st_ptr[id] + pos] = k;
+            if (pos < kFix) fix[(int64_t)id * kFix + pos] = (uint16_t)k;
         }
     }
 }
@@ -134,15 +146,17 @@ __device__ inline void wave_sync() {
 // Everything a merge step reads is in LDS, so the chain of dependent steps waits on no memory and
 // runs without branches: the decay tables (FULL_TAB: all of them; else their first entries, longer
 // step deltas reading the table in memory), the row's presynaptic spike steps once per wave and each
-// lane's target spike steps (as [entry][lane], all of a lane's loads in flight together). Steps are
-// 16-bit local steps (a call covers at most 65535 steps); each list ends with the sentinel 65535,
-// past every step of the window, so a step only compares the two heads with the window end. A wave
-// whose row or one of whose targets has a longer list runs the same merge from memory.
+// lane's target spike steps (as [entry][lane]). Steps are 16-bit local steps (a call covers at most
+// 65535 steps); each list ends with the sentinel 65535, past every step of the window, so a step
+// only compares the two heads with the window end.
+//
+// The 64 synapses of the next group are loaded while the current group merges (index, plastic
+// flag, weight, trace record), and a target's list comes from its fixed-stride copy (one load level
+// after the index instead of two), so each group waits on one L2 round trip. A wave whose row or one
+// of whose targets has kFix or more spikes runs the same merge from the lists in memory.
 constexpr int32_t kPreCap = 64;
-constexpr int32_t kPostCap = 20;
-constexpr int32_t kTabPart = 2048;   // entries staged when the whole table does not fit
-constexpr int32_t kTabFull = 4352;   // entries that fit (34 KB)
-constexpr uint16_t kEnd = 0xFFFF;
+constexpr int32_t kTabPart = 2048;  // entries staged when the whole table does not fit
+constexpr int32_t kTabFull = 4352;  // entries that fit (34 KB)
 constexpr int kRsPassThreads = 1024;
 constexpr int32_t kRsPassWaves = kRsPassThreads / 64;
 
@@ -165,15 +179,32 @@ __device__ inline void merge_step(const ReplayStdpArgs& X, MergeState& s, bool p
     s.w = stdp_clip(s.w + (pre_side ? -X.max_ltd * s.tr.apost : X.max_ltp * s.tr.apre), s.tr.wmax);
 }
 
-// Two 1024-thread workgroups per CU (LDS 76 KB each with the whole table): 8 waves per SIMD to
-// overlap the dependent loads that open each group of 64 synapses (index, then the target's list
-// bounds, then its spike steps).
+// One synapse group's loads: the first level (index, flag, weight, record) of group j0.
+struct Group {
+    uint32_t en;
+    uint32_t pl;
+    float w;
+    TraceHead tr;
+};
+
+__device__ inline void load_group(const ReplayStdpArgs& X, int64_t j, int64_t re, Group& g) {
+    const int64_t jc = j < re ? j : re - 1;  // in bounds for every lane (re > rb)
+    g.en = X.syn[jc];
+    g.pl = j < re ? (uint32_t)X.plastic[jc] : 0u;
+    g.w = 0.0f;
+    g.tr = TraceHead{};
+    if (g.pl) {  // a non-plastic synapse's weight and record stay in memory
+        g.w = X.w[jc];
+        g.tr = load_head(X.heads, X.head_stride, jc);
+    }
+}
+
 template <bool FULL_TAB>
 __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(8))) void nbx_replay_stdp(
     ReplayStdpArgs X) {
     __shared__ float2 s_tab[FULL_TAB ? kTabFull : kTabPart];
     __shared__ uint16_t s_pre[kRsPassWaves][kPreCap];
-    __shared__ uint16_t s_post[kRsPassWaves][kPostCap * 64];
+    __shared__ uint16_t s_post[kRsPassWaves][kFix * 64];
     if (*X.err & kErrRaster) return;  // the same flag for every thread: a bad raster changes no synapse
     constexpr int32_t kTab = FULL_TAB ? kTabFull : kTabPart;
     const int32_t ntab = min(X.tab_len, kTab);
@@ -193,6 +224,7 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
     unsigned long long upd = 0;
     for (int32_t pre = (int32_t)blockIdx.x * kRsPassWaves + wave; pre < X.n; pre += nw) {
         const int64_t rb = X.row_ptr[pre], re = X.row_ptr[pre + 1];
+        if (rb == re) continue;  // wave-uniform
         const int32_t pb = X.st_ptr[pre], na = X.st_ptr[pre + 1] - pb;
         const bool pre_lds = na < kPreCap;  // wave-uniform
         // every lane of the wave is here (the row loop is wave-uniform)
@@ -201,33 +233,30 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
             if (lane == 0) sp[na] = kEnd;
         }
         wave_sync();
-        for (int64_t j0 = rb; j0 < re; j0 += 64) {  // wave-uniform chunk loop
+        // group j0 with its loads in g; loads group j0 + 64 into gn while it merges
+        auto group = [&](int64_t j0, const Group& g, Group& gn) {
             const int64_t j = j0 + lane;
-            const bool pl = j < re && X.plastic[j];
-            uint32_t en = 0;
-            int32_t qb = 0, nb = 0;
-            float w = 0.0f;
-            TraceHead tr{};
-            if (pl) {
-                en = X.syn[j];
-                w = X.w[j];
-                tr = load_head(X.trace + j);
-                qb = X.st_ptr[syn_post(en)];
-                nb = X.st_ptr[syn_post(en) + 1] - qb;
+            const int32_t d = syn_delay(g.en), post = syn_post(g.en);
+            const bool pl = g.pl != 0;
+            const bool any = __any(pl);  // wave-uniform; rows without plastic synapses skip the lists
+            // the lists first, then the next group: waiting for the lists leaves the group in flight
+            uint2 lst[kFix / 4];
+            if (any) {
+#pragma unroll
+                for (int e = 0; e < kFix / 4; ++e) lst[e] = X.fix[(int64_t)post * (kFix / 4) + e];
             }
-            const int32_t d = syn_delay(en);
-            const bool fast = pre_lds && !__any(pl && nb >= kPostCap);  // wave-uniform
+            if (j0 + 64 < re) load_group(X, j0 + 64 + lane, re, gn);
+            if (!any) return;
+            const bool fast = pre_lds && !__any(pl && (lst[kFix / 4 - 1].y >> 16) != kEnd);  // wave-uniform
             if (fast) {
-                // stage the target's spike steps (all loads before the first store) and the sentinel
                 if (pl) {
-                    int32_t qv[kPostCap];
 #pragma unroll
-                    for (int e = 0; e < kPostCap; ++e) qv[e] = e < nb ? X.st_k[qb + e] : (int32_t)kEnd;
-#pragma unroll
-                    for (int e = 0; e < kPostCap; ++e)
-                        if (e <= nb) sq[64 * e] = (uint16_t)qv[e];
-                }
-                if (pl) {
+                    for (int e = 0; e < kFix / 4; ++e) {
+                        sq[64 * (4 * e)] = (uint16_t)(lst[e].x & 0xFFFFu);
+                        sq[64 * (4 * e + 1)] = (uint16_t)(lst[e].x >> 16);
+                        sq[64 * (4 * e + 2)] = (uint16_t)(lst[e].y & 0xFFFFu);
+                        sq[64 * (4 * e + 3)] = (uint16_t)(lst[e].y >> 16);
+                    }
                     int32_t ia = 0, ib = 0;
                     if (lead > 0) {
                         while ((int32_t)sp[ia] + d < lead) ++ia;  // arrived before the window: applied earlier
@@ -235,7 +264,7 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
                     }
                     const int32_t ia0 = ia, ib0 = ib;
                     int32_t a = (int32_t)sp[ia] + d, b = sq[64 * ib];
-                    MergeState s{w, tr, tr.slast - X.base};
+                    MergeState s{g.w, g.tr, g.tr.slast - X.base};
                     while (min(a, b) < kend) {
                         const bool pre_side = a <= b;
                         const int32_t t = pre_side ? a : b;
@@ -249,11 +278,12 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
                     if (ia != ia0 || ib != ib0) {
                         upd += (unsigned long long)(ia - ia0 + ib - ib0);
                         s.tr.slast = s.since + X.base;
-                        store_head(X.trace + j, s.tr);
+                        store_head(X.heads, X.head_stride, j, s.tr);
                         X.w[j] = s.w;
                     }
                 }
             } else if (pl) {  // the same merge with the lists read from memory
+                const int32_t qb = X.st_ptr[post], nb = X.st_ptr[post + 1] - qb;
                 auto arr = [&](int32_t i) { return i < na ? X.st_k[pb + i] + d : kNever; };
                 auto spk = [&](int32_t i) { return i < nb ? X.st_k[qb + i] : kNever; };
                 int32_t ia = 0, ib = 0;
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
                 while (spk(ib) < lead) ++ib;
                 const int32_t ia0 = ia, ib0 = ib;
                 int32_t a = arr(ia), b = spk(ib);
-                MergeState s{w, tr, tr.slast - X.base};
+                MergeState s{g.w, g.tr, g.tr.slast - X.base};
                 while (min(a, b) < kend) {
                     const bool pre_side = a <= b;
                     const int32_t t = pre_side ? a : b;
@@ -273,10 +303,18 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
                 if (ia != ia0 || ib != ib0) {
                     upd += (unsigned long long)(ia - ia0 + ib - ib0);
                     s.tr.slast = s.since + X.base;
-                    store_head(X.trace + j, s.tr);
+                    store_head(X.heads, X.head_stride, j, s.tr);
                     X.w[j] = s.w;
                 }
             }
+        };
+        // groups in pairs, so the two load buffers alternate without copies
+        Group g0, g1;
+        load_group(X, rb + lane, re, g0);
+        for (int64_t j0 = rb; j0 < re; j0 += 128) {  // wave-uniform
+            group(j0, g0, g1);
+            if (j0 + 64 >= re) break;
+            group(j0 + 64, g1, g0);
         }
         wave_sync();  // the next row's staging follows every read of this one
     }
@@ -332,6 +370,7 @@ static int32_t replay_stdp_workspace(nbx_net* net, int64_t cap, int32_t blocks) 
         NBX_PASS(regrow(net, &R.k, (size_t)cap));
         R.cap = cap;
     }
+    if (!R.fix) NBX_PASS(net->alloc(&R.fix, (size_t)n * kFix));
     const int64_t m = (int64_t)n * blocks;
     if (m > R.mcap) {
         NBX_PASS(regrow(net, &R.mask, (size_t)m));
@@ -360,7 +399,9 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     NBX_HIP_TRY(hipGetLastError());
     size_t bytes = R.scan_bytes;
     NBX_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(R.scan_tmp, bytes, R.cnt, R.ptr, n + 1, st));
-    rs_place<<<kgrid, 256, 0, st>>>(d_ids, d_tick_ptr, K, n, blocks, R.mask, R.blk, R.ptr, R.k, net->d_err);
+    rs_fill16<<<std::max(1, std::min((n * kFix + 255) / 256, 4096)), 256, 0, st>>>(R.fix, (int64_t)n * kFix);
+    NBX_HIP_TRY(hipGetLastError());
+    rs_place<<<kgrid, 256, 0, st>>>(d_ids, d_tick_ptr, K, n, blocks, R.mask, R.blk, R.ptr, R.k, R.fix, net->d_err);
     NBX_HIP_TRY(hipGetLastError());
 
     ReplayStdpArgs X{};
@@ -368,13 +409,15 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     X.syn = net->d_syn;
     X.plastic = net->d_plastic;
     X.w = net->d_w;
-    X.trace = net->d_trace;
+    X.heads = net->d_heads;
+    X.head_stride = net->head_stride;
     X.tab = R.tab2;
     X.tab_len = net->tab_len;
     X.max_ltp = net->sd.max_ltp;
     X.max_ltd = net->sd.max_ltd;
     X.st_ptr = R.ptr;
     X.st_k = R.k;
+    X.fix = reinterpret_cast<const uint2*>(R.fix);
     X.n = n;
     X.lead = lead;
     X.kend = K;

@@ -285,7 +285,7 @@ __device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s
             const uint32_t en = A.syn[j];
             float w = A.w[j];
             // every synapse has a trace record: load it with the flag rather than after it
-            TraceHead tr = A.stdp ? load_head(A.trace + j) : TraceHead{};
+            TraceHead tr = A.stdp ? load_head(A.heads, A.head_stride, j) : TraceHead{};
             const bool plastic = A.stdp && A.plastic[j];
             if (MERGED && plastic && (A.fmask[syn_post(en)] & 1ull)) {
                 // STDP.cpp:37-59 for step t - 1 first: the target fired then
@@ -299,7 +299,7 @@ __device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s
             if (plastic) {  // STDP.cpp:19-35: decay, Apre += 1, w -= A- * Apost, clip
                 stdp_decay(A, tr, t);
                 tr.apre += 1.0f;
-                store_head(A.trace + j, tr);
+                store_head(A.heads, A.head_stride, j, tr);
                 A.w[j] = stdp_clip(w + -A.max_ltd * tr.apost, tr.wmax);
             }
             if (A.nrules) {
@@ -361,18 +361,18 @@ __device__ inline void ltp_items(const StepArgs& A, int32_t t, int32_t bi, int32
                     one = !((A.fmask[pd0 & 0xFFFFFF] >> ((pd0 >> 24) - 1)) & 1ull);
                     two = two && !((A.fmask[pd1 & 0xFFFFFF] >> ((pd1 >> 24) - 1)) & 1ull);
                 }
-                TraceHead r0 = load_head(A.trace + s0), r1 = load_head(A.trace + s1);
+                TraceHead r0 = load_head(A.heads, A.head_stride, s0), r1 = load_head(A.heads, A.head_stride, s1);
                 const float w0 = A.w[s0], w1 = A.w[s1];
                 stdp_decay(A, r0, t);
                 stdp_decay(A, r1, t);
                 r0.apost += 1.0f;
                 r1.apost += 1.0f;
                 if (one) {
-                    store_head(A.trace + s0, r0);
+                    store_head(A.heads, A.head_stride, s0, r0);
                     A.w[s0] = stdp_clip(w0 + A.max_ltp * r0.apre, r0.wmax);
                 }
                 if (two) {
-                    store_head(A.trace + s1, r1);
+                    store_head(A.heads, A.head_stride, s1, r1);
                     A.w[s1] = stdp_clip(w1 + A.max_ltp * r1.apre, r1.wmax);
                 }
             }
