@@ -152,3 +152,40 @@ def test_replay_stdp_device_entry_rejects_bad_raster(gpu):
     run = OracleRun(wl)
     assert run.replay_stdp([[4, 9], [1, 4], [], [2]]) == int(cnt.item()) > 0
     assert_same(net.weights(), run.w, "weights")
+
+
+@pytest.mark.parametrize("p_spike", [0.002, 0.03, 0.2], ids=["sparse", "medium", "dense"])
+def test_replay_stdp_random_rasters_against_oracle(gpu, p_spike):
+    """Synthetic rasters of every density: short lists (merged from LDS), lists of 20 and more
+    spikes and rows of 64 and more (merged from memory), against orc_replay_stdp, in two chained
+    calls."""
+    T = 400
+    wl = W.stdp_izh(n=1200, k_in=50, steps=10, w_bg=2.0)
+    n, D = wl.spec.n, wl.d_slots
+    rng = np.random.default_rng(int(p_spike * 1e4))
+    lists = [np.nonzero(rng.random(n) < p_spike)[0].astype(np.int32) for _ in range(T)]
+    net = wl.make_net()
+    run = OracleRun(wl)
+    t1 = 173
+    for t0, lead, part in ((0, 0, lists[:t1]), (t1, D - 1, lists[t1 - (D - 1):])):
+        assert net.replay_stdp(part, t0=t0, lead=lead) == run.replay_stdp(part, t0=t0, lead=lead)
+    assert_same(net.weights(), run.w, "weights")
+    apre, apost, last = net.stdp_state()
+    assert_same(apre, run.apre, "Apre")
+    assert_same(apost, run.apost, "Apost")
+    assert_same(last, run.slast, "LastUpdate")
+
+
+def test_replay_stdp_long_decay_tables(gpu):
+    # dt 0.1 ms: the decay tables (exp(-k dt / 20 ms) down to 0) are longer than the part staged in LDS
+    T = 600
+    wl = W.stdp_izh(n=1500, k_in=60, steps=T, w_bg=2.0)
+    wl.spec.dt = 0.1
+    ref, lists = closed_loop(wl, T)
+    assert sum(len(x) for x in lists) > 100
+    net = wl.make_net()
+    net.replay_stdp(lists[:300])
+    net.replay_stdp(lists[300 - (wl.d_slots - 1):], t0=300, lead=wl.d_slots - 1)
+    got = replayed_state(net)
+    for a, b, name in zip(got, ref, ("weights", "Apre", "Apost", "LastUpdate")):
+        assert_same(a, b, name)
