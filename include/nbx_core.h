@@ -270,7 +270,9 @@ int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick
  * (lead <= t0; D - 1 of them continue an earlier call exactly) only supply arrivals. Neuron
  * state, delay ring, firing history and the step counter are not touched. *updates (may be
  * NULL) receives the number of plastic updates applied. lead + nsteps <= 65535 (longer
- * rasters: chained calls). Networks with trigger rules are rejected. Host arrays. */
+ * rasters: chained calls). The network keeps a workspace of about 12 * n * ceil((lead + nsteps)
+ * / 64) + 4 * spikes + 48 * n bytes for later calls. Networks with trigger rules are rejected.
+ * Host arrays. */
 int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,
                         int32_t nsteps, int64_t* updates);
 /* Device-resident variant for timing: device pointers, enqueued on the network's stream;
