@@ -291,28 +291,38 @@ def dense_bench(steps: int, noise_scale: float = 1.0) -> dict:
     from neuronbridge_amd import workloads as W
     from neuronbridge_amd.hiptime import Event
     wl = W.dense_izh(steps=steps + 100, noise_scale=noise_scale)
-    net = wl.make_net()
-    engine = net.pick_engine()
-    net.run(100, engine)
-    net.record(net.time, steps, counts=True, raster=False)
-    ev0 = net.events()
-    e0, e1 = Event(), Event()
-    net.sync()
-    e0.record(net.stream)
-    net.run_async(steps, engine)
-    e1.record(net.stream)
-    e1.synchronize()
-    net.sync()
-    ms = e0.elapsed_ms(e1)
-    ev = net.events() - ev0
-    spikes = int(net.counts().sum())
-    net.close()
-    return {"workload": f"dense Izhikevich {wl.spec.n} neurons all-to-all, Gaussian currents (std 5 exc / 2 inh "
-                        f"per step x {noise_scale:.3g}), dt 0.1 ms, {steps} steps",
-            "engine": {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels"}.get(engine, str(engine)),
-            "ms_per_step": ms / steps, "neuron_steps_per_s": wl.spec.n * steps / (ms * 1e-3),
-            "syn_events_per_s": ev / (ms * 1e-3),
-            "spike_rate_hz": spikes / wl.spec.n / (steps * 0.1e-3)}
+    names = {L.ENGINE_GRAPH: "graph", L.ENGINE_KERNELS: "kernels", L.ENGINE_PERSISTENT: "persistent"}
+
+    def timed(engine):
+        net = wl.make_net()
+        net.run(100, engine)
+        net.record(net.time, steps, counts=True, raster=True)
+        ev0 = net.events()
+        e0, e1 = Event(), Event()
+        net.sync()
+        e0.record(net.stream)
+        net.run_async(steps, engine)
+        e1.record(net.stream)
+        e1.synchronize()
+        net.sync()
+        out = (e0.elapsed_ms(e1), net.events() - ev0, int(net.counts().sum()), net.raster())
+        net.close()
+        return out
+
+    engine = wl.make_net().pick_engine()
+    ms, ev, spikes, raster = timed(engine)
+    res = {"workload": f"dense Izhikevich {wl.spec.n} neurons all-to-all, Gaussian currents (std 5 exc / 2 inh "
+                       f"per step x {noise_scale:.3g}), dt 0.1 ms, {steps} steps",
+           "engine": names.get(engine, str(engine)),
+           "ms_per_step": ms / steps, "neuron_steps_per_s": wl.spec.n * steps / (ms * 1e-3),
+           "syn_events_per_s": ev / (ms * 1e-3),
+           "spike_rate_hz": spikes / wl.spec.n / (steps * 0.1e-3)}
+    if engine != L.ENGINE_GRAPH:  # the graph engine on the same steps, and the same raster
+        gms, _, _, graster = timed(L.ENGINE_GRAPH)
+        res["graph_ms_per_step"] = gms / steps
+        res["speedup_vs_graph"] = round(gms / ms, 2)
+        res["raster_identical_to_graph"] = bool(np.array_equal(raster, graster))
+    return res
 
 
 def models_bench(steps: int) -> dict:

@@ -128,7 +128,8 @@ typedef struct nbx_rule_desc {
 enum {
     NBX_ENGINE_KERNELS = 0,    /* one launch per phase per step */
     NBX_ENGINE_GRAPH = 1,      /* the same launches captured in a hipGraph of many steps */
-    NBX_ENGINE_PERSISTENT = 2, /* one cooperative launch; post-blocked delivery into an LDS ring */
+    NBX_ENGINE_PERSISTENT = 2, /* one cooperative launch; post-blocked delivery into an LDS ring
+                                  (dense networks of up to 1024 neurons: one workgroup for all steps) */
     NBX_ENGINE_AUTO = 3        /* persistent when the network fits it, else graph */
 };
 

@@ -276,6 +276,8 @@ std::string persistent_reason(const nbx_net* net);
 int32_t build_persistent_layout(nbx_net* net);
 int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
+std::string small_dense_reason(const nbx_net* net);  // empty when the single-workgroup engine fits
+int32_t run_small_dense(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
 int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLOTS
 int32_t clear_raster_error(nbx_net* net);

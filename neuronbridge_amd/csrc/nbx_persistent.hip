@@ -831,6 +831,10 @@ Geometry geometry(const nbx_net* net) {
 }  // namespace
 
 std::string persistent_reason(const nbx_net* net) {
+    if (net->has_dense && !net->has_sparse) {  // small dense networks: the single-workgroup engine
+        const std::string why = small_dense_reason(net);
+        return why.empty() ? why : "dense delivery: " + why;
+    }
     if (!net->has_sparse) return "needs sparse synapses";
     if (net->has_dense) return "dense delivery runs on the graph engine";
     if (net->stdp) return "STDP networks use arrival-time delivery";
@@ -1003,6 +1007,7 @@ static int64_t longest_row(const nbx_net* net) {
 }
 
 int32_t run_persistent(nbx_net* net, int32_t nsteps) {
+    if (net->has_dense && !net->has_sparse) return run_small_dense(net, nsteps);
     int32_t rc = build_persistent_layout(net);
     if (rc) return rc;
     if (nsteps == 0) return NBX_OK;

@@ -73,8 +73,9 @@ def test_model_replay_matches_brute_force(gpu, model, slots, monkeypatch):
     assert net.events() - e0 == events
 
 
+@pytest.mark.parametrize("engine", [L.ENGINE_GRAPH, L.ENGINE_PERSISTENT], ids=["graph", "persistent"])
 @pytest.mark.parametrize("model", [L.LIF_EXP2, L.LIF_EXP3], ids=["lif2", "lif3"])
-def test_dense_signed_weights(gpu, model):
+def test_dense_signed_weights(gpu, model, engine):
     # negative dense weights are inhibitory conductance inputs on their own channel
     n, T = 400, 400
     wl = W.model_sparse(model, n=n, k_in=10, seed=5)
@@ -84,7 +85,7 @@ def test_dense_signed_weights(gpu, model):
     wl.syn, wl.dense_wt, wl.dense_delay = None, wt, 2
     net = wl.make_net()
     net.record(0, T, counts=True, raster=True)
-    net.run(T, L.ENGINE_GRAPH)
+    net.run(T, engine)
     run = OracleRun(wl, fixed_shift=net.fixed_shift)
     counts_o, raster_o = run.run(T)
     assert counts_o.sum() > 100

@@ -120,13 +120,21 @@ def assert_history(net, raster_o, n):
         assert_bits_equal(np.sort(net.history(t)), np.nonzero(bits[t])[0].astype(np.int32), f"history of step {t}")
 
 
-def test_dense_izhikevich_current_config(gpu):
-    # dense all-to-all current-based Izhikevich at the configuration's full size (N=1000)
+@pytest.mark.parametrize("engine", ["graph", "persistent", "mixed"])
+def test_dense_izhikevich_current_config(gpu, engine):
+    # dense all-to-all current-based Izhikevich at the configuration's full size (N=1000); the
+    # persistent engine runs it in one workgroup, "mixed" alternates all three engines
     T = 1500
     wl = W.dense_izh(n=1000, steps=T, noise_scale=10 ** 0.5)
     net = wl.make_net()
     net.record(0, T, counts=True, raster=True)
-    net.run(T, L.ENGINE_GRAPH)
+    if engine == "mixed":
+        for k, e in ((411, L.ENGINE_PERSISTENT), (300, L.ENGINE_GRAPH), (1, L.ENGINE_PERSISTENT),
+                     (250, L.ENGINE_KERNELS), (T - 962, L.ENGINE_PERSISTENT)):
+            net.run(k, e)
+    else:
+        assert net.pick_engine() == L.ENGINE_PERSISTENT
+        net.run(T, ENGINES[engine])
     run = OracleRun(wl, fixed_shift=net.fixed_shift)
     counts_o, raster_o = run.run(T)
     assert counts_o.sum() > 100
