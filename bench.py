@@ -457,7 +457,7 @@ def main() -> int:
     ap.add_argument("--integrator", default="rk4", choices=["rk4", "euler"])
     ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--replay-reps", type=int, default=5)
-    ap.add_argument("--replay-steps", type=int, default=1024)
+    ap.add_argument("--replay-steps", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-replay", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
