@@ -279,7 +279,7 @@ int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_pt
 /* Device-resident variant for timing: device pointers, enqueued on the network's stream;
  * nspikes = tick_ptr[lead + nsteps] sizes the workspace. *d_updates (device, may be NULL) is
  * incremented. An invalid raster changes no synapse and is reported (NBX_ERR_INVALID) by the
- * next nbx_net_sync (or nbx_net_run). */
+ * next nbx_net_sync (or nbx_net_run); the next replay-STDP call starts with the report cleared. */
 int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
                                int32_t nsteps, int64_t nspikes, uint64_t* d_updates);
 /* ---------------- .nbnet connection tables ----------------
