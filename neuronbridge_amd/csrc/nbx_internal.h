@@ -430,11 +430,13 @@ __host__ __device__ inline float hash_uniform01(int32_t neuron, int32_t t) {
 
 // Applies one neuron's arrivals (a0/a1/a2 fixed point) and external inputs for step t,
 // then advances it one step. Shared by every engine so their arithmetic is identical.
-template <int MODEL>
+// POIS_GIVEN: the step's Poisson background count was drawn ahead by the caller (pc).
+template <int MODEL, bool LOAD_CUR = true, bool POIS_GIVEN = false>
 __device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, int32_t a2,
-                                   float& V, float& S1, float& GA, float& GG, float& I, int32_t& last) {
-    if (A.cur && t >= A.cur_t0 && t < A.cur_t0 + A.cur_n) I = A.cur[(int64_t)(t - A.cur_t0) * A.n + i];
-    if (A.pois) a0 += poisson_count(A.seed_lo, A.seed_hi, i, t, A.pois_tab, A.kmax) * A.q_bg;
+                                   float& V, float& S1, float& GA, float& GG, float& I, int32_t& last,
+                                   int32_t pc = 0) {
+    if (LOAD_CUR && A.cur && t >= A.cur_t0 && t < A.cur_t0 + A.cur_n) I = A.cur[(int64_t)(t - A.cur_t0) * A.n + i];
+    if (A.pois) a0 += (POIS_GIVEN ? pc : poisson_count(A.seed_lo, A.seed_hi, i, t, A.pois_tab, A.kmax)) * A.q_bg;
     const float f0 = (float)a0 * A.inv_scale;
     if (MODEL == NBX_MODEL_LIF_EXP2) {
         GA += f0;

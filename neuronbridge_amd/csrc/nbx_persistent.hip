@@ -80,7 +80,9 @@ __device__ inline void st_dev(int32_t* p, int32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline bool grid_barrier(Barrier* bar, int32_t k, int32_t nblocks, int32_t* err, int32_t* s_abort) {
+// The grid barrier in two halves, so a workgroup can do work that depends on no other workgroup
+// between its arrival and its wait.
+__device__ inline void grid_arrive(Barrier* bar, int32_t k, int32_t nblocks) {
     // every wave waits for its own published stores (__syncthreads waits for LDS only), so all
     // of the workgroup's stores are performed before its arrival is counted
     __builtin_amdgcn_s_waitcnt(0);
@@ -95,6 +97,12 @@ __device__ inline bool grid_barrier(Barrier* bar, int32_t k, int32_t nblocks, in
             if (o2 == groups * (k + 1) - 1)
                 for (int q = 0; q < kXcds; ++q) st_dev(&bar->gen[q * kLine], k + 1);
         }
+    }
+}
+
+__device__ inline bool grid_wait(Barrier* bar, int32_t k, int32_t* err, int32_t* s_abort) {
+    if (threadIdx.x == 0) {
+        const int32_t g = blockIdx.x % kXcds;
         uint32_t spins = 0;
         bool abort = false;
         while (ld_dev(&bar->gen[g * kLine]) < k + 1) {
@@ -159,6 +167,17 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         }
     }
     unsigned long long ev = 0;
+    // Poisson background counts of the next step: they depend on no other workgroup, so they are
+    // drawn between this workgroup's barrier arrival and its wait
+    int32_t pc[NPT];
+    auto draw = [&](int32_t t) {
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) {
+            const int32_t pl = tid + u * T;
+            pc[u] = (A.pois && pl < np) ? poisson_count(A.seed_lo, A.seed_hi, p0 + pl, t, A.pois_tab, A.kmax) : 0;
+        }
+    };
+    draw(X.t0);
     __syncthreads();
 
     for (int32_t k = 0; k <= X.nsteps; ++k) {
@@ -229,7 +248,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                     a2 = s_ring[(slot * nch + 2) * P + pl];
                     s_ring[(slot * nch + 2) * P + pl] = 0;
                 }
-                fired = neuron_tick<MODEL>(A, i, t, a0, a1, a2, V[u], S1[u], GA[u], GG[u], I[u], last[u]);
+                fired = neuron_tick<MODEL, true, true>(A, i, t, a0, a1, a2, V[u], S1[u], GA[u], GG[u], I[u], last[u],
+                                                       pc[u]);
             }
             const uint64_t mask = __ballot(fired);
             if (lane == 0) s_wcnt[wave] = __popcll(mask);
@@ -255,7 +275,9 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         }
         // slot of step t+1 held step t+1-D: nobody reads it in this phase (D >= 3)
         if (b == 0 && tid == 0) st_dev(&A.hist_cnt[(t + 1) % D], 0);
-        if (!grid_barrier(X.bar, k, X.B, A.err, &s_abort)) break;
+        grid_arrive(X.bar, k, X.B);
+        draw(t + 1);
+        if (!grid_wait(X.bar, k, A.err, &s_abort)) break;
     }
     // write back state and ring; the ring then holds every delivery up to the last step
 #pragma unroll
