@@ -431,6 +431,15 @@ void nbx_batch_parts(BatchArgs B, const PartDev* __restrict__ parts) {
     __syncthreads();
     uint32_t nspk = 0;  // spikes of this thread's words
     const uint16_t* wbase = B.w + I.w_off + c0;
+    // Poisson background counts of the coming step: they depend on no other part, so each step's
+    // are drawn while the part waits for the instance's spike words of the step before
+    int32_t pc[NPT];
+    auto draw = [&](int32_t t) {
+#pragma unroll
+        for (int j = 0; j < NPT; ++j)
+            pc[j] = (owner && c0 + j < n) ? poisson_count(I.seed_lo, I.seed_hi, opaque(c0 + j), t, I.tab, I.kmax) : 0;
+    };
+    draw(B.t0);
     for (int32_t k = 0; k < B.nsteps; ++k) {
         const int32_t t = B.t0 + k;
         const int32_t cnt = s_cnt;
@@ -470,7 +479,7 @@ void nbx_batch_parts(BatchArgs B, const PartDev* __restrict__ parts) {
             for (int j = 0; j < NPT; ++j) {
                 const int32_t i = c0 + j;
                 if (i < n) {
-                    const int32_t a0 = acc[j] + poisson_count(I.seed_lo, I.seed_hi, opaque(i), t, I.tab, I.kmax) * I.q_bg;
+                    const int32_t a0 = acc[j] + pc[j] * I.q_bg;
                     if (last[j] > P.t_ref) V[j] += (float)a0 * I.inv_scale;
                     float g = 0.0f;
                     if (lif_step<false>(P, V[j], g, 0.0f, last[j])) fired |= 1u << j;
@@ -490,6 +499,7 @@ void nbx_batch_parts(BatchArgs B, const PartDev* __restrict__ parts) {
                 B.raster[I.r_off + (int64_t)(t - B.rec_t0) * I.words + wg0 + tid] = word;
             nspk += (uint32_t)__popc(word);
         }
+        draw(t + 1);
         // the instance's spike words of step t
         uint32_t b0 = 0, b1 = 0;
         if (pd.parts > 1) {
