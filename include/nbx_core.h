@@ -174,7 +174,8 @@ int32_t nbx_net_get_rule_state(nbx_net* net, float* apre, float* apost, float* e
  * (reference: ProcessCurrent / InterfaceChannelValue with overwrite). Host array, copied. */
 int32_t nbx_net_set_current(nbx_net* net, const float* cur, int32_t t_begin, int32_t nsteps);
 /* Arrivals accumulate as int32 in units of 2^-shift; 0 picks the largest shift
- * that cannot overflow for this network. Must precede nbx_net_finalize. */
+ * that cannot overflow for this network. Must precede nbx_net_finalize, which rejects a
+ * requested shift whose int32 sums could overflow (NBX_ERR_INVALID). */
 int32_t nbx_net_set_fixed_shift(nbx_net* net, int32_t shift);
 int32_t nbx_net_get_fixed_shift(const nbx_net* net, int32_t* shift);
 /* Upload and build device layouts; required before stepping. */
@@ -223,7 +224,10 @@ int32_t nbx_net_apply_post_learning(nbx_net* net);
 /* Push this step's firings into the delay ring and close the step. */
 int32_t nbx_net_commit_firings(nbx_net* net);
 /* Inject spikes as if `ids` fired at the current step (reference: QueueInterfaceSpike /
- * AddExternalSpikeActivity); call between nbx_net_neuron_step and nbx_net_commit_firings. */
+ * AddExternalSpikeActivity); call between nbx_net_neuron_step and nbx_net_commit_firings.
+ * A neuron fires at most once per step: an id repeated in the call or already in the step's
+ * firing list (fired in the update, or injected earlier) is rejected (NBX_ERR_INVALID, nothing
+ * injected). */
 int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count);
 
 /* Delay ring contents, int32 fixed point ring[slot][channel][n] (slot = step % slots). */
@@ -246,7 +250,17 @@ int32_t nbx_net_pending_view(nbx_net* net, nbx_pending_view* view);
 
 /* Weights and STDP state (reference: Download/UploadSynapticWeights, Get/SetSynapseWeight). */
 int32_t nbx_net_get_weights(nbx_net* net, float* w, int64_t count);
+/* Replaces every sparse weight (CSR order), between steps. Arrivals of spikes already fired but
+ * not yet delivered take the new weights, as the reference reads a synapse's weight at delivery
+ * (PropogatedSpikeGroup.cpp:33-71). Rejected (NBX_ERR_INVALID, weights unchanged) when a weight is
+ * not finite or when the new weights could overflow the int32 arrival sums at the fixed shift
+ * chosen at nbx_net_finalize (sum of |w| into one neuron, plastic synapses at their bound);
+ * NBX_ERR_STATE inside an open per-phase step. */
 int32_t nbx_net_set_weights(nbx_net* net, const float* w, int64_t count);
+/* One synapse by CSR index (reference: GpuPropagationRuntime::Get/SetSynapseWeight,
+ * GpuPropagationRuntime.h:139-141), with the checks and delivery rule of nbx_net_set_weights. */
+int32_t nbx_net_get_synapse_weight(nbx_net* net, int64_t synapse, float* w);
+int32_t nbx_net_set_synapse_weight(nbx_net* net, int64_t synapse, float w);
 int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t* last_update, int64_t count);
 
 /* ---------------- replay delivery (multi-step, injected spikes) ----------------

@@ -162,6 +162,8 @@ SIGNATURES = {
     "nbx_net_pending_view": [_NET, C.POINTER(PendingView)],
     "nbx_net_get_weights": [_NET, _f32p, C.c_int64],
     "nbx_net_set_weights": [_NET, _f32p, C.c_int64],
+    "nbx_net_get_synapse_weight": [_NET, C.c_int64, _f32p],
+    "nbx_net_set_synapse_weight": [_NET, C.c_int64, C.c_float],
     "nbx_net_get_stdp_state": [_NET, _f32p, _f32p, _i32p, C.c_int64],
     "nbx_replay_deliver": [_NET, _i32p, _i32p, C.c_int32, C.c_int32, _i32p],
     "nbx_replay_deliver_device": [_NET, _vp, _vp, C.c_int32, _vp],

@@ -623,18 +623,13 @@ struct nbx_batch {
         void* kargs[] = {&A};
         return hipLaunchKernel(fn, dim3(count), dim3(thr), kargs, lds, stream);
     }
-    // Every part of a launch must be co-resident: the cooperative launch has the runtime check it
-    // (capacity is also checked when the split is set). NBX_COOPERATIVE=0 selects a regular
-    // launch for tools that cannot trace cooperative dispatches; the polls are bounded either way.
+    // Every part of a launch must be co-resident: the cooperative launch has the runtime guarantee
+    // it (capacity is also checked when the split is set).
     hipError_t launch_parts(int32_t t0, int32_t nsteps) {
         BatchArgs A = args(t0, nsteps);
         const PartDev* parts = d_parts;
         void* kargs[] = {&A, &parts};
         const void* fn = parts_kernel(part_npt);
-        const char* coop = std::getenv("NBX_COOPERATIVE");
-        if (coop && coop[0] == '0')
-            return hipLaunchKernel(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs, part_lds(part_npt),
-                                   stream);
         return hipLaunchCooperativeKernel(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs,
                                           (unsigned)part_lds(part_npt), stream);
     }

@@ -329,13 +329,14 @@ int32_t run_kernels(nbx_net* net, int32_t nsteps) {
     return NBX_OK;
 }
 
-int32_t auto_shift(nbx_net* net) {
-    // largest shift such that the sum of |contributions| into one (step, channel, neuron)
-    // stays below 2^31 for any spike pattern
+// Largest sum of |contributions| that can land in one (step, channel, neuron) for any spike
+// pattern: every synapse into the neuron (plastic ones at their weight bound), the dense column and
+// the largest Poisson count. `w` holds the sparse weights in CSR order.
+double arrival_bound(const nbx_net* net, const float* w_csr) {
     std::vector<double> bound((size_t)net->n, 0.0);
     if (net->has_sparse) {
         for (int64_t j = 0; j < net->S; ++j) {
-            double w = std::fabs((double)net->h_w[(size_t)j]);
+            double w = std::fabs((double)w_csr[(size_t)j]);
             if (net->stdp && !net->h_plastic.empty() && net->h_plastic[(size_t)j])
                 w = std::max(w, net->h_smaxw.empty() ? (double)net->sd.w_max : std::fabs((double)net->h_smaxw[(size_t)j]));
             if (!net->h_rule.empty() && net->h_rule[(size_t)j] >= 0)
@@ -352,10 +353,22 @@ int32_t auto_shift(nbx_net* net) {
     double B = 0.0;
     for (double b : bound) B = std::max(B, b);
     if (net->pois) B += (double)net->kmax * std::fabs((double)net->w_bg);
+    return B;
+}
+
+// Largest shift (at most 24) whose int32 sums cannot overflow for the bound B; the margin leaves
+// room for the rounding of each quantized term (0.5 ulp of 2^-s each).
+int32_t shift_for_bound(double B) {
     int32_t s = 24;
-    // ceil-safe: also leave room for rounding of each quantized term (0.5 ulp of 2^-s each)
     while (s > 0 && B * std::ldexp(1.0, s) + 1e6 >= 2147483647.0) --s;
     return s;
+}
+
+// Whether shift s keeps every int32 sum of the bound B in range.
+bool shift_fits(double B, int32_t s) { return B * std::ldexp(1.0, s) + 1e6 < 2147483647.0; }
+
+int32_t auto_shift(const nbx_net* net) {
+    return shift_for_bound(arrival_bound(net, net->has_sparse ? net->h_w.data() : nullptr));
 }
 
 }  // namespace
@@ -650,6 +663,15 @@ int32_t nbx_net_finalize(nbx_net* net) {
         for (size_t j = 0; j < (size_t)net->S; ++j)
             NBX_CHECK(!(net->h_plastic[j] && (net->h_rule[j] >= 0 || net->h_trig[j] >= 0)),
                       "a synapse is either STDP-plastic, rule-plastic or a trigger");
+    if (net->shift_req) {
+        const double B = arrival_bound(net, net->has_sparse ? net->h_w.data() : nullptr);
+        if (!shift_fits(B, net->shift_req)) {
+            set_error("fixed shift " + std::to_string(net->shift_req) +
+                      " can overflow the int32 arrival sums of this network (largest safe shift " +
+                      std::to_string(shift_for_bound(B)) + ")");
+            return NBX_ERR_INVALID;
+        }
+    }
     net->shift = net->shift_req ? net->shift_req : auto_shift(net);
     net->words = (n + 31) / 32;
     // conductance inputs present (the reference models' CheckType flags)
@@ -703,6 +725,7 @@ int32_t nbx_net_finalize(nbx_net* net) {
     NBX_RC(net->alloc(&net->d_events, 1));
     NBX_HIP_TRY(hipMemsetAsync(net->d_events, 0, sizeof(unsigned long long), net->stream));
     NBX_RC(net->alloc(&net->d_tbase, 1));
+    NBX_RC(net->alloc(&net->d_stage, (size_t)n));
     NBX_RC(net->alloc(&net->d_err, 1));
     NBX_HIP_TRY(hipMemsetAsync(net->d_err, 0, sizeof(int32_t), net->stream));
     // the stream is non-blocking: finish the clears before anything else touches these buffers
@@ -992,18 +1015,24 @@ int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count) {
     NBX_CHECK(count >= 0 && (ids || count == 0), "bad spike list");
     if (count == 0) return NBX_OK;
     for (int32_t k = 0; k < count; ++k) NBX_CHECK(ids[k] >= 0 && ids[k] < net->n, "injected id out of range");
+    NBX_CHECK(count <= net->n, "too many spikes in one step");
     NBX_RC(device_scope(net));
+    // A neuron fires at most once per step: an id repeated in the call, or already in the step's
+    // list (it fired in the update or was injected before), would be delivered twice.
+    std::vector<int32_t> listed((size_t)net->n);
     int32_t cur = 0;
-    const int32_t slot = net->t % net->D;
-    NBX_HIP_TRY(hipMemcpy(&cur, net->d_hist_cnt + slot, sizeof(int32_t), hipMemcpyDeviceToHost));
+    NBX_RC(nbx_net_get_history(net, net->t, listed.data(), net->n, &cur));
     NBX_CHECK(cur + count <= net->n, "too many spikes in one step");
-    int32_t* d_ids = nullptr;
-    NBX_HIP_TRY(hipMalloc(&d_ids, sizeof(int32_t) * count));
-    hipError_t e = hipMemcpy(d_ids, ids, sizeof(int32_t) * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch_inject(net->args(), d_ids, count, net->t, net->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
-    hipFree(d_ids);
-    NBX_HIP_TRY(e);
+    std::vector<uint8_t> seen((size_t)net->n, 0);
+    for (int32_t k = 0; k < cur; ++k) seen[(size_t)listed[(size_t)k]] = 1;
+    for (int32_t k = 0; k < count; ++k) {
+        NBX_CHECK(!seen[(size_t)ids[k]], "injected neuron already fires in this step (listed twice or fired in the "
+                                         "update): a neuron fires at most once per step");
+        seen[(size_t)ids[k]] = 1;
+    }
+    NBX_HIP_TRY(hipMemcpy(net->d_stage, ids, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    NBX_HIP_TRY(launch_inject(net->args(), net->d_stage, count, net->t, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     return NBX_OK;
 }
 
@@ -1108,17 +1137,110 @@ int32_t nbx_net_get_weights(nbx_net* net, float* w, int64_t count) {
     return NBX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Replaces the weights of synapses [j0, j0 + count) (CSR order). Push-mode networks hold the
+// arrivals of spikes already fired in the delay ring, quantized with the weights of their spike
+// step; the reference delivers each arrival with the weight current at its arrival
+// (PropogatedSpikeGroup.cpp:33-71 reads Interconnections::weight when the group is processed),
+// so the pending arrivals of changed synapses are moved to the new weights: every spike of the
+// last D - 1 steps (the firing-history ring) whose arrival step is not yet past adds
+// q(new) - q(old) to its ring entry. Pull-mode networks read the weights at arrival already.
+int32_t set_weight_range(nbx_net* net, int64_t j0, int64_t count, const float* w) {
+    for (int64_t k = 0; k < count; ++k) NBX_CHECK(std::isfinite(w[k]), "weights must be finite");
+    if (net->phase != 0) {
+        set_error("weights change between steps (the per-phase step is open)");
+        return NBX_ERR_STATE;
+    }
+    std::vector<float> nw;
+    if (!net->pull_mode) {
+        nw = net->h_w;  // host copy is current: no learning changes push-mode weights
+    } else {
+        nw.resize((size_t)net->S);
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        NBX_HIP_TRY(hipMemcpy(nw.data(), net->d_w, sizeof(float) * nw.size(), hipMemcpyDeviceToHost));
+    }
+    std::copy(w, w + count, nw.begin() + j0);
+    {  // the fixed-point shift was chosen at finalize: the new weights must not overflow its sums
+        const double B = arrival_bound(net, nw.data());
+        if (!shift_fits(B, net->shift)) {
+            set_error("new weights can overflow the int32 arrival sums at fixed shift " + std::to_string(net->shift) +
+                      " (largest safe shift " + std::to_string(shift_for_bound(B)) + "); weights unchanged");
+            return NBX_ERR_INVALID;
+        }
+    }
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    if (!net->pull_mode && net->t > 0) {
+        const int32_t n = net->n, D = net->D, t = net->t;
+        std::vector<int32_t> hist((size_t)D * n), cnt((size_t)D);
+        std::vector<int32_t> ring((size_t)D * net->nch * n);
+        NBX_HIP_TRY(hipMemcpy(cnt.data(), net->d_hist_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(hist.data(), net->d_hist, sizeof(int32_t) * hist.size(), hipMemcpyDeviceToHost));
+        NBX_HIP_TRY(hipMemcpy(ring.data(), net->d_ring, sizeof(int32_t) * ring.size(), hipMemcpyDeviceToHost));
+        const float scale = std::ldexp(1.0f, net->shift);
+        bool touched = false;
+        for (int32_t st = std::max(0, t - D + 1); st < t; ++st) {
+            const int32_t slot = st % D;
+            for (int32_t k = 0; k < cnt[(size_t)slot]; ++k) {
+                const int32_t pre = hist[(size_t)slot * n + k];
+                const int64_t b = std::max(net->h_row_ptr[(size_t)pre], j0);
+                const int64_t e = std::min(net->h_row_ptr[(size_t)pre + 1], j0 + count);
+                for (int64_t j = b; j < e; ++j) {
+                    const uint32_t en = net->h_syn[(size_t)j];
+                    const int32_t arr = st + syn_delay(en);
+                    if (arr < t) continue;  // already delivered
+                    int32_t dq = (int32_t)std::nearbyint(nw[(size_t)j] * scale) -
+                                 (int32_t)std::nearbyint(net->h_w[(size_t)j] * scale);
+                    if (dq == 0) continue;
+                    const int32_t ty = syn_type(en);
+                    if (arrival_negated(net->P.model, ty)) dq = -dq;
+                    ring[((size_t)(arr % D) * net->nch + arrival_channel(net->P.model, ty)) * n + syn_post(en)] += dq;
+                    touched = true;
+                }
+            }
+        }
+        if (touched)
+            NBX_HIP_TRY(hipMemcpy(net->d_ring, ring.data(), sizeof(int32_t) * ring.size(), hipMemcpyHostToDevice));
+    }
+    NBX_HIP_TRY(hipMemcpy(net->d_w + j0, w, sizeof(float) * (size_t)count, hipMemcpyHostToDevice));
+    std::copy(w, w + count, net->h_w.begin() + j0);
+    return refresh_persistent_weights(net);
+}
+
+}  // namespace
+
+extern "C" {
+
 int32_t nbx_net_set_weights(nbx_net* net, const float* w, int64_t count) {
     int32_t rc = require_finalized(net);
     if (rc) return rc;
     NBX_CHECK(net->has_sparse, "network has no sparse synapses");
     NBX_CHECK(w && count == net->S, "weight buffer size must equal the synapse count");
     NBX_RC(device_scope(net));
+    return set_weight_range(net, 0, count, w);
+}
+
+int32_t nbx_net_get_synapse_weight(nbx_net* net, int64_t synapse, float* w) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "network has no sparse synapses");
+    NBX_CHECK(w, "null argument");
+    NBX_CHECK(synapse >= 0 && synapse < net->S, "synapse index out of range");
+    NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-    NBX_HIP_TRY(hipMemcpy(net->d_w, w, sizeof(float) * (size_t)count, hipMemcpyHostToDevice));
-    net->h_w.assign(w, w + count);
-    rc = refresh_persistent_weights(net);
-    return rc;
+    NBX_HIP_TRY(hipMemcpy(w, net->d_w + synapse, sizeof(float), hipMemcpyDeviceToHost));
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_synapse_weight(nbx_net* net, int64_t synapse, float w) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->has_sparse, "network has no sparse synapses");
+    NBX_CHECK(synapse >= 0 && synapse < net->S, "synapse index out of range");
+    NBX_RC(device_scope(net));
+    return set_weight_range(net, synapse, 1, &w);
 }
 
 int32_t nbx_net_get_stdp_state(nbx_net* net, float* apre, float* apost, int32_t* last_update, int64_t count) {

@@ -144,6 +144,7 @@ struct nbx_net {
     bool rstdp = false;
     unsigned long long* d_events = nullptr;
     int32_t* d_tbase = nullptr;
+    int32_t* d_stage = nullptr;  // [n] staging of host-given neuron ids (injected spikes)
     int32_t* d_err = nullptr;
     int32_t* d_counts = nullptr;
     uint32_t* d_raster = nullptr;

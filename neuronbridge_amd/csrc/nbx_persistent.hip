@@ -1059,15 +1059,10 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.nsteps = nsteps;
     X.bar = reinterpret_cast<Barrier*>(L.d_bar);
     void* kargs[] = {&X};
-    // Co-residency of all L.B workgroups is checked above (occupancy x CUs). The cooperative
-    // launch additionally has the runtime verify it; NBX_COOPERATIVE=0 selects a regular launch
-    // for tools that cannot trace cooperative dispatches (the barrier spin is bounded either way).
-    const char* coop = std::getenv("NBX_COOPERATIVE");
-    if (coop && coop[0] == '0')
-        NBX_HIP_TRY(hipLaunchKernel(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
-    else
-        NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes,
-                                               net->stream));
+    // Co-residency of all L.B workgroups is checked above (occupancy x CUs); the cooperative
+    // launch additionally has the runtime guarantee it, which the grid barrier relies on.
+    NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes,
+                                           net->stream));
     net->t += nsteps;
     return NBX_OK;
 }

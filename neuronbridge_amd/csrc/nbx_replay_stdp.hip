@@ -380,11 +380,12 @@ static int32_t replay_stdp_workspace(nbx_net* net, int64_t cap, int32_t blocks) 
     return NBX_OK;
 }
 
-int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
-                           int32_t nsteps, int64_t nspikes, unsigned long long* d_updates) {
+// The launch sequence over a workspace R sized for n neurons, `blocks` 64-step blocks and R.cap
+// spikes (cnt[n] must read 0).
+static int32_t enqueue_replay_stdp(nbx_net* net, const ReplayStdpWork& R, const int32_t* d_ids,
+                                   const int32_t* d_tick_ptr, int32_t t0, int32_t lead, int32_t nsteps,
+                                   unsigned long long* d_updates) {
     const int32_t n = net->n, K = lead + nsteps, blocks = (K + 63) / 64;
-    NBX_PASS(replay_stdp_workspace(net, std::max<int64_t>(nspikes, 1), blocks));
-    ReplayStdpWork& R = net->rsw;
     hipStream_t st = net->stream;
     const int64_t m = (int64_t)n * blocks;
     const int32_t kgrid = std::max(1, std::min(K, 4096));
@@ -429,6 +430,13 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     else nbx_replay_stdp<false><<<grid, kRsPassThreads, 0, st>>>(X);
     NBX_HIP_TRY(hipGetLastError());
     return NBX_OK;
+}
+
+int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
+                           int32_t nsteps, int64_t nspikes, unsigned long long* d_updates) {
+    const int32_t K = lead + nsteps, blocks = (K + 63) / 64;
+    NBX_PASS(replay_stdp_workspace(net, std::max<int64_t>(nspikes, 1), blocks));
+    return enqueue_replay_stdp(net, net->rsw, d_ids, d_tick_ptr, t0, lead, nsteps, d_updates);
 }
 
 }  // namespace nbx

@@ -487,8 +487,8 @@ __global__ void nbx_advance(int32_t* tbase, int32_t by) {
     if (threadIdx.x == 0) *tbase += by;
 }
 
-// Injected spikes of step t: appended to the step's firing list (ids are distinct
-// from the neurons that fired on their own only if the caller makes them so).
+// Injected spikes of step t: appended to the step's firing list (the host entry point has
+// checked that no id is repeated or already listed).
 __global__ void nbx_inject(StepArgs A, const int32_t* ids, int32_t count, int32_t t) {
     const int32_t slot = t % A.D;
     __shared__ int32_t s_base;

@@ -311,6 +311,14 @@ class Net:
         w = np.ascontiguousarray(w, dtype=np.float32)
         L.call("nbx_net_set_weights", self.h, _p(w, C.c_float), w.size)
 
+    def synapse_weight(self, j: int) -> float:
+        out = C.c_float()
+        L.call("nbx_net_get_synapse_weight", self.h, int(j), C.byref(out))
+        return float(out.value)
+
+    def set_synapse_weight(self, j: int, w: float) -> None:
+        L.call("nbx_net_set_synapse_weight", self.h, int(j), float(w))
+
     def stdp_state(self):
         S = self.syn_count
         apre = np.zeros(S, np.float32)

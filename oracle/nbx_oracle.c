@@ -22,6 +22,14 @@
  *   orc_replay_stdp        orc_run's own final weights and traces when fed orc_run's raster
  *                          (single and chained calls); per synapse, orc_stdp_replay on that
  *                          synapse's arrival / spike schedule
+ *   conductance terms      closed forms with the conductances held (tests/test_oracle_pins.py):
+ *                          LIF_EXP / LIF_EXP2 / LIF_EXP3 contract to V_inf = (sum g E + R I +
+ *                          V_rest) / (1 + sum g) by 1 - h (Euler) or the RK4 polynomial in h;
+ *                          Izhikevich AMPA / GABA / NMDA currents vanish at their reversal
+ *                          potential, change sign across it and add linearly (scaled by R);
+ *                          decays equal exp(-n dt / tau)
+ *   orc_stable_unit_random bit-exact against the reference's own StableUnitRandom compiled from
+ *                          its source (oracle/_ref, tests/golden/stable_unit_random.npz)
  */
 #include "nbx_oracle.h"
 

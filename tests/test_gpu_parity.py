@@ -82,10 +82,13 @@ def test_sparse_lif_closed_loop_bitexact(gpu, engine, integrator):
     compare(net, run, wl, counts_o, raster_o)
 
 
-@pytest.mark.parametrize("model", [L.LIF_EXP, L.IZH], ids=["lif_exp", "izh_cond"])
-@pytest.mark.parametrize("engine", ["graph", "persistent"])
-def test_other_models_closed_loop_bitexact(gpu, model, engine):
-    wl = small_sparse(model, L.RK4 if model == L.IZH else L.EULER)
+@pytest.mark.parametrize("model,integrator", [(L.LIF_EXP, L.EULER), (L.LIF_EXP, L.RK4), (L.IZH, L.RK4),
+                                               (L.IZH, L.EULER)],
+                         ids=["lif_exp_euler", "lif_exp_rk4", "izh_cond_rk4", "izh_cond_euler"])
+@pytest.mark.parametrize("engine", ["kernels", "graph", "persistent"])
+def test_other_models_closed_loop_bitexact(gpu, model, integrator, engine):
+    # the conductance models with both integrators, the reference's (Euler) included, on every engine
+    wl = small_sparse(model, integrator)
     T = 300
     net = wl.make_net()
     net.record(0, T, counts=True, raster=True)
