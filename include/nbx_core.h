@@ -296,6 +296,19 @@ int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_pt
  * next nbx_net_sync (or nbx_net_run); the next replay-STDP call starts with the report cleared. */
 int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
                                int32_t nsteps, int64_t nspikes, uint64_t* d_updates);
+/* Allocation-free variant of nbx_replay_stdp_device. It has no counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, semantics and results; the call's
+ * workspace (per-neuron spike lists, spike bit words, scan storage, decay table) lives in one
+ * static device array of NBX_REPLAY_STDP_SCRATCH_BYTES per GPU, held until the process ends, and
+ * the call makes no hipMalloc / hipFree or other allocation. A call whose workspace would exceed
+ * the array (about 12 * n * ceil((lead + nsteps) / 64) + 4 * nspikes + 48 * n bytes) is rejected
+ * with NBX_ERR_INVALID, as is a network whose blocked delivery layouts are built (freeing them
+ * would deallocate). Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from any network or thread) race and must be serialised by the caller. */
+#define NBX_REPLAY_STDP_SCRATCH_BYTES ((int64_t)32 << 20)
+int32_t nbx_replay_stdp_device_noalloc(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0,
+                                       int32_t lead, int32_t nsteps, int64_t nspikes, uint64_t* d_updates);
 /* ---------------- .nbnet connection tables ----------------
  * The reference's binary network description (python/src/neuronbridge/nbnet.py:34-38,
  * read by src/native/shared/src/streaming_build/NbnetReader.cpp): header, metadata JSON,

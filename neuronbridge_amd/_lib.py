@@ -170,6 +170,7 @@ SIGNATURES = {
     "nbx_replay_path": [_NET, _i32p],
     "nbx_replay_stdp": [_NET, _i32p, _i32p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
     "nbx_replay_stdp_device": [_NET, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _vp],
+    "nbx_replay_stdp_device_noalloc": [_NET, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _vp],
     "nbx_nbnet_info": [C.c_char_p, C.c_int32, C.POINTER(NbnetHeader)],
     "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],

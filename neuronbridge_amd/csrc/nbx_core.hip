@@ -1377,6 +1377,22 @@ int32_t nbx_replay_stdp_device(nbx_net* net, const int32_t* d_ids, const int32_t
                               reinterpret_cast<unsigned long long*>(d_updates));
 }
 
+int32_t nbx_replay_stdp_device_noalloc(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0,
+                                       int32_t lead, int32_t nsteps, int64_t nspikes, uint64_t* d_updates) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(net->stdp && net->has_sparse, "replay STDP needs sparse synapses with STDP (nbx_net_set_stdp)");
+    NBX_CHECK(net->rules.empty(), "replay STDP does not run trigger-driven learning rules");
+    NBX_CHECK(d_tick_ptr && nsteps > 0 && lead >= 0 && lead <= t0 && nspikes >= 0, "bad replay arguments");
+    NBX_CHECK((int64_t)lead + nsteps <= 65535, "replay STDP covers at most 65535 steps per call (chain calls)");
+    NBX_CHECK(nspikes == 0 || d_ids, "null spike ids");
+    // the blocked layouts carry pre-quantized weights; invalidating them would free memory
+    NBX_CHECK(!net->pl.built && !net->rs.built, "blocked delivery layouts are built for this network");
+    NBX_RC(device_scope(net));
+    return launch_replay_stdp_noalloc(net, d_ids, d_tick_ptr, t0, lead, nsteps, nspikes,
+                                      reinterpret_cast<unsigned long long*>(d_updates));
+}
+
 int32_t nbx_replay_path(nbx_net* net, int32_t* path) {
     int32_t rc = require_finalized(net);
     if (rc) return rc;

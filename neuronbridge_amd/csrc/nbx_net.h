@@ -284,5 +284,12 @@ int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLO
 int32_t clear_raster_error(nbx_net* net);
 int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
                            int32_t nsteps, int64_t nspikes, unsigned long long* d_updates);
+// Same, with the workspace in the library's static device array (no allocation); rejects calls
+// whose workspace exceeds it.
+constexpr size_t kReplayStdpScratchBytes = NBX_REPLAY_STDP_SCRATCH_BYTES;
+int32_t launch_replay_stdp_noalloc(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0,
+                                   int32_t lead, int32_t nsteps, int64_t nspikes, unsigned long long* d_updates);
+// Workspace bytes of one replay-STDP call (-1 when the scan storage query fails).
+int64_t replay_stdp_scratch_bytes(const nbx_net* net, int32_t K, int64_t nspikes, size_t* scan_bytes);
 const char* rule_desc_error(const nbx_rule_desc& q, bool allow_stdp);  // null when valid
 }  // namespace nbx

@@ -21,6 +21,9 @@
 // popcount of the lower bits of its word.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+#include <string>
+
 #include "nbx_internal.h"
 #include "nbx_net.h"
 
@@ -437,6 +440,101 @@ int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_
     const int32_t K = lead + nsteps, blocks = (K + 63) / 64;
     NBX_PASS(replay_stdp_workspace(net, std::max<int64_t>(nspikes, 1), blocks));
     return enqueue_replay_stdp(net, net->rsw, d_ids, d_tick_ptr, t0, lead, nsteps, d_updates);
+}
+
+// ---------------------------------------------------------------- allocation-free variant
+// The workspace of one call in a file-scope device array: each GPU holds its own copy for the
+// life of the process. Host code never passes the symbol itself (in host code its name is only a
+// stand-in): the device address comes from hipGetSymbolAddress, looked up once per device with
+// that device current, on the entry point's first call there.
+static __device__ __align__(256) unsigned char scratch[kReplayStdpScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int32_t scratch_address(int32_t device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device index beyond the scratch table");
+        return NBX_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        NBX_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned a null scratch address");
+            return NBX_ERR_HIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+    return NBX_OK;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int64_t replay_stdp_scratch_bytes(const nbx_net* net, int32_t K, int64_t nspikes, size_t* scan_bytes) {
+    const int32_t n = net->n, blocks = (K + 63) / 64;
+    size_t sb = 0;
+    // a size query only: hipcub computes the temporary storage of the scan on the host
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (int32_t*)nullptr, (int32_t*)nullptr, n + 1, net->stream) !=
+        hipSuccess)
+        return -1;
+    if (scan_bytes) *scan_bytes = sb;
+    const size_t m = (size_t)n * blocks;
+    return (int64_t)(round256(sizeof(int32_t) * ((size_t)n + 1)) * 2 +
+                     round256(sizeof(int32_t) * (size_t)std::max<int64_t>(nspikes, 1)) +
+                     round256(sizeof(unsigned long long) * m) + round256(sizeof(int32_t) * m) +
+                     round256(sizeof(uint16_t) * (size_t)n * kFix) + round256(sizeof(float2) * (size_t)net->tab_len) +
+                     round256(std::max<size_t>(sb, 1)));
+}
+
+int32_t launch_replay_stdp_noalloc(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0,
+                                   int32_t lead, int32_t nsteps, int64_t nspikes, unsigned long long* d_updates) {
+    const int32_t n = net->n, K = lead + nsteps, blocks = (K + 63) / 64;
+    size_t sb = 0;
+    const int64_t need = replay_stdp_scratch_bytes(net, K, nspikes, &sb);
+    if (need < 0) {
+        set_error("replay STDP: scan storage query failed");
+        return NBX_ERR_HIP;
+    }
+    if (need > (int64_t)kReplayStdpScratchBytes) {
+        set_error("replay STDP without allocation: the call needs " + std::to_string(need) +
+                  " bytes of workspace, more than the static " + std::to_string(kReplayStdpScratchBytes) +
+                  " (fewer steps or spikes per call)");
+        return NBX_ERR_INVALID;
+    }
+    unsigned char* base = nullptr;
+    NBX_PASS(scratch_address(net->device, &base));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += round256(bytes);
+        return p;
+    };
+    const size_t m = (size_t)n * blocks;
+    ReplayStdpWork R;
+    R.cnt = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * ((size_t)n + 1)));
+    R.ptr = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * ((size_t)n + 1)));
+    R.cap = std::max<int64_t>(nspikes, 1);
+    R.k = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (size_t)R.cap));
+    R.mask = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * m));
+    R.blk = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * m));
+    R.mcap = (int64_t)m;
+    R.fix = reinterpret_cast<uint16_t*>(take(sizeof(uint16_t) * (size_t)n * kFix));
+    R.tab2 = reinterpret_cast<float2*>(take(sizeof(float2) * (size_t)net->tab_len));
+    R.scan_tmp = take(std::max<size_t>(sb, 1));
+    R.scan_bytes = sb;
+    // the array keeps the previous call's contents: the scan's last input and the decay table are
+    // rewritten here (fill kernels, no memset nodes, so the call can be captured)
+    NBX_HIP_TRY(launch_fill_i32(R.cnt + n, 0, 1, net->stream));
+    rs_interleave<<<std::max(1, std::min((net->tab_len + 255) / 256, 256)), 256, 0, net->stream>>>(
+        net->d_ltp_tab, net->d_ltd_tab, net->tab_len, R.tab2);
+    NBX_HIP_TRY(hipGetLastError());
+    return enqueue_replay_stdp(net, R, d_ids, d_tick_ptr, t0, lead, nsteps, d_updates);
 }
 
 }  // namespace nbx
