@@ -139,6 +139,19 @@ typedef struct nbx_net nbx_net;
  * AllocateDeviceBuffers, legacy InitStateVector). State starts like
  * Neuron_State_Vector::InitNeuronState with sigma 0 (LastSpike = 10000). */
 int32_t nbx_net_create(const nbx_model_desc* model, int32_t device, nbx_net** out);
+/* A network of `count` (1..64) neuron populations (reference: DenseUnifiedNeuronRuntime::Initialize
+ * with one DenseNeuronModelSpec per range, DenseUnifiedNeuronRuntime.h:20-23; the legacy engine's
+ * several NeuronModel objects joined by Interconnections). Population k owns the neurons
+ * [pops[0].n + ... + pops[k-1].n, + pops[k].n) and keeps its own model, integrator and
+ * parameters; every population steps with the same dt, and synapses of the network join any two
+ * neurons (each synapse's type must be an input of its target's model). Arrivals are kept by
+ * synapse type (ring channel = type: 0 excitatory, 1 inhibitory, 2 NMDA, magnitudes as stored)
+ * and each population folds them into its model's inputs, so a population computes exactly what a
+ * network of its model alone computes from the same arrivals. State is x[i * nstate + j] with
+ * nstate the largest of the populations' (columns a model does not have read 0). Such networks run
+ * on the kernel and graph engines and the per-phase calls; dense all-to-all weights need one
+ * population. nbx_net_create(md, ...) is the one-population case. */
+int32_t nbx_net_create_populations(const nbx_model_desc* pops, int32_t count, int32_t device, nbx_net** out);
 int32_t nbx_net_destroy(nbx_net* net);
 
 /* Sparse synapses (reference: GpuPropagationRuntime::Initialize(layout)). */
@@ -219,6 +232,16 @@ int32_t nbx_net_run_propagation(nbx_net* net);
 /* Neuron update of the current step; optionally copies out the firing ids (ascending)
  * and count (reference: DenseUnifiedNeuronRuntime::Step output_firing_ids). */
 int32_t nbx_net_neuron_step(nbx_net* net, int32_t* ids_out, int32_t ids_cap, int32_t* count_out);
+/* Output neurons (reference: DenseUnifiedNeuronRuntime::ConfigureOutputNeuronMask,
+ * DenseUnifiedNeuronRuntime.h:26-27): mask[i] != 0 marks neuron i as an output of the network;
+ * NULL marks every neuron. The mask selects the output list of nbx_net_neuron_step_outputs; every
+ * firing is still delivered, recorded and committed. */
+int32_t nbx_net_set_output_mask(nbx_net* net, const uint8_t* mask);
+/* The neuron update of nbx_net_neuron_step, copying out the step's firings among the output
+ * neurons and (full_ids non-NULL) all of them, each ascending (reference: Step(common, t,
+ * output_firing_ids, full_firing_ids, export_full_firings_to_host)). */
+int32_t nbx_net_neuron_step_outputs(nbx_net* net, int32_t* out_ids, int32_t out_cap, int32_t* out_count,
+                                    int32_t* full_ids, int32_t full_cap, int32_t* full_count);
 /* Post-side STDP for the neurons that fired this step. */
 int32_t nbx_net_apply_post_learning(nbx_net* net);
 /* Push this step's firings into the delay ring and close the step. */
@@ -229,6 +252,77 @@ int32_t nbx_net_commit_firings(nbx_net* net);
  * firing list (fired in the update, or injected earlier) is rejected (NBX_ERR_INVALID, nothing
  * injected). */
 int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count);
+
+/* ---------------- interface channels (inputs from outside the network) ----------------
+ * GpuPropagationRuntime::SetInterfaceTargetNeurons / UploadInterfaceChannelValues /
+ * UploadInterfaceChannelValuesDevice / ApplyInterfaceStateVectors (GpuPropagationRuntime.h:103-111,
+ * GpuPropagationRuntime.cu:68-140,1431-1560). A record names its target neuron directly
+ * (target_neuron >= 0) or through an interface slot (target_neuron < 0: the neuron mapped to
+ * interface_slot); records whose target resolves to no neuron are skipped, as the reference does.
+ * Channels: NBX_PENDING_EXC / _INH / _NMDA add value * scale as an arrival of synapse type 0 / 1 / 2
+ * for the current step (int32 fixed point like every arrival; an inhibitory value is a magnitude);
+ * NBX_PENDING_CURRENT with overwrite set replaces the target's I_ext by value * scale (the
+ * reference's stateful current slots; a per-step current table of nbx_net_set_current still wins
+ * for the steps it covers); NBX_PENDING_MODULATORY and channels the target's model does not take
+ * are skipped. The current step is the one being run (after nbx_net_begin_step, before the neuron
+ * update) or, between steps, the next one. Records sharing a (target, channel) accumulate; an
+ * overwrite record of a conductance channel adds its value, since the reference's pending buffer
+ * holds nothing of the step when the interface is applied. Values whose fixed-point image could
+ * overflow the arrival sums are rejected (host records: NBX_ERR_INVALID, nothing applied; device
+ * copies: the value is dropped and the next call reports NBX_ERR_STATE). */
+enum { NBX_PENDING_EXC = 0, NBX_PENDING_INH = 1, NBX_PENDING_CURRENT = 2, NBX_PENDING_NMDA = 3,
+       NBX_PENDING_MODULATORY = 4 };
+typedef struct nbx_interface_value {   /* reference InterfaceChannelValue */
+    int32_t interface_slot, target_neuron;
+    uint8_t pending_channel, overwrite;
+    float value, scale;
+} nbx_interface_value;
+typedef struct nbx_interface_copy {    /* reference DeviceToDeviceInterfaceCopySpec */
+    int32_t interface_slot, target_neuron;
+    uint8_t pending_channel, overwrite;
+    int32_t source_index;
+    float scale;
+} nbx_interface_copy;
+/* Slot -> target neuron map (ids in [0, n), or -1 for an unmapped slot). */
+int32_t nbx_net_set_interface_targets(nbx_net* net, const int32_t* neuron_ids, int32_t count);
+/* Stages host records for nbx_net_apply_interface (the record count must equal the slot count
+ * when a slot map is set; 0 records clears the stage). A zero value is skipped, as the reference's
+ * kernel skips it; a current record must have overwrite set. */
+int32_t nbx_net_upload_interface_values(nbx_net* net, const nbx_interface_value* values, int32_t count);
+/* Applies the staged host records to the current step (they stay staged until replaced). */
+int32_t nbx_net_apply_interface(nbx_net* net);
+/* Applies device-to-device records at once: value = d_values[source_index] * scale, read on the
+ * device (d_values: device pointer); a zero value is skipped unless overwrite is set, and a current
+ * record must have overwrite set. */
+int32_t nbx_net_upload_interface_values_device(nbx_net* net, const float* d_values, const nbx_interface_copy* specs,
+                                               int32_t count);
+
+/* Arrivals from outside the network's synapses (reference: ProcessSpike calls of other models'
+ * and queues' spike groups on this model, which the legacy GPU models stage in AuxStateCPU,
+ * TimeDrivenLIF_Exponential_Decay_GPU_Interface.cu:50-58): adds weights[k] to neuron targets[k] as
+ * an arrival of synapse type types[k] (0 exc, 1 inh, 2 NMDA; magnitudes, signed for current-based
+ * Izhikevich) delivered at step t, exactly like a synapse's arrival at that step. t must lie in
+ * [current step, current step + slots - 1] (the delay ring's reach; the current step only before its
+ * neuron update). Weights whose fixed-point sum could overflow are rejected (NBX_ERR_INVALID). */
+int32_t nbx_net_add_arrivals(nbx_net* net, const int32_t* targets, const uint8_t* types, const float* weights,
+                             int32_t count, int32_t t);
+
+/* Back to the state after nbx_net_finalize with the synapses kept (reference:
+ * GpuPropagationRuntime::ResetState + DenseUnifiedNeuronRuntime::ResetState): neuron state and
+ * LastSpike to their initial values, delay ring, firing history and event count cleared, learning
+ * traces (STDP and rule records, eligibilities, cerebellar arrival histories) cleared, the step
+ * counter to 0. Weights are kept, as the reference keeps d_syn_weight_. */
+int32_t nbx_net_reset_state(nbx_net* net);
+
+/* Pending inputs of the current step as floats, channel-major pending[ch * n + i] =
+ * arrivals * 2^-shift (reference: DownloadPendingChannels, GpuPropagationRuntime.h:131-134). */
+int32_t nbx_net_download_pending(nbx_net* net, float* pending, int64_t count, int32_t* channel_count,
+                                 int32_t* channel_stride);
+
+/* Device float snapshot of the current step's pending arrivals, [ch][n] (the float pending pool of
+ * GetDeviceCommonBuffersView, DenseNeuronDeviceViews.h:6-13); the buffer belongs to the network
+ * and is overwritten by the next call. */
+int32_t nbx_net_pending_device_float(nbx_net* net, float** d_pending);
 
 /* Delay ring contents, int32 fixed point ring[slot][channel][n] (slot = step % slots). */
 int32_t nbx_net_ring_shape(const nbx_net* net, int32_t* slots, int32_t* channels);

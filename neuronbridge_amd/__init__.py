@@ -8,9 +8,9 @@ from . import _lib
 from ._lib import (EULER, RK4, LIF_VJ, LIF_EXP, IZH, IZH_CUR, LIF_EXP2, LIF_EXP3, POISSON, TRIGGER_RELAY,
                    ENGINE_KERNELS, ENGINE_GRAPH, ENGINE_PERSISTENT,
                    ENGINE_AUTO, NbxError, lib)
-from .network import ModelSpec, Net, Synapses, build_fixed_indegree
+from .network import ModelSpec, Net, PopulationsSpec, Synapses, build_fixed_indegree
 
-__all__ = ["EULER", "RK4", "LIF_VJ", "LIF_EXP", "IZH", "IZH_CUR", "LIF_EXP2", "LIF_EXP3", "POISSON",
+__all__ = ["PopulationsSpec", "EULER", "RK4", "LIF_VJ", "LIF_EXP", "IZH", "IZH_CUR", "LIF_EXP2", "LIF_EXP3", "POISSON",
            "TRIGGER_RELAY", "ENGINE_KERNELS", "ENGINE_GRAPH",
            "ENGINE_PERSISTENT", "ENGINE_AUTO", "NbxError", "lib", "ModelSpec", "Net", "Synapses",
            "build_fixed_indegree"]

@@ -122,6 +122,18 @@ class MonitorDesc(C.Structure):
                 ("record_pending_channels", C.c_int32), ("neuron_ids", _i32p), ("neuron_count", C.c_int32)]
 
 
+class InterfaceValue(C.Structure):
+    _fields_ = [("interface_slot", C.c_int32), ("target_neuron", C.c_int32), ("pending_channel", C.c_uint8),
+                ("overwrite", C.c_uint8), ("value", C.c_float), ("scale", C.c_float)]
+
+
+class InterfaceCopy(C.Structure):
+    _fields_ = [("interface_slot", C.c_int32), ("target_neuron", C.c_int32), ("pending_channel", C.c_uint8),
+                ("overwrite", C.c_uint8), ("source_index", C.c_int32), ("scale", C.c_float)]
+
+
+PENDING_EXC, PENDING_INH, PENDING_CURRENT, PENDING_NMDA, PENDING_MODULATORY = 0, 1, 2, 3, 4
+
 _NET = _vp
 # name -> argtypes (all return int32 status unless listed in _RESTYPES)
 SIGNATURES = {
@@ -129,6 +141,17 @@ SIGNATURES = {
     "nbx_version": [],
     "nbx_device_count": [],
     "nbx_net_create": [C.POINTER(ModelDesc), C.c_int32, C.POINTER(_vp)],
+    "nbx_net_create_populations": [C.POINTER(ModelDesc), C.c_int32, C.c_int32, C.POINTER(_vp)],
+    "nbx_net_set_output_mask": [_NET, _u8p],
+    "nbx_net_neuron_step_outputs": [_NET, _i32p, C.c_int32, _i32p, _i32p, C.c_int32, _i32p],
+    "nbx_net_set_interface_targets": [_NET, _i32p, C.c_int32],
+    "nbx_net_upload_interface_values": [_NET, C.POINTER(InterfaceValue), C.c_int32],
+    "nbx_net_apply_interface": [_NET],
+    "nbx_net_upload_interface_values_device": [_NET, _vp, C.POINTER(InterfaceCopy), C.c_int32],
+    "nbx_net_add_arrivals": [_NET, _i32p, _u8p, _f32p, C.c_int32, C.c_int32],
+    "nbx_net_reset_state": [_NET],
+    "nbx_net_download_pending": [_NET, _f32p, C.c_int64, _i32p, _i32p],
+    "nbx_net_pending_device_float": [_NET, C.POINTER(_vp)],
     "nbx_net_destroy": [_NET],
     "nbx_net_set_sparse": [_NET, C.POINTER(SparseDesc)],
     "nbx_net_set_dense": [_NET, _f32p, C.c_int32],

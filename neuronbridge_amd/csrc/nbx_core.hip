@@ -56,18 +56,72 @@ int32_t model_nstate(int32_t model) {
     }
 }
 
-// Device array behind each column of the reference's AoS state (null: a column this build does
-// not keep, read as 0; the Izhikevich NMDA conductance)
-std::vector<float*> state_columns(nbx_net* net) {
-    switch (net->md.model) {
+// Device array behind each column of the reference's AoS state of `model` (null: a column this
+// build does not keep, read as 0)
+std::vector<float*> model_columns(nbx_net* net, int32_t model) {
+    switch (model) {
         case NBX_MODEL_LIF_VJ: return {net->d_v, net->d_iext};
         case NBX_MODEL_LIF_EXP: return {net->d_v, net->d_s1, net->d_iext};
         case NBX_MODEL_LIF_EXP2: return {net->d_v, net->d_ga, net->d_gg, net->d_iext};
         case NBX_MODEL_LIF_EXP3: return {net->d_v, net->d_ga, net->d_gg, net->d_s1, net->d_iext};
         case NBX_MODEL_POISSON: return {net->d_v, net->d_iext};
         case NBX_MODEL_TRIGGER_RELAY: return {net->d_v};
+        case NBX_MODEL_IZH: return {net->d_v, net->d_s1, net->d_ga, net->d_gg, net->d_g3, net->d_iext};
         default: return {net->d_v, net->d_s1, net->d_ga, net->d_gg, nullptr, net->d_iext};
     }
+}
+
+// Parameters of one population as the kernels take them.
+ModelParams make_params(const nbx_model_desc* md, int32_t begin) {
+    ModelParams P{};
+    P.model = md->model;
+    P.integrator = md->integrator;
+    P.n = md->n;
+    P.begin = begin;
+    P.t_ref = md->t_ref;
+    P.dt = md->dt;
+    P.hdt = md->dt * 0.5f;
+    P.dt6 = md->dt / 6.0f;
+    P.v_rest = md->v_rest;
+    P.tau = md->tau;
+    P.v_th = md->v_th;
+    P.r = md->r;
+    P.v_reset = md->v_reset;
+    P.e_rev = md->e_rev;
+    P.izh_v_th = md->izh_v_th;
+    P.izh_r = md->izh_r;
+    P.e_ampa = md->e_ampa;
+    P.e_gaba = md->e_gaba;
+    P.g_decay = decay_factor(md->dt, md->g_tau);
+    P.ampa_decay = decay_factor(md->dt, md->ampa_tau);
+    P.gaba_decay = decay_factor(md->dt, md->gaba_tau);
+    // Izhikevich NMDA: the single-entry lookup exp(-dt * (1/nmda_tau))
+    // (TimeDrivenIzhikevic_Exponential_Decay.cpp:32-39)
+    P.nmda_decay = md->model == NBX_MODEL_IZH ? decay_factor(md->dt, md->nmda_tau) : 1.0f;
+    if (md->model == NBX_MODEL_LIF_EXP3) {
+        P.ampa_decay = decay_factor_div(md->dt, md->ampa_tau);
+        P.gaba_decay = decay_factor_div(md->dt, md->gaba_tau);
+        P.nmda_decay = decay_factor_div(md->dt, md->nmda_tau);
+    }
+    P.rate_bias = md->rate_bias_hz;
+    P.rate_gain = md->rate_gain_hz;
+    P.cond = 0;
+    return P;
+}
+
+const char* model_desc_error(const nbx_model_desc* md) {
+    if (!(md->model >= NBX_MODEL_LIF_VJ && md->model <= NBX_MODEL_TRIGGER_RELAY)) return "unknown model id";
+    if (!(md->integrator == NBX_EULER || md->integrator == NBX_RK4)) return "unknown integrator";
+    if (!(md->n > 0 && md->n < kMaxNeurons)) return "neuron count must be in [1, 2^24)";
+    if (!(md->dt > 0.0f)) return "dt must be positive";
+    const bool izh = md->model == NBX_MODEL_IZH || md->model == NBX_MODEL_IZH_CUR;
+    if (izh && !(md->izh_a && md->izh_b && md->izh_c && md->izh_d)) return "Izhikevich models need per-neuron a,b,c,d";
+    const bool lif = md->model == NBX_MODEL_LIF_VJ || md->model == NBX_MODEL_LIF_EXP ||
+                     md->model == NBX_MODEL_LIF_EXP2 || md->model == NBX_MODEL_LIF_EXP3;
+    if (lif && md->tau == 0.0f) return "LIF tau must be non-zero";
+    if (md->model == NBX_MODEL_LIF_EXP3 && !(md->ampa_tau != 0.0f && md->gaba_tau != 0.0f && md->nmda_tau != 0.0f))
+        return "LIF_EXP3 conductance time constants must be non-zero";
+    return nullptr;
 }
 
 // Trace decay tables by integer step delta: exp(-(delta * dt) * (1/tau)) in float, the arithmetic
@@ -185,6 +239,7 @@ int32_t finalize_rules(nbx_net* net) {
             conn[j] = net->h_src_index.size() == S ? (int32_t)net->h_src_index[j] : (int32_t)j;
             if (net->h_rule[j] >= 0 && rd[(size_t)net->h_rule[j]].kind == NBX_RULE_CEREBELLAR) cidx[j] = rows++;
         }
+        net->crows = rows;
         NBX_PASS(net->upload(&net->d_cidx, cidx.data(), S));
         NBX_PASS(net->upload(&net->d_conn_index, conn.data(), S));
         std::vector<int32_t> zero((size_t)std::max(rows, 1), 0);
@@ -195,6 +250,36 @@ int32_t finalize_rules(nbx_net* net) {
     NBX_PASS(net->upload(&net->d_rin_syn, rin_syn.data(), rin_syn.size()));
     NBX_PASS(net->upload(&net->d_tin_ptr, tin_ptr.data(), tin_ptr.size()));
     NBX_PASS(net->upload(&net->d_tin_syn, tin_syn.data(), tin_syn.size()));
+    return NBX_OK;
+}
+
+// Per-synapse learning records in their initial state (traces 0, last update 0, the weight bounds):
+// one 32-byte record per synapse for networks with learning rules (STDP_State traces shared with
+// STDP), one 16-byte head for pair STDP alone. alloc: allocate (finalize), else overwrite (reset).
+int32_t upload_learning_records(nbx_net* net, bool alloc) {
+    const size_t S = (size_t)net->S;
+    if (!net->rules.empty()) {
+        std::vector<SynTrace> tr(S);
+        for (size_t j = 0; j < S; ++j)
+            tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j], 0.0f,
+                             net->h_maxw.empty() ? 0.0f : net->h_maxw[j], 0, 0};
+        if (alloc) {
+            NBX_PASS(net->upload(&net->d_trace, tr.data(), S));
+            net->d_heads = reinterpret_cast<TraceHead*>(net->d_trace);
+            net->head_stride = (int32_t)(sizeof(SynTrace) / sizeof(TraceHead));
+        } else {
+            NBX_HIP_TRY(hipMemcpy(net->d_trace, tr.data(), sizeof(SynTrace) * S, hipMemcpyHostToDevice));
+        }
+    } else if (net->stdp) {
+        std::vector<TraceHead> hd(S);
+        for (size_t j = 0; j < S; ++j) hd[j] = TraceHead{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j]};
+        if (alloc) {
+            NBX_PASS(net->upload(&net->d_heads, hd.data(), S));
+            net->head_stride = 1;
+        } else {
+            NBX_HIP_TRY(hipMemcpy(net->d_heads, hd.data(), sizeof(TraceHead) * S, hipMemcpyHostToDevice));
+        }
+    }
     return NBX_OK;
 }
 
@@ -235,6 +320,11 @@ int32_t check_device_error(nbx_net* net) {
     if (e & kErrHistory) {
         set_error("cerebellar arrival history overflow: an arrival still inside the kernel window was "
                   "overwritten (raise nbx_rule_desc.history)");
+        return NBX_ERR_STATE;
+    }
+    if (e & kErrInput) {
+        NBX_PASS(clear_error_bits(net, kErrInput));
+        set_error("device interface values could overflow the int32 arrival sums; those values were dropped");
         return NBX_ERR_STATE;
     }
     if (e & kErrRaster) {  // an argument error of one replay call: reported once
@@ -386,66 +476,72 @@ int32_t nbx_device_count(void) {
 }
 
 int32_t nbx_net_create(const nbx_model_desc* md, int32_t device, nbx_net** out) {
-    NBX_CHECK(md && out, "null argument");
+    return nbx_net_create_populations(md, 1, device, out);
+}
+
+int32_t nbx_net_create_populations(const nbx_model_desc* pops, int32_t count, int32_t device, nbx_net** out) {
+    NBX_CHECK(pops && out, "null argument");
     *out = nullptr;
-    NBX_CHECK(md->model >= NBX_MODEL_LIF_VJ && md->model <= NBX_MODEL_TRIGGER_RELAY, "unknown model id");
-    NBX_CHECK(md->integrator == NBX_EULER || md->integrator == NBX_RK4, "unknown integrator");
-    NBX_CHECK(md->n > 0 && md->n < kMaxNeurons, "neuron count must be in [1, 2^24)");
-    NBX_CHECK(md->dt > 0.0f, "dt must be positive");
-    const bool izh = md->model == NBX_MODEL_IZH || md->model == NBX_MODEL_IZH_CUR;
-    NBX_CHECK(!izh || (md->izh_a && md->izh_b && md->izh_c && md->izh_d), "Izhikevich models need per-neuron a,b,c,d");
-    const bool lif = md->model == NBX_MODEL_LIF_VJ || md->model == NBX_MODEL_LIF_EXP ||
-                     md->model == NBX_MODEL_LIF_EXP2 || md->model == NBX_MODEL_LIF_EXP3;
-    NBX_CHECK(!lif || md->tau != 0.0f, "LIF tau must be non-zero");
-    NBX_CHECK(md->model != NBX_MODEL_LIF_EXP3 ||
-                  (md->ampa_tau != 0.0f && md->gaba_tau != 0.0f && md->nmda_tau != 0.0f),
-              "LIF_EXP3 conductance time constants must be non-zero");
+    NBX_CHECK(count >= 1 && count <= kMaxPops, "population count must be in [1, 64]");
+    int64_t total = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        const char* why = model_desc_error(&pops[k]);
+        if (why) {
+            set_error(count > 1 ? "population " + std::to_string(k) + ": " + why : std::string(why));
+            return NBX_ERR_INVALID;
+        }
+        NBX_CHECK(pops[k].dt == pops[0].dt, "every population must step with the same dt");
+        total += pops[k].n;
+    }
+    NBX_CHECK(total < kMaxNeurons, "neuron count must be in [1, 2^24)");
     int ndev = 0;
     NBX_HIP_TRY(hipGetDeviceCount(&ndev));
     NBX_CHECK(device >= 0 && device < ndev, "device index out of range");
     std::unique_ptr<nbx_net> net(new nbx_net());
     net->device = device;
-    net->md = *md;
-    net->md.izh_a = net->md.izh_b = net->md.izh_c = net->md.izh_d = nullptr;
-    net->n = md->n;
-    net->ns = model_nstate(md->model);
-    net->nch = model_channels(md->model);
-    if (izh) {
-        net->h_a.assign(md->izh_a, md->izh_a + md->n);
-        net->h_b.assign(md->izh_b, md->izh_b + md->n);
-        net->h_c.assign(md->izh_c, md->izh_c + md->n);
-        net->h_d.assign(md->izh_d, md->izh_d + md->n);
+    net->n = (int32_t)total;
+    bool any_izh = false;
+    for (int32_t k = 0; k < count; ++k) any_izh |= pops[k].model == NBX_MODEL_IZH || pops[k].model == NBX_MODEL_IZH_CUR;
+    if (any_izh) {
+        net->h_a.assign((size_t)total, 0.0f);
+        net->h_b.assign((size_t)total, 0.0f);
+        net->h_c.assign((size_t)total, 0.0f);
+        net->h_d.assign((size_t)total, 0.0f);
     }
-    ModelParams& P = net->P;
-    P.model = md->model;
-    P.integrator = md->integrator;
-    P.n = md->n;
-    P.t_ref = md->t_ref;
-    P.dt = md->dt;
-    P.hdt = md->dt * 0.5f;
-    P.dt6 = md->dt / 6.0f;
-    P.v_rest = md->v_rest;
-    P.tau = md->tau;
-    P.v_th = md->v_th;
-    P.r = md->r;
-    P.v_reset = md->v_reset;
-    P.e_rev = md->e_rev;
-    P.izh_v_th = md->izh_v_th;
-    P.izh_r = md->izh_r;
-    P.e_ampa = md->e_ampa;
-    P.e_gaba = md->e_gaba;
-    P.g_decay = decay_factor(md->dt, md->g_tau);
-    P.ampa_decay = decay_factor(md->dt, md->ampa_tau);
-    P.gaba_decay = decay_factor(md->dt, md->gaba_tau);
-    P.nmda_decay = 1.0f;
-    if (md->model == NBX_MODEL_LIF_EXP3) {
-        P.ampa_decay = decay_factor_div(md->dt, md->ampa_tau);
-        P.gaba_decay = decay_factor_div(md->dt, md->gaba_tau);
-        P.nmda_decay = decay_factor_div(md->dt, md->nmda_tau);
+    int32_t begin = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        const nbx_model_desc* md = &pops[k];
+        NetPop pp;
+        pp.begin = begin;
+        pp.md = *md;
+        pp.md.izh_a = pp.md.izh_b = pp.md.izh_c = pp.md.izh_d = nullptr;
+        pp.P = make_params(md, begin);
+        if (md->model == NBX_MODEL_IZH || md->model == NBX_MODEL_IZH_CUR) {
+            std::copy(md->izh_a, md->izh_a + md->n, net->h_a.begin() + begin);
+            std::copy(md->izh_b, md->izh_b + md->n, net->h_b.begin() + begin);
+            std::copy(md->izh_c, md->izh_c + md->n, net->h_c.begin() + begin);
+            std::copy(md->izh_d, md->izh_d + md->n, net->h_d.begin() + begin);
+        }
+        net->pops.push_back(pp);
+        begin += md->n;
     }
-    P.rate_bias = md->rate_bias_hz;
-    P.rate_gain = md->rate_gain_hz;
-    P.cond = 0;
+    net->md = net->pops[0].md;
+    net->P = net->pops[0].P;
+    int32_t ns = 0;
+    for (const NetPop& pp : net->pops) ns = std::max(ns, model_nstate(pp.md.model));
+    net->ns = ns;
+    if (net->multi()) {
+        net->md.n = net->n;
+        net->P.model = kModelTyped;
+        net->P.n = net->n;
+        net->nch = model_channels(kModelTyped);
+        net->h_pop_of.resize((size_t)total);
+        for (int32_t k = 0; k < count; ++k)
+            std::fill(net->h_pop_of.begin() + net->pops[(size_t)k].begin,
+                      net->h_pop_of.begin() + net->pops[(size_t)k].begin + net->pops[(size_t)k].md.n, (uint8_t)k);
+    } else {
+        net->nch = model_channels(net->md.model);
+    }
     NBX_HIP_TRY(hipSetDevice(device));
     NBX_HIP_TRY(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
     *out = net.release();
@@ -488,10 +584,10 @@ int32_t nbx_net_set_sparse(nbx_net* net, const nbx_sparse_desc* syn) {
             const int32_t d = syn->delay[j], post = syn->post[j], ty = syn->type[j];
             NBX_CHECK(d >= 1 && d <= syn->max_delay, "synapse delay outside [1, max_delay]");
             NBX_CHECK(post >= 0 && post < n, "synapse post index out of range");
-            NBX_CHECK(ty >= 0 && ty <= model_max_type(net->md.model),
-                      net->md.model == NBX_MODEL_LIF_EXP3
-                          ? "synapse type must be 0 (AMPA), 1 (GABA) or 2 (NMDA)"
-                          : "synapse type must be 0 (excitatory) or 1 (inhibitory)");
+            const int32_t tm = net->model_of(post);
+            NBX_CHECK(ty >= 0 && ty <= model_max_type(tm),
+                      model_max_type(tm) == 2 ? "synapse type must be 0 (AMPA), 1 (GABA) or 2 (NMDA)"
+                                              : "synapse type must be 0 (excitatory) or 1 (inhibitory)");
             NBX_CHECK(d >= prev_d, "each row must be sorted by delay");
             prev_d = d;
             net->h_syn[(size_t)j] = pack_syn(post, d, ty);
@@ -507,6 +603,7 @@ int32_t nbx_net_set_dense(nbx_net* net, const float* wt, int32_t delay) {
     NBX_CHECK(net && wt, "null argument");
     NBX_CHECK(!net->finalized, "dense weights must be set before nbx_net_finalize");
     NBX_CHECK(delay >= 1 && delay <= kMaxDelay, "dense delay must be in [1, 63]");
+    NBX_CHECK(!net->multi(), "dense all-to-all weights need a one-population network (use sparse synapses)");
     net->h_wt.assign(wt, wt + (size_t)net->n * net->n);
     net->dense_delay = delay;
     net->max_delay = std::max(net->max_delay, delay);
@@ -656,6 +753,12 @@ int32_t nbx_net_finalize(nbx_net* net) {
     // at least 3 so the slot being filled, the one being read and the one being freed differ
     net->D = std::max(net->max_delay + 1, 3);
     net->pull_mode = net->stdp || !net->rules.empty();
+    if (net->multi())  // each synapse's type must be one its target's model takes (.nbnet tables too)
+        for (int64_t j = 0; j < net->S; ++j) {
+            const uint32_t e = net->h_syn[(size_t)j];
+            NBX_CHECK(syn_type(e) <= model_max_type(net->model_of(syn_post(e))),
+                      "a synapse type is not an input of its target population's model");
+        }
     NBX_CHECK(!(net->stdp && !net->has_sparse), "STDP needs sparse synapses");
     NBX_CHECK(net->rules.empty() || (net->has_sparse && net->h_rule.size() == (size_t)net->S),
               "learning rules must be set after the synapses they refer to");
@@ -674,28 +777,42 @@ int32_t nbx_net_finalize(nbx_net* net) {
     }
     net->shift = net->shift_req ? net->shift_req : auto_shift(net);
     net->words = (n + 31) / 32;
-    // conductance inputs present (the reference models' CheckType flags)
+    // conductance inputs present in each population (the reference models' CheckType flags): the
+    // types of the synapses into it, dense weight signs and the (excitatory) background
     {
-        int32_t cond = net->pois ? kCondExc : 0;
-        for (int64_t j = 0; j < net->S; ++j) cond |= 1 << syn_type(net->h_syn[(size_t)j]);
+        std::vector<int32_t> cond(net->pops.size(), net->pois ? kCondExc : 0);
+        for (int64_t j = 0; j < net->S; ++j) {
+            const uint32_t e = net->h_syn[(size_t)j];
+            cond[net->multi() ? net->h_pop_of[(size_t)syn_post(e)] : 0] |= 1 << syn_type(e);
+        }
         if (net->has_dense)
-            for (float w : net->h_wt) cond |= (w < 0.0f) ? kCondInh : kCondExc;
-        net->P.cond = cond & (kCondExc | kCondInh | kCondNmda);
+            for (float w : net->h_wt) cond[0] |= (w < 0.0f) ? kCondInh : kCondExc;
+        for (size_t k = 0; k < net->pops.size(); ++k)
+            net->pops[k].P.cond = cond[k] & (kCondExc | kCondInh | kCondNmda);
+        if (!net->multi()) net->P.cond = net->pops[0].P.cond;
     }
+    // Izhikevich NMDA synapses (type 2): a third ring channel and the g_nmda state array
+    bool izh_nmda = false;
+    for (const NetPop& pp : net->pops) izh_nmda |= pp.md.model == NBX_MODEL_IZH && (pp.P.cond & kCondNmda);
+    if (!net->multi() && izh_nmda) net->nch = 3;
     // state (SoA) initialised like Neuron_State_Vector::InitNeuronState with sigma 0
     std::vector<float> v((size_t)n), s1((size_t)n, 0.0f), zero((size_t)n, 0.0f);
     std::vector<int32_t> last((size_t)n, 10000);
-    for (int32_t i = 0; i < n; ++i) {
-        switch (net->md.model) {
-            case NBX_MODEL_LIF_VJ:  // TimeDrivenLIF_Voltage_jump.cpp:37
-            case NBX_MODEL_LIF_EXP3: v[(size_t)i] = net->md.v_reset; break;  // ..._triple.cpp:71
-            case NBX_MODEL_POISSON:  // PoissonRate.cpp:45
-            case NBX_MODEL_TRIGGER_RELAY: v[(size_t)i] = 0.0f; break;
-            default: v[(size_t)i] = net->md.v_rest;
+    for (const NetPop& pp : net->pops) {
+        for (int32_t i = pp.begin; i < pp.begin + pp.md.n; ++i) {
+            switch (pp.md.model) {
+                case NBX_MODEL_LIF_VJ:  // TimeDrivenLIF_Voltage_jump.cpp:37
+                case NBX_MODEL_LIF_EXP3: v[(size_t)i] = pp.md.v_reset; break;  // ..._triple.cpp:71
+                case NBX_MODEL_POISSON:  // PoissonRate.cpp:45
+                case NBX_MODEL_TRIGGER_RELAY: v[(size_t)i] = 0.0f; break;
+                default: v[(size_t)i] = pp.md.v_rest;
+            }
+            if (pp.md.model == NBX_MODEL_IZH || pp.md.model == NBX_MODEL_IZH_CUR)
+                s1[(size_t)i] = net->h_b[(size_t)i] * pp.md.v_rest;
         }
-        if (net->md.model == NBX_MODEL_IZH || net->md.model == NBX_MODEL_IZH_CUR)
-            s1[(size_t)i] = net->h_b[(size_t)i] * net->md.v_rest;
     }
+    net->h_init_v = v;
+    net->h_init_s1 = s1;
 #define NBX_RC(x)            \
     do {                     \
         rc = (x);            \
@@ -707,6 +824,13 @@ int32_t nbx_net_finalize(nbx_net* net) {
     NBX_RC(net->upload(&net->d_gg, zero.data(), (size_t)n));
     NBX_RC(net->upload(&net->d_iext, zero.data(), (size_t)n));
     NBX_RC(net->upload(&net->d_last, last.data(), (size_t)n));
+    if (izh_nmda) NBX_RC(net->upload(&net->d_g3, zero.data(), (size_t)n));
+    if (net->multi()) {
+        std::vector<ModelParams> pp;
+        for (const NetPop& q : net->pops) pp.push_back(q.P);
+        NBX_RC(net->upload(&net->d_pops, pp.data(), pp.size()));
+        NBX_RC(net->upload(&net->d_pop_of, net->h_pop_of.data(), net->h_pop_of.size()));
+    }
     if (!net->h_a.empty()) {
         NBX_RC(net->upload(&net->d_a, net->h_a.data(), (size_t)n));
         NBX_RC(net->upload(&net->d_b, net->h_b.data(), (size_t)n));
@@ -752,22 +876,7 @@ int32_t nbx_net_finalize(nbx_net* net) {
     if (net->has_dense) NBX_RC(net->upload(&net->d_wt, net->h_wt.data(), net->h_wt.size()));
     if (net->pois) NBX_RC(net->upload(&net->d_pois_tab, net->pois_tab.data(), net->pois_tab.size()));
     if (net->cur_n) NBX_RC(net->upload(&net->d_cur, net->h_cur.data(), net->h_cur.size()));
-    if (!net->rules.empty()) {  // per-synapse learning records (STDP_State traces shared with STDP)
-        const size_t S = (size_t)net->S;
-        std::vector<SynTrace> tr(S);
-        for (size_t j = 0; j < S; ++j)
-            tr[j] = SynTrace{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j], 0.0f,
-                             net->h_maxw.empty() ? 0.0f : net->h_maxw[j], 0, 0};
-        NBX_RC(net->upload(&net->d_trace, tr.data(), S));
-        net->d_heads = reinterpret_cast<TraceHead*>(net->d_trace);
-        net->head_stride = (int32_t)(sizeof(SynTrace) / sizeof(TraceHead));
-    } else if (net->stdp) {  // pair STDP only: 16-byte records
-        const size_t S = (size_t)net->S;
-        std::vector<TraceHead> hd(S);
-        for (size_t j = 0; j < S; ++j) hd[j] = TraceHead{0.0f, 0.0f, 0, net->h_smaxw.empty() ? net->sd.w_max : net->h_smaxw[j]};
-        NBX_RC(net->upload(&net->d_heads, hd.data(), S));
-        net->head_stride = 1;
-    }
+    NBX_RC(upload_learning_records(net, true));
     if (!net->rules.empty()) NBX_RC(finalize_rules(net));
     if (net->stdp) {
         NBX_CHECK(!net->h_plastic.empty(), "STDP needs per-synapse plastic flags");
@@ -806,15 +915,18 @@ int32_t nbx_net_set_state(nbx_net* net, const float* x, const int32_t* last) {
     NBX_CHECK(x && last, "null argument");
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-    const int32_t n = net->n, ns = net->ns;
-    const std::vector<float*> dst = state_columns(net);
-    std::vector<float> col((size_t)n);
-    for (int j = 0; j < ns; ++j) {
-        if (!dst[(size_t)j]) continue;
-        for (int32_t i = 0; i < n; ++i) col[(size_t)i] = x[(size_t)i * ns + j];
-        NBX_HIP_TRY(hipMemcpy(dst[(size_t)j], col.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    const int32_t ns = net->ns;
+    std::vector<float> col;
+    for (const NetPop& pp : net->pops) {  // each population's columns (several: x[i * ns + j], ns the largest)
+        const std::vector<float*> dst = model_columns(net, pp.md.model);
+        col.resize((size_t)pp.md.n);
+        for (size_t j = 0; j < dst.size(); ++j) {
+            if (!dst[j]) continue;
+            for (int32_t i = 0; i < pp.md.n; ++i) col[(size_t)i] = x[(size_t)(pp.begin + i) * ns + j];
+            NBX_HIP_TRY(hipMemcpy(dst[j] + pp.begin, col.data(), sizeof(float) * pp.md.n, hipMemcpyHostToDevice));
+        }
     }
-    NBX_HIP_TRY(hipMemcpy(net->d_last, last, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    NBX_HIP_TRY(hipMemcpy(net->d_last, last, sizeof(int32_t) * net->n, hipMemcpyHostToDevice));
     return NBX_OK;
 }
 
@@ -824,18 +936,19 @@ int32_t nbx_net_get_state(nbx_net* net, float* x, int32_t* last) {
     NBX_CHECK(x && last, "null argument");
     NBX_RC(device_scope(net));
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
-    const int32_t n = net->n, ns = net->ns;
-    std::vector<float> col((size_t)n);
-    auto get = [&](const float* d, int j) -> hipError_t {
-        hipError_t e = hipMemcpy(col.data(), d, sizeof(float) * n, hipMemcpyDeviceToHost);
-        for (int32_t i = 0; i < n; ++i) x[(size_t)i * ns + j] = col[(size_t)i];
-        return e;
-    };
-    for (size_t k = 0; k < (size_t)n * ns; ++k) x[k] = 0.0f;
-    const std::vector<float*> src = state_columns(net);
-    for (int j = 0; j < ns; ++j)
-        if (src[(size_t)j]) NBX_HIP_TRY(get(src[(size_t)j], j));
-    NBX_HIP_TRY(hipMemcpy(last, net->d_last, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    const int32_t ns = net->ns;
+    for (size_t k = 0; k < (size_t)net->n * ns; ++k) x[k] = 0.0f;
+    std::vector<float> col;
+    for (const NetPop& pp : net->pops) {
+        const std::vector<float*> src = model_columns(net, pp.md.model);
+        col.resize((size_t)pp.md.n);
+        for (size_t j = 0; j < src.size(); ++j) {
+            if (!src[j]) continue;
+            NBX_HIP_TRY(hipMemcpy(col.data(), src[j] + pp.begin, sizeof(float) * pp.md.n, hipMemcpyDeviceToHost));
+            for (int32_t i = 0; i < pp.md.n; ++i) x[(size_t)(pp.begin + i) * ns + j] = col[(size_t)i];
+        }
+    }
+    NBX_HIP_TRY(hipMemcpy(last, net->d_last, sizeof(int32_t) * net->n, hipMemcpyDeviceToHost));
     return NBX_OK;
 }
 
@@ -1066,6 +1179,298 @@ int32_t nbx_net_commit_firings(nbx_net* net) {
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     net->t += 1;
     net->phase = 0;
+    return NBX_OK;
+}
+
+int32_t nbx_net_set_output_mask(nbx_net* net, const uint8_t* mask) {
+    NBX_CHECK(net, "null argument");
+    if (!mask) net->out_mask.clear();
+    else net->out_mask.assign(mask, mask + net->n);
+    return NBX_OK;
+}
+
+int32_t nbx_net_neuron_step_outputs(nbx_net* net, int32_t* out_ids, int32_t out_cap, int32_t* out_count,
+                                    int32_t* full_ids, int32_t full_cap, int32_t* full_count) {
+    std::vector<int32_t> ids((size_t)std::max(net ? net->n : 1, 1));
+    int32_t cnt = 0;
+    int32_t rc = nbx_net_neuron_step(net, ids.data(), (int32_t)ids.size(), &cnt);
+    if (rc) return rc;
+    if (full_count) *full_count = cnt;
+    if (full_ids) {
+        NBX_CHECK(full_cap >= cnt, "full firing id buffer too small");
+        std::copy(ids.begin(), ids.begin() + cnt, full_ids);
+    }
+    int32_t k = 0;
+    for (int32_t j = 0; j < cnt; ++j) {
+        if (!net->out_mask.empty() && !net->out_mask[(size_t)ids[(size_t)j]]) continue;
+        if (out_ids) {
+            NBX_CHECK(k < out_cap, "output firing id buffer too small");
+            out_ids[k] = ids[(size_t)j];
+        }
+        ++k;
+    }
+    if (out_count) *out_count = k;
+    return NBX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Room left in every int32 arrival sum after the largest synaptic input (the fixed-point shift
+// keeps B * 2^shift + 1e6 below 2^31): the bound on the fixed-point image of a step's external
+// inputs into one neuron and channel.
+double input_headroom(const nbx_net* net) {
+    std::vector<float> w;
+    const float* wp = nullptr;
+    if (net->has_sparse) {
+        if (net->pull_mode) {
+            w.resize((size_t)net->S);
+            if (hipMemcpy(w.data(), net->d_w, sizeof(float) * w.size(), hipMemcpyDeviceToHost) != hipSuccess) return 0.0;
+            wp = w.data();
+        } else {
+            wp = net->h_w.data();
+        }
+    }
+    const double B = arrival_bound(net, wp);
+    return std::max(0.0, 2147483647.0 - 1e6 - B * std::ldexp(1.0, net->shift));
+}
+
+// Target of an interface record: the direct target, else the neuron of its slot; -1 for none.
+int32_t resolve_target(const nbx_net* net, int32_t slot, int32_t target) {
+    if (target >= 0) return target < net->n ? target : -1;
+    if (slot >= 0 && slot < (int32_t)net->iface_targets.size()) return net->iface_targets[(size_t)slot];
+    return -1;
+}
+
+// Input kind of a pending channel for a target neuron: synapse type 0..2, kInputCurrent, or -1
+// when the target's model takes no such input.
+int32_t channel_kind(const nbx_net* net, int32_t channel, int32_t target) {
+    int32_t type;
+    switch (channel) {
+        case NBX_PENDING_EXC: type = 0; break;
+        case NBX_PENDING_INH: type = 1; break;
+        case NBX_PENDING_NMDA: type = 2; break;
+        case NBX_PENDING_CURRENT: return net->model_of(target) == NBX_MODEL_TRIGGER_RELAY ? -1 : kInputCurrent;
+        default: return -1;
+    }
+    if (type > model_max_type(net->model_of(target))) return -1;
+    if (type >= net->nch && arrival_channel(net->P.model, type) >= net->nch) return -1;
+    return type;
+}
+
+int32_t ensure_input_stage(nbx_net* net, int64_t count) {
+    if (count <= net->inputs_cap) return NBX_OK;
+    if (net->d_inputs) {
+        auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)net->d_inputs);
+        if (it != net->allocs.end()) net->allocs.erase(it);
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        NBX_HIP_TRY(hipFree(net->d_inputs));
+        net->d_inputs = nullptr;
+        net->inputs_cap = 0;
+    }
+    NBX_PASS(net->alloc(&net->d_inputs, (size_t)count));
+    net->inputs_cap = count;
+    return NBX_OK;
+}
+
+// The step external inputs go to now: the step being run before its update, else the next one.
+int32_t input_step(const nbx_net* net, int32_t* t) {
+    if (net->phase >= 3) {
+        set_error("external inputs of a step come before its neuron update");
+        return NBX_ERR_STATE;
+    }
+    *t = net->t;
+    return NBX_OK;
+}
+
+// Checks host records' fixed-point images against the headroom (per target and channel) and
+// launches them for step t.
+int32_t run_inputs(nbx_net* net, const std::vector<InputRec>& recs, int32_t t) {
+    if (recs.empty()) return NBX_OK;
+    const double H = input_headroom(net);
+    const double scale = std::ldexp(1.0, net->shift);
+    std::vector<std::pair<int64_t, double>> sums;
+    sums.reserve(recs.size());
+    for (const InputRec& r : recs)
+        if (r.kind != kInputCurrent && r.src < 0)
+            sums.push_back({(int64_t)r.target * 4 + r.kind, std::fabs((double)r.value * scale) + 0.5});
+    std::sort(sums.begin(), sums.end());
+    for (size_t a = 0; a < sums.size();) {
+        double tot = 0.0;
+        size_t b = a;
+        for (; b < sums.size() && sums[b].first == sums[a].first; ++b) tot += sums[b].second;
+        if (tot > H) {
+            set_error("external inputs into neuron " + std::to_string(sums[a].first / 4) +
+                      " can overflow the int32 arrival sums at fixed shift " + std::to_string(net->shift));
+            return NBX_ERR_INVALID;
+        }
+        a = b;
+    }
+    NBX_PASS(ensure_input_stage(net, (int64_t)recs.size()));
+    NBX_HIP_TRY(hipMemcpy(net->d_inputs, recs.data(), sizeof(InputRec) * recs.size(), hipMemcpyHostToDevice));
+    NBX_HIP_TRY(launch_inputs(net->args(), net->d_inputs, (int32_t)recs.size(), nullptr, t, (float)H, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    return NBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nbx_net_set_interface_targets(nbx_net* net, const int32_t* neuron_ids, int32_t count) {
+    NBX_CHECK(net && (neuron_ids || count == 0) && count >= 0, "bad interface target list");
+    for (int32_t k = 0; k < count; ++k)
+        NBX_CHECK(neuron_ids[k] >= -1 && neuron_ids[k] < net->n, "interface target neuron out of range");
+    net->iface_targets.assign(neuron_ids, neuron_ids + count);
+    return NBX_OK;
+}
+
+int32_t nbx_net_upload_interface_values(nbx_net* net, const nbx_interface_value* values, int32_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(count >= 0 && (values || count == 0), "bad interface record list");
+    NBX_CHECK(count == 0 || net->iface_targets.empty() || count == (int32_t)net->iface_targets.size(),
+              "host interface record count must match interface target mapping count");
+    std::vector<InputRec> st;
+    for (int32_t k = 0; k < count; ++k) {
+        const nbx_interface_value& v = values[k];
+        NBX_CHECK(std::isfinite(v.value) && std::isfinite(v.scale), "interface values must be finite");
+        NBX_CHECK(v.pending_channel != NBX_PENDING_CURRENT || v.overwrite,
+                  "current inputs replace I_ext: set overwrite on current records");
+        if (v.value == 0.0f) continue;  // ApplyBoundaryStateVectorKernel skips zero values
+        const int32_t tgt = resolve_target(net, v.interface_slot, v.target_neuron);
+        if (tgt < 0) continue;
+        const int32_t kind = channel_kind(net, v.pending_channel, tgt);
+        if (kind < 0) continue;
+        st.push_back(InputRec{tgt, kind, -1, v.overwrite ? 1 : 0, v.value * v.scale});
+    }
+    net->iface_staged.swap(st);
+    return NBX_OK;
+}
+
+int32_t nbx_net_apply_interface(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    int32_t t = 0;
+    NBX_RC(input_step(net, &t));
+    NBX_RC(device_scope(net));
+    return run_inputs(net, net->iface_staged, t);
+}
+
+int32_t nbx_net_upload_interface_values_device(nbx_net* net, const float* d_values, const nbx_interface_copy* specs,
+                                               int32_t count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(d_values, "device_values must not be null");
+    NBX_CHECK(count >= 0 && (specs || count == 0), "bad interface copy list");
+    int32_t t = 0;
+    NBX_RC(input_step(net, &t));
+    std::vector<InputRec> recs;
+    for (int32_t k = 0; k < count; ++k) {
+        const nbx_interface_copy& c = specs[k];
+        NBX_CHECK(std::isfinite(c.scale), "interface scales must be finite");
+        NBX_CHECK(c.pending_channel != NBX_PENDING_CURRENT || c.overwrite,
+                  "current inputs replace I_ext: set overwrite on current records");
+        const int32_t tgt = resolve_target(net, c.interface_slot, c.target_neuron);
+        if (tgt < 0 || c.source_index < 0) continue;
+        const int32_t kind = channel_kind(net, c.pending_channel, tgt);
+        if (kind < 0) continue;
+        recs.push_back(InputRec{tgt, kind, c.source_index, c.overwrite ? 1 : 0, c.scale});
+    }
+    if (recs.empty()) return NBX_OK;
+    NBX_RC(device_scope(net));
+    NBX_RC(ensure_input_stage(net, (int64_t)recs.size()));
+    const double H = input_headroom(net);
+    NBX_HIP_TRY(hipMemcpy(net->d_inputs, recs.data(), sizeof(InputRec) * recs.size(), hipMemcpyHostToDevice));
+    NBX_HIP_TRY(launch_inputs(net->args(), net->d_inputs, (int32_t)recs.size(), d_values, t, (float)H, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    int32_t e = 0;
+    NBX_HIP_TRY(hipMemcpy(&e, net->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e & kErrInput) {
+        NBX_PASS(clear_error_bits(net, kErrInput));
+        set_error("device interface values could overflow the int32 arrival sums; those values were dropped");
+        return NBX_ERR_STATE;
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_add_arrivals(nbx_net* net, const int32_t* targets, const uint8_t* types, const float* weights,
+                             int32_t count, int32_t t) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(count >= 0 && (count == 0 || (targets && types && weights)), "bad arrival list");
+    const int32_t lo = net->phase >= 3 ? net->t + 1 : net->t;
+    NBX_CHECK(t >= lo && t <= net->t + net->D - 1,
+              "arrival step outside the delay ring's reach (current step .. current step + slots - 1)");
+    std::vector<InputRec> recs((size_t)count);
+    for (int32_t k = 0; k < count; ++k) {
+        NBX_CHECK(targets[k] >= 0 && targets[k] < net->n, "arrival target out of range");
+        NBX_CHECK(types[k] <= model_max_type(net->model_of(targets[k])), "arrival type is not an input of its target's model");
+        NBX_CHECK(arrival_channel(net->P.model, types[k]) < net->nch,
+                  "the network keeps no channel for this arrival type (no synapse of the type was given)");
+        NBX_CHECK(std::isfinite(weights[k]), "arrival weights must be finite");
+        recs[(size_t)k] = InputRec{targets[k], types[k], -1, 0, weights[k]};
+    }
+    NBX_RC(device_scope(net));
+    return run_inputs(net, recs, t);
+}
+
+int32_t nbx_net_reset_state(nbx_net* net) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_RC(device_scope(net));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    const size_t n = (size_t)net->n;
+    std::vector<float> zero(n, 0.0f);
+    std::vector<int32_t> last(n, 10000);
+    NBX_HIP_TRY(hipMemcpy(net->d_v, net->h_init_v.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    NBX_HIP_TRY(hipMemcpy(net->d_s1, net->h_init_s1.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    for (float* p : {net->d_ga, net->d_gg, net->d_iext, net->d_g3})
+        if (p) NBX_HIP_TRY(hipMemcpy(p, zero.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    NBX_HIP_TRY(hipMemcpy(net->d_last, last.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    hipStream_t st = net->stream;
+    NBX_HIP_TRY(launch_fill_i32(net->d_ring, 0, (int64_t)net->D * net->nch * net->n, st));
+    NBX_HIP_TRY(launch_fill_i32(net->d_hist_cnt, 0, net->D, st));
+    if (net->d_fmask) NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(net->d_fmask), 0, 2 * (int64_t)n, st));
+    NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(net->d_events), 0, 2, st));
+    NBX_HIP_TRY(launch_fill_i32(net->d_err, 0, 1, st));
+    if (!net->rules.empty()) {
+        NBX_HIP_TRY(launch_fill_i32(net->d_tstamp, -1, net->S, st));
+        NBX_HIP_TRY(launch_fill_i32(net->d_tpost_stamp, -1, net->n, st));
+        NBX_HIP_TRY(launch_fill_i32(net->d_tcount, 0, 2, st));
+        if (net->d_chead) NBX_HIP_TRY(launch_fill_i32(net->d_chead, 0, std::max(net->crows, 1), st));
+    }
+    NBX_HIP_TRY(hipStreamSynchronize(st));
+    NBX_RC(upload_learning_records(net, false));
+    net->t = 0;
+    net->phase = 0;
+    return NBX_OK;
+}
+
+int32_t nbx_net_download_pending(nbx_net* net, float* pending, int64_t count, int32_t* channel_count,
+                                 int32_t* channel_stride) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(pending && count == (int64_t)net->nch * net->n, "pending buffer size must be channels * n");
+    NBX_RC(device_scope(net));
+    float* d = nullptr;
+    NBX_RC(nbx_net_pending_device_float(net, &d));
+    NBX_HIP_TRY(hipMemcpy(pending, d, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
+    if (channel_count) *channel_count = net->nch;
+    if (channel_stride) *channel_stride = net->n;
+    return NBX_OK;
+}
+
+int32_t nbx_net_pending_device_float(nbx_net* net, float** d_pending) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    NBX_CHECK(d_pending, "null argument");
+    NBX_RC(device_scope(net));
+    if (!net->d_pending_f) NBX_RC(net->alloc(&net->d_pending_f, (size_t)net->nch * net->n));
+    NBX_HIP_TRY(launch_pending_float(net->args(), net->t, net->d_pending_f, net->stream));
+    NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+    *d_pending = net->d_pending_f;
     return NBX_OK;
 }
 

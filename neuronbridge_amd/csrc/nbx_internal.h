@@ -51,6 +51,7 @@ __host__ __device__ inline int32_t syn_type(uint32_t e) { return (int32_t)(e >> 
 // ----------------------------------------------------------------- model parameters (by value)
 struct ModelParams {
     int32_t model, integrator, n, t_ref;
+    int32_t begin;  // first neuron of the population (0 for a single-population network)
     float dt, hdt, dt6;
     float v_rest, tau, v_th, r, v_reset, e_rev;
     float izh_v_th, izh_r, e_ampa, e_gaba;
@@ -70,20 +71,27 @@ constexpr int32_t kCondExc = 1, kCondInh = 2, kCondNmda = 4;
 //   IZH, LIF_EXP2, TRIGGER_RELAY: channel 0 excitatory, channel 1 inhibitory, both positive
 //   LIF_EXP3: channel = type (AMPA, GABA, NMDA), positive
 //   IZH_CUR: one channel, weights already signed
+//   IZH: channel 2 carries NMDA arrivals (type 2) when the network has them
+// Networks of several populations deliver by type instead (kModelTyped below): channel = type,
+// magnitudes as stored, and each population's update folds the channels into its model's inputs.
+constexpr int32_t kModelTyped = 15;
 __host__ __device__ inline int32_t model_channels(int32_t model) {
-    return model == NBX_MODEL_LIF_EXP3 ? 3
+    return (model == NBX_MODEL_LIF_EXP3 || model == kModelTyped) ? 3
          : (model == NBX_MODEL_IZH || model == NBX_MODEL_LIF_EXP2 || model == NBX_MODEL_TRIGGER_RELAY) ? 2 : 1;
 }
 __host__ __device__ inline int32_t arrival_channel(int32_t model, int32_t type) {
-    if (model == NBX_MODEL_LIF_EXP3) return type;
+    if (model == NBX_MODEL_LIF_EXP3 || model == kModelTyped) return type;
+    if (model == NBX_MODEL_IZH) return type;
     if (model_channels(model) == 2) return type == 1 ? 1 : 0;
     return 0;
 }
 __host__ __device__ inline bool arrival_negated(int32_t model, int32_t type) {
     return type == 1 && (model == NBX_MODEL_LIF_VJ || model == NBX_MODEL_LIF_EXP || model == NBX_MODEL_POISSON);
 }
-// Largest synapse type a model accepts.
-__host__ __device__ inline int32_t model_max_type(int32_t model) { return model == NBX_MODEL_LIF_EXP3 ? 2 : 1; }
+// Largest synapse type a model accepts (NMDA for the triple-exponential LIF and Izhikevich).
+__host__ __device__ inline int32_t model_max_type(int32_t model) {
+    return (model == NBX_MODEL_LIF_EXP3 || model == NBX_MODEL_IZH) ? 2 : 1;
+}
 
 // Which SoA state arrays a model keeps besides v, I_ext and LastSpike.
 __host__ __device__ constexpr bool model_uses_s1(int32_t m) {
@@ -105,8 +113,17 @@ struct RuleDev {
     uint32_t seed;
 };
 // Device error codes (nbx_net::d_err; the batched engine raises kErrExchange, replay STDP kErrRaster)
-constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4, kErrRaster = 8;
+constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4, kErrRaster = 8, kErrInput = 16;
+
+// One external input of a step: an arrival of synapse type kind (0..2) or (kInputCurrent) a new
+// I_ext; value is the applied value, or the scale of d_values[src] when src >= 0.
+constexpr int32_t kInputCurrent = 3;
+struct InputRec {
+    int32_t target, kind, src, overwrite;
+    float value;
+};
 constexpr int32_t kMaxRules = 64;
+constexpr int32_t kMaxPops = 64;  // populations per network
 
 // Device pointers and scalars the step kernels need; passed by value.
 // Per-synapse learning state in one 32-byte record, so an update touches this line and the
@@ -160,6 +177,7 @@ struct StepArgs {
     float* ga;    // IZH: g_ampa
     float* gg;    // IZH: g_gaba
     float* iext;  // I_ext
+    float* g3;    // IZH: g_nmda (networks with Izhikevich NMDA synapses), else null
     int32_t* last;
     const float *a, *b, *c, *d;
     // delivery
@@ -222,6 +240,10 @@ struct StepArgs {
     int32_t rec_t0, rec_n;
     // error flag (device-side failures such as a barrier time-out)
     int32_t* err;
+    // populations of a network with several (P.model == kModelTyped): per-neuron population and
+    // the populations' parameters; null for one population
+    const uint8_t* pop_of;
+    const ModelParams* pops;
 };
 
 // ----------------------------------------------------------------- device helpers
@@ -272,13 +294,23 @@ __device__ inline float lif_rhs(const ModelParams& P, float V, float I, float g)
     return cur + (P.v_rest - V) / P.tau;
 }
 
+// ---- NMDA magnesium block 1 / (1 + exp(-0.062 V) * (1.2 / 3.57)). The exponential is evaluated in
+// double and rounded to float (the oracle does the same); the reference calls float std::exp.
+__device__ inline float nmda_block(float V) {
+    const float e = (float)exp((double)(-0.062f * V));
+    return 1.0f / (1.0f + e * (1.2f / 3.57f));
+}
+
+// TimeDrivenIzhikevic_Exponential_Decay.cpp:62-75: I_tot = g_ampa (E_ampa - v) + g_gaba (E_gaba - v)
+// [+ g_nmda B(v) (E_ampa - v) when the network has NMDA synapses] + I, left to right
 template <bool COND>
-__device__ inline void izh_rhs(const ModelParams& P, float a, float b, float V, float U, float ga, float gg, float I,
-                               float& dv, float& du) {
+__device__ inline void izh_rhs(const ModelParams& P, float a, float b, float V, float U, float ga, float gg, float gn,
+                               float I, float& dv, float& du) {
     float tot = 0.0f;
     if (COND) {
         tot += ga * (P.e_ampa - V);
         tot += gg * (P.e_gaba - V);
+        if (P.cond & kCondNmda) tot += gn * nmda_block(V) * (P.e_ampa - V);
     }
     tot += I;
     dv = 0.04f * V * V + 5.0f * V + 140.0f - U + P.izh_r * tot;
@@ -314,19 +346,19 @@ __device__ inline bool lif_step(const ModelParams& P, float& V, float& g, float 
 
 template <bool COND>
 __device__ inline bool izh_step(const ModelParams& P, float a, float b, float c, float d, float& V, float& U,
-                                float& ga, float& gg, float I, int32_t& last) {
+                                float& ga, float& gg, float& gn, float I, int32_t& last) {
     if (last > 0) {
         if (P.integrator == NBX_EULER) {
             float dv, du;
-            izh_rhs<COND>(P, a, b, V, U, ga, gg, I, dv, du);
+            izh_rhs<COND>(P, a, b, V, U, ga, gg, gn, I, dv, du);
             V += dv * P.dt;
             U += du * P.dt;
         } else {
             float v1, u1, v2, u2, v3, u3, v4, u4;
-            izh_rhs<COND>(P, a, b, V, U, ga, gg, I, v1, u1);
-            izh_rhs<COND>(P, a, b, V + v1 * P.hdt, U + u1 * P.hdt, ga, gg, I, v2, u2);
-            izh_rhs<COND>(P, a, b, V + v2 * P.hdt, U + u2 * P.hdt, ga, gg, I, v3, u3);
-            izh_rhs<COND>(P, a, b, V + v3 * P.dt, U + u3 * P.dt, ga, gg, I, v4, u4);
+            izh_rhs<COND>(P, a, b, V, U, ga, gg, gn, I, v1, u1);
+            izh_rhs<COND>(P, a, b, V + v1 * P.hdt, U + u1 * P.hdt, ga, gg, gn, I, v2, u2);
+            izh_rhs<COND>(P, a, b, V + v2 * P.hdt, U + u2 * P.hdt, ga, gg, gn, I, v3, u3);
+            izh_rhs<COND>(P, a, b, V + v3 * P.dt, U + u3 * P.dt, ga, gg, gn, I, v4, u4);
             float av = v1 + 2.0f * v2;
             av = av + 2.0f * v3;
             av = av + v4;
@@ -340,6 +372,7 @@ __device__ inline bool izh_step(const ModelParams& P, float a, float b, float c,
     if (COND) {
         ga *= P.ampa_decay;
         gg *= P.gaba_decay;
+        if (P.cond & kCondNmda) gn *= P.nmda_decay;  // :96-97
     }
     last += 1;
     if (V >= P.izh_v_th) {
@@ -358,13 +391,6 @@ __device__ inline float lif2_rhs(const ModelParams& P, float V, float I, float g
     cur += ge * (P.e_ampa - V) / P.tau;
     cur += gi * (P.e_gaba - V) / P.tau;
     return cur + (P.v_rest - V) / P.tau;
-}
-
-// ---- NMDA magnesium block 1 / (1 + exp(-0.062 V) * (1.2 / 3.57)). The exponential is evaluated in
-// double and rounded to float (the oracle does the same); the reference calls float std::exp.
-__device__ inline float nmda_block(float V) {
-    const float e = (float)exp((double)(-0.062f * V));
-    return 1.0f / (1.0f + e * (1.2f / 3.57f));
 }
 
 // ---- triple-exponential LIF (TimeDrivenLIF_Exponential_triple.cpp:79-105):
@@ -429,12 +455,13 @@ __host__ __device__ inline float hash_uniform01(int32_t neuron, int32_t t) {
 }
 
 // Applies one neuron's arrivals (a0/a1/a2 fixed point) and external inputs for step t,
-// then advances it one step. Shared by every engine so their arithmetic is identical.
+// then advances it one step with the parameters P of its population. Shared by every engine so
+// their arithmetic is identical. GN is the Izhikevich NMDA conductance (unused otherwise).
 // POIS_GIVEN: the step's Poisson background count was drawn ahead by the caller (pc).
 template <int MODEL, bool LOAD_CUR = true, bool POIS_GIVEN = false>
-__device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, int32_t a2,
-                                   float& V, float& S1, float& GA, float& GG, float& I, int32_t& last,
-                                   int32_t pc = 0) {
+__device__ inline bool neuron_tick_p(const StepArgs& A, const ModelParams& P, int32_t i, int32_t t, int32_t a0,
+                                     int32_t a1, int32_t a2, float& V, float& S1, float& GA, float& GG, float& GN,
+                                     float& I, int32_t& last, int32_t pc = 0) {
     if (LOAD_CUR && A.cur && t >= A.cur_t0 && t < A.cur_t0 + A.cur_n) I = A.cur[(int64_t)(t - A.cur_t0) * A.n + i];
     if (A.pois) a0 += (POIS_GIVEN ? pc : poisson_count(A.seed_lo, A.seed_hi, i, t, A.pois_tab, A.kmax)) * A.q_bg;
     const float f0 = (float)a0 * A.inv_scale;
@@ -442,20 +469,20 @@ __device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int3
         GA += f0;
         GG += (float)a1 * A.inv_scale;
         float unused = 0.0f;
-        return lif_cond_step<2>(A.P, V, GA, GG, unused, I, last);
+        return lif_cond_step<2>(P, V, GA, GG, unused, I, last);
     } else if (MODEL == NBX_MODEL_LIF_EXP3) {
         GA += f0;
         GG += (float)a1 * A.inv_scale;
         S1 += (float)a2 * A.inv_scale;
-        return lif_cond_step<3>(A.P, V, GA, GG, S1, I, last);
+        return lif_cond_step<3>(P, V, GA, GG, S1, I, last);
     } else if (MODEL == NBX_MODEL_POISSON) {  // PoissonRate.cpp:55-77; V holds the transient rate input
         V += f0;
-        float rate = A.P.rate_bias + V + A.P.rate_gain * I;
+        float rate = P.rate_bias + V + P.rate_gain * I;
         rate = (0.0f < rate) ? rate : 0.0f;
-        float p = rate * A.P.dt * 0.001f;
+        float p = rate * P.dt * 0.001f;
         p = (p < 1.0f) ? p : 1.0f;
         V = 0.0f;
-        if (hash_uniform01(i, t) < p) {
+        if (hash_uniform01(i - P.begin, t) < p) {  // the hash takes the neuron's index within its model
             last = 0;
             return true;
         }
@@ -463,26 +490,51 @@ __device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int3
         return false;
     } else if (MODEL == NBX_MODEL_TRIGGER_RELAY) {  // V holds last_exc_input
         V = f0;
-        if (f0 > A.P.v_th) {
+        if (f0 > P.v_th) {
             last = 0;
             return true;
         }
         last += 1;
         return false;
     } else if (MODEL == NBX_MODEL_LIF_VJ) {
-        if (last > A.P.t_ref) V += f0;
+        if (last > P.t_ref) V += f0;
         float g = 0.0f;
-        return lif_step<false>(A.P, V, g, I, last);
+        return lif_step<false>(P, V, g, I, last);
     } else if (MODEL == NBX_MODEL_LIF_EXP) {
         S1 += f0;
-        return lif_step<true>(A.P, V, S1, I, last);
+        return lif_step<true>(P, V, S1, I, last);
     } else if (MODEL == NBX_MODEL_IZH) {
         GA += f0;
         GG += (float)a1 * A.inv_scale;
-        return izh_step<true>(A.P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, I, last);
+        GN += (float)a2 * A.inv_scale;
+        return izh_step<true>(P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, GN, I, last);
     } else {
         const float Itot = I + f0;
-        return izh_step<false>(A.P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, Itot, last);
+        float gn = 0.0f;
+        return izh_step<false>(P, A.a[i], A.b[i], A.c[i], A.d[i], V, S1, GA, GG, gn, Itot, last);
+    }
+}
+
+// The single-population form (the network's own parameters, no NMDA conductance of Izhikevich).
+template <int MODEL, bool LOAD_CUR = true, bool POIS_GIVEN = false>
+__device__ inline bool neuron_tick(const StepArgs& A, int32_t i, int32_t t, int32_t a0, int32_t a1, int32_t a2,
+                                   float& V, float& S1, float& GA, float& GG, float& I, int32_t& last,
+                                   int32_t pc = 0) {
+    float gn = 0.0f;
+    return neuron_tick_p<MODEL, LOAD_CUR, POIS_GIVEN>(A, A.P, i, t, a0, a1, a2, V, S1, GA, GG, gn, I, last, pc);
+}
+
+// Typed arrivals (channel = synapse type, magnitudes as stored) folded into a model's inputs, the
+// same integers a network of that model alone accumulates: one signed channel (inhibitory
+// subtracted) for LIF_VJ / LIF_EXP / POISSON, the signed sum for current-based Izhikevich,
+// channels as they are for the conductance models.
+__device__ inline void fold_typed(int32_t model, int32_t& a0, int32_t& a1, int32_t& a2) {
+    if (model == NBX_MODEL_LIF_VJ || model == NBX_MODEL_LIF_EXP || model == NBX_MODEL_POISSON) {
+        a0 = a0 - a1;
+        a1 = 0;
+    } else if (model == NBX_MODEL_IZH_CUR) {
+        a0 = a0 + a1;
+        a1 = 0;
     }
 }
 
@@ -497,5 +549,8 @@ hipError_t launch_trigger(const StepArgs& A, const int32_t* tbase, int32_t k, hi
 hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s);
 hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t s);
+hipError_t launch_inputs(const StepArgs& A, const InputRec* d_recs, int32_t count, const float* d_vals, int32_t t,
+                         float qmax, hipStream_t s);
+hipError_t launch_pending_float(const StepArgs& A, int32_t t, float* d_out, hipStream_t s);
 
 }  // namespace nbx

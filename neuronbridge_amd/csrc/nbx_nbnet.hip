@@ -438,7 +438,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         NBX_HIP_TRY(hipMemcpyAsync(rec, sc.pinned[cur], (size_t)c * kRecordSize, hipMemcpyHostToDevice, st));
         NBX_HIP_TRY(hipEventRecord(copied[cur], st));
         nbx_nbnet_extract<<<blocks_for(c), 256, 0, st>>>(rec, c, base, n, key, idx, type, w, pl, rules, cnt, bad, maxd,
-                                                          model_max_type(net->md.model));
+                                                          net_max_type(net));
         NBX_HIP_TRY(hipGetLastError());
         cur ^= 1;
     }
@@ -460,7 +460,7 @@ int32_t nbx_net_load_nbnet(nbx_net* net, const char* path, int32_t verify_checks
         return NBX_ERR_UNSUPPORTED;
     }
     if (bad_h & kBadType) {
-        set_error(net->md.model == NBX_MODEL_LIF_EXP3
+        set_error(net_max_type(net) == 2
                       ? "nbnet synapse type must be 0 (AMPA), 1 (GABA) or 2 (NMDA)"
                       : "nbnet synapse type must be 0 (excitatory) or 1 (inhibitory)");
         return NBX_ERR_UNSUPPORTED;

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "nbx_internal.h"
@@ -65,11 +67,31 @@ struct ReplayStdpWork {
     unsigned long long* upd = nullptr;  // update counter of the host entry point
 };
 
+// One neuron population: neurons [begin, begin + md.n) with one model and its parameters.
+struct NetPop {
+    int32_t begin = 0;
+    nbx_model_desc md{};
+    ModelParams P{};
+};
+
 struct nbx_net {
     int32_t device = 0;
     hipStream_t stream = nullptr;
-    nbx_model_desc md{};
-    ModelParams P{};
+    nbx_model_desc md{};  // the first population's (dt, integrator and the model of a one-population net)
+    ModelParams P{};      // its parameters; P.model = kModelTyped (typed delivery) for several populations
+    std::vector<NetPop> pops;
+    std::vector<uint8_t> h_pop_of;  // [n] population of each neuron (several populations)
+    bool multi() const { return pops.size() > 1; }
+    int32_t model_of(int32_t i) const {
+        return multi() ? pops[h_pop_of[(size_t)i]].md.model : md.model;
+    }
+    // largest synapse type any population accepts (each synapse is checked against its target's
+    // population at finalize)
+    friend int32_t net_max_type(const nbx_net* net) {
+        int32_t m = 0;
+        for (const NetPop& p : net->pops) m = std::max(m, nbx::model_max_type(p.md.model));
+        return m;
+    }
     int32_t n = 0, ns = 0, nch = 1;
     std::vector<float> h_a, h_b, h_c, h_d;
 
@@ -117,6 +139,18 @@ struct nbx_net {
     float *d_v = nullptr, *d_s1 = nullptr, *d_ga = nullptr, *d_gg = nullptr, *d_iext = nullptr;
     int32_t* d_last = nullptr;
     float *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_d = nullptr;
+    float* d_g3 = nullptr;          // Izhikevich NMDA conductance (networks with such synapses)
+    uint8_t* d_pop_of = nullptr;    // several populations: [n] population of each neuron
+    ModelParams* d_pops = nullptr;  // and their parameters
+    std::vector<float> h_init_v, h_init_s1;  // state at finalize (ResetState)
+    // external inputs: interface slot map, staged host records, device staging, output mask
+    std::vector<int32_t> iface_targets;
+    std::vector<nbx::InputRec> iface_staged;
+    nbx::InputRec* d_inputs = nullptr;  // device staging of input records (grown on demand)
+    int64_t inputs_cap = 0;
+    float* d_pending_f = nullptr;       // float snapshot of a step's pending arrivals
+    std::vector<uint8_t> out_mask;      // output neurons (empty: all)
+    int32_t crows = 0;                  // cerebellar arrival-history rows
     int32_t *d_ring = nullptr, *d_hist = nullptr, *d_hist_cnt = nullptr;
     uint64_t* d_fmask = nullptr;
     int64_t* d_row_ptr = nullptr;
@@ -175,7 +209,10 @@ struct nbx_net {
         A.ga = d_ga;
         A.gg = d_gg;
         A.iext = d_iext;
+        A.g3 = d_g3;
         A.last = d_last;
+        A.pop_of = d_pop_of;
+        A.pops = d_pops;
         A.a = d_a;
         A.b = d_b;
         A.c = d_c;
@@ -282,6 +319,7 @@ int32_t run_small_dense(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);
 int32_t replay_path(const nbx_net* net);  // NBX_REPLAY_GLOBAL / _OFFSETS / _SLOTS
 int32_t clear_raster_error(nbx_net* net);
+int32_t clear_error_bits(nbx_net* net, int32_t bits);  // clears device error flags (a kernel, then a sync)
 int32_t launch_replay_stdp(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t t0, int32_t lead,
                            int32_t nsteps, int64_t nspikes, unsigned long long* d_updates);
 // Same, with the workspace in the library's static device array (no allocation); rejects calls

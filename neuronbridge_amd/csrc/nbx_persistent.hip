@@ -858,6 +858,9 @@ std::string persistent_reason(const nbx_net* net) {
         return why.empty() ? why : "dense delivery: " + why;
     }
     if (!net->has_sparse) return "needs sparse synapses";
+    if (net->multi()) return "networks of several populations run on the graph engine";
+    if (net->md.model == NBX_MODEL_IZH && (net->P.cond & kCondNmda))
+        return "Izhikevich NMDA conductances run on the graph engine";
     if (net->has_dense) return "dense delivery runs on the graph engine";
     if (net->stdp) return "STDP networks use arrival-time delivery";
     if (!net->rules.empty()) return "networks with learning rules use arrival-time delivery";
@@ -916,8 +919,8 @@ int32_t build_persistent_layout(nbx_net* net) {
             const uint32_t e = net->h_syn[(size_t)j];
             const int32_t post = syn_post(e), ty = syn_type(e), blk = post / P;
             int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
-            const int32_t ch = arrival_channel(net->md.model, ty);
-            if (arrival_negated(net->md.model, ty)) qv = -qv;
+            const int32_t ch = arrival_channel(net->P.model, ty);
+            if (arrival_negated(net->P.model, ty)) qv = -qv;
             const uint32_t meta = (uint32_t)(post % P) | ((uint32_t)syn_delay(e) << 16) | ((uint32_t)ch << 22) |
                                   (((uint32_t)blk & 0x7Fu) << 25);
             seg[(size_t)cnt[(size_t)blk]++] = make_uint2(meta, (uint32_t)qv);
@@ -998,8 +1001,8 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
             const uint32_t e = net->h_syn[(size_t)j];
             const int32_t post = syn_post(e), ty = syn_type(e), qb = post / Pr;
             int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
-            const int32_t ch = arrival_channel(net->md.model, ty);
-            if (arrival_negated(net->md.model, ty)) qv = -qv;
+            const int32_t ch = arrival_channel(net->P.model, ty);
+            if (arrival_negated(net->P.model, ty)) qv = -qv;
             const uint2 en = make_uint2((uint32_t)(ch * Pr + post - qb * Pr) | ((uint32_t)(syn_delay(e) - 1) << 16),
                                         (uint32_t)qv);
             const int32_t k = fill[(size_t)qb]++;
@@ -1142,7 +1145,7 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
 }
 
 int32_t replay_path(const nbx_net* net) {
-    if (net->stdp || !net->rules.empty()) return NBX_REPLAY_GLOBAL;
+    if (net->stdp || !net->rules.empty() || net->multi()) return NBX_REPLAY_GLOBAL;
     const ReplayGeometry r = replay_geometry(net);
     if (!r.ok) return NBX_REPLAY_GLOBAL;
     if (r.slots) return NBX_REPLAY_SLOTS;

@@ -331,8 +331,10 @@ __global__ __launch_bounds__(kRsPassThreads) __attribute__((amdgpu_waves_per_eu(
 
 // Clears the raster error once it has been reported: an atomic on the network's stream, like the
 // kernels that raise it (a host copy into the flag could race with the line the atomics hold).
-int32_t clear_raster_error(nbx_net* net) {
-    rs_clear_err<<<1, 1, 0, net->stream>>>(net->d_err, kErrRaster);
+int32_t clear_raster_error(nbx_net* net) { return clear_error_bits(net, kErrRaster); }
+
+int32_t clear_error_bits(nbx_net* net, int32_t bits) {
+    rs_clear_err<<<1, 1, 0, net->stream>>>(net->d_err, bits);
     NBX_HIP_TRY(hipGetLastError());
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     return NBX_OK;

@@ -14,6 +14,8 @@
 //                     GpuPropagationRuntime.cu:150-316)
 //   nbx_ltp           post-side STDP over each fired neuron's plastic inputs
 //                     (reference: STDP::ApplyPostSynaticSpike, STDP.cpp:37-61)
+#include <algorithm>
+
 #include "nbx_internal.h"
 
 namespace nbx {
@@ -59,6 +61,46 @@ __device__ inline void push_row(const StepArgs& A, int32_t t, int64_t b, int64_t
     }
 }
 
+// Reads and clears neuron i's ring entries of step t, advances it with the parameters P and
+// stores its state. TYPED: the ring holds typed arrivals (several populations), folded first.
+template <int MODEL, bool TYPED>
+__device__ inline bool update_neuron(const StepArgs& A, const ModelParams& P, int32_t i, int32_t t, int32_t slot) {
+    int32_t* r = A.ring + (int64_t)slot * A.nch * A.n;
+    int32_t a0 = r[i];
+    r[i] = 0;
+    int32_t a1 = 0, a2 = 0;
+    if (A.nch > 1) {
+        a1 = r[A.n + i];
+        r[A.n + i] = 0;
+    }
+    if (A.nch > 2) {
+        a2 = r[2 * A.n + i];
+        r[2 * A.n + i] = 0;
+    }
+    if (TYPED) fold_typed(MODEL, a0, a1, a2);
+    constexpr bool izh_nmda = MODEL == NBX_MODEL_IZH;
+    float V = A.v[i];
+    float S1 = model_uses_s1(MODEL) ? A.s1[i] : 0.0f;
+    float GA = model_uses_g2(MODEL) ? A.ga[i] : 0.0f;
+    float GG = model_uses_g2(MODEL) ? A.gg[i] : 0.0f;
+    float GN = (izh_nmda && A.g3) ? A.g3[i] : 0.0f;
+    float I = A.iext[i];
+    int32_t last = A.last[i];
+    const bool fired = neuron_tick_p<MODEL>(A, P, i, t, a0, a1, a2, V, S1, GA, GG, GN, I, last);
+    A.v[i] = V;
+    if (model_uses_s1(MODEL)) A.s1[i] = S1;
+    if (model_uses_g2(MODEL)) {
+        A.ga[i] = GA;
+        A.gg[i] = GG;
+    }
+    if (izh_nmda && A.g3) A.g3[i] = GN;
+    A.iext[i] = I;
+    A.last[i] = last;
+    return fired;
+}
+
+// MODEL = kModelTyped: a network of several populations; each neuron takes its population's
+// model and parameters from the population table.
 template <int MODEL>
 __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const int32_t* tbase, int32_t k) {
     const int32_t t = step_of(tbase, k);
@@ -66,33 +108,21 @@ __global__ __launch_bounds__(kUpdateThreads) void nbx_update(StepArgs A, const i
     const int32_t slot = t % A.D;
     bool fired = false;
     if (i < A.n) {
-        int32_t* r = A.ring + (int64_t)slot * A.nch * A.n;
-        const int32_t a0 = r[i];
-        r[i] = 0;
-        int32_t a1 = 0, a2 = 0;
-        if (A.nch > 1) {
-            a1 = r[A.n + i];
-            r[A.n + i] = 0;
+        if constexpr (MODEL == kModelTyped) {
+            const ModelParams P = A.pops[A.pop_of[i]];
+            switch (P.model) {
+                case NBX_MODEL_LIF_VJ: fired = update_neuron<NBX_MODEL_LIF_VJ, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_LIF_EXP: fired = update_neuron<NBX_MODEL_LIF_EXP, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_IZH: fired = update_neuron<NBX_MODEL_IZH, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_IZH_CUR: fired = update_neuron<NBX_MODEL_IZH_CUR, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_LIF_EXP2: fired = update_neuron<NBX_MODEL_LIF_EXP2, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_LIF_EXP3: fired = update_neuron<NBX_MODEL_LIF_EXP3, true>(A, P, i, t, slot); break;
+                case NBX_MODEL_POISSON: fired = update_neuron<NBX_MODEL_POISSON, true>(A, P, i, t, slot); break;
+                default: fired = update_neuron<NBX_MODEL_TRIGGER_RELAY, true>(A, P, i, t, slot); break;
+            }
+        } else {
+            fired = update_neuron<MODEL, false>(A, A.P, i, t, slot);
         }
-        if (A.nch > 2) {
-            a2 = r[2 * A.n + i];
-            r[2 * A.n + i] = 0;
-        }
-        float V = A.v[i];
-        float S1 = model_uses_s1(MODEL) ? A.s1[i] : 0.0f;
-        float GA = model_uses_g2(MODEL) ? A.ga[i] : 0.0f;
-        float GG = model_uses_g2(MODEL) ? A.gg[i] : 0.0f;
-        float I = A.iext[i];
-        int32_t last = A.last[i];
-        fired = neuron_tick<MODEL>(A, i, t, a0, a1, a2, V, S1, GA, GG, I, last);
-        A.v[i] = V;
-        if (model_uses_s1(MODEL)) A.s1[i] = S1;
-        if (model_uses_g2(MODEL)) {
-            A.ga[i] = GA;
-            A.gg[i] = GG;
-        }
-        A.iext[i] = I;
-        A.last[i] = last;
         if (A.fmask) A.fmask[i] = (A.fmask[i] << 1) | (fired ? 1ull : 0ull);
     }
     // ---- spike compaction: ballot per wavefront, prefix over the block's wavefronts
@@ -513,6 +543,44 @@ __global__ void nbx_inject(StepArgs A, const int32_t* ids, int32_t count, int32_
     if (ev) atomicAdd(A.events, ev);
 }
 
+// External inputs of step t (interface channels, arrivals from outside the synapses), resolved on
+// the host: an arrival of synapse type r.kind (0..2) adds rint(value * 2^shift) to the target's ring
+// entry exactly as a synapse's arrival does; kind kInputCurrent replaces I_ext. A record with
+// src >= 0 takes its value from the device array at src times r.value (device-to-device copies;
+// zero values skipped unless r.overwrite). |value * 2^shift| above qmax raises kErrInput.
+__global__ void nbx_inputs(StepArgs A, const InputRec* __restrict__ recs, int32_t count,
+                           const float* __restrict__ vals, int32_t t, float qmax) {
+    int32_t* r0 = A.ring + (int64_t)(t % A.D) * A.nch * A.n;
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+        const InputRec r = recs[j];
+        float applied = r.value;
+        if (r.src >= 0) {
+            applied = vals[r.src] * r.value;
+            if (applied == 0.0f && !r.overwrite) continue;
+        }
+        if (r.kind == kInputCurrent) {
+            A.iext[r.target] = applied;
+            continue;
+        }
+        const float qs = applied * A.scale;
+        if (!(fabsf(qs) <= qmax)) {
+            atomicOr(A.err, kErrInput);
+            continue;
+        }
+        int32_t q = __float2int_rn(qs);
+        if (arrival_negated(A.P.model, r.kind)) q = -q;
+        atomicAdd(&r0[(int64_t)arrival_channel(A.P.model, r.kind) * A.n + r.target], q);
+    }
+}
+
+// Pending arrivals of step t as floats, channel-major [ch][n] (a snapshot of the ring slot).
+__global__ void nbx_pending_float(StepArgs A, int32_t t, float* out) {
+    const int32_t* r0 = A.ring + (int64_t)(t % A.D) * A.nch * A.n;
+    const int64_t total = (int64_t)A.nch * A.n;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x)
+        out[j] = (float)r0[j] * A.inv_scale;
+}
+
 __global__ void nbx_fill_i32(int32_t* p, int32_t value, int64_t count) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
         p[j] = value;
@@ -525,6 +593,7 @@ int deliver_grid() { return 1024; }
 hipError_t launch_update(const StepArgs& A, const int32_t* tbase, int32_t k, hipStream_t s) {
     const int grid = (A.n + kUpdateThreads - 1) / kUpdateThreads;
     switch (A.P.model) {
+        case kModelTyped: nbx_update<kModelTyped><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
         case NBX_MODEL_LIF_VJ: nbx_update<NBX_MODEL_LIF_VJ><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
         case NBX_MODEL_LIF_EXP: nbx_update<NBX_MODEL_LIF_EXP><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
         case NBX_MODEL_IZH: nbx_update<NBX_MODEL_IZH><<<grid, kUpdateThreads, 0, s>>>(A, tbase, k); break;
@@ -580,6 +649,19 @@ hipError_t launch_advance(int32_t* tbase, int32_t by, hipStream_t s) {
 
 hipError_t launch_inject(const StepArgs& A, const int32_t* d_ids, int32_t count, int32_t t, hipStream_t s) {
     nbx_inject<<<1, 256, 0, s>>>(A, d_ids, count, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_inputs(const StepArgs& A, const InputRec* d_recs, int32_t count, const float* d_vals, int32_t t,
+                         float qmax, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    nbx_inputs<<<std::min((count + 255) / 256, 1024), 256, 0, s>>>(A, d_recs, count, d_vals, t, qmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_pending_float(const StepArgs& A, int32_t t, float* d_out, hipStream_t s) {
+    const int64_t total = (int64_t)A.nch * A.n;
+    nbx_pending_float<<<(int)std::min<int64_t>((total + 255) / 256, 4096), 256, 0, s>>>(A, t, d_out);
     return hipGetLastError();
 }
 

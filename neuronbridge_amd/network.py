@@ -85,6 +85,38 @@ class ModelSpec:
 
 
 @dataclass
+class PopulationsSpec:
+    """Several neuron populations in one network (nbx_net_create_populations): population k owns
+    the next pops[k].n neurons; every population steps with the same dt. Arrivals are kept by
+    synapse type, so the delay ring has the three typed channels."""
+    pops: list
+
+    @property
+    def n(self) -> int:
+        return int(sum(p.n for p in self.pops))
+
+    @property
+    def nstate(self) -> int:
+        return max(p.nstate for p in self.pops)
+
+    @property
+    def channels(self) -> int:
+        return 3
+
+    @property
+    def dt(self) -> float:
+        return self.pops[0].dt
+
+    @property
+    def begins(self) -> list:
+        out, b = [], 0
+        for p in self.pops:
+            out.append(b)
+            b += p.n
+        return out
+
+
+@dataclass
 class Synapses:
     """CSR by presynaptic neuron, rows sorted by (delay, post)."""
     row_ptr: np.ndarray  # int64 [n+1]
@@ -116,11 +148,16 @@ def build_fixed_indegree(n: int, k_in: int, d_min: int, d_max: int, seed: int, n
 class Net:
     """One network resident on a GPU."""
 
-    def __init__(self, spec: ModelSpec, device: int = 0):
+    def __init__(self, spec, device: int = 0):
+        """spec: a ModelSpec (one population) or a PopulationsSpec (several)."""
         self.spec = spec
         h = C.c_void_p()
-        desc = spec.desc()
-        L.call("nbx_net_create", C.byref(desc), device, C.byref(h))
+        if isinstance(spec, PopulationsSpec):
+            descs = (L.ModelDesc * len(spec.pops))(*[p.desc() for p in spec.pops])
+            L.call("nbx_net_create_populations", descs, len(spec.pops), device, C.byref(h))
+        else:
+            desc = spec.desc()
+            L.call("nbx_net_create", C.byref(desc), device, C.byref(h))
         self.h = h
         self.n = spec.n
         self.nstate = spec.nstate
@@ -287,6 +324,66 @@ class Net:
 
     def commit_firings(self) -> None:
         L.call("nbx_net_commit_firings", self.h)
+
+    def set_output_mask(self, mask) -> None:
+        """ConfigureOutputNeuronMask: None marks every neuron as an output."""
+        if mask is None:
+            L.call("nbx_net_set_output_mask", self.h, None)
+            return
+        self._mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        assert self._mask.shape == (self.n,)
+        L.call("nbx_net_set_output_mask", self.h, _p(self._mask, C.c_uint8))
+
+    def neuron_step_outputs(self):
+        """Neuron update of the current step; (output firings, all firings), each ascending."""
+        out = np.zeros(self.n, dtype=np.int32)
+        full = np.zeros(self.n, dtype=np.int32)
+        oc, fc = C.c_int32(), C.c_int32()
+        L.call("nbx_net_neuron_step_outputs", self.h, _p(out, C.c_int32), self.n, C.byref(oc), _p(full, C.c_int32),
+               self.n, C.byref(fc))
+        return out[:oc.value].copy(), full[:fc.value].copy()
+
+    def set_interface_targets(self, ids) -> None:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        L.call("nbx_net_set_interface_targets", self.h, _p(ids, C.c_int32), int(ids.shape[0]))
+
+    def upload_interface_values(self, records) -> None:
+        """records: (interface_slot, target_neuron, pending_channel, value, scale, overwrite) tuples."""
+        arr = (L.InterfaceValue * max(1, len(records)))()
+        for k, (slot, tgt, ch, val, sc, ow) in enumerate(records):
+            arr[k] = L.InterfaceValue(int(slot), int(tgt), int(ch), int(bool(ow)), float(val), float(sc))
+        L.call("nbx_net_upload_interface_values", self.h, arr, len(records))
+
+    def apply_interface(self) -> None:
+        L.call("nbx_net_apply_interface", self.h)
+
+    def upload_interface_values_device(self, d_values: int, specs) -> None:
+        """specs: (interface_slot, target_neuron, pending_channel, source_index, scale, overwrite)
+        tuples; d_values is a device pointer to the source floats."""
+        arr = (L.InterfaceCopy * max(1, len(specs)))()
+        for k, (slot, tgt, ch, src, sc, ow) in enumerate(specs):
+            arr[k] = L.InterfaceCopy(int(slot), int(tgt), int(ch), int(bool(ow)), int(src), float(sc))
+        ptr = d_values if isinstance(d_values, C.c_void_p) else C.c_void_p(int(d_values))
+        L.call("nbx_net_upload_interface_values_device", self.h, ptr, arr, len(specs))
+
+    def add_arrivals(self, targets, types, weights, t: int) -> None:
+        tg = np.ascontiguousarray(targets, dtype=np.int32)
+        ty = np.ascontiguousarray(types, dtype=np.uint8)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        assert tg.shape == ty.shape == w.shape
+        L.call("nbx_net_add_arrivals", self.h, _p(tg, C.c_int32), _p(ty, C.c_uint8), _p(w, C.c_float),
+               int(tg.shape[0]), int(t))
+
+    def reset_state(self) -> None:
+        L.call("nbx_net_reset_state", self.h)
+
+    def download_pending(self) -> np.ndarray:
+        slots, ch = C.c_int32(), C.c_int32()
+        L.call("nbx_net_ring_shape", self.h, C.byref(slots), C.byref(ch))
+        out = np.zeros((ch.value, self.n), dtype=np.float32)
+        cc, cs = C.c_int32(), C.c_int32()
+        L.call("nbx_net_download_pending", self.h, _p(out, C.c_float), out.size, C.byref(cc), C.byref(cs))
+        return out
 
     def history(self, t: int) -> np.ndarray:
         ids = np.zeros(self.n, dtype=np.int32)

@@ -151,6 +151,78 @@ def model_sparse(model: int, n: int = 2000, k_in: int = 100, integrator: int = L
                     last0=last0, steps=0, extra={"n_exc": n_exc, "k_in": k_in})
 
 
+def populations(models=(L.LIF_VJ, L.IZH), sizes=(1600, 400), k_in: int = 100, integrator: int = L.EULER,
+                seed: int = SEED_BASE + 8, nmda_fraction: float = 0.25) -> Workload:
+    """Several populations of different models in one network, joined by one fixed in-degree
+    topology (delays U{1..20}; presynaptic neurons of the last 20% of each population inhibitory).
+    Each synapse's weight follows its target's model: voltage jumps of 0.12 / 0.45 mV into LIF
+    voltage-jump, conductance increments of 0.02 / 0.05 into LIF exp-decay and the double / triple
+    exponential LIFs, 0.05 / 0.1 into conductance Izhikevich, signed currents 1.5 / -3 into
+    current-based Izhikevich, rate increments of 200 / 400 Hz into PoissonRate, 0.002 / 0.5 into the
+    trigger relay (all for in-degree 100, scaled by 100 / k_in). nmda_fraction of the excitatory
+    synapses into conductance Izhikevich and the triple-exponential LIF are NMDA (type 2).
+    Background: Poisson(1.0) per step of weight 0.1 into every neuron."""
+    from .network import PopulationsSpec
+    rng = np.random.default_rng(seed)
+    n = int(sum(sizes))
+    begins = np.cumsum([0] + list(sizes))
+    exc = np.ones(n, dtype=bool)
+    for b, m in zip(begins[:-1], sizes):
+        exc[b + int(round(0.8 * m)):b + m] = False
+    syn = build_fixed_indegree(n, k_in, 1, 20, seed, n)  # types set below from each presynaptic neuron
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(syn.row_ptr))
+    syn.type = np.where(exc[pre], 0, 1).astype(np.uint8)
+    g = np.float32(100.0 / k_in)
+    pops, x0cols = [], []
+    w = np.zeros(syn.count, np.float32)
+    tgt_pop = np.searchsorted(begins, syn.post, side="right") - 1
+    for k, (model, m) in enumerate(zip(models, sizes)):
+        into = tgt_pop == k
+        if model == L.LIF_VJ:
+            ps = ModelSpec(model=model, n=m, dt=0.1, integrator=integrator, t_ref=20, v_rest=-65.0, tau=20.0,
+                           v_th=-50.0, r=1.0, v_reset=-65.0)
+            we, wi = 0.12, 0.45
+        elif model == L.LIF_EXP:
+            ps = ModelSpec(model=model, n=m, dt=0.1, integrator=integrator, t_ref=20, v_rest=-65.0, tau=20.0,
+                           v_th=-50.0, r=1.0, v_reset=-65.0, g_tau=5.0, e_rev=0.0)
+            we, wi = 0.02, 0.05
+        elif model in (L.LIF_EXP2, L.LIF_EXP3):
+            ps = ModelSpec(model=model, n=m, dt=0.1, integrator=integrator, t_ref=20, v_rest=-60.0, tau=20.0,
+                           v_th=-50.0, r=1.0, v_reset=-60.0, e_ampa=0.0, e_gaba=-80.0, ampa_tau=5.0,
+                           gaba_tau=10.0, nmda_tau=10.0)
+            we, wi = 0.02, 0.05
+        elif model in (L.IZH, L.IZH_CUR):
+            a, b, c, d = izh_params(m, int(round(0.8 * m)), rng)
+            ps = ModelSpec(model=model, n=m, dt=0.1, integrator=integrator, v_rest=-65.0, a=a, b=b, c=c, d=d,
+                           nmda_tau=100.0)
+            we, wi = (0.05, 0.1) if model == L.IZH else (1.5, -3.0)
+        elif model == L.POISSON:
+            ps = ModelSpec(model=model, n=m, dt=0.1, rate_bias_hz=30.0, rate_gain_hz=1.0)
+            we, wi = 200.0, 400.0
+        else:
+            ps = ModelSpec(model=model, n=m, dt=0.1, v_th=0.25)
+            we, wi = 0.002, 0.5
+        w[into] = np.where(syn.type[into] == 0, np.float32(we), np.float32(wi)) * g
+        if model in (L.IZH, L.LIF_EXP3):
+            nmda = into & (syn.type == 0) & (rng.random(syn.count) < nmda_fraction)
+            syn.type = np.where(nmda, 2, syn.type).astype(np.uint8)
+        pops.append(ps)
+    syn.weight = w.astype(np.float32)
+    spec = PopulationsSpec(pops)
+    x0 = np.zeros((n, spec.nstate), np.float32)
+    for b, ps in zip(begins[:-1], pops):
+        if ps.model in (L.LIF_VJ, L.LIF_EXP):
+            x0[b:b + ps.n, 0] = rng.uniform(-65.0, -50.0, ps.n)
+        elif ps.model in (L.LIF_EXP2, L.LIF_EXP3):
+            x0[b:b + ps.n, 0] = rng.uniform(-60.0, -50.0, ps.n)
+        elif ps.model in (L.IZH, L.IZH_CUR):
+            x0[b:b + ps.n, 0] = rng.uniform(-70.0, -55.0, ps.n)
+            x0[b:b + ps.n, 1] = np.asarray(ps.b, np.float32) * x0[b:b + ps.n, 0]
+    last0 = np.full(n, 10000, dtype=np.int32)
+    return Workload("populations", spec, syn=syn, pois_seed=seed, pois_lambda=1.0, w_bg=0.1, x0=x0, last0=last0,
+                    steps=0, extra={"begins": begins.tolist(), "k_in": k_in})
+
+
 def trigger_rules(n: int = 2000, k_in: int = 100, seed: int = SEED_BASE + 7, trigger_fraction: float = 0.03,
                   with_stdp: bool = False, cerebellar: bool = False, history: int = 0) -> Workload:
     """The config-2 LIF network with the legacy trigger-driven rules on its E->E synapses:

@@ -62,7 +62,16 @@ class OrcNet(C.Structure):
         ("nrules", C.c_int32), ("rules", C.c_void_p), ("syn_rule", C.POINTER(C.c_int8)),
         ("syn_trig", C.POINTER(C.c_int8)), ("maxw", _f32p), ("elig", _f32p), ("stdp_maxw", _f32p),
         ("conn_index", _i32p),
+        ("npops", C.c_int32), ("pops", C.POINTER(OrcModel)), ("pop_begin", _i32p),
+        ("inputs", C.c_void_p), ("input_ptr", _i32p), ("input_t0", C.c_int32), ("input_nsteps", C.c_int32),
     ]
+
+
+class OrcInput(C.Structure):
+    _fields_ = [("target", C.c_int32), ("kind", C.c_int32), ("value", C.c_float)]
+
+
+INPUT_CURRENT = 3
 
 
 RULE_RSTDP, RULE_ADDITIVE, RULE_CEREBELLAR = 1, 2, 3

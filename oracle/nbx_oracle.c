@@ -73,7 +73,8 @@ void orc_model_prepare(orc_model* m) {
     m->g_decay = ref_decay(m->dt, m->g_tau);
     m->ampa_decay = ref_decay(m->dt, m->ampa_tau);
     m->gaba_decay = ref_decay(m->dt, m->gaba_tau);
-    m->nmda_decay = 1.0f;
+    /* Izhikevich NMDA: the same single-entry lookup (TimeDrivenIzhikevic_Exponential_Decay.cpp:32-39) */
+    m->nmda_decay = m->model == ORC_IZH ? ref_decay(m->dt, m->nmda_tau) : 1.0f;
     if (m->model == ORC_LIF_EXP3) { /* TimeDrivenLIF_Exponential_triple.cpp:56-60: exp(-dt / tau) */
         m->ampa_decay = expf(-m->dt / m->ampa_tau);
         m->gaba_decay = expf(-m->dt / m->gaba_tau);
@@ -86,7 +87,7 @@ void orc_model_prepare(orc_model* m) {
  *   IZH g_ampa / g_gaba; LIF_EXP2 gexc / ginh (..._double.cpp:105-113); TRIGGER_RELAY exc / (unused) inh
  *   LIF_EXP3 ampa / gaba / nmda by type (..._triple.cpp:142-155); IZH_CUR signed current */
 static int orc_channel(int model, int type) {
-    if (model == ORC_LIF_EXP3) return type;
+    if (model == ORC_LIF_EXP3 || model == ORC_IZH) return type; /* Izhikevich NMDA: channel 2 (:119-131) */
     if (model == ORC_IZH || model == ORC_LIF_EXP2 || model == ORC_TRIGGER_RELAY) return type == 1 ? 1 : 0;
     return 0;
 }
@@ -215,7 +216,14 @@ typedef struct {
     float* isyn;
     float scale;
     int ns;
+    int typed;              /* several populations: fixed-point arrivals kept per synapse type */
+    const uint8_t* pop_of;  /* population of each neuron (typed) */
 } run_ctx;
+
+/* the model (and parameters) of neuron i */
+static const orc_model* model_of(const run_ctx* c, int32_t i) {
+    return c->typed ? &c->net->pops[c->pop_of[i]] : &c->net->m;
+}
 
 /* One synaptic arrival. ProcessSpike semantics:
  *   voltage jump: V += w (exc) / V -= w (inh) only outside refractory (Voltage_jump.cpp:100-113)
@@ -223,12 +231,16 @@ typedef struct {
  *   Izhikevich:   g_ampa += w (exc), g_gaba += w (inh) (Izhikevic .cpp:119-131)
  *   current-based Izhikevich: signed w into this tick's synaptic current. */
 static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
-    const orc_model* m = &c->net->m;
+    const orc_model* m = model_of(c, post);
     float* s = c->x + (size_t)post * c->ns;
     if (c->acc_mode == ORC_ACC_FIXED) {
         int32_t q = (int32_t)lrintf(w * c->scale);
-        if (orc_negated(m->model, type)) q = -q;
         int32_t* acc[3] = {c->acc0, c->acc1, c->acc2};
+        if (c->typed) { /* by synapse type; apply_fixed folds them per model */
+            acc[type][post] += q;
+            return;
+        }
+        if (orc_negated(m->model, type)) q = -q;
         acc[orc_channel(m->model, type)][post] += q;
         return;
     }
@@ -237,10 +249,7 @@ static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
             if (c->last[post] > m->t_ref) s[0] += (type == 1) ? -w : w;
             break;
         case ORC_LIF_EXP: s[1] += (type == 1) ? -w : w; break;
-        case ORC_IZH:
-            if (type == 1) s[3] += w;
-            else s[2] += w;
-            break;
+        case ORC_IZH: s[2 + type] += w; break; /* g_ampa / g_gaba / g_nmda by type (Izhikevic .cpp:119-131) */
         case ORC_LIF_EXP2: s[1 + (type == 1)] += w; break;  /* ..._double.cpp:105-113 */
         case ORC_LIF_EXP3: s[1 + type] += w; break;         /* ..._triple.cpp:142-155 */
         case ORC_POISSON: s[0] += (type == 1) ? -w : w; break;
@@ -252,12 +261,17 @@ static void arrive(run_ctx* c, int32_t post, int32_t type, float w) {
 }
 
 /* Fixed-point arrivals are applied once per tick, before the update. */
-static void apply_fixed(run_ctx* c) {
-    const orc_model* m = &c->net->m;
+static void apply_fixed(run_ctx* c, int32_t n) {
     const float inv = 1.0f / c->scale;
-    for (int i = 0; i < m->n; ++i) {
+    for (int i = 0; i < n; ++i) {
+        const orc_model* m = model_of(c, i);
         float* s = c->x + (size_t)i * c->ns;
-        const float a0 = (float)c->acc0[i] * inv;
+        int32_t q0 = c->acc0[i];
+        if (c->typed) { /* the integers a network of this model alone accumulates */
+            if (m->model == ORC_LIF_VJ || m->model == ORC_LIF_EXP || m->model == ORC_POISSON) q0 -= c->acc1[i];
+            else if (m->model == ORC_IZH_CUR) q0 += c->acc1[i];
+        }
+        const float a0 = (float)q0 * inv;
         switch (m->model) {
             case ORC_LIF_VJ:
                 if (c->last[i] > m->t_ref) s[0] += a0;
@@ -266,6 +280,7 @@ static void apply_fixed(run_ctx* c) {
             case ORC_IZH:
                 s[2] += a0;
                 s[3] += (float)c->acc1[i] * inv;
+                s[4] += (float)c->acc2[i] * inv;
                 break;
             case ORC_LIF_EXP2:
                 s[1] += a0;
@@ -559,7 +574,14 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
                 int64_t* events) {
     orc_model* m = &net->m;
-    orc_model_prepare(m);
+    const int typed = net->npops > 0;
+    if (typed) { /* network-wide step and size for the learning code */
+        m->dt = net->pops[0].dt;
+        m->n = net->pop_begin[net->npops];
+        for (int k = 0; k < net->npops; ++k) orc_model_prepare(&net->pops[k]);
+    } else {
+        orc_model_prepare(m);
+    }
     const int n = m->n;
     const int D = net->d_slots;
     const int words = (n + 31) / 32;
@@ -568,7 +590,19 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     c.acc_mode = acc_mode;
     c.x = x;
     c.last = last;
-    c.ns = orc_nstate(m->model);
+    c.typed = typed;
+    uint8_t* pop_of = NULL;
+    c.ns = 0;
+    if (typed) {
+        pop_of = (uint8_t*)malloc((size_t)(n > 0 ? n : 1));
+        for (int k = 0; k < net->npops; ++k) {
+            for (int32_t i = net->pop_begin[k]; i < net->pop_begin[k + 1]; ++i) pop_of[i] = (uint8_t)k;
+            if (orc_nstate(net->pops[k].model) > c.ns) c.ns = orc_nstate(net->pops[k].model);
+        }
+    } else {
+        c.ns = orc_nstate(m->model);
+    }
+    c.pop_of = pop_of;
     c.scale = ldexpf(1.0f, net->fixed_shift);
     c.acc0 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
     c.acc1 = (int32_t*)calloc((size_t)n, sizeof(int32_t));
@@ -580,16 +614,23 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     if (net->pois_on) kmax = orc_poisson_thresholds(net->pois_lambda, tab, 64);
     const float inv_ltp = (float)(1.0 / (double)net->ltp_tau); /* STDP_State.cpp:6-7 */
     const float inv_ltd = (float)(1.0 / (double)net->ltd_tau);
-    const int iidx = orc_iext_index(m->model);
     int64_t ev = 0;
     int32_t status = 0;
-    /* conductance inputs present (CheckType): synapse types, dense weight signs, background */
-    m->cond = net->pois_on ? ORC_COND_EXC : 0;
-    if (net->row_ptr)
-        for (int64_t s = 0; s < net->row_ptr[n]; ++s) m->cond |= 1 << net->type[s];
-    if (net->dense_wt)
-        for (size_t k = 0; k < (size_t)n * n; ++k) m->cond |= net->dense_wt[k] < 0.0f ? ORC_COND_INH : ORC_COND_EXC;
-    m->cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
+    /* conductance inputs present in each model (CheckType): types of the synapses into it, dense
+     * weight signs, background */
+    if (typed) {
+        for (int k = 0; k < net->npops; ++k) net->pops[k].cond = net->pois_on ? ORC_COND_EXC : 0;
+        if (net->row_ptr)
+            for (int64_t s = 0; s < net->row_ptr[n]; ++s) net->pops[pop_of[net->post[s]]].cond |= 1 << net->type[s];
+        for (int k = 0; k < net->npops; ++k) net->pops[k].cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
+    } else {
+        m->cond = net->pois_on ? ORC_COND_EXC : 0;
+        if (net->row_ptr)
+            for (int64_t s = 0; s < net->row_ptr[n]; ++s) m->cond |= 1 << net->type[s];
+        if (net->dense_wt)
+            for (size_t k = 0; k < (size_t)n * n; ++k) m->cond |= net->dense_wt[k] < 0.0f ? ORC_COND_INH : ORC_COND_EXC;
+        m->cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
+    }
     rule_lists rl;
     memset(&rl, 0, sizeof(rl));
     const int rules_on = net->nrules > 0 && net->row_ptr;
@@ -600,10 +641,24 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
         memset(c.acc1, 0, sizeof(int32_t) * (size_t)n);
         memset(c.acc2, 0, sizeof(int32_t) * (size_t)n);
         memset(c.isyn, 0, sizeof(float) * (size_t)n);
+        /* inputs from outside the synapses: new external currents (stateful, ProcessCurrent); their
+         * arrivals follow the synaptic ones below */
+        if (net->inputs && t >= net->input_t0 && t < net->input_t0 + net->input_nsteps) {
+            for (int32_t k = net->input_ptr[t - net->input_t0]; k < net->input_ptr[t - net->input_t0 + 1]; ++k) {
+                const orc_input* in = &net->inputs[k];
+                if (in->kind == ORC_INPUT_CURRENT) {
+                    const int ix = orc_iext_index(model_of(&c, in->target)->model);
+                    if (ix >= 0) x[(size_t)in->target * c.ns + ix] = in->value;
+                }
+            }
+        }
         /* external current replaces I_ext for the tick (stateful current input) */
-        if (net->cur && iidx >= 0) {
+        if (net->cur) {
             const float* cr = net->cur + (size_t)(t - net->cur_t_begin) * n;
-            for (int i = 0; i < n; ++i) x[(size_t)i * c.ns + iidx] = cr[i];
+            for (int i = 0; i < n; ++i) {
+                const int ix = orc_iext_index(model_of(&c, i)->model);
+                if (ix >= 0) x[(size_t)i * c.ns + ix] = cr[i];
+            }
         }
         /* sparse arrivals: spikes fired at t-d through synapses of delay d
          * (group time = t_spike + delay, TimeDrivenInternalSpike.cpp:17; delivery
@@ -673,6 +728,13 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 r[2 * n + i] = 0;
             }
         }
+        /* arrivals from outside the synapses (not part of the push-ring image) */
+        if (net->inputs && t >= net->input_t0 && t < net->input_t0 + net->input_nsteps) {
+            for (int32_t k = net->input_ptr[t - net->input_t0]; k < net->input_ptr[t - net->input_t0 + 1]; ++k) {
+                const orc_input* in = &net->inputs[k];
+                if (in->kind != ORC_INPUT_CURRENT) arrive(&c, in->target, in->kind, in->value);
+            }
+        }
         /* Poisson background: count excitatory arrivals of weight w_bg */
         if (net->pois_on) {
             for (int i = 0; i < n; ++i) {
@@ -680,11 +742,23 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 for (int k = 0; k < cnt; ++k) arrive(&c, i, 0, net->w_bg);
             }
         }
-        if (acc_mode == ORC_ACC_FIXED) apply_fixed(&c);
+        if (acc_mode == ORC_ACC_FIXED) apply_fixed(&c, n);
 
-        m->step = t;
-        const int32_t nspk = update_f32(
-            m, x, last, (m->model == ORC_IZH_CUR || m->model == ORC_TRIGGER_RELAY) ? c.isyn : NULL, spikes);
+        int32_t nspk = 0;
+        if (typed) { /* each model over its own neurons, in population order (ascending ids) */
+            for (int k = 0; k < net->npops; ++k) {
+                orc_model* pm = &net->pops[k];
+                const int32_t b = net->pop_begin[k];
+                pm->step = t;
+                const int cur_in = pm->model == ORC_IZH_CUR || pm->model == ORC_TRIGGER_RELAY;
+                nspk += update_range_f32(pm, x + (size_t)b * c.ns, last + b, cur_in ? c.isyn + b : NULL, spikes + nspk,
+                                         c.ns, b);
+            }
+        } else {
+            m->step = t;
+            nspk = update_f32(m, x, last, (m->model == ORC_IZH_CUR || m->model == ORC_TRIGGER_RELAY) ? c.isyn : NULL,
+                              spikes);
+        }
 
         /* post-synaptic learning at the spike tick (TimeDrivenInternalSpike.cpp:70-82, STDP.cpp:37-61) */
         if (net->stdp_on && net->in_ptr) {
@@ -709,6 +783,10 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                     for (int64_t s = net->row_ptr[pre]; s < net->row_ptr[pre + 1]; ++s) {
                         int32_t* r = ring + (size_t)((t + net->delay[s]) % D) * 3 * n;
                         int32_t q = (int32_t)lrintf(net->w[s] * c.scale);
+                        if (typed) {
+                            r[(size_t)net->type[s] * n + net->post[s]] += q;
+                            continue;
+                        }
                         if (orc_negated(m->model, net->type[s])) q = -q;
                         r[(size_t)orc_channel(m->model, net->type[s]) * n + net->post[s]] += q;
                     }
@@ -746,6 +824,7 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     }
     if (events) *events += ev;
     if (rules_on) rule_lists_free(&rl);
+    free(pop_of);
     free(c.acc0);
     free(c.acc1);
     free(c.acc2);

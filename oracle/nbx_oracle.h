@@ -114,7 +114,29 @@ typedef struct orc_net {
     float* elig;             /* per synapse R-STDP eligibility */
     const float* stdp_maxw;  /* per synapse STDP weight bound (STDP.cpp:29-34), NULL: w_max for all */
     const int32_t* conn_index; /* per synapse connection index (Interconnections::Index), NULL: CSR position */
+    /* several neuron populations (npops > 0; m is then unused): population k owns neurons
+     * [pop_begin[k], pop_begin[k+1]) with its own model pops[k] (the legacy engine's several
+     * NeuronModel objects joined by Interconnections). Each synapse's arrival goes to its target's
+     * model; fixed-point arrivals accumulate per synapse type and each model folds them. The AoS
+     * state x[i * ns + j] uses ns = the largest orc_nstate of the populations. */
+    int32_t npops;
+    orc_model* pops;
+    const int32_t* pop_begin;
+    /* inputs from outside the synapses for ticks [input_t0, input_t0 + input_nsteps): those of tick t
+     * are inputs[input_ptr[t - input_t0] .. input_ptr[t - input_t0 + 1]) */
+    const struct orc_input* inputs;
+    const int32_t* input_ptr;
+    int32_t input_t0, input_nsteps;
 } orc_net;
+
+/* One external input of a tick: kind 0 / 1 / 2 an arrival of that synapse type (value the weight,
+ * as a synapse's arrival), kind 3 a new I_ext (value replaces the target's external current, before
+ * the tick's current table). */
+enum { ORC_INPUT_CURRENT = 3 };
+typedef struct orc_input {
+    int32_t target, kind;
+    float value;
+} orc_input;
 
 enum { ORC_RULE_RSTDP = 1, ORC_RULE_ADDITIVE = 2, ORC_RULE_CEREBELLAR = 3 };
 typedef struct orc_rule {

@@ -8,6 +8,7 @@ import numpy as np
 
 import oracle as O
 from neuronbridge_amd import _lib as L
+from neuronbridge_amd.network import PopulationsSpec
 
 MODEL_MAP = {L.LIF_VJ: O.LIF_VJ, L.LIF_EXP: O.LIF_EXP, L.IZH: O.IZH, L.IZH_CUR: O.IZH_CUR,
              L.LIF_EXP2: O.LIF_EXP2, L.LIF_EXP3: O.LIF_EXP3, L.POISSON: O.POISSON, L.TRIGGER_RELAY: O.TRIGGER_RELAY}
@@ -43,9 +44,17 @@ def initial_state(wl):
     n, ns = wl.spec.n, wl.spec.nstate
     if wl.x0 is not None:
         return np.ascontiguousarray(wl.x0, dtype=np.float32).reshape(n, ns).copy(), wl.last0.astype(np.int32).copy()
-    x = np.zeros((n, ns), dtype=np.float32)
     last = np.full(n, 10000, dtype=np.int32)
-    sp = wl.spec
+    if isinstance(wl.spec, PopulationsSpec):
+        x = np.zeros((n, ns), dtype=np.float32)
+        for b, ps in zip(wl.spec.begins, wl.spec.pops):
+            x[b:b + ps.n, :ps.nstate] = _initial_columns(ps, ps.n)
+        return x, last
+    return _initial_columns(wl.spec, n), last
+
+
+def _initial_columns(sp, n):
+    x = np.zeros((n, sp.nstate), dtype=np.float32)
     if sp.model in (L.LIF_VJ, L.LIF_EXP3):
         x[:, 0] = sp.v_reset
     elif sp.model in (L.LIF_EXP, L.LIF_EXP2):
@@ -55,7 +64,22 @@ def initial_state(wl):
     else:
         x[:, 0] = sp.v_rest
         x[:, 1] = (np.asarray(sp.b, np.float32) * np.float32(sp.v_rest)).astype(np.float32)
-    return x, last
+    return x
+
+
+def orc_model(sp, keep: list):
+    """An OrcModel of one population's parameters (per-neuron arrays kept alive in `keep`)."""
+    m = O.OrcModel(model=MODEL_MAP[sp.model], integrator=sp.integrator, n=sp.n, t_ref=sp.t_ref, dt=sp.dt,
+                   v_rest=sp.v_rest, tau=sp.tau, v_th=sp.v_th, r=sp.r, v_reset=sp.v_reset, g_tau=sp.g_tau,
+                   e_rev=sp.e_rev, izh_v_th=sp.izh_v_th, izh_r=sp.izh_r, e_ampa=sp.e_ampa, e_gaba=sp.e_gaba,
+                   ampa_tau=sp.ampa_tau, gaba_tau=sp.gaba_tau, nmda_tau=sp.nmda_tau,
+                   rate_bias=sp.rate_bias_hz, rate_gain=sp.rate_gain_hz)
+    if sp.a is not None:
+        for name in "abcd":
+            arr = np.ascontiguousarray(getattr(sp, name), dtype=np.float32)
+            keep.append(arr)
+            setattr(m, name, O.ptr(arr, C.c_float))
+    return m
 
 
 def raster_lists(raster: np.ndarray, n: int) -> list:
@@ -81,17 +105,17 @@ class OracleRun:
         sp = wl.spec
         self.keep = []
         k = self.keep.append
-        m = O.OrcModel(model=MODEL_MAP[sp.model], integrator=sp.integrator, n=sp.n, t_ref=sp.t_ref, dt=sp.dt,
-                       v_rest=sp.v_rest, tau=sp.tau, v_th=sp.v_th, r=sp.r, v_reset=sp.v_reset, g_tau=sp.g_tau,
-                       e_rev=sp.e_rev, izh_v_th=sp.izh_v_th, izh_r=sp.izh_r, e_ampa=sp.e_ampa, e_gaba=sp.e_gaba,
-                       ampa_tau=sp.ampa_tau, gaba_tau=sp.gaba_tau, nmda_tau=sp.nmda_tau,
-                       rate_bias=sp.rate_bias_hz, rate_gain=sp.rate_gain_hz)
-        if sp.a is not None:
-            for name in "abcd":
-                arr = np.ascontiguousarray(getattr(sp, name), dtype=np.float32)
-                k(arr)
-                setattr(m, name, O.ptr(arr, C.c_float))
+        multi = isinstance(sp, PopulationsSpec)
+        m = orc_model(sp.pops[0] if multi else sp, self.keep)
         net = O.OrcNet(m=m, d_slots=wl.d_slots, fixed_shift=auto_shift(wl) if fixed_shift is None else fixed_shift)
+        if multi:
+            pops = (O.OrcModel * len(sp.pops))(*[orc_model(ps, self.keep) for ps in sp.pops])
+            begins = np.ascontiguousarray(sp.begins + [sp.n], dtype=np.int32)
+            k(pops)
+            k(begins)
+            net.npops = len(sp.pops)
+            net.pops = C.cast(pops, C.POINTER(O.OrcModel))
+            net.pop_begin = O.ptr(begins, C.c_int32)
         self.shift = net.fixed_shift
         self.w = None
         if wl.syn is not None:
@@ -192,6 +216,23 @@ class OracleRun:
         self.ring = np.zeros((D, 3, n), dtype=np.int32)
         self.t = 0
         self.events = np.zeros(1, dtype=np.int64)
+
+    def set_inputs(self, per_step: list, t0: int) -> None:
+        """Inputs from outside the synapses for steps t0, t0 + 1, ...: per step a list of
+        (target, kind, value) with kind 0..2 an arrival of that synapse type, 3 a new I_ext."""
+        ptr = np.zeros(len(per_step) + 1, dtype=np.int32)
+        ptr[1:] = np.cumsum([len(x) for x in per_step])
+        arr = (O.OrcInput * max(1, int(ptr[-1])))()
+        j = 0
+        for step in per_step:
+            for tgt, kind, val in step:
+                arr[j] = O.OrcInput(int(tgt), int(kind), float(np.float32(val)))
+                j += 1
+        self._inputs = (arr, ptr)
+        self.net.inputs = C.cast(arr, C.c_void_p)
+        self.net.input_ptr = O.ptr(ptr, C.c_int32)
+        self.net.input_t0 = int(t0)
+        self.net.input_nsteps = len(per_step)
 
     def run(self, nsteps: int, acc_mode: int = O.ACC_FIXED, ring: bool = True, raster: bool = True):
         n = self.wl.spec.n
