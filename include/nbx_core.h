@@ -246,6 +246,19 @@ int32_t nbx_net_neuron_step_outputs(nbx_net* net, int32_t* out_ids, int32_t out_
 int32_t nbx_net_apply_post_learning(nbx_net* net);
 /* Push this step's firings into the delay ring and close the step. */
 int32_t nbx_net_commit_firings(nbx_net* net);
+/* Commit with the caller's full firing list of the current step (reference:
+ * CommitCurrentDeviceFiringsFromExternal(t, d_external_firing_ids, d_external_firing_count),
+ * GpuPropagationRuntime.h:96-99; device pointers). The list replaces the step's list: ids beyond the
+ * neurons that fired in the update are injected as firings of the step (as nbx_net_inject_spikes),
+ * and then the step is committed (nbx_net_commit_firings). A list that omits a neuron that fired,
+ * repeats an id or holds one out of range is rejected (NBX_ERR_INVALID, nothing committed). The
+ * pointers of nbx_net_firing_device_view commit the step's own list. */
+int32_t nbx_net_commit_external(nbx_net* net, int32_t t, const int32_t* d_ids, const int32_t* d_count);
+/* Device views of the current step's firings after its neuron update (reference:
+ * DenseUnifiedNeuronRuntime::device_full_firing_ids / device_full_firing_count /
+ * device_current_did_fire): the step's id list (unordered) and count inside the firing-history
+ * ring, and per-neuron did-fire flags (a uint8 array of the network, rebuilt by this call). */
+int32_t nbx_net_firing_device_view(nbx_net* net, int32_t** d_ids, int32_t** d_count, uint8_t** d_did_fire);
 /* Inject spikes as if `ids` fired at the current step (reference: QueueInterfaceSpike /
  * AddExternalSpikeActivity); call between nbx_net_neuron_step and nbx_net_commit_firings.
  * A neuron fires at most once per step: an id repeated in the call or already in the step's

@@ -152,6 +152,8 @@ SIGNATURES = {
     "nbx_net_reset_state": [_NET],
     "nbx_net_download_pending": [_NET, _f32p, C.c_int64, _i32p, _i32p],
     "nbx_net_pending_device_float": [_NET, C.POINTER(_vp)],
+    "nbx_net_commit_external": [_NET, C.c_int32, _vp, _vp],
+    "nbx_net_firing_device_view": [_NET, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)],
     "nbx_net_destroy": [_NET],
     "nbx_net_set_sparse": [_NET, C.POINTER(SparseDesc)],
     "nbx_net_set_dense": [_NET, _f32p, C.c_int32],

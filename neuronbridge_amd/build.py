@@ -69,18 +69,25 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-NATIVE_TESTS = ["dense_runtime_smoke"]
+NATIVE_TESTS = ["dense_runtime_smoke", "dense_runtime_parity"]
 
 
 def build_native_tests() -> list:
-    """g++ builds of the C++ checks under tests/native against libnbx.so (host code only)."""
+    """g++ builds of the C++ checks under tests/native (host code only): each links the reference-
+    interface runtime classes (shim/npgr_runtime.cpp) over libnbx.so, the HIP runtime for its own
+    device buffers, and the CPU oracle as the checker."""
     out = []
     src_dir = os.path.join(ROOT, "tests", "native")
+    oracle = os.path.join(ROOT, "oracle")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     for name in NATIVE_TESTS:
         exe = os.path.join(src_dir, name)
-        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
-               os.path.join(PKG, "shim"), os.path.join(src_dir, name + ".cpp"), "-o", exe, "-L", PKG, "-lnbx",
-               "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../neuronbridge_amd"]
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+               "-I", os.path.join(PKG, "shim", "npgr"), "-I", oracle, "-I", os.path.join(rocm, "include"),
+               os.path.join(src_dir, name + ".cpp"), os.path.join(PKG, "shim", "npgr_runtime.cpp"), "-o", exe,
+               "-L", PKG, "-lnbx", "-L", oracle, "-lorc", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+               "-Wl,-rpath," + PKG, "-Wl,-rpath," + oracle, "-Wl,-rpath,$ORIGIN/../../neuronbridge_amd",
+               "-Wl,-rpath,$ORIGIN/../../oracle", "-Wl,-rpath," + os.path.join(rocm, "lib")]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"g++ failed for {name}:\n{r.stdout}{r.stderr}")

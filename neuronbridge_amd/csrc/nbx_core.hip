@@ -1149,6 +1149,68 @@ int32_t nbx_net_inject_spikes(nbx_net* net, const int32_t* ids, int32_t count) {
     return NBX_OK;
 }
 
+int32_t nbx_net_firing_device_view(nbx_net* net, int32_t** d_ids, int32_t** d_count, uint8_t** d_did_fire) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 3 && net->phase != 4) {
+        set_error("the step's firings exist after its neuron update");
+        return NBX_ERR_STATE;
+    }
+    NBX_RC(device_scope(net));
+    const int32_t slot = net->t % net->D;
+    if (d_ids) *d_ids = net->d_hist + (size_t)slot * net->n;
+    if (d_count) *d_count = net->d_hist_cnt + slot;
+    if (d_did_fire) {
+        if (!net->d_fire_flags) NBX_RC(net->alloc(&net->d_fire_flags, (size_t)net->n));
+        NBX_HIP_TRY(launch_fire_flags(net->args(), net->t, net->d_fire_flags, net->stream));
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        *d_did_fire = net->d_fire_flags;
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_net_commit_external(nbx_net* net, int32_t t, const int32_t* d_ids, const int32_t* d_count) {
+    int32_t rc = require_finalized(net);
+    if (rc) return rc;
+    if (net->phase != 3 && net->phase != 4) {
+        set_error("commit must follow the neuron step");
+        return NBX_ERR_STATE;
+    }
+    if (t != net->t) {
+        set_error("commit for step " + std::to_string(t) + " while the network is at step " + std::to_string(net->t));
+        return NBX_ERR_STATE;
+    }
+    NBX_CHECK(d_ids && d_count, "null firing list");
+    NBX_RC(device_scope(net));
+    const int32_t slot = net->t % net->D;
+    if (d_ids != net->d_hist + (size_t)slot * net->n || d_count != net->d_hist_cnt + slot) {
+        int32_t cnt = 0;
+        NBX_HIP_TRY(hipStreamSynchronize(net->stream));
+        NBX_HIP_TRY(hipMemcpy(&cnt, d_count, sizeof(int32_t), hipMemcpyDeviceToHost));
+        NBX_CHECK(cnt >= 0 && cnt <= net->n, "external firing count out of range");
+        std::vector<int32_t> ext((size_t)std::max(cnt, 1)), own((size_t)net->n);
+        if (cnt) NBX_HIP_TRY(hipMemcpy(ext.data(), d_ids, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        int32_t own_cnt = 0;
+        NBX_RC(nbx_net_get_history(net, net->t, own.data(), net->n, &own_cnt));
+        std::vector<uint8_t> in_ext((size_t)net->n, 0);
+        for (int32_t k = 0; k < cnt; ++k) {
+            NBX_CHECK(ext[(size_t)k] >= 0 && ext[(size_t)k] < net->n, "external firing id out of range");
+            NBX_CHECK(!in_ext[(size_t)ext[(size_t)k]], "external firing list repeats an id");
+            in_ext[(size_t)ext[(size_t)k]] = 1;
+        }
+        std::vector<uint8_t> in_own((size_t)net->n, 0);
+        for (int32_t k = 0; k < own_cnt; ++k) {
+            NBX_CHECK(in_ext[(size_t)own[(size_t)k]], "external firing list omits a neuron that fired in the update");
+            in_own[(size_t)own[(size_t)k]] = 1;
+        }
+        std::vector<int32_t> extra;
+        for (int32_t k = 0; k < cnt; ++k)
+            if (!in_own[(size_t)ext[(size_t)k]]) extra.push_back(ext[(size_t)k]);
+        if (!extra.empty()) NBX_RC(nbx_net_inject_spikes(net, extra.data(), (int32_t)extra.size()));
+    }
+    return nbx_net_commit_firings(net);
+}
+
 int32_t nbx_net_apply_post_learning(nbx_net* net) {
     int32_t rc = require_finalized(net);
     if (rc) return rc;

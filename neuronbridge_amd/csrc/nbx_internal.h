@@ -552,5 +552,6 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t count, hipStream_t
 hipError_t launch_inputs(const StepArgs& A, const InputRec* d_recs, int32_t count, const float* d_vals, int32_t t,
                          float qmax, hipStream_t s);
 hipError_t launch_pending_float(const StepArgs& A, int32_t t, float* d_out, hipStream_t s);
+hipError_t launch_fire_flags(const StepArgs& A, int32_t t, uint8_t* d_flags, hipStream_t s);
 
 }  // namespace nbx

@@ -149,6 +149,7 @@ struct nbx_net {
     nbx::InputRec* d_inputs = nullptr;  // device staging of input records (grown on demand)
     int64_t inputs_cap = 0;
     float* d_pending_f = nullptr;       // float snapshot of a step's pending arrivals
+    uint8_t* d_fire_flags = nullptr;    // did-fire flags of a step (device view)
     std::vector<uint8_t> out_mask;      // output neurons (empty: all)
     int32_t crows = 0;                  // cerebellar arrival-history rows
     int32_t *d_ring = nullptr, *d_hist = nullptr, *d_hist_cnt = nullptr;

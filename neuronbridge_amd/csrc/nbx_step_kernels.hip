@@ -581,6 +581,19 @@ __global__ void nbx_pending_float(StepArgs A, int32_t t, float* out) {
         out[j] = (float)r0[j] * A.inv_scale;
 }
 
+// Did-fire flags of step t from its firing list (the flags were cleared before).
+__global__ void nbx_fire_flags(StepArgs A, int32_t t, uint8_t* flags) {
+    const int32_t slot = t % A.D;
+    const int32_t cnt = A.hist_cnt[slot];
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x)
+        flags[A.hist[(int64_t)slot * A.n + j]] = 1;
+}
+
+__global__ void nbx_fill_u8(uint8_t* p, int64_t count) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = 0;
+}
+
 __global__ void nbx_fill_i32(int32_t* p, int32_t value, int64_t count) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (int64_t)gridDim.x * blockDim.x)
         p[j] = value;
@@ -662,6 +675,12 @@ hipError_t launch_inputs(const StepArgs& A, const InputRec* d_recs, int32_t coun
 hipError_t launch_pending_float(const StepArgs& A, int32_t t, float* d_out, hipStream_t s) {
     const int64_t total = (int64_t)A.nch * A.n;
     nbx_pending_float<<<(int)std::min<int64_t>((total + 255) / 256, 4096), 256, 0, s>>>(A, t, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fire_flags(const StepArgs& A, int32_t t, uint8_t* d_flags, hipStream_t s) {
+    nbx_fill_u8<<<(int)std::min<int64_t>(((int64_t)A.n + 255) / 256, 4096), 256, 0, s>>>(d_flags, A.n);
+    nbx_fire_flags<<<std::max(1, std::min((A.n + 255) / 256, 1024)), 256, 0, s>>>(A, t, d_flags);
     return hipGetLastError();
 }
 

@@ -1,4 +1,6 @@
-"""Runs the native C++ checks of the C++ runtime mirror (tests/native/*.cpp, built by build())."""
+"""Runs the native C++ checks of the reference-interface runtime classes (tests/native/*.cpp, built by
+build()): the delay known answer and STDP trajectory, and the two-population RunStep parity run
+against the CPU oracle."""
 import os
 import subprocess
 
@@ -8,7 +10,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-@pytest.mark.parametrize("name", ["dense_runtime_smoke"])
+@pytest.mark.parametrize("name", ["dense_runtime_smoke", "dense_runtime_parity"])
 def test_native_executable(gpu, name):
     exe = os.path.join(HERE, "native", name)
     assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
