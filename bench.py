@@ -15,8 +15,9 @@ HBM before the timed region. Prints one JSON line.
              --replay-steps steps injected in one launch; 8 B per event + 4 B per output arrival), and
              a third replay STDP (the raster of --stdp-steps steps of the STDP network in one pass over
              its synapses; 36 B per plastic synapse).
-  cpu_baseline  the CPU oracle (reference-faithful per-event float delivery), one core,
-             on the first --cpu-steps steps of the same network
+  cpu_baseline  the CPU oracle in the legacy engine's queue-parallel structure (per-event float
+             delivery), on every usable core for --cpu-steps steps and on one core for
+             --cpu-steps-one steps of the same network, best of 3 after a warm-up, lscpu recorded
 """
 from __future__ import annotations
 
@@ -54,19 +55,65 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, steps: int) -> dict:
+def host_cpu() -> dict:
+    """lscpu model, sockets, cores and threads of this host, and the CPUs this process may use."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None, "threads_per_core": None}
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core"}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except Exception:
+        pass
+    info["usable_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def cpu_baseline(wl, steps: int, steps_one: int) -> dict:
+    """The CPU oracle in the legacy engine's structure (a thread per queue of target neurons,
+    per-queue synapse lists, per-event float delivery, orc_run_queues): `steps` steps on every usable
+    core and `steps_one` steps on one; each is the best of 3 runs from the initial state after one
+    warm-up run; the stepping loop only is timed (no network build)."""
+    import ctypes as C
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import oracle as O
     from harness import OracleRun
-    run = OracleRun(wl)
-    t0 = time.perf_counter()
-    run.run(steps, O.ACC_FLOAT_EVENTS, ring=False, raster=False)
-    dt = time.perf_counter() - t0
+    cpu = host_cpu()
+    threads = max(1, min(cpu["usable_cpus"], int(os.environ.get("OMP_NUM_THREADS", "0") or cpu["usable_cpus"])))
     n = wl.spec.n
-    return {"value": n * steps / dt, "unit": "neuron-steps/s", "cores": 1, "kind": "port",
-            "syn_events_per_s": float(run.events[0]) / dt,
-            "sample": f"CPU oracle, reference-faithful per-event float delivery, steps 0..{steps - 1} of the "
-                      f"same {n}-neuron / {wl.syn.count}-synapse network, 1 thread, {dt:.1f} s"}
+
+    def best_of(nq: int, nsteps: int):
+        times, events = [], 0
+        for rep in range(4):  # 1 warm-up + 3 timed
+            run = OracleRun(wl)
+            counts = np.zeros(nsteps, np.int32)
+            secs = C.c_double()
+            st = O.lib().orc_run_queues(C.byref(run.net), nq, O.ptr(run.x, C.c_float), O.ptr(run.last, C.c_int32),
+                                        O.ptr(run.hist, C.c_int32), O.ptr(run.hist_cnt, C.c_int32), 0, nsteps,
+                                        O.ptr(counts, C.c_int32), O.ptr(run.events, C.c_int64), C.byref(secs))
+            assert st == 0, "orc_run_queues failed"
+            if rep:
+                times.append(secs.value)
+            events = int(run.events[0])
+        return min(times), events
+
+    t_all, ev_all = best_of(threads, steps)
+    t_one, ev_one = best_of(1, steps_one)
+    return {"value": n * steps / t_all, "unit": "neuron-steps/s", "cores": threads, "kind": "port",
+            "syn_events_per_s": ev_all / t_all,
+            "sample": f"CPU oracle in the legacy engine's queue structure (one OpenMP thread per queue of target "
+                      f"neurons, per-event float delivery), steps 0..{steps - 1} of the same {n}-neuron / "
+                      f"{wl.syn.count}-synapse network on {threads} threads, best of 3 after a warm-up: {t_all:.2f} s",
+            "single_thread": {"value": n * steps_one / t_one, "unit": "neuron-steps/s", "cores": 1,
+                              "syn_events_per_s": ev_one / t_one,
+                              "sample": f"steps 0..{steps_one - 1} on 1 thread, best of 3 after a warm-up: "
+                                        f"{t_one:.2f} s"},
+            "cpu": cpu}
 
 
 def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
@@ -455,7 +502,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "graph", "kernels"])
     ap.add_argument("--integrator", default="rk4", choices=["rk4", "euler"])
-    ap.add_argument("--cpu-steps", type=int, default=3000)
+    ap.add_argument("--cpu-steps", type=int, default=10000, help="CPU baseline steps on every usable core")
+    ap.add_argument("--cpu-steps-one", type=int, default=1000, help="CPU baseline steps on one core")
     ap.add_argument("--replay-reps", type=int, default=5)
     ap.add_argument("--replay-steps", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
@@ -571,7 +619,7 @@ def main() -> int:
         log(f"rules: {out['rules']}")
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU oracle baseline over {args.cpu_steps} steps ...")
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps, max(1, args.cpu_steps_one))
     print(json.dumps(out), flush=True)
     return 0
 

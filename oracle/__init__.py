@@ -119,6 +119,9 @@ def lib():
         L.orc_stdp_replay.argtypes = [C.c_float] * 7 + [_u8p, _u8p, C.c_int32, _f32p]
         L.orc_replay_stdp.restype = C.c_int64
         L.orc_replay_stdp.argtypes = [C.POINTER(OrcNet), _i32p, _i32p, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_run_queues.restype = C.c_int32
+        L.orc_run_queues.argtypes = [C.POINTER(OrcNet), C.c_int32, _f32p, _i32p, _i32p, _i32p, C.c_int32, C.c_int32,
+                                     _i32p, _i64p, C.POINTER(C.c_double)]
         L.orc_stable_unit_random.restype = C.c_float
         L.orc_stable_unit_random.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32]
         L.orc_cerebellar_kernel.restype = C.c_int32

@@ -31,9 +31,15 @@
  *   orc_stable_unit_random bit-exact against the reference's own StableUnitRandom compiled from
  *                          its source (oracle/_ref, tests/golden/stable_unit_random.npz)
  */
+#define _POSIX_C_SOURCE 200809L /* clock_gettime */
 #include "nbx_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <math.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -831,6 +837,150 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
     free(c.isyn);
     free(spikes);
     return status;
+}
+
+/* ---------------------------------------------------------------- queue-parallel run
+ * The legacy engine's OpenMP structure (the CPU baseline): neurons are split into nq queues, one
+ * thread each (Simulation.cu:635-639, NumberOfOpenMPQueues); each presynaptic neuron keeps its
+ * synapses per target queue (NeuronPropogationStructure.h:16-24, built per queue by
+ * Network.cu:483-560), so a queue's thread delivers every spike's arrivals into its own neurons
+ * as per-event float adds (ProcessSpike, PropogatedSpikeGroup.cpp:33-71), draws their background,
+ * updates them, and the queues meet once per tick. Per target the events arrive in the order of
+ * orc_run's float-event mode, so the results are identical to it. Supports what the sparse
+ * configurations use (sparse synapses, Poisson background, external current table; no learning,
+ * dense weights, populations or external inputs): returns -3 otherwise. */
+static double orc_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int32_t orc_run_queues(orc_net* net, int32_t nq, float* x, int32_t* last, int32_t* hist, int32_t* hist_cnt,
+                       int32_t t0, int32_t nticks, int32_t* spike_counts, int64_t* events, double* loop_seconds) {
+    if (nq < 1 || net->stdp_on || net->nrules > 0 || net->dense_wt || net->npops > 0 || net->inputs || !net->row_ptr)
+        return -3;
+    orc_model* m = &net->m;
+    orc_model_prepare(m);
+    const int n = m->n, D = net->d_slots, ns = orc_nstate(m->model);
+    if (nq > n) nq = n;
+    m->cond = net->pois_on ? ORC_COND_EXC : 0;
+    for (int64_t s = 0; s < net->row_ptr[n]; ++s) m->cond |= 1 << net->type[s];
+    m->cond &= ORC_COND_EXC | ORC_COND_INH | ORC_COND_NMDA;
+    int32_t* qbeg = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nq + 1));
+    for (int q = 0; q <= nq; ++q) qbeg[q] = (int32_t)((int64_t)n * q / nq);
+    /* per queue: CSR by presynaptic neuron of the synapses into the queue, in CSR order */
+    int64_t** qptr = (int64_t**)malloc(sizeof(int64_t*) * (size_t)nq);
+    int64_t** qsyn = (int64_t**)malloc(sizeof(int64_t*) * (size_t)nq);
+    uint32_t tab[64];
+    const int32_t kmax = net->pois_on ? orc_poisson_thresholds(net->pois_lambda, tab, 64) : 0;
+    float* isyn = (float*)calloc((size_t)n, sizeof(float));
+    int32_t* spk = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* qcnt = (int32_t*)calloc((size_t)nq, sizeof(int32_t));
+    const int iidx = orc_iext_index(m->model);
+    int64_t ev = 0;
+#pragma omp parallel num_threads(nq)
+    {
+#ifdef _OPENMP
+        const int q = omp_get_thread_num();
+#else
+        const int q = 0;
+#endif
+        int64_t cnt = 0;
+        for (int64_t s = 0; s < net->row_ptr[n]; ++s)
+            if (net->post[s] >= qbeg[q] && net->post[s] < qbeg[q + 1]) ++cnt;
+        qptr[q] = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+        qsyn[q] = (int64_t*)malloc(sizeof(int64_t) * (size_t)(cnt > 0 ? cnt : 1));
+        int64_t k = 0;
+        for (int p = 0; p < n; ++p) {
+            qptr[q][p] = k;
+            for (int64_t s = net->row_ptr[p]; s < net->row_ptr[p + 1]; ++s)
+                if (net->post[s] >= qbeg[q] && net->post[s] < qbeg[q + 1]) qsyn[q][k++] = s;
+        }
+        qptr[q][n] = k;
+        run_ctx c;
+        memset(&c, 0, sizeof(c));
+        c.net = net;
+        c.acc_mode = ORC_ACC_FLOAT_EVENTS;
+        c.x = x;
+        c.last = last;
+        c.isyn = isyn;
+        c.ns = ns;
+        c.scale = 1.0f;
+        const int32_t b = qbeg[q], e = qbeg[q + 1];
+        double t_begin = 0.0;
+#pragma omp barrier
+        if (q == 0) t_begin = orc_now(); /* the stepping loop only, after the per-queue lists */
+        for (int32_t t = t0; t < t0 + nticks; ++t) {
+            for (int32_t i = b; i < e; ++i) isyn[i] = 0.0f;
+            if (net->cur && iidx >= 0) {
+                const float* cr = net->cur + (size_t)(t - net->cur_t_begin) * n;
+                for (int32_t i = b; i < e; ++i) x[(size_t)i * ns + iidx] = cr[i];
+            }
+            for (int d = 1; d < D; ++d) { /* spikes of t - d through this queue's delay-d synapses */
+                const int32_t ts = t - d;
+                if (ts < 0) continue;
+                const int slot = ts % D;
+                const int32_t* hl = hist + (size_t)slot * n;
+                for (int kk = 0; kk < hist_cnt[slot]; ++kk) {
+                    const int32_t pre = hl[kk];
+                    int64_t lo = qptr[q][pre], hi = qptr[q][pre + 1];
+                    while (lo < hi) {
+                        const int64_t mid = lo + (hi - lo) / 2;
+                        if (net->delay[qsyn[q][mid]] < d) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    for (int64_t j = lo; j < qptr[q][pre + 1] && net->delay[qsyn[q][j]] == d; ++j) {
+                        const int64_t s = qsyn[q][j];
+                        arrive(&c, net->post[s], net->type[s], net->w[s]);
+                    }
+                }
+            }
+            if (net->pois_on)
+                for (int32_t i = b; i < e; ++i) {
+                    const int32_t pc = orc_poisson_count(net->seed_lo, net->seed_hi, i, t, tab, kmax);
+                    for (int kk = 0; kk < pc; ++kk) arrive(&c, i, 0, net->w_bg);
+                }
+            orc_model mq = *m;
+            mq.n = e - b;
+            mq.step = t;
+            if (mq.a) {
+                mq.a = m->a + b;
+                mq.b = m->b + b;
+                mq.c = m->c + b;
+                mq.d = m->d + b;
+            }
+            const int cur_in = m->model == ORC_IZH_CUR || m->model == ORC_TRIGGER_RELAY;
+            qcnt[q] = update_range_f32(&mq, x + (size_t)b * ns, last + b, cur_in ? isyn + b : NULL, spk + b, ns, b);
+#pragma omp barrier
+#pragma omp single
+            { /* the tick's spike list in ascending order, its events and counts */
+                const int slot = t % D;
+                int32_t tot = 0;
+                for (int qq = 0; qq < nq; ++qq) {
+                    memmove(hist + (size_t)slot * n + tot, spk + qbeg[qq], sizeof(int32_t) * (size_t)qcnt[qq]);
+                    tot += qcnt[qq];
+                }
+                hist_cnt[slot] = tot;
+                for (int32_t kk = 0; kk < tot; ++kk) {
+                    const int32_t p = hist[(size_t)slot * n + kk];
+                    ev += net->row_ptr[p + 1] - net->row_ptr[p];
+                }
+                if (spike_counts) spike_counts[t - t0] = tot;
+            }
+        }
+#pragma omp barrier
+        if (q == 0 && loop_seconds) *loop_seconds = orc_now() - t_begin;
+        free(qptr[q]);
+        free(qsyn[q]);
+    }
+    if (events) *events += ev;
+    free(qptr);
+    free(qsyn);
+    free(qbeg);
+    free(isyn);
+    free(spk);
+    free(qcnt);
+    return 0;
 }
 
 /* Replay STDP over a recorded raster: the STDP part of orc_run (STDP.cpp:19-61 at the delivery

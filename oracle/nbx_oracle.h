@@ -186,6 +186,14 @@ int32_t orc_run(orc_net* net, int32_t acc_mode, float* x, int32_t* last, int32_t
                 int32_t* ring, int32_t t0, int32_t nticks, int32_t* spike_counts, uint32_t* raster,
                 int64_t* events);
 
+/* The legacy engine's queue-parallel OpenMP run (one thread per queue of neurons, per-queue
+ * synapse lists, per-event float delivery): the CPU baseline. Results equal orc_run in
+ * ORC_ACC_FLOAT_EVENTS mode. Sparse networks without learning, dense weights, populations or
+ * external inputs (else returns -3). loop_seconds (may be NULL): wall time of the stepping loop,
+ * without building the per-queue synapse lists. */
+int32_t orc_run_queues(orc_net* net, int32_t nq, float* x, int32_t* last, int32_t* hist, int32_t* hist_cnt,
+                       int32_t t0, int32_t nticks, int32_t* spike_counts, int64_t* events, double* loop_seconds);
+
 /* Replay STDP over a recorded raster: ids[tick_ptr[k] .. tick_ptr[k+1]) fired at step
  * t0 - lead + k (k < lead + nsteps); applies the pair-STDP updates of steps [t0, t0 + nsteps)
  * in orc_run's order (arrivals of a step, then its spikes). Returns the number of updates. */
