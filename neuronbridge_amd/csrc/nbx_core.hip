@@ -334,7 +334,7 @@ int32_t check_device_error(nbx_net* net) {
         return NBX_ERR_INVALID;
     }
     if (e != 0) {
-        set_error("device-side error flag " + std::to_string(e) + " (grid barrier time-out)");
+        set_error("device-side error flag " + std::to_string(e) + " (a wait of the persistent engine timed out)");
         return NBX_ERR_STATE;
     }
     return NBX_OK;

@@ -34,7 +34,8 @@ struct PersistentLayout {
     int64_t max_row = 0;             // longest presynaptic row
     int32_t* d_off = nullptr;        // [n][nr * gs]
     uint2* d_seg = nullptr;          // {post_local | delay << 16 (6 bits) | channel << 22 (2 bits) | (block & 127) << 25, q}
-    int32_t* d_bar = nullptr;        // grid barrier counter
+    int32_t* d_bar = nullptr;        // grid barrier counters (zero between launches)
+    bool launch_checked = false;     // LDS attribute set and co-residency of the B workgroups checked
 };
 
 // Fixed-slot layout of replay delivery: the synapses of presynaptic neuron i onto the m*P

@@ -34,6 +34,7 @@ constexpr int kMaxThreads = 1024;
 constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
 constexpr int kLanes = 4;                  // lanes sharing one spike's synapse segment
+constexpr int kTabCap = 64;                // Poisson threshold table entries held in LDS
 constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;  // replay slot entry: unused
 constexpr uint32_t kSlotLink = 0x80000000u;   // replay slot entry: link to the overflow list
 constexpr int kSlotCap = 16;                  // entries per replay slot (128 bytes)
@@ -61,6 +62,7 @@ struct Barrier {
     int32_t group[kXcds * kLine];
     int32_t top[kLine];
     int32_t gen[kXcds * kLine];
+    int32_t retired[kLine];  // workgroups that have left the kernel
 };
 
 struct PersistArgs {
@@ -100,19 +102,39 @@ __device__ inline void grid_arrive(Barrier* bar, int32_t k, int32_t nblocks) {
     }
 }
 
+// Called once by thread 0 of every workgroup after its last barrier access (aborted or not): the
+// last workgroup to retire returns the barrier to zero, so the next launch on the stream starts
+// from a clean barrier without a separate clearing dispatch. Every earlier barrier atomic of a
+// workgroup returned its value (the gen stores are waited for here) before its retirement is
+// counted, so no barrier access of this launch is performed after the reset.
+__device__ inline void grid_retire(Barrier* bar, int32_t nblocks) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const int32_t old = __hip_atomic_fetch_add(&bar->retired[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (old == nblocks - 1) {
+        for (int q = 0; q < kXcds; ++q) {
+            st_dev(&bar->group[q * kLine], 0);
+            st_dev(&bar->gen[q * kLine], 0);
+        }
+        st_dev(&bar->top[0], 0);
+        st_dev(&bar->retired[0], 0);
+    }
+}
+
 __device__ inline bool grid_wait(Barrier* bar, int32_t k, int32_t* err, int32_t* s_abort) {
     if (threadIdx.x == 0) {
         const int32_t g = blockIdx.x % kXcds;
         uint32_t spins = 0;
         bool abort = false;
         while (ld_dev(&bar->gen[g * kLine]) < k + 1) {
-            if (ld_dev(err) != 0) {
+            if ((ld_dev(err) & kErrBarrier) != 0) {  // another workgroup's wait timed out
                 abort = true;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
-                atomicExch(err, 1);
+                atomicOr(err, kErrBarrier);
                 abort = true;
                 break;
             }
@@ -130,6 +152,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     __shared__ int32_t s_base;
     __shared__ int32_t s_abort;
     __shared__ unsigned long long s_ev;
+    __shared__ uint32_t s_tab[kTabCap];
     const StepArgs& A = X.A;
     const int32_t P = X.P, D = A.D, nch = A.nch, n = A.n;
     const int32_t b = block_of(blockIdx.x, X.B);
@@ -141,11 +164,30 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
     const int32_t i0 = (b / X.rb) * X.gs + b % X.rb;  // this block's entry in an offset row; i0 + 1 ends it
 
-    // load this block's slice of the delay ring and its neurons' state
-    for (int32_t j = tid; j < D * nch * P; j += T) {
-        const int32_t pl = j % P, sc = j / P;  // sc = slot * nch + ch
-        s_ring[j] = (pl < np) ? A.ring[(int64_t)sc * n + p0 + pl] : 0;
+    // load this block's slice of the delay ring (rows of P neurons, many loads in flight), the
+    // Poisson thresholds and its neurons' state
+    {
+        const int32_t rows = D * nch;
+        for (int32_t r0 = 0; r0 < rows; r0 += 8) {
+            int32_t v[8][NPT];
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int u = 0; u < NPT; ++u) {
+                    const int32_t pl = tid + u * T;
+                    v[r][u] = (r0 + r < rows && pl < np) ? A.ring[(int64_t)(r0 + r) * n + p0 + pl] : 0;
+                }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int u = 0; u < NPT; ++u) {
+                    const int32_t pl = tid + u * T;
+                    if (r0 + r < rows && pl < P) s_ring[(r0 + r) * P + pl] = v[r][u];
+                }
+        }
     }
+    const int32_t kmax = min(A.kmax, kTabCap - 1);
+    if (A.pois && tid <= kmax) s_tab[tid] = A.pois_tab[tid];
     if (tid == 0) s_ev = 0;
     float V[NPT], S1[NPT], GA[NPT], GG[NPT], I[NPT];
     int32_t last[NPT];
@@ -174,14 +216,15 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
 #pragma unroll
         for (int u = 0; u < NPT; ++u) {
             const int32_t pl = tid + u * T;
-            pc[u] = (A.pois && pl < np) ? poisson_count(A.seed_lo, A.seed_hi, p0 + pl, t, A.pois_tab, A.kmax) : 0;
+            pc[u] = (A.pois && pl < np) ? poisson_count(A.seed_lo, A.seed_hi, p0 + pl, t, s_tab, kmax) : 0;
         }
     };
+    __syncthreads();  // s_tab and s_ring are loaded
     draw(X.t0);
-    __syncthreads();
 
     for (int32_t k = 0; k <= X.nsteps; ++k) {
         const int32_t t = X.t0 + k;
+        const int32_t sb = t % D;  // ring slot of step t
         if (k > 0) {
             // spikes of step t-1 were complete when the previous barrier opened
             const int32_t ps = (t - 1) % D;
@@ -212,9 +255,9 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                         e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
                     }
                 auto deposit = [&](const uint2 x) {
-                    const int32_t d = (int32_t)((x.x >> 16) & 0x3Fu);
-                    atomicAdd(&s_ring[(((t - 1 + d) % D) * nch + (int32_t)((x.x >> 22) & 3u)) * P +
-                                      (int32_t)(x.x & 0xFFFFu)],
+                    int32_t s = sb + (int32_t)((x.x >> 16) & 0x3Fu) - 1;  // slot of step t - 1 + delay
+                    s -= (s >= D) ? D : 0;
+                    atomicAdd(&s_ring[(s * nch + (int32_t)((x.x >> 22) & 3u)) * P + (int32_t)(x.x & 0xFFFFu)],
                               (int32_t)x.y);
                 };
 #pragma unroll
@@ -302,7 +345,10 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     }
     if (ev) atomicAdd(&s_ev, ev);
     __syncthreads();
-    if (tid == 0 && s_ev) atomicAdd(A.events, s_ev);
+    if (tid == 0) {
+        if (s_ev) atomicAdd(A.events, s_ev);
+        grid_retire(X.bar, X.B);
+    }
 }
 
 // Replay delivery on the post-blocked layout.
@@ -939,6 +985,8 @@ int32_t build_persistent_layout(nbx_net* net) {
     if (rc) return rc;
     rc = net->alloc(&L.d_bar, sizeof(Barrier) / sizeof(int32_t));
     if (rc) return rc;
+    // zeroed once here; every launch leaves it zero again (grid_retire)
+    NBX_HIP_TRY(launch_fill_i32(L.d_bar, 0, (int64_t)(sizeof(Barrier) / sizeof(int32_t)), net->stream));
     L.B = B;
     L.P = P;
     L.threads = g.threads;
@@ -1038,17 +1086,18 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     if (nsteps == 0) return NBX_OK;
     PersistentLayout& L = net->pl;
     void* fn = persistent_kernel_for(net->md.model, L.npt);
-    int per_cu = 0, cus = 0;
-    NBX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, L.threads, (size_t)L.lds_bytes));
-    NBX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
-    if ((int64_t)per_cu * cus < L.B) {
-        set_error("persistent engine: " + std::to_string(L.B) + " workgroups cannot be co-resident");
-        return NBX_ERR_UNSUPPORTED;
+    if (!L.launch_checked) {  // the kernel, its block size and LDS are fixed with the layout
+        int per_cu = 0, cus = 0;
+        if (L.lds_bytes > 64 * 1024)
+            NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.lds_bytes));
+        NBX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, L.threads, (size_t)L.lds_bytes));
+        NBX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
+        if ((int64_t)per_cu * cus < L.B) {
+            set_error("persistent engine: " + std::to_string(L.B) + " workgroups cannot be co-resident");
+            return NBX_ERR_UNSUPPORTED;
+        }
+        L.launch_checked = true;
     }
-    if (L.lds_bytes > 64 * 1024)
-        NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.lds_bytes));
-    nbx_zero_i32<<<1, 256, 0, net->stream>>>(L.d_bar, (int64_t)(sizeof(Barrier) / sizeof(int32_t)));
-    NBX_HIP_TRY(hipGetLastError());
     PersistArgs X;
     X.A = net->args();
     X.off = L.d_off;
