@@ -30,11 +30,12 @@ namespace nbx {
 namespace {
 
 constexpr int kXcds = 8;
-constexpr int kMaxThreads = 1024;
+constexpr int kMaxThreads = 512;  // persistent workgroup bound: 256 VGPRs per thread, no spills
 constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
 constexpr int kLanes = 4;                  // lanes sharing one spike's synapse segment
 constexpr int kTabCap = 64;                // Poisson threshold table entries held in LDS
+constexpr int kRound = 2;                  // spikes per lane group in flight per delivery round
 constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;  // replay slot entry: unused
 constexpr uint32_t kSlotLink = 0x80000000u;   // replay slot entry: link to the overflow list
 constexpr int kSlotCap = 16;                  // entries per replay slot (128 bytes)
@@ -45,25 +46,25 @@ __device__ inline int32_t block_of(int32_t w, int32_t B) {
     return (w % kXcds) * per + (w / kXcds);
 }
 
-// Grid barrier, two-level: workgroups of one XCD group (blockIdx % 8) arrive on their
-// group counter; the last arrival of a group arrives on the top counter; the last top
-// arrival raises every group's generation flag. Counters only grow (targets scale with
-// the step), each sits on its own 128-byte line, and the spin is bounded: on time-out
-// the error flag is raised and every waiter leaves.
+// Step synchronisation by counters. Workgroup w belongs to group w % 8 (its XCD); per step and
+// group one 64-bit counter {spikes << 32 | arrivals} sits on its own 128-byte line. Publishing
+// step t, a workgroup adds {its spikes << 32 | 1} to its group's counter of slot t % D with one
+// returning atomic: the old value's high half is where its spike ids go in the group's list, and
+// its arrival is counted. A workgroup needs step t when the 8 counters of the slot all show their
+// group's workgroup count; it polls those 8 words (8 lanes, one load each per round) and then has
+// the step's per-group spike counts too. Few workgroups share one counter (atomics on one address
+// serialise) and nobody polls a line that every workgroup polls more than once per round.
 //
-// Memory ordering: the only data other workgroups read within a launch are the firing
-// ids and counts of the previous step. They are written with device-scope atomic
-// stores/adds (coherence-point writes, `sc1`) and read with device-scope atomic loads,
-// and every wave executes s_waitcnt 0 before the workgroup barrier that precedes the
-// arrival, so every store of the workgroup has been performed before its arrival is
-// counted. No L2 write-back or invalidate is needed.
-constexpr int kLine = 32;  // int32 per 128-byte line
-struct Barrier {
-    int32_t group[kXcds * kLine];
-    int32_t top[kLine];
-    int32_t gen[kXcds * kLine];
-    int32_t retired[kLine];  // workgroups that have left the kernel
-};
+// Memory ordering: a workgroup's list entries are stored after its atomic, so a reader may see
+// the arrival first; entries are 64-bit {step tag << 32 | id}, written and read with device-scope
+// atomic stores and loads (single-copy atomic), and a reader re-reads an entry until it carries
+// the step's tag. Tags are unique over the network's lifetime (the host advances the epoch by
+// nsteps + 1 per launch). The counters of slot (t + 1) % D are cleared by workgroup 0 in step t,
+// before its own atomic for step t (it waits for the clearing stores), so every atomic for step
+// t + 1 (made after seeing workgroup 0's arrival for step t) finds them cleared; nobody reads
+// them any more (their last readers finished before step t - 1 ended). Spins are bounded: on
+// time-out the error flag's kErrBarrier bit is raised and every waiter that sees it leaves.
+constexpr int kCtrLine = 16;  // uint64 per 128-byte line
 
 struct PersistArgs {
     StepArgs A;
@@ -72,83 +73,106 @@ struct PersistArgs {
     int32_t B, P;
     int32_t rb, gs, os;  // blocks per region, ints per region group, ints per offset row
     int32_t t0, nsteps;
-    Barrier* bar;
+    unsigned long long* gctr;   // [D][8][kCtrLine] group counters {spikes << 32 | arrivals}
+    unsigned long long* glist;  // [D][8][gcap] group spike lists {tag << 32 | id}
+    int32_t gcap;               // list capacity: the neurons of one group
+    uint32_t epoch;             // tag of step t0 - 1 (step t0 + k carries epoch + 1 + k)
 };
 
 __device__ inline int32_t ld_dev(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void st_dev(int32_t* p, int32_t v) {
+__device__ inline unsigned long long ld_dev64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev64(unsigned long long* p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline uint32_t tag_of(unsigned long long x) { return (uint32_t)(x >> 32); }
 
-// The grid barrier in two halves, so a workgroup can do work that depends on no other workgroup
-// between its arrival and its wait.
-__device__ inline void grid_arrive(Barrier* bar, int32_t k, int32_t nblocks) {
-    // every wave waits for its own published stores (__syncthreads waits for LDS only), so all
-    // of the workgroup's stores are performed before its arrival is counted
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int32_t g = blockIdx.x % kXcds;
-        const int32_t per = nblocks / kXcds + (g < nblocks % kXcds ? 1 : 0);  // workgroups in group g
-        const int32_t groups = min(nblocks, kXcds);                            // non-empty groups
-        const int32_t old = __hip_atomic_fetch_add(&bar->group[g * kLine], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == per * (k + 1) - 1) {
-            const int32_t o2 = __hip_atomic_fetch_add(&bar->top[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (o2 == groups * (k + 1) - 1)
-                for (int q = 0; q < kXcds; ++q) st_dev(&bar->gen[q * kLine], k + 1);
+// Re-reads every non-null p[r] whose word does not carry tag, all together per round, until all
+// do; x[r] holds the first loads on entry. False if the wait was abandoned (kErrBarrier raised by
+// another workgroup, or by this one at the spin limit).
+template <int N>
+__device__ inline bool await_tags(const unsigned long long* const (&p)[N], uint32_t tag, unsigned long long (&x)[N],
+                                  int32_t* err) {
+    uint32_t pend = 0;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+        if (p[r] && tag_of(x[r]) != tag) pend |= 1u << r;
+    uint32_t spins = 0;
+    while (pend) {
+        if ((spins & 63u) == 0 && (ld_dev(err) & kErrBarrier) != 0) return false;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit) {
+            atomicOr(err, kErrBarrier);
+            return false;
         }
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+            if ((pend >> r) & 1u) x[r] = ld_dev64(p[r]);
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+            if (((pend >> r) & 1u) && tag_of(x[r]) == tag) pend &= ~(1u << r);
     }
+    return true;
 }
 
-// Called once by thread 0 of every workgroup after its last barrier access (aborted or not): the
-// last workgroup to retire returns the barrier to zero, so the next launch on the stream starts
-// from a clean barrier without a separate clearing dispatch. Every earlier barrier atomic of a
-// workgroup returned its value (the gen stores are waited for here) before its retirement is
-// counted, so no barrier access of this launch is performed after the reset.
-__device__ inline void grid_retire(Barrier* bar, int32_t nblocks) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const int32_t old = __hip_atomic_fetch_add(&bar->retired[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (old == nblocks - 1) {
-        for (int q = 0; q < kXcds; ++q) {
-            st_dev(&bar->group[q * kLine], 0);
-            st_dev(&bar->gen[q * kLine], 0);
-        }
-        st_dev(&bar->top[0], 0);
-        st_dev(&bar->retired[0], 0);
-    }
-}
-
-__device__ inline bool grid_wait(Barrier* bar, int32_t k, int32_t* err, int32_t* s_abort) {
-    if (threadIdx.x == 0) {
-        const int32_t g = blockIdx.x % kXcds;
+// Wave 0 of a workgroup: waits until all groups' workgroups have published the step of slot sl
+// and leaves the groups' list starts (step numbering group-major) in s_gstart[0..8], the step's
+// spike count in s_gstart[8]. False if the wait was abandoned.
+__device__ inline bool await_step(const PersistArgs& X, int32_t sl, int32_t lane, int32_t* s_gstart, int32_t* err) {
+    const int32_t per = X.B / kXcds;
+    unsigned long long c = 0;
+    bool ok = true;
+    if (lane < kXcds) {
+        const unsigned long long* p = X.gctr + ((int64_t)sl * kXcds + lane) * kCtrLine;
+        c = ld_dev64(p);
         uint32_t spins = 0;
-        bool abort = false;
-        while (ld_dev(&bar->gen[g * kLine]) < k + 1) {
-            if ((ld_dev(err) & kErrBarrier) != 0) {  // another workgroup's wait timed out
-                abort = true;
+        while ((int32_t)(uint32_t)c != per) {
+            if ((spins & 63u) == 0 && (ld_dev(err) & kErrBarrier) != 0) {
+                ok = false;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
                 atomicOr(err, kErrBarrier);
-                abort = true;
+                ok = false;
                 break;
             }
+            c = ld_dev64(p);
         }
-        *s_abort = abort ? 1 : 0;
     }
-    __syncthreads();
-    return *s_abort == 0;
+    const int32_t gc = (lane < kXcds) ? (int32_t)(c >> 32) : 0;
+    int32_t gp = gc;  // inclusive prefix over lanes 0..7
+#pragma unroll
+    for (int o = 1; o < kXcds; o <<= 1) {
+        const int32_t z = __shfl_up(gp, o, 64);
+        if (lane >= o) gp += z;
+    }
+    if (lane < kXcds) s_gstart[lane] = gp - gc;
+    if (lane == kXcds - 1) s_gstart[kXcds] = gp;
+    return __all(ok ? 1 : 0) != 0;
+}
+
+// Where spike number f of slot sl (f < the step's count) is listed; gs holds the group list starts.
+__device__ inline const unsigned long long* spike_ptr(const PersistArgs& X, int32_t sl, const int32_t (&gs)[kXcds],
+                                                      int32_t f) {
+    int32_t g = 0, start = 0;
+#pragma unroll
+    for (int q = 1; q < kXcds; ++q)
+        if (f >= gs[q]) {
+            g = q;
+            start = gs[q];
+        }
+    return X.glist + ((int64_t)sl * kXcds + g) * X.gcap + (f - start);
 }
 
 template <int MODEL, int NPT>
 __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     extern __shared__ int32_t s_ring[];  // [D][nch][P]
-    __shared__ int32_t s_wcnt[kMaxThreads / 64];
+    __shared__ int32_t s_wc[kMaxNpt * (kMaxThreads / 64)];  // spikes per (neuron chunk, wave) of the update
+    __shared__ int32_t s_gstart[kXcds + 1];                 // the step's group list starts and count
     __shared__ int32_t s_base;
     __shared__ int32_t s_abort;
     __shared__ unsigned long long s_ev;
@@ -156,6 +180,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const StepArgs& A = X.A;
     const int32_t P = X.P, D = A.D, nch = A.nch, n = A.n;
     const int32_t b = block_of(blockIdx.x, X.B);
+    const int32_t xg = (int32_t)(blockIdx.x % kXcds);  // this workgroup's group
     const int32_t p0 = b * P;
     const int32_t np = max(0, min(P, n - p0));
     const int32_t T = blockDim.x;
@@ -188,7 +213,10 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     }
     const int32_t kmax = min(A.kmax, kTabCap - 1);
     if (A.pois && tid <= kmax) s_tab[tid] = A.pois_tab[tid];
-    if (tid == 0) s_ev = 0;
+    if (tid == 0) {
+        s_ev = 0;
+        s_abort = 0;
+    }
     float V[NPT], S1[NPT], GA[NPT], GG[NPT], I[NPT];
     int32_t last[NPT];
 #pragma unroll
@@ -210,7 +238,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     }
     unsigned long long ev = 0;
     // Poisson background counts of the next step: they depend on no other workgroup, so they are
-    // drawn between this workgroup's barrier arrival and its wait
+    // drawn between this workgroup's publication of a step and its wait for the others'
     int32_t pc[NPT];
     auto draw = [&](int32_t t) {
 #pragma unroll
@@ -221,24 +249,43 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     };
     __syncthreads();  // s_tab and s_ring are loaded
     draw(X.t0);
+    bool aborted = false;
 
     for (int32_t k = 0; k <= X.nsteps; ++k) {
         const int32_t t = X.t0 + k;
         const int32_t sb = t % D;  // ring slot of step t
         if (k > 0) {
-            // spikes of step t-1 were complete when the previous barrier opened
+            // 1. the spikes of step t-1 (published by every workgroup once the slot's counters are full)
             const int32_t ps = (t - 1) % D;
-            const int32_t cnt = ld_dev(&A.hist_cnt[ps]);
-            const int32_t* ids = A.hist + (int64_t)ps * n;
+            const uint32_t want = X.epoch + (uint32_t)k;
+            if (wave == 0 && !await_step(X, ps, lane, s_gstart, A.err) && lane == 0) s_abort = 1;
+            __syncthreads();
+            if (s_abort) {  // uniform: read after the barrier, set before it
+                aborted = true;
+                break;
+            }
+            const int32_t cnt = s_gstart[kXcds];
+            int32_t gs[kXcds];
+#pragma unroll
+            for (int q = 0; q < kXcds; ++q) gs[q] = s_gstart[q];
             if (b == 0 && tid == 0 && A.counts && t - 1 >= A.rec_t0 && t - 1 < A.rec_t0 + A.rec_n)
                 A.counts[t - 1 - A.rec_t0] = cnt;
+            bool ok = true;
             // lanes in groups of kLanes share one spike so its segment entries coalesce
-            for (int32_t base = grp; base < cnt; base += ngrp * 2) {
-                int32_t pre[2], o0[2], o1[2];
+            for (int32_t base = grp; base < cnt; base += ngrp * kRound) {
+                int32_t pre[kRound], o0[kRound], o1[kRound];
+                unsigned long long x[kRound];
+                const unsigned long long* sp[kRound];
 #pragma unroll
-                for (int r = 0; r < 2; ++r) pre[r] = (base + r * ngrp < cnt) ? ld_dev(&ids[base + r * ngrp]) : -1;
+                for (int r = 0; r < kRound; ++r) {
+                    sp[r] = (base + r * ngrp < cnt) ? spike_ptr(X, ps, gs, base + r * ngrp) : nullptr;
+                    x[r] = sp[r] ? ld_dev64(sp[r]) : 0ull;
+                }
+                if (ok) ok = await_tags(sp, want, x, A.err);
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
+                for (int r = 0; r < kRound; ++r) pre[r] = (sp[r] && ok) ? (int32_t)(uint32_t)x[r] : -1;
+#pragma unroll
+                for (int r = 0; r < kRound; ++r) {
                     o0[r] = o1[r] = 0;
                     if (pre[r] >= 0) {
                         const int32_t* op = X.off + (int64_t)pre[r] * X.os + i0;
@@ -246,22 +293,22 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                         o1[r] = op[1];
                     }
                 }
-                uint2 e[2][2];
+                uint2 e[kRound][2];
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < kRound; ++r)
 #pragma unroll
                     for (int u = 0; u < 2; ++u) {
                         const int32_t j = o0[r] + gl + u * kLanes;
                         e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
                     }
-                auto deposit = [&](const uint2 x) {
-                    int32_t s = sb + (int32_t)((x.x >> 16) & 0x3Fu) - 1;  // slot of step t - 1 + delay
+                auto deposit = [&](const uint2 y) {
+                    int32_t s = sb + (int32_t)((y.x >> 16) & 0x3Fu) - 1;  // slot of step t - 1 + delay
                     s -= (s >= D) ? D : 0;
-                    atomicAdd(&s_ring[(s * nch + (int32_t)((x.x >> 22) & 3u)) * P + (int32_t)(x.x & 0xFFFFu)],
-                              (int32_t)x.y);
+                    atomicAdd(&s_ring[(s * nch + (int32_t)((y.x >> 22) & 3u)) * P + (int32_t)(y.x & 0xFFFFu)],
+                              (int32_t)y.y);
                 };
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
+                for (int r = 0; r < kRound; ++r) {
 #pragma unroll
                     for (int u = 0; u < 2; ++u)
                         if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u]);
@@ -269,11 +316,21 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                     if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
                 }
             }
+            if (!ok) s_abort = 1;
+        }
+        __syncthreads();  // every delivery into this step's ring slot is in; s_abort is uniform
+        if (s_abort) {
+            aborted = true;
+            break;
         }
         if (k == X.nsteps) break;
-        __syncthreads();
-        const int32_t slot = t % D;
+        // 2. update this block's neurons for step t and publish their spikes
+        const int32_t slot = sb;
         const bool rec = A.raster && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n;
+        const uint32_t tag = X.epoch + 1u + (uint32_t)k;
+        if (b == 0 && tid < kXcds)  // the next step's counters (see above)
+            st_dev64(X.gctr + ((int64_t)((t + 1) % D) * kXcds + tid) * kCtrLine, 0ull);
+        uint64_t msk[NPT];
 #pragma unroll
         for (int u = 0; u < NPT; ++u) {
             const int32_t pl = tid + u * T;
@@ -294,33 +351,48 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 fired = neuron_tick<MODEL, true, true>(A, i, t, a0, a1, a2, V[u], S1[u], GA[u], GG[u], I[u], last[u],
                                                        pc[u]);
             }
-            const uint64_t mask = __ballot(fired);
-            if (lane == 0) s_wcnt[wave] = __popcll(mask);
+            msk[u] = __ballot(fired);
+            if (lane == 0) s_wc[u * nwaves + wave] = __popcll(msk[u]);
             if (rec && lane == 0) {
                 const int32_t pw = u * T + wave * 64;  // first local neuron of this wavefront
                 uint32_t* row = A.raster + (int64_t)(t - A.rec_t0) * A.words;
-                if (pw < np) row[(p0 + pw) >> 5] = (uint32_t)mask;
-                if (pw + 32 < np) row[((p0 + pw) >> 5) + 1] = (uint32_t)(mask >> 32);
+                if (pw < np) row[(p0 + pw) >> 5] = (uint32_t)msk[u];
+                if (pw + 32 < np) row[((p0 + pw) >> 5) + 1] = (uint32_t)(msk[u] >> 32);
             }
-            __syncthreads();
-            if (tid == 0) {
-                int32_t tot = 0;
-                for (int w = 0; w < nwaves; ++w) {
-                    const int32_t c = s_wcnt[w];
-                    s_wcnt[w] = tot;
-                    tot += c;
-                }
-                s_base = tot ? atomicAdd(&A.hist_cnt[slot], tot) : 0;
-            }
-            __syncthreads();
-            if (fired) st_dev(&A.hist[(int64_t)slot * n + s_base + s_wcnt[wave] + __popcll(mask & ((1ull << lane) - 1ull))], i);
-            __syncthreads();
         }
-        // slot of step t+1 held step t+1-D: nobody reads it in this phase (D >= 3)
-        if (b == 0 && tid == 0) st_dev(&A.hist_cnt[(t + 1) % D], 0);
-        grid_arrive(X.bar, k, X.B);
+        // workgroup 0 waits for its clearing stores before its arrival is counted (the only data
+        // ordered by the counters; the list entries carry their tag instead)
+        if (b == 0) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        {
+            // this workgroup's spikes, numbered by (neuron chunk, wave, lane), go into its group's
+            // list at the place its arrival atomic returns; each wave scans the per-(chunk, wave) counts
+            const int32_t ne = NPT * nwaves;
+            const int32_t mine = (lane < ne) ? s_wc[lane] : 0;
+            int32_t y = mine;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t z = __shfl_up(y, o, 64);
+                if (lane >= o) y += z;
+            }
+            const int32_t tot = __shfl(y, ne - 1, 64);
+            if (tid == 0) {
+                const unsigned long long old = __hip_atomic_fetch_add(
+                    X.gctr + ((int64_t)slot * kXcds + xg) * kCtrLine, ((unsigned long long)(uint32_t)tot << 32) | 1ull,
+                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_base = (int32_t)(old >> 32);
+            }
+            __syncthreads();
+            unsigned long long* lst = X.glist + ((int64_t)slot * kXcds + xg) * X.gcap + s_base;
+            const unsigned long long hi = (unsigned long long)tag << 32;
+#pragma unroll
+            for (int u = 0; u < NPT; ++u) {
+                const int32_t start = __shfl(y - mine, u * nwaves + wave, 64);
+                if ((msk[u] >> lane) & 1ull)
+                    st_dev64(&lst[start + __popcll(msk[u] & ((1ull << lane) - 1ull))], hi | (uint32_t)(p0 + tid + u * T));
+            }
+        }
         draw(t + 1);
-        if (!grid_wait(X.bar, k, A.err, &s_abort)) break;
     }
     // write back state and ring; the ring then holds every delivery up to the last step
 #pragma unroll
@@ -344,11 +416,35 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         if (pl < np) A.ring[(int64_t)sc * n + p0 + pl] = s_ring[j];
     }
     if (ev) atomicAdd(&s_ev, ev);
-    __syncthreads();
-    if (tid == 0) {
-        if (s_ev) atomicAdd(A.events, s_ev);
-        grid_retire(X.bar, X.B);
+    // The firing-history ring for the steps it still holds (the launch's last D - 1), from the group
+    // lists: workgroup j takes the step t_end - 1 - j (every workgroup saw every step complete, and
+    // an entry not yet seen with its tag is waited for). Each history entry is written by one
+    // workgroup.
+    if (!aborted) {
+        const int32_t nh = min(X.nsteps, D - 1);
+        for (int32_t j = b; j < nh; j += X.B) {
+            const int32_t s = X.t0 + X.nsteps - 1 - j, sl = s % D;
+            const uint32_t tag = X.epoch + (uint32_t)(X.nsteps - j);
+            __syncthreads();  // s_gstart of the previous step is read
+            if (wave == 0) await_step(X, sl, lane, s_gstart, A.err);
+            __syncthreads();
+            const int32_t cnt = s_gstart[kXcds];
+            int32_t gs[kXcds];
+#pragma unroll
+            for (int q = 0; q < kXcds; ++q) gs[q] = s_gstart[q];
+            for (int32_t f = tid; f < cnt; f += T) {
+                const unsigned long long* sp[1] = {spike_ptr(X, sl, gs, f)};
+                unsigned long long x[1] = {ld_dev64(sp[0])};
+                await_tags(sp, tag, x, A.err);
+                A.hist[(int64_t)sl * n + f] = (int32_t)(uint32_t)x[0];
+            }
+            if (tid == 0) A.hist_cnt[sl] = cnt;
+        }
+        // the next step's updates start counting from zero (the slot held step t_end - D)
+        if (b == 0 && tid == 0) A.hist_cnt[(X.t0 + X.nsteps) % D] = 0;
     }
+    __syncthreads();
+    if (tid == 0 && s_ev) atomicAdd(A.events, s_ev);
 }
 
 // Replay delivery on the post-blocked layout.
@@ -983,10 +1079,13 @@ int32_t build_persistent_layout(nbx_net* net) {
     if (rc) return rc;
     rc = net->upload(&L.d_seg, seg.data(), seg.size());
     if (rc) return rc;
-    rc = net->alloc(&L.d_bar, sizeof(Barrier) / sizeof(int32_t));
+    L.gcap = (B / kXcds) * P;  // the neurons of one group (B is a multiple of 8)
+    rc = net->alloc(&L.d_glist, (size_t)net->D * kXcds * L.gcap);
     if (rc) return rc;
-    // zeroed once here; every launch leaves it zero again (grid_retire)
-    NBX_HIP_TRY(launch_fill_i32(L.d_bar, 0, (int64_t)(sizeof(Barrier) / sizeof(int32_t)), net->stream));
+    rc = net->alloc(&L.d_gctr, (size_t)net->D * kXcds * kCtrLine);
+    if (rc) return rc;
+    L.t_next = -1;  // the counters are cleared before the first launch
+    L.epoch = 0;
     L.B = B;
     L.P = P;
     L.threads = g.threads;
@@ -1008,7 +1107,8 @@ static int32_t free_owned(nbx_net* net, void* p) {
 int32_t refresh_persistent_weights(nbx_net* net) {
     PersistentLayout& L = net->pl;
     if (L.built) {
-        for (void* p : {(void*)L.d_off, (void*)L.d_seg, (void*)L.d_bar}) NBX_PASS(free_owned(net, p));
+        for (void* p : {(void*)L.d_off, (void*)L.d_seg, (void*)L.d_glist, (void*)L.d_gctr})
+            NBX_PASS(free_owned(net, p));
         L = PersistentLayout();
     }
     ReplaySlots& RS = net->rs;
@@ -1109,13 +1209,30 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.os = L.nr * L.gs;
     X.t0 = net->t;
     X.nsteps = nsteps;
-    X.bar = reinterpret_cast<Barrier*>(L.d_bar);
+    // the first step's counters must be zero: each launch clears the slot of the step after its
+    // last, so this holds when the previous persistent launch ended at this step; list entries
+    // carry unique tags, so only a tag wrap needs them cleared
+    const bool wrap = (uint64_t)L.epoch + (uint64_t)nsteps + 2u >= ((uint64_t)1 << 32);
+    if (net->t != L.t_next || wrap)
+        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, (int64_t)2 * net->D * kXcds * kCtrLine,
+                                    net->stream));
+    if (wrap) {
+        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_glist), 0, (int64_t)2 * net->D * kXcds * L.gcap,
+                                    net->stream));
+        L.epoch = 0;
+    }
+    X.gctr = L.d_gctr;
+    X.glist = L.d_glist;
+    X.gcap = L.gcap;
+    X.epoch = L.epoch;
     void* kargs[] = {&X};
     // Co-residency of all L.B workgroups is checked above (occupancy x CUs); the cooperative
     // launch additionally has the runtime guarantee it, which the grid barrier relies on.
     NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes,
                                            net->stream));
     net->t += nsteps;
+    L.t_next = net->t;
+    L.epoch += (uint32_t)nsteps + 1u;  // the launch used tags epoch + 1 .. epoch + nsteps
     return NBX_OK;
 }
 
