@@ -35,7 +35,10 @@ constexpr int kMaxNpt = 4;
 constexpr uint32_t kSpinLimit = 1u << 24;  // ~0.5 s of s_sleep(1) spins before giving up
 constexpr int kLanes = 4;                  // lanes sharing one spike's synapse segment
 constexpr int kTabCap = 64;                // Poisson threshold table entries held in LDS
-constexpr int kRound = 2;                  // spikes per lane group in flight per delivery round
+// Spikes per lane group in flight per delivery round: 4 covers a step of 4 * 112 spikes on a
+// 448-thread workgroup in one round (the first, busier steps of a run); 2 is 3% faster once the
+// 1e5-neuron network settles at ~160 spikes per step but 9% slower over its first 25 steps.
+constexpr int kRound = 4;
 constexpr uint32_t kSlotEmpty = 0xFFFFFFFFu;  // replay slot entry: unused
 constexpr uint32_t kSlotLink = 0x80000000u;   // replay slot entry: link to the overflow list
 constexpr int kSlotCap = 16;                  // entries per replay slot (128 bytes)
@@ -65,6 +68,7 @@ __device__ inline int32_t block_of(int32_t w, int32_t B) {
 // them any more (their last readers finished before step t - 1 ended). Spins are bounded: on
 // time-out the error flag's kErrBarrier bit is raised and every waiter that sees it leaves.
 constexpr int kCtrLine = 16;  // uint64 per 128-byte line
+constexpr int kGroups = 16;   // counter groups: workgroup w belongs to group w % kGroups
 
 struct PersistArgs {
     StepArgs A;
@@ -122,11 +126,11 @@ __device__ inline bool await_tags(const unsigned long long* const (&p)[N], uint3
 // and leaves the groups' list starts (step numbering group-major) in s_gstart[0..8], the step's
 // spike count in s_gstart[8]. False if the wait was abandoned.
 __device__ inline bool await_step(const PersistArgs& X, int32_t sl, int32_t lane, int32_t* s_gstart, int32_t* err) {
-    const int32_t per = X.B / kXcds;
+    const int32_t per = X.B / kGroups + ((lane < X.B % kGroups) ? 1 : 0);  // workgroups w with w % kGroups == lane
     unsigned long long c = 0;
     bool ok = true;
-    if (lane < kXcds) {
-        const unsigned long long* p = X.gctr + ((int64_t)sl * kXcds + lane) * kCtrLine;
+    if (lane < kGroups) {
+        const unsigned long long* p = X.gctr + ((int64_t)sl * kGroups + lane) * kCtrLine;
         c = ld_dev64(p);
         uint32_t spins = 0;
         while ((int32_t)(uint32_t)c != per) {
@@ -143,36 +147,36 @@ __device__ inline bool await_step(const PersistArgs& X, int32_t sl, int32_t lane
             c = ld_dev64(p);
         }
     }
-    const int32_t gc = (lane < kXcds) ? (int32_t)(c >> 32) : 0;
+    const int32_t gc = (lane < kGroups) ? (int32_t)(c >> 32) : 0;
     int32_t gp = gc;  // inclusive prefix over lanes 0..7
 #pragma unroll
-    for (int o = 1; o < kXcds; o <<= 1) {
+    for (int o = 1; o < kGroups; o <<= 1) {
         const int32_t z = __shfl_up(gp, o, 64);
         if (lane >= o) gp += z;
     }
-    if (lane < kXcds) s_gstart[lane] = gp - gc;
-    if (lane == kXcds - 1) s_gstart[kXcds] = gp;
+    if (lane < kGroups) s_gstart[lane] = gp - gc;
+    if (lane == kGroups - 1) s_gstart[kGroups] = gp;
     return __all(ok ? 1 : 0) != 0;
 }
 
 // Where spike number f of slot sl (f < the step's count) is listed; gs holds the group list starts.
-__device__ inline const unsigned long long* spike_ptr(const PersistArgs& X, int32_t sl, const int32_t (&gs)[kXcds],
+__device__ inline const unsigned long long* spike_ptr(const PersistArgs& X, int32_t sl, const int32_t (&gs)[kGroups],
                                                       int32_t f) {
     int32_t g = 0, start = 0;
 #pragma unroll
-    for (int q = 1; q < kXcds; ++q)
+    for (int q = 1; q < kGroups; ++q)
         if (f >= gs[q]) {
             g = q;
             start = gs[q];
         }
-    return X.glist + ((int64_t)sl * kXcds + g) * X.gcap + (f - start);
+    return X.glist + ((int64_t)sl * kGroups + g) * X.gcap + (f - start);
 }
 
 template <int MODEL, int NPT>
 __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     extern __shared__ int32_t s_ring[];  // [D][nch][P]
     __shared__ int32_t s_wc[kMaxNpt * (kMaxThreads / 64)];  // spikes per (neuron chunk, wave) of the update
-    __shared__ int32_t s_gstart[kXcds + 1];                 // the step's group list starts and count
+    __shared__ int32_t s_gstart[kGroups + 1];                 // the step's group list starts and count
     __shared__ int32_t s_base;
     __shared__ int32_t s_abort;
     __shared__ unsigned long long s_ev;
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const StepArgs& A = X.A;
     const int32_t P = X.P, D = A.D, nch = A.nch, n = A.n;
     const int32_t b = block_of(blockIdx.x, X.B);
-    const int32_t xg = (int32_t)(blockIdx.x % kXcds);  // this workgroup's group
+    const int32_t xg = (int32_t)(blockIdx.x % kGroups);  // this workgroup's group
     const int32_t p0 = b * P;
     const int32_t np = max(0, min(P, n - p0));
     const int32_t T = blockDim.x;
@@ -264,58 +268,63 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 aborted = true;
                 break;
             }
-            const int32_t cnt = s_gstart[kXcds];
-            int32_t gs[kXcds];
+            const int32_t cnt = s_gstart[kGroups];
+            int32_t gs[kGroups];
 #pragma unroll
-            for (int q = 0; q < kXcds; ++q) gs[q] = s_gstart[q];
+            for (int q = 0; q < kGroups; ++q) gs[q] = s_gstart[q];
             if (b == 0 && tid == 0 && A.counts && t - 1 >= A.rec_t0 && t - 1 < A.rec_t0 + A.rec_n)
                 A.counts[t - 1 - A.rec_t0] = cnt;
             bool ok = true;
-            // lanes in groups of kLanes share one spike so its segment entries coalesce
-            for (int32_t base = grp; base < cnt; base += ngrp * kRound) {
-                int32_t pre[kRound], o0[kRound], o1[kRound];
-                unsigned long long x[kRound];
-                const unsigned long long* sp[kRound];
+            // lanes in groups of kLanes share one spike so its segment entries coalesce; each round
+            // keeps kRound spikes per lane group in flight
+            auto deliver = [&](auto rounds) {
+                constexpr int R = decltype(rounds)::value;
+                unsigned long long x[R];
+                const unsigned long long* sp[R];
+                for (int32_t base = grp; base < cnt; base += ngrp * R) {
+                    int32_t pre[R], o0[R], o1[R];
 #pragma unroll
-                for (int r = 0; r < kRound; ++r) {
-                    sp[r] = (base + r * ngrp < cnt) ? spike_ptr(X, ps, gs, base + r * ngrp) : nullptr;
-                    x[r] = sp[r] ? ld_dev64(sp[r]) : 0ull;
-                }
-                if (ok) ok = await_tags(sp, want, x, A.err);
+                    for (int r = 0; r < R; ++r) {
+                        sp[r] = (base + r * ngrp < cnt) ? spike_ptr(X, ps, gs, base + r * ngrp) : nullptr;
+                        x[r] = sp[r] ? ld_dev64(sp[r]) : 0ull;
+                    }
+                    if (ok) ok = await_tags(sp, want, x, A.err);
 #pragma unroll
-                for (int r = 0; r < kRound; ++r) pre[r] = (sp[r] && ok) ? (int32_t)(uint32_t)x[r] : -1;
+                    for (int r = 0; r < R; ++r) pre[r] = (sp[r] && ok) ? (int32_t)(uint32_t)x[r] : -1;
 #pragma unroll
-                for (int r = 0; r < kRound; ++r) {
-                    o0[r] = o1[r] = 0;
-                    if (pre[r] >= 0) {
-                        const int32_t* op = X.off + (int64_t)pre[r] * X.os + i0;
-                        o0[r] = op[0];
-                        o1[r] = op[1];
+                    for (int r = 0; r < R; ++r) {
+                        o0[r] = o1[r] = 0;
+                        if (pre[r] >= 0) {
+                            const int32_t* op = X.off + (int64_t)pre[r] * X.os + i0;
+                            o0[r] = op[0];
+                            o1[r] = op[1];
+                        }
+                    }
+                    uint2 e[R][2];
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const int32_t j = o0[r] + gl + u * kLanes;
+                            e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
+                        }
+                    auto deposit = [&](const uint2 y) {
+                        int32_t s = sb + (int32_t)((y.x >> 16) & 0x3Fu) - 1;  // slot of step t - 1 + delay
+                        s -= (s >= D) ? D : 0;
+                        atomicAdd(&s_ring[(s * nch + (int32_t)((y.x >> 22) & 3u)) * P + (int32_t)(y.x & 0xFFFFu)],
+                                  (int32_t)y.y);
+                    };
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+#pragma unroll
+                        for (int u = 0; u < 2; ++u)
+                            if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u]);
+                        for (int32_t j = o0[r] + gl + 2 * kLanes; j < o1[r]; j += kLanes) deposit(X.seg[j]);
+                        if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
                     }
                 }
-                uint2 e[kRound][2];
-#pragma unroll
-                for (int r = 0; r < kRound; ++r)
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const int32_t j = o0[r] + gl + u * kLanes;
-                        e[r][u] = (j < o1[r]) ? X.seg[j] : make_uint2(0u, 0u);
-                    }
-                auto deposit = [&](const uint2 y) {
-                    int32_t s = sb + (int32_t)((y.x >> 16) & 0x3Fu) - 1;  // slot of step t - 1 + delay
-                    s -= (s >= D) ? D : 0;
-                    atomicAdd(&s_ring[(s * nch + (int32_t)((y.x >> 22) & 3u)) * P + (int32_t)(y.x & 0xFFFFu)],
-                              (int32_t)y.y);
-                };
-#pragma unroll
-                for (int r = 0; r < kRound; ++r) {
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-                        if (o0[r] + gl + u * kLanes < o1[r]) deposit(e[r][u]);
-                    for (int32_t j = o0[r] + gl + 2 * kLanes; j < o1[r]; j += kLanes) deposit(X.seg[j]);
-                    if (gl == 0) ev += (unsigned long long)(o1[r] - o0[r]);
-                }
-            }
+            };
+            deliver(std::integral_constant<int, kRound>{});
             if (!ok) s_abort = 1;
         }
         __syncthreads();  // every delivery into this step's ring slot is in; s_abort is uniform
@@ -328,8 +337,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         const int32_t slot = sb;
         const bool rec = A.raster && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n;
         const uint32_t tag = X.epoch + 1u + (uint32_t)k;
-        if (b == 0 && tid < kXcds)  // the next step's counters (see above)
-            st_dev64(X.gctr + ((int64_t)((t + 1) % D) * kXcds + tid) * kCtrLine, 0ull);
+        if (b == 0 && tid < kGroups)  // the next step's counters (see above)
+            st_dev64(X.gctr + ((int64_t)((t + 1) % D) * kGroups + tid) * kCtrLine, 0ull);
         uint64_t msk[NPT];
 #pragma unroll
         for (int u = 0; u < NPT; ++u) {
@@ -378,12 +387,12 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             const int32_t tot = __shfl(y, ne - 1, 64);
             if (tid == 0) {
                 const unsigned long long old = __hip_atomic_fetch_add(
-                    X.gctr + ((int64_t)slot * kXcds + xg) * kCtrLine, ((unsigned long long)(uint32_t)tot << 32) | 1ull,
+                    X.gctr + ((int64_t)slot * kGroups + xg) * kCtrLine, ((unsigned long long)(uint32_t)tot << 32) | 1ull,
                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 s_base = (int32_t)(old >> 32);
             }
             __syncthreads();
-            unsigned long long* lst = X.glist + ((int64_t)slot * kXcds + xg) * X.gcap + s_base;
+            unsigned long long* lst = X.glist + ((int64_t)slot * kGroups + xg) * X.gcap + s_base;
             const unsigned long long hi = (unsigned long long)tag << 32;
 #pragma unroll
             for (int u = 0; u < NPT; ++u) {
@@ -428,10 +437,10 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             __syncthreads();  // s_gstart of the previous step is read
             if (wave == 0) await_step(X, sl, lane, s_gstart, A.err);
             __syncthreads();
-            const int32_t cnt = s_gstart[kXcds];
-            int32_t gs[kXcds];
+            const int32_t cnt = s_gstart[kGroups];
+            int32_t gs[kGroups];
 #pragma unroll
-            for (int q = 0; q < kXcds; ++q) gs[q] = s_gstart[q];
+            for (int q = 0; q < kGroups; ++q) gs[q] = s_gstart[q];
             for (int32_t f = tid; f < cnt; f += T) {
                 const unsigned long long* sp[1] = {spike_ptr(X, sl, gs, f)};
                 unsigned long long x[1] = {ld_dev64(sp[0])};
@@ -1079,10 +1088,10 @@ int32_t build_persistent_layout(nbx_net* net) {
     if (rc) return rc;
     rc = net->upload(&L.d_seg, seg.data(), seg.size());
     if (rc) return rc;
-    L.gcap = (B / kXcds) * P;  // the neurons of one group (B is a multiple of 8)
-    rc = net->alloc(&L.d_glist, (size_t)net->D * kXcds * L.gcap);
+    L.gcap = (B + kGroups - 1) / kGroups * P;  // the neurons of one group
+    rc = net->alloc(&L.d_glist, (size_t)net->D * kGroups * L.gcap);
     if (rc) return rc;
-    rc = net->alloc(&L.d_gctr, (size_t)net->D * kXcds * kCtrLine);
+    rc = net->alloc(&L.d_gctr, (size_t)net->D * kGroups * kCtrLine);
     if (rc) return rc;
     L.t_next = -1;  // the counters are cleared before the first launch
     L.epoch = 0;
@@ -1214,10 +1223,10 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     // carry unique tags, so only a tag wrap needs them cleared
     const bool wrap = (uint64_t)L.epoch + (uint64_t)nsteps + 2u >= ((uint64_t)1 << 32);
     if (net->t != L.t_next || wrap)
-        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, (int64_t)2 * net->D * kXcds * kCtrLine,
+        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, (int64_t)2 * net->D * kGroups * kCtrLine,
                                     net->stream));
     if (wrap) {
-        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_glist), 0, (int64_t)2 * net->D * kXcds * L.gcap,
+        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_glist), 0, (int64_t)2 * net->D * kGroups * L.gcap,
                                     net->stream));
         L.epoch = 0;
     }
