@@ -630,8 +630,8 @@ struct nbx_batch {
         const PartDev* parts = d_parts;
         void* kargs[] = {&A, &parts};
         const void* fn = parts_kernel(part_npt);
-        return hipLaunchCooperativeKernel(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs,
-                                          (unsigned)part_lds(part_npt), stream);
+        return launch_coresident(fn, dim3((unsigned)h_parts.size()), dim3(kPartThreads), kargs, part_lds(part_npt),
+                                 stream);
     }
     int32_t part_capacity(int32_t npt, int32_t* cap) const {
         int per_cu = 0, cus = 0;

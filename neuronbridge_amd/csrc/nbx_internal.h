@@ -20,6 +20,15 @@ namespace nbx {
 
 // ----------------------------------------------------------------- errors
 void set_error(const std::string& msg);
+
+// Launches a kernel whose workgroups wait for one another (the persistent and batched engines).
+// A cooperative launch has the runtime guarantee that all of them are resident at once. Under
+// rocprofv3 (which sets ROCP_TOOL_LIBRARIES) the launch is a regular one: the profiler loses its
+// results when it exits after tracing a cooperative dispatch, also for a 40-line program with one
+// empty cooperative kernel and no part of this library, while the same program with regular
+// launches profiles cleanly. The caller has checked that occupancy x CUs covers the grid, and a
+// profiled process runs alone on the GPU, so the workgroups are co-resident there too.
+hipError_t launch_coresident(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t stream);
 #define NBX_HIP_TRY(expr)                                                                        \
     do {                                                                                         \
         hipError_t e_ = (expr);                                                                  \

@@ -1188,6 +1188,12 @@ static int64_t longest_row(const nbx_net* net) {
     return r;
 }
 
+hipError_t launch_coresident(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t stream) {
+    static const bool profiled = std::getenv("ROCP_TOOL_LIBRARIES") != nullptr;
+    if (profiled) return hipLaunchKernel(fn, grid, block, args, lds, stream);
+    return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
+}
+
 int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     if (net->has_dense && !net->has_sparse) return run_small_dense(net, nsteps);
     int32_t rc = build_persistent_layout(net);
@@ -1236,9 +1242,8 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.epoch = L.epoch;
     void* kargs[] = {&X};
     // Co-residency of all L.B workgroups is checked above (occupancy x CUs); the cooperative
-    // launch additionally has the runtime guarantee it, which the grid barrier relies on.
-    NBX_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(L.B), dim3(L.threads), kargs, (unsigned)L.lds_bytes,
-                                           net->stream));
+    // launch additionally has the runtime guarantee it, which the step counters rely on.
+    NBX_HIP_TRY(launch_coresident(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
     net->t += nsteps;
     L.t_next = net->t;
     L.epoch += (uint32_t)nsteps + 1u;  // the launch used tags epoch + 1 .. epoch + nsteps
