@@ -142,7 +142,8 @@ def replay_roofline(net, wl, raster: np.ndarray, reps: int) -> dict:
     ms = e0.elapsed_ms(e1) / reps
     alg = events * EVENT_BYTES + K * n * 4
     ach = alg / (ms * 1e-3) / 1e9
-    kernel = {L.REPLAY_SLOTS: "nbx_replay_slots", L.REPLAY_OFFSETS: "nbx_replay_ring"}.get(net.replay_path(),
+    kernel = {L.REPLAY_STREAM: "nbx_replay_stream", L.REPLAY_SLOTS: "nbx_replay_slots",
+              L.REPLAY_OFFSETS: "nbx_replay_ring"}.get(net.replay_path(),
                                                                                            "nbx_replay_global")
     return {"kernel": kernel, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, K),

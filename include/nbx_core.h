@@ -456,9 +456,11 @@ int32_t nbx_net_export_monitor(nbx_net* net, const nbx_monitor_desc* desc);
 
 /* Which replay kernel nbx_replay_deliver uses for this network (launch-shape overrides
  * in the environment included): NBX_REPLAY_GLOBAL (global atomics; STDP networks and
- * layouts that do not fit), NBX_REPLAY_OFFSETS (post-blocked offset layout, LDS ring) or
- * NBX_REPLAY_SLOTS (fixed 128-byte slot per presynaptic neuron and post block). */
-enum { NBX_REPLAY_GLOBAL = 0, NBX_REPLAY_OFFSETS = 1, NBX_REPLAY_SLOTS = 2 };
+ * layouts that do not fit), NBX_REPLAY_OFFSETS (post-blocked offset layout, LDS ring),
+ * NBX_REPLAY_SLOTS (fixed 128-byte slot per presynaptic neuron and post block, windows of input
+ * steps between workgroup barriers) or NBX_REPLAY_STREAM (the same slots, every wave of a
+ * workgroup pipelining its own spikes with no barriers; the default where the slots fit). */
+enum { NBX_REPLAY_GLOBAL = 0, NBX_REPLAY_OFFSETS = 1, NBX_REPLAY_SLOTS = 2, NBX_REPLAY_STREAM = 3 };
 int32_t nbx_replay_path(nbx_net* net, int32_t* path);
 /* Device-resident variant for timing: all pointers are device pointers and the
  * call only enqueues the launch on the network's stream. */

@@ -20,7 +20,7 @@ NBX_ERR_STATE = -4
 LIF_VJ, LIF_EXP, IZH, IZH_CUR, LIF_EXP2, LIF_EXP3, POISSON, TRIGGER_RELAY = 0, 1, 2, 3, 4, 5, 6, 7
 EULER, RK4 = 0, 1
 ENGINE_KERNELS, ENGINE_GRAPH, ENGINE_PERSISTENT, ENGINE_AUTO = 0, 1, 2, 3
-REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS = 0, 1, 2
+REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS, REPLAY_STREAM = 0, 1, 2, 3
 RECORD_COUNTS, RECORD_RASTER = 1, 2
 
 _f32p = C.POINTER(C.c_float)

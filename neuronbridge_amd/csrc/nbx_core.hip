@@ -1779,7 +1779,7 @@ int32_t nbx_replay_deliver(nbx_net* net, const int32_t* ids, const int32_t* tick
     hipFree(d_out);
     if (rc) return rc;
     NBX_HIP_TRY(e);
-    return NBX_OK;
+    return check_device_error(net);  // a bounded wait of the streamed replay kernel that gave up
 }
 
 int32_t nbx_replay_stdp(nbx_net* net, const int32_t* ids, const int32_t* tick_ptr, int32_t t0, int32_t lead,

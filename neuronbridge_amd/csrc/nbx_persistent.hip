@@ -44,6 +44,28 @@ constexpr uint32_t kSlotLink = 0x80000000u;   // replay slot entry: link to the 
 constexpr int kSlotCap = 16;                  // entries per replay slot (128 bytes)
 
 
+// Raw buffer loads: an offset at or past the resource's byte count reads zeros without a memory
+// request, so a masked-off lane needs no branch around its load (a load under a branch makes the
+// compiler wait for it before the branch ends, serialising the loads a lane keeps in flight).
+constexpr uint32_t kBufOff = 0x80000000u;         // offset of a lane with nothing to load
+constexpr int32_t kBufDword3 = 0x00020000;       // resource word 3 for gfx9 (raw, no swizzle)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int32_t)min(bytes, 0x7FFFFFFFu), kBufDword3);
+}
+__device__ inline uint4 buf_load16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int32_t)off, 0, 0);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int64_t buf_load_i64(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int32_t)off, 0, 0);
+    return (int64_t)(((uint64_t)v.y << 32) | v.x);
+}
+__device__ inline int32_t buf_load_i32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int32_t)off, 0, 0);
+}
+
 __device__ inline int32_t block_of(int32_t w, int32_t B) {
     const int32_t per = B / kXcds;  // B is a multiple of kXcds
     return (w % kXcds) * per + (w / kXcds);
@@ -491,6 +513,7 @@ struct ReplayArgs {
     const uint4* slots;  // fixed-slot layout: 8 uint4 (16 entries) per (pre, post block)
     const uint2* ovf;
     const int64_t* row_ptr;  // CSR row starts (row lengths count the synaptic events)
+    int32_t* err;            // device error flags (a bounded wait that gave up)
 };
 
 template <int LANES>
@@ -722,6 +745,8 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
     // counted by the workgroup with q == f % Ba (Ba = blocks that hold neurons; the others have
     // returned above), which spreads the extra row reads evenly
     const int32_t Ba = min(X.Bq, (X.n + Pr - 1) / Pr);
+    const __amdgpu_buffer_rsrc_t slot_rsrc =
+        buf_rsrc(X.slots + (int64_t)q * X.n * (kSlotCap / 2), (uint32_t)X.n * 128u);
 
     for (int32_t j = tid; j < R * words; j += T) s_ring[j] = 0;
     unsigned long long ev = 0;
@@ -775,6 +800,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
         };
 
         // one slot batch of the staged spikes [base, base + kSR * ngrp) of staging buffer sb
+        // (a lane group without a spike reads zeros: an entry of weight 0)
         auto fetch = [&](int32_t sb, int32_t cc, int32_t base, int32_t (&sid)[kSR], uint4 (&v)[kSR]) {
 #pragma unroll
             for (int r = 0; r < kSR; ++r) {
@@ -783,8 +809,8 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
             }
 #pragma unroll
             for (int r = 0; r < kSR; ++r)
-                v[r] = (sid[r] >= 0) ? X.slots[((int64_t)q * X.n + (sid[r] & 0xFFFFFF)) * (kSlotCap / 2) + gl]
-                                     : make_uint4(kSlotEmpty, 0u, kSlotEmpty, 0u);
+                v[r] = buf_load16(slot_rsrc, sid[r] >= 0 ? (uint32_t)(sid[r] & 0xFFFFFF) * 128u + (uint32_t)gl * 16u
+                                                         : kBufOff);
         };
         // the first batch of the next window, fetched before this window's flush so its loads
         // are in flight while the completed steps are written out
@@ -840,6 +866,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
                             atomicAdd(&s_ring[((tkp + (meta >> 16)) & rmask) * (uint32_t)words + (meta & 0xFFFFu)],
                                       (int32_t)qv);
                         };
+                        if (sid[r] < 0) continue;
                         if (v[r].x != kSlotEmpty) deposit(v[r].x, v[r].y);
                         if ((v[r].z & 0xFF000000u) == kSlotLink) {  // the rest of a long segment
                             const uint32_t cnt = v[r].z & 0xFFFFFFu;
@@ -907,6 +934,255 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
     if (ev) atomicAdd(X.events, ev);
 }
 
+// Replay delivery on the fixed-slot layout without workgroup barriers.
+//
+// Same ownership as nbx_replay_slots (workgroup = post block q x time group g, arrivals summed in
+// an LDS ring of R output steps), but the 16 waves never wait for one another at a window
+// boundary, so every wave keeps slot loads in flight all the time:
+//  - input steps are taken in super steps of kss steps (about 1024 spikes each); in super step i
+//    lane group h of wave v delivers the spikes lo_i + 8v + h + 128r, r = 0, 1, ..., in chunks
+//    of 8 rounds: one load per lane fetches a chunk's 64 spike ids, and the input step of each
+//    (a search over the super step's kss tick pointers in LDS) rides in the id's top byte;
+//  - a wave pipelines its own work: the ids of the next chunk, the slots of the next half-chunk
+//    (4 rounds: one 128-byte slot per lane group and round) and the LDS deposits of the current
+//    one are in flight together, across super-step boundaries;
+//  - wave v owns slice v of the ring's words: it writes out (and clears) that slice of every
+//    output step whose inputs all waves have delivered, read from per-wave progress counters;
+//  - before depositing super step i a wave waits until every slice of the ring slots that the
+//    super step lands in has been cleared (per-wave flush counters). A waiting wave keeps
+//    writing out its slice, and R >= kss + D - 1 leaves room for one super step past the
+//    completed outputs, so the waits always end.
+// Progress and flush counters are LDS words published with workgroup-scope release stores and
+// read with acquire loads, so a slice is read after every deposit into it and cleared before
+// the next deposit into the same ring slot.
+constexpr int kStreamWaves = 16;    // waves per workgroup (1024 threads)
+constexpr int kStreamRounds = 4;    // rounds (spikes per lane group) per half-chunk
+constexpr int kStreamSpikes = 1024; // spikes per super step the super-step length aims for
+
+__device__ inline int32_t wave_min16(const int32_t* a, int32_t lane) {
+    int32_t x = __hip_atomic_load(a + (lane & 15), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = min(x, __shfl_xor(x, 8));
+    x = min(x, __shfl_xor(x, 4));
+    x = min(x, __shfl_xor(x, 2));
+    x = min(x, __shfl_xor(x, 1));
+    return x;
+}
+
+__global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
+    constexpr int SR = kStreamRounds;
+    extern __shared__ int32_t s_mem[];
+    __shared__ int32_t s_tp[kTpCap + 1];
+    __shared__ int32_t s_prog[kStreamWaves], s_fl[kStreamWaves];
+    const int32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int32_t gl = lane & 7, h = lane >> 3;
+    const int32_t Pr = X.m * X.P, words = X.nch * Pr, R = X.R;
+    int32_t* s_ring = s_mem;  // [R][nch][Pr]
+
+    const int32_t nw = gridDim.x;
+    const int32_t lin = (nw % kXcds == 0) ? (int32_t)(blockIdx.x % kXcds) * (nw / kXcds) + (int32_t)blockIdx.x / kXcds
+                                          : (int32_t)blockIdx.x;
+    const int32_t g = lin / X.Bq, q = lin % X.Bq;
+    const int32_t p0 = q * Pr;
+    const int32_t np = max(0, min(Pr, X.n - p0));
+    const int32_t Kg = (X.K + X.G - 1) / X.G;
+    const int32_t o_begin = g * Kg, o_end = min(X.K, o_begin + Kg);
+    if (np == 0 || o_begin >= o_end) return;  // uniform over the workgroup
+    const int32_t span = X.D - 2;             // output step - input step lies in [0, D - 2]
+    const int32_t k_begin = max(0, o_begin - span);
+    // super-step cap: W from the host (>= 1), at most R - D + 1 so the waits always end
+    const int32_t kss_max = max(1, min(R - span - 1, X.W));
+    // spike f is counted (its row length) by the workgroup of its time group with q == f % Ba
+    const int32_t Ba = min(X.Bq, (X.n + Pr - 1) / Pr);
+    // slices of whole 128-byte lines (32 words), so no output line is written in two pieces
+    const int32_t per = 32 * (((words + 31) / 32 + kStreamWaves - 1) / kStreamWaves);
+    const int32_t j0 = min(words, wv * per), j1 = min(words, j0 + per);
+    const __amdgpu_buffer_rsrc_t slot_rsrc =
+        buf_rsrc(X.slots + (int64_t)q * X.n * (kSlotCap / 2), (uint32_t)X.n * 128u);
+    const __amdgpu_buffer_rsrc_t ids_rsrc = buf_rsrc(X.ids, 0x7FFFFFFFu);
+    const __amdgpu_buffer_rsrc_t rowp_rsrc = buf_rsrc(X.row_ptr, (uint32_t)(X.n + 1) * 8u);
+
+    for (int32_t j = tid; j < R * words; j += blockDim.x) s_ring[j] = 0;
+    if (tid < kStreamWaves) s_fl[tid] = k_begin;
+    unsigned long long ev = 0;
+    int32_t fl = k_begin, fslot = 0;  // this wave's first output step not yet written, its ring slot
+    bool bad = false;
+
+    // write out and clear this wave's slice of the output steps [fl, upto)
+    auto flush_to = [&](int32_t upto) {
+        if (fl >= upto) return;
+        for (; fl < upto; ++fl) {
+            int32_t* row = s_ring + fslot * words;
+            const bool own = fl >= o_begin;
+            int64_t obase = (int64_t)fl * X.nch * X.n + p0;
+            for (int32_t j = j0 + lane; j < j1; j += 64) {
+                int32_t ch = 0, pl = j;
+                while (pl >= Pr) {
+                    pl -= Pr;
+                    ++ch;
+                }
+                if (own && pl < np) X.out[obase + (int64_t)ch * X.n + pl] = row[j];
+                row[j] = 0;
+            }
+            if (++fslot == R) fslot = 0;
+        }
+        __hip_atomic_store(&s_fl[wv], fl, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+
+    for (int32_t s0 = k_begin; s0 < o_end; s0 += kTpCap) {
+        const int32_t sl = min(kTpCap, o_end - s0), s1 = s0 + sl;
+        __syncthreads();  // every wave has finished the previous pass
+        for (int32_t j = tid; j <= sl; j += blockDim.x) s_tp[j] = X.tick_ptr[s0 + j];
+        if (tid < kStreamWaves) s_prog[tid] = 0;
+        __syncthreads();
+        const int32_t tot = s_tp[sl] - s_tp[0];
+        const int32_t kss =
+            max(1, min(kss_max, tot > 0 ? (int32_t)min<int64_t>(kss_max, (int64_t)kStreamSpikes * sl / tot) : kss_max));
+        const int32_t nss = (sl + kss - 1) / kss;
+        auto ss_lo = [&](int32_t i) { return s_tp[min(i * kss, sl)]; };
+        auto ss_hi = [&](int32_t i) { return s_tp[min(i * kss + kss, sl)]; };
+        // rounds of this wave in super step i (its lane group 0 has the most spikes)
+        auto rounds = [&](int32_t i) {
+            const int32_t c = ss_hi(i) - ss_lo(i) - 8 * wv;
+            return c > 0 ? (c + 127) / 128 : 0;
+        };
+        auto next_chunk = [&](int32_t& i, int32_t& c) {
+            if (8 * (c + 1) < rounds(i)) {
+                ++c;
+                return;
+            }
+            c = 0;
+            for (++i; i < nss && rounds(i) == 0; ++i) {
+            }
+        };
+        // lane's spike of chunk (i, c): its id (a buffer load left in flight; zero when the lane
+        // has no spike), its input step within the super step (tk, -1 when none) and whether this
+        // workgroup counts its synaptic events
+        struct Spk {
+            int32_t id, tk;
+            bool cnt;
+        };
+        auto load_ids = [&](int32_t i, int32_t c) {
+            Spk r{0, -1, false};
+            int32_t f = -1;
+            if (i < nss) {
+                f = ss_lo(i) + 8 * wv + (lane & 7) + 128 * (8 * c + (lane >> 3));
+                if (f < ss_hi(i)) {
+                    const int32_t t0 = i * kss;
+                    int32_t a = 0, b = min(kss, sl - t0);
+                    while (b - a > 1) {
+                        const int32_t mid = (a + b) >> 1;
+                        if (s_tp[t0 + mid] <= f) a = mid;
+                        else b = mid;
+                    }
+                    r.tk = a;
+                    r.cnt = (f % Ba == q) && (s0 + t0 + a >= o_begin);
+                } else {
+                    f = -1;
+                }
+            }
+            r.id = buf_load_i32(ids_rsrc, f >= 0 ? (uint32_t)f * 4u : kBufOff);
+            return r;
+        };
+        // slots of half `half` (rounds 4 half .. 4 half + 3) of a chunk: group h's spike of each round
+        // comes from the lane that loaded it; a round without a spike reads zeros
+        auto fetch = [&](const Spk& c, int32_t half, int32_t (&sid)[SR], uint4 (&v)[SR]) {
+            const int32_t pk = c.tk >= 0 ? (c.id | (c.tk << 24)) : -1;
+#pragma unroll
+            for (int r = 0; r < SR; ++r) sid[r] = __shfl(pk, ((half * SR + r) << 3) | h);
+#pragma unroll
+            for (int r = 0; r < SR; ++r)
+                v[r] = buf_load16(slot_rsrc, sid[r] >= 0 ? (uint32_t)(sid[r] & 0xFFFFFF) * 128u + (uint32_t)gl * 16u
+                                                         : kBufOff);
+        };
+        auto consume = [&](const int32_t (&sid)[SR], const uint4 (&v)[SR], int32_t base) {
+#pragma unroll
+            for (int r = 0; r < SR; ++r) {
+                if (sid[r] < 0) continue;
+                const int32_t b2 = base + ((uint32_t)sid[r] >> 24);
+                auto deposit = [&](uint32_t meta, uint32_t qv) {
+                    int32_t s = b2 + (int32_t)(meta >> 16);
+                    if (s >= R) s -= R;
+                    atomicAdd(&s_ring[s * words + (int32_t)(meta & 0xFFFFu)], (int32_t)qv);
+                };
+                if (v[r].x != kSlotEmpty) deposit(v[r].x, v[r].y);
+                if ((v[r].z & 0xFF000000u) == kSlotLink) {  // the rest of a long segment
+                    const uint32_t n = v[r].z & 0xFFFFFFu;
+                    for (uint32_t k = 0; k < n; ++k) {
+                        const uint2 e = X.ovf[(int64_t)v[r].w + k];
+                        deposit(e.x, e.y);
+                    }
+                } else if (v[r].z != kSlotEmpty) {
+                    deposit(v[r].z, v[r].w);
+                }
+            }
+        };
+        auto try_flush = [&]() { flush_to(min(s1, s0 + kss * wave_min16(s_prog, lane))); };
+        auto publish = [&](int32_t done) {
+            __hip_atomic_store(&s_prog[wv], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            try_flush();
+        };
+        // every ring slot super step i lands in is cleared in every slice
+        auto wait_ring = [&](int32_t i) {
+            const int32_t need = s0 + kss * (i + 1) + span - R;
+            uint32_t spins = 0;
+            while (!bad && wave_min16(s_fl, lane) < need) {
+                try_flush();
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) bad = true;
+            }
+        };
+
+        int32_t ci = 0, cc = 0;
+        while (ci < nss && rounds(ci) == 0) ++ci;
+        if (ci > 0) publish(min(ci, nss));
+        int32_t ni = ci, nc = cc;
+        if (ni < nss) next_chunk(ni, nc);
+        Spk P = load_ids(ci, cc), Q = load_ids(ni, nc);
+        int32_t sidA[SR], sidB[SR];
+        uint4 vA[SR], vB[SR];
+        // Fetches are unconditional (a lane without a spike reads zeros) and the chunk records P, Q
+        // swap roles by unrolling rather than by copies: a load under a branch, or a copy of a load
+        // still in flight, makes the compiler wait for it.
+        fetch(P, 0, sidA, vA);
+        // one chunk: cur's second half and the first half of nxt are fetched while cur is deposited;
+        // cur's record is then reloaded with the chunk after nxt
+        auto step = [&](Spk& cur, const Spk& nxt) {
+            fetch(cur, 1, sidB, vB);
+            // row bounds of the counted spikes, summed after the deposits (no wait here)
+            const uint32_t ro = cur.cnt ? (uint32_t)cur.id * 8u : kBufOff;
+            const int64_t r0 = buf_load_i64(rowp_rsrc, ro), r1 = buf_load_i64(rowp_rsrc, cur.cnt ? ro + 8u : kBufOff);
+            const int32_t base = (s0 - k_begin + ci * kss) % R;
+            if (cc == 0) wait_ring(ci);
+            consume(sidA, vA, base);
+            fetch(nxt, 0, sidA, vA);
+            consume(sidB, vB, base);
+            ev += (unsigned long long)(r1 - r0);
+            if (ni != ci) publish(min(ni, nss));
+            ci = ni;
+            cc = nc;
+            if (ni < nss) next_chunk(ni, nc);
+            cur = load_ids(ni, nc);
+        };
+        while (ci < nss) {
+            step(P, Q);
+            if (ci >= nss) break;
+            step(Q, P);
+        }
+        // drain: write out this wave's slice of every output step of the pass once all waves
+        // are done (others may still wait on this wave's slice)
+        uint32_t spins = 0;
+        while (!bad && fl < s1) {
+            try_flush();
+            if (fl < s1) __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) bad = true;
+        }
+    }
+    // one atomic per wave for the event count
+    for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);
+    if (lane == 0 && ev) atomicAdd(X.events, ev);
+    if (bad && lane == 0) atomicOr(X.err, kErrBarrier);
+}
+
 // Replay with global atomics (networks the post-blocked layout does not fit).
 __global__ __launch_bounds__(256) void nbx_replay_global(StepArgs A, const int32_t* __restrict__ ids,
                                                          const int32_t* __restrict__ tick_ptr, int32_t K,
@@ -969,6 +1245,7 @@ struct Geometry {
 struct ReplayGeometry {
     bool ok = false;
     bool slots = false;
+    bool stream = false;  // slots delivered by nbx_replay_stream (no window barriers)
     int32_t P = 0, B = 0;  // layout block size and count
     int32_t m = 1, Bq = 0, G = 1, W = 0, R = 0, C = 0, lanes = 4;
     int64_t lds = 0;
@@ -1277,20 +1554,46 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
     // tuning and tests
     const int32_t m_req = env("NBX_REPLAY_M", 0), w_req = env("NBX_REPLAY_W", 0), g_req = env("NBX_REPLAY_G", 0);
     const int32_t c_req = env("NBX_REPLAY_C", 0), slots_req = env("NBX_REPLAY_SLOTS", -1);
+    const int32_t stream_req = env("NBX_REPLAY_STREAM", 1);
     const int32_t c_min = 2048, c_max = kStageU * 1024;
     if (L.max_row >= (1 << 24)) return r;  // segment lengths share a word with the window step
     if (net->n > (1 << 24)) return r;        // staged ids share a word with the window step
     const double outdeg = (double)net->S / std::max<int32_t>(1, net->n);
-    for (int32_t m : {2, 1}) {
+    for (int32_t m : {3, 2, 1}) {
         if (m_req && m != m_req) continue;
+        if (m == 3 && m_req != 3) continue;  // three blocks per workgroup only on request
         if (m > L.B) continue;
         const double mean_seg = outdeg * m * L.P / std::max<int32_t>(1, net->n);
         // a 16-entry slot suits mean segments of ~3..10 entries (longer ones overflow often,
         // shorter ones waste most of the 128 bytes); otherwise the offset layout is used
-        const bool slots = slots_req >= 0 ? slots_req != 0 : (mean_seg >= 3.0 && mean_seg <= 10.5);
+        // (the streamed kernel on three blocks per workgroup takes up to ~13: its deeper pipeline
+        // absorbs the ~12 % of slots that continue in the overflow list)
+        const double seg_hi = (m == 3 && stream_req) ? 13.5 : 10.5;
+        const bool slots = slots_req >= 0 ? slots_req != 0 : (mean_seg >= 3.0 && mean_seg <= seg_hi);
         if (!slots && L.rb % m != 0) continue;
         const int64_t words = (int64_t)net->nch * m * L.P;
         if (slots && words > 65536) continue;  // a slot entry holds its LDS word in 16 bits
+        if (slots && stream_req) {  // ring of as many steps as fit, no staging buffers
+            const int64_t sbudget = (int64_t)max_lds - (int64_t)(kTpCap + 1) * 4 - 2 * kStreamWaves * 4 - 256;
+            const int64_t Rs = std::min<int64_t>(sbudget / (words * 4), 64);
+            if (Rs < net->D) continue;  // one super step of at least one input step past D - 2
+            r.slots = r.stream = true;
+            r.P = L.P;
+            r.B = L.B;
+            r.m = m;
+            r.R = (int32_t)Rs;
+            // super steps of at most half the ring's free steps: waves may drift apart by the rest
+            r.W = w_req > 0 ? w_req : std::max<int32_t>(1, (r.R - net->D + 1) / 2);
+            r.lds = Rs * words * 4;
+            r.Bq = (L.B + m - 1) / m;
+            int cus = 0;  // one workgroup per CU: as many time groups as the CUs hold
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device) != hipSuccess ||
+                cus <= 0)
+                cus = 256;
+            r.G = g_req > 0 ? g_req : std::max<int32_t>(1, cus / r.Bq);
+            r.ok = true;
+            return r;
+        }
         const int64_t stage_bytes = slots ? 8 : 16;  // per staged spike, two buffers
         const int32_t steps_fit = (int32_t)((budget - c_min * stage_bytes) / (words * 4));
         int32_t R = 0, W = 0;
@@ -1328,7 +1631,7 @@ int32_t replay_path(const nbx_net* net) {
     if (net->stdp || !net->rules.empty() || net->multi()) return NBX_REPLAY_GLOBAL;
     const ReplayGeometry r = replay_geometry(net);
     if (!r.ok) return NBX_REPLAY_GLOBAL;
-    if (r.slots) return NBX_REPLAY_SLOTS;
+    if (r.slots) return r.stream ? NBX_REPLAY_STREAM : NBX_REPLAY_SLOTS;
     return persistent_reason(net).empty() || net->pl.built ? NBX_REPLAY_OFFSETS : NBX_REPLAY_GLOBAL;
 }
 
@@ -1364,6 +1667,7 @@ int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_
             X.slots = nullptr;
             X.ovf = nullptr;
             X.row_ptr = net->d_row_ptr;
+            X.err = net->d_err;
             const void* fn = r.lanes == 16 ? (const void*)nbx_replay_ring<16>
                              : r.lanes == 8 ? (const void*)nbx_replay_ring<8>
                                             : (const void*)nbx_replay_ring<4>;
@@ -1372,7 +1676,7 @@ int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_
                 X.slots = reinterpret_cast<const uint4*>(net->rs.d_slots);
                 X.ovf = net->rs.d_ovf;
                 X.Bq = net->rs.Bq;
-                fn = (const void*)nbx_replay_slots;
+                fn = r.stream ? (const void*)nbx_replay_stream : (const void*)nbx_replay_slots;
             }
             if (r.lds > 64 * 1024)
                 NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)r.lds));

@@ -466,7 +466,7 @@ class Net:
         L.call("nbx_net_export_monitor", self.h, C.byref(d))
 
     def replay_path(self) -> int:
-        """Replay kernel this network uses: REPLAY_GLOBAL, REPLAY_OFFSETS or REPLAY_SLOTS."""
+        """Replay kernel this network uses: REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS or REPLAY_STREAM."""
         p = C.c_int32()
         L.call("nbx_replay_path", self.h, C.byref(p))
         return p.value

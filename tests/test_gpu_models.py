@@ -65,7 +65,7 @@ def test_model_replay_matches_brute_force(gpu, model, slots, monkeypatch):
     spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
     e0 = net.events()
     if slots is not None:
-        assert net.replay_path() == (L.REPLAY_SLOTS if slots == "1" else L.REPLAY_OFFSETS)
+        assert net.replay_path() == (L.REPLAY_STREAM if slots == "1" else L.REPLAY_OFFSETS)
     out = net.replay(spikes)
     expect, events = replay_expect(wl, net.fixed_shift, spikes)
     assert out.shape[1] == wl.spec.channels

@@ -245,17 +245,22 @@ def replay_expect(wl, fixed_shift, spikes):
 
 
 # Replay launch shapes forced through the library's launch-time knobs: (merge factor, window,
-# time groups, staged spikes, slot layout); None = the geometry the library picks. "c64" makes
-# most windows hold more spikes than one staging buffer (the chunked path).
-REPLAY_KNOBS = ("NBX_REPLAY_M", "NBX_REPLAY_W", "NBX_REPLAY_G", "NBX_REPLAY_C", "NBX_REPLAY_SLOTS")
+# time groups, staged spikes, slot layout, streamed slot kernel); None = the geometry the library
+# picks. "c64" makes most windows hold more spikes than one staging buffer (the chunked path);
+# the streamed kernel has no windows or staging (its ring holds as many steps as fit).
+REPLAY_KNOBS = ("NBX_REPLAY_M", "NBX_REPLAY_W", "NBX_REPLAY_G", "NBX_REPLAY_C", "NBX_REPLAY_SLOTS",
+                "NBX_REPLAY_STREAM")
 REPLAY_SHAPES = {
     "auto": None,
-    "ring-m2w8g2": ("2", "8", "2", "", "0"),
-    "ring-m1w8g3": ("1", "8", "3", "", "0"),
-    "ring-m2w16g5": ("2", "16", "5", "", "0"),
-    "ring-m1w12g2c64": ("1", "12", "2", "64", "0"),
-    "slots-m2w8g2": ("2", "8", "2", "", "1"),
-    "slots-m1w12g3c64": ("1", "12", "3", "64", "1"),
+    "ring-m2w8g2": ("2", "8", "2", "", "0", "0"),
+    "ring-m1w8g3": ("1", "8", "3", "", "0", "0"),
+    "ring-m2w16g5": ("2", "16", "5", "", "0", "0"),
+    "ring-m1w12g2c64": ("1", "12", "2", "64", "0", "0"),
+    "slots-m2w8g2": ("2", "8", "2", "", "1", "0"),
+    "slots-m1w12g3c64": ("1", "12", "3", "64", "1", "0"),
+    "stream-m2g2": ("2", "", "2", "", "1", "1"),
+    "stream-m3g3": ("3", "", "3", "", "1", "1"),
+    "stream-m1g7": ("1", "", "7", "", "1", "1"),
 }
 
 
@@ -279,7 +284,8 @@ def test_replay_delivery_brute_force(gpu, shape, monkeypatch):
     spikes = random_spikes(rng, wl.spec.n, K, 0, 60)
     spikes[7] = spikes[7][:0]  # an empty step
     if shape:
-        assert net.replay_path() == (L.REPLAY_SLOTS if shape[4] == "1" else L.REPLAY_OFFSETS)
+        expect_path = L.REPLAY_OFFSETS if shape[4] == "0" else (L.REPLAY_STREAM if shape[5] == "1" else L.REPLAY_SLOTS)
+        assert net.replay_path() == expect_path
     e0 = net.events()
     out = net.replay(spikes)
     expect, events = replay_expect(wl, net.fixed_shift, spikes)
@@ -287,10 +293,28 @@ def test_replay_delivery_brute_force(gpu, shape, monkeypatch):
     assert net.events() - e0 == events
 
 
-def test_replay_slot_overflow(gpu, monkeypatch):
+@pytest.mark.parametrize("stream", ["0", "1"], ids=["window", "stream"])
+def test_replay_dense_steps(gpu, stream, monkeypatch):
+    # 600-1300 spikes per step: the streamed kernel takes one input step per super step and
+    # several chunks of 8 rounds per wave; the window kernel stages in several chunks
+    set_replay_shape(monkeypatch, ("2", "8" if stream == "0" else "", "3", "", "1", stream))
+    wl = small_sparse(L.LIF_VJ, L.EULER, n=2500, k_in=80, seed=11)
+    net = wl.make_net()
+    rng = np.random.default_rng(7)
+    K = 40
+    spikes = random_spikes(rng, wl.spec.n, K, 600, 1300)
+    e0 = net.events()
+    out = net.replay(spikes)
+    expect, events = replay_expect(wl, net.fixed_shift, spikes)
+    assert_bits_equal(out.astype(np.int64), expect, "replay arrivals of dense steps")
+    assert net.events() - e0 == events
+
+
+@pytest.mark.parametrize("stream", ["0", "1"], ids=["window", "stream"])
+def test_replay_slot_overflow(gpu, stream, monkeypatch):
     # 640 neurons with in-degree 400: a pair of 32-neuron blocks receives ~40 synapses per
     # presynaptic neuron, so most slots continue in the overflow list
-    set_replay_shape(monkeypatch, ("2", "8", "2", "", "1"))
+    set_replay_shape(monkeypatch, ("2", "8" if stream == "0" else "", "2", "", "1", stream))
     wl = small_sparse(L.LIF_VJ, L.EULER, n=640, k_in=400, seed=4)
     net = wl.make_net()
     rng = np.random.default_rng(2)
@@ -303,10 +327,10 @@ def test_replay_slot_overflow(gpu, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("slots", ["0", "1"], ids=["ring", "slots"])
+@pytest.mark.parametrize("slots", ["0", "1", "stream"], ids=["ring", "slots", "stream"])
 def test_replay_two_channels(gpu, slots, monkeypatch):
     # conductance Izhikevich: inhibitory arrivals go to their own channel
-    set_replay_shape(monkeypatch, ("2", "8", "2", "", slots))
+    set_replay_shape(monkeypatch, ("2", "", "2", "", "1", "1") if slots == "stream" else ("2", "8", "2", "", slots, "0"))
     wl = small_sparse(L.IZH, L.EULER, n=2500, k_in=80, seed=6)
     net = wl.make_net()
     rng = np.random.default_rng(3)
@@ -319,14 +343,17 @@ def test_replay_two_channels(gpu, slots, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("slots", ["auto", "0"], ids=["auto", "ring"])
+@pytest.mark.parametrize("slots", ["auto", "0", "window"], ids=["auto", "ring", "window"])
 def test_replay_full_size(gpu, slots, monkeypatch):
     # configuration-size network (1e5 neurons, 1e8 synapses), 96 steps of 100-220 spikes
-    if slots != "auto":
+    if slots == "window":
+        monkeypatch.setenv("NBX_REPLAY_STREAM", "0")
+    elif slots != "auto":
         monkeypatch.setenv("NBX_REPLAY_SLOTS", slots)
     wl = W.sparse_lif()
     net = wl.make_net()
-    assert net.replay_path() == (L.REPLAY_SLOTS if slots == "auto" else L.REPLAY_OFFSETS)
+    expect_path = {"auto": L.REPLAY_STREAM, "0": L.REPLAY_OFFSETS, "window": L.REPLAY_SLOTS}[slots]
+    assert net.replay_path() == expect_path
     rng = np.random.default_rng(5)
     K = 96
     spikes = random_spikes(rng, wl.spec.n, K, 100, 220)
