@@ -50,8 +50,9 @@ struct PersistentLayout {
 // the entry's LDS word within one ring step, and its output step offset.
 struct ReplaySlots {
     bool built = false;
-    int32_t m = 0, Bq = 0;
-    uint2* d_slots = nullptr;  // [Bq][n][16]
+    bool packed = false;       // 6-byte entries (20 per slot), meta = (delay - 1) << wbits | LDS word
+    int32_t m = 0, Bq = 0, wbits = 0;
+    uint2* d_slots = nullptr;  // [Bq][n][16] 8-byte entries, or [Bq][n] 128-byte packed slots
     uint2* d_ovf = nullptr;
     int64_t ovf_count = 0;
 };

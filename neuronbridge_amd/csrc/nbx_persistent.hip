@@ -514,6 +514,7 @@ struct ReplayArgs {
     const uint2* ovf;
     const int64_t* row_ptr;  // CSR row starts (row lengths count the synaptic events)
     int32_t* err;            // device error flags (a bounded wait that gave up)
+    int32_t wbits;           // packed slots: bits of the LDS word in an entry's meta
 };
 
 template <int LANES>
@@ -956,7 +957,6 @@ __global__ __launch_bounds__(1024) void nbx_replay_slots(ReplayArgs X) {
 // read with acquire loads, so a slice is read after every deposit into it and cleared before
 // the next deposit into the same ring slot.
 constexpr int kStreamWaves = 16;    // waves per workgroup (1024 threads)
-constexpr int kStreamRounds = 4;    // rounds (spikes per lane group) per half-chunk
 constexpr int kStreamSpikes = 1024; // spikes per super step the super-step length aims for
 
 __device__ inline int32_t wave_min16(const int32_t* a, int32_t lane) {
@@ -968,13 +968,20 @@ __device__ inline int32_t wave_min16(const int32_t* a, int32_t lane) {
     return x;
 }
 
+// PACKED: 6-byte entries read by 4 lanes of 32 bytes per slot (build_replay_slots_packed);
+// otherwise 8-byte entries read by 8 lanes of 16 bytes.
+template <bool PACKED>
 __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
-    constexpr int SR = kStreamRounds;
+    constexpr int LPS = PACKED ? 4 : 8;  // lanes per slot
+    constexpr int GW = 64 / LPS;         // lane groups per wave
+    constexpr int NG = kStreamWaves * GW; // lane groups per workgroup
+    constexpr int CH = 64 / GW;          // rounds per chunk (one id load per lane)
+    constexpr int SR = CH / 2;           // rounds per half-chunk
     extern __shared__ int32_t s_mem[];
     __shared__ int32_t s_tp[kTpCap + 1];
     __shared__ int32_t s_prog[kStreamWaves], s_fl[kStreamWaves];
     const int32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int32_t gl = lane & 7, h = lane >> 3;
+    const int32_t gl = lane % LPS, h = lane / LPS;
     const int32_t Pr = X.m * X.P, words = X.nch * Pr, R = X.R;
     int32_t* s_ring = s_mem;  // [R][nch][Pr]
 
@@ -1042,11 +1049,11 @@ __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
         auto ss_hi = [&](int32_t i) { return s_tp[min(i * kss + kss, sl)]; };
         // rounds of this wave in super step i (its lane group 0 has the most spikes)
         auto rounds = [&](int32_t i) {
-            const int32_t c = ss_hi(i) - ss_lo(i) - 8 * wv;
-            return c > 0 ? (c + 127) / 128 : 0;
+            const int32_t c = ss_hi(i) - ss_lo(i) - GW * wv;
+            return c > 0 ? (c + NG - 1) / NG : 0;
         };
         auto next_chunk = [&](int32_t& i, int32_t& c) {
-            if (8 * (c + 1) < rounds(i)) {
+            if (CH * (c + 1) < rounds(i)) {
                 ++c;
                 return;
             }
@@ -1065,7 +1072,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
             Spk r{0, -1, false};
             int32_t f = -1;
             if (i < nss) {
-                f = ss_lo(i) + 8 * wv + (lane & 7) + 128 * (8 * c + (lane >> 3));
+                f = ss_lo(i) + GW * wv + (lane % GW) + NG * (CH * c + lane / GW);
                 if (f < ss_hi(i)) {
                     const int32_t t0 = i * kss;
                     int32_t a = 0, b = min(kss, sl - t0);
@@ -1083,18 +1090,23 @@ __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
             r.id = buf_load_i32(ids_rsrc, f >= 0 ? (uint32_t)f * 4u : kBufOff);
             return r;
         };
-        // slots of half `half` (rounds 4 half .. 4 half + 3) of a chunk: group h's spike of each round
-        // comes from the lane that loaded it; a round without a spike reads zeros
-        auto fetch = [&](const Spk& c, int32_t half, int32_t (&sid)[SR], uint4 (&v)[SR]) {
+        // slots of half `half` (rounds SR half .. SR half + SR - 1) of a chunk: group h's spike of
+        // each round comes from the lane that loaded it; a round without a spike reads zeros
+        constexpr int VL = PACKED ? 2 : 1;  // 16-byte loads per lane and slot
+        auto fetch = [&](const Spk& c, int32_t half, int32_t (&sid)[SR], uint4 (&v)[SR][VL]) {
             const int32_t pk = c.tk >= 0 ? (c.id | (c.tk << 24)) : -1;
 #pragma unroll
-            for (int r = 0; r < SR; ++r) sid[r] = __shfl(pk, ((half * SR + r) << 3) | h);
+            for (int r = 0; r < SR; ++r) sid[r] = __shfl(pk, (half * SR + r) * GW + h);
 #pragma unroll
             for (int r = 0; r < SR; ++r)
-                v[r] = buf_load16(slot_rsrc, sid[r] >= 0 ? (uint32_t)(sid[r] & 0xFFFFFF) * 128u + (uint32_t)gl * 16u
-                                                         : kBufOff);
+#pragma unroll
+                for (int u = 0; u < VL; ++u)
+                    v[r][u] = buf_load16(slot_rsrc, sid[r] >= 0 ? (uint32_t)(sid[r] & 0xFFFFFF) * 128u +
+                                                                      (uint32_t)(gl * VL + u) * 16u
+                                                                : kBufOff);
         };
-        auto consume = [&](const int32_t (&sid)[SR], const uint4 (&v)[SR], int32_t base) {
+        const uint32_t wmask = (1u << X.wbits) - 1u;
+        auto consume = [&](const int32_t (&sid)[SR], const uint4 (&v)[SR][VL], int32_t base) {
 #pragma unroll
             for (int r = 0; r < SR; ++r) {
                 if (sid[r] < 0) continue;
@@ -1104,15 +1116,38 @@ __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
                     if (s >= R) s -= R;
                     atomicAdd(&s_ring[s * words + (int32_t)(meta & 0xFFFFu)], (int32_t)qv);
                 };
-                if (v[r].x != kSlotEmpty) deposit(v[r].x, v[r].y);
-                if ((v[r].z & 0xFF000000u) == kSlotLink) {  // the rest of a long segment
-                    const uint32_t n = v[r].z & 0xFFFFFFu;
+                if constexpr (PACKED) {
+                    auto deposit_p = [&](uint32_t meta, uint32_t qv) {
+                        int32_t s = b2 + (int32_t)(meta >> X.wbits);
+                        if (s >= R) s -= R;
+                        atomicAdd(&s_ring[s * words + (int32_t)(meta & wmask)], (int32_t)qv);
+                    };
+                    const uint4 a = v[r][0], b = v[r][1];
+                    const uint32_t lw = b.w >> 16, c = lw & 7u;
+                    if (c > 0) deposit_p(b.y & 0xFFFFu, a.x);
+                    if (c > 1) deposit_p(b.y >> 16, a.y);
+                    if (c > 2) deposit_p(b.z & 0xFFFFu, a.z);
+                    if (c > 3) deposit_p(b.z >> 16, a.w);
+                    if (c > 4) deposit_p(b.w & 0xFFFFu, b.x);
+                    if (lw & 0x8000u) {  // the rest of a long segment (8-byte entries)
+                        const uint32_t n = (lw >> 3) & 0xFFFu;
+                        for (uint32_t k = 0; k < n; ++k) {
+                            const uint2 e = X.ovf[(int64_t)b.x + k];
+                            deposit(e.x, e.y);
+                        }
+                    }
+                    continue;
+                }
+                const uint4 w4 = v[r][0];
+                if (w4.x != kSlotEmpty) deposit(w4.x, w4.y);
+                if ((w4.z & 0xFF000000u) == kSlotLink) {  // the rest of a long segment
+                    const uint32_t n = w4.z & 0xFFFFFFu;
                     for (uint32_t k = 0; k < n; ++k) {
-                        const uint2 e = X.ovf[(int64_t)v[r].w + k];
+                        const uint2 e = X.ovf[(int64_t)w4.w + k];
                         deposit(e.x, e.y);
                     }
-                } else if (v[r].z != kSlotEmpty) {
-                    deposit(v[r].z, v[r].w);
+                } else if (w4.z != kSlotEmpty) {
+                    deposit(w4.z, w4.w);
                 }
             }
         };
@@ -1139,7 +1174,7 @@ __global__ __launch_bounds__(1024) void nbx_replay_stream(ReplayArgs X) {
         if (ni < nss) next_chunk(ni, nc);
         Spk P = load_ids(ci, cc), Q = load_ids(ni, nc);
         int32_t sidA[SR], sidB[SR];
-        uint4 vA[SR], vB[SR];
+        uint4 vA[SR][VL], vB[SR][VL];
         // Fetches are unconditional (a lane without a spike reads zeros) and the chunk records P, Q
         // swap roles by unrolling rather than by copies: a load under a branch, or a copy of a load
         // still in flight, makes the compiler wait for it.
@@ -1246,6 +1281,8 @@ struct ReplayGeometry {
     bool ok = false;
     bool slots = false;
     bool stream = false;  // slots delivered by nbx_replay_stream (no window barriers)
+    bool packed = false;  // streamed slots with 6-byte entries
+    int32_t wbits = 0;
     int32_t P = 0, B = 0;  // layout block size and count
     int32_t m = 1, Bq = 0, G = 1, W = 0, R = 0, C = 0, lanes = 4;
     int64_t lds = 0;
@@ -1407,7 +1444,7 @@ int32_t refresh_persistent_weights(nbx_net* net) {
 
 static int32_t build_replay_slots(nbx_net* net, int32_t m) {
     ReplaySlots& RS = net->rs;
-    if (RS.built && RS.m == m) return NBX_OK;
+    if (RS.built && !RS.packed && RS.m == m) return NBX_OK;
     if (RS.built) {
         for (void* p : {(void*)RS.d_slots, (void*)RS.d_ovf}) NBX_PASS(free_owned(net, p));
         RS = ReplaySlots();
@@ -1454,6 +1491,89 @@ static int32_t build_replay_slots(nbx_net* net, int32_t m) {
     RS.ovf_count = (int64_t)ovf.size();
     RS.m = m;
     RS.Bq = Bq;
+    RS.built = true;
+    return NBX_OK;
+}
+
+// Packed slots: 128 bytes per (presynaptic neuron, post block of m*P neurons), read by four lanes
+// of 32 bytes. Lane l holds entries 5l .. 5l+4 as q[0..4] (words 0-4, the quantized weights) and
+// metas m[0..4] (16-bit halves of words 5-7: (delay - 1) << wbits | LDS word), and in the last
+// half a lane word: entry count (bits 0-2); the last lane of a segment longer than 20 holds 4
+// entries, the overflow list offset in q[4], the number of further entries in bits 3-14 and
+// the link flag in bit 15. Overflow entries keep the 8-byte format.
+static int32_t build_replay_slots_packed(nbx_net* net, int32_t m, int32_t wbits) {
+    ReplaySlots& RS = net->rs;
+    if (RS.built && RS.packed && RS.m == m && RS.wbits == wbits) return NBX_OK;
+    if (RS.built) {
+        for (void* p : {(void*)RS.d_slots, (void*)RS.d_ovf}) NBX_PASS(free_owned(net, p));
+        RS = ReplaySlots();
+    }
+    const Geometry L = geometry(net);
+    const int32_t n = net->n, Pr = m * L.P, Bq = (L.B + m - 1) / m;
+    const float scale = std::ldexp(1.0f, net->shift);
+    constexpr int kPer = 5, kLanes = 4, kCap = kPer * kLanes;
+    std::vector<uint32_t> slots((size_t)Bq * n * 32, 0u);
+    std::vector<uint2> ovf;
+    std::vector<int32_t> cnt((size_t)Bq), fill((size_t)Bq);
+    std::vector<int64_t> ovf_at((size_t)Bq);
+    for (int32_t p = 0; p < n; ++p) {
+        const int64_t rbeg = net->h_row_ptr[(size_t)p], rend = net->h_row_ptr[(size_t)p + 1];
+        std::fill(cnt.begin(), cnt.end(), 0);
+        std::fill(fill.begin(), fill.end(), 0);
+        for (int64_t j = rbeg; j < rend; ++j) cnt[(size_t)(syn_post(net->h_syn[(size_t)j]) / Pr)]++;
+        auto slot = [&](int32_t qb) { return slots.data() + ((size_t)qb * n + (size_t)p) * 32; };
+        for (int32_t qb = 0; qb < Bq; ++qb) {
+            const int32_t c = cnt[(size_t)qb];
+            uint32_t* sp = slot(qb);
+            const int32_t inslot = c > kCap ? kCap - 1 : c;
+            for (int32_t l = 0; l < kLanes; ++l) {
+                const int32_t lc = std::max(0, std::min(kPer, inslot - kPer * l));
+                sp[l * 8 + 7] = (uint32_t)lc << 16;
+            }
+            if (c > kCap) {
+                const int64_t excess = c - (kCap - 1);
+                if (excess > 4095) {
+                    set_error("replay slots: a segment too long for the packed format");
+                    return NBX_ERR_UNSUPPORTED;
+                }
+                sp[3 * 8 + 4] = (uint32_t)ovf.size();  // lane 3, q[4]
+                sp[3 * 8 + 7] |= (0x8000u | (uint32_t)excess << 3) << 16;
+                ovf_at[(size_t)qb] = (int64_t)ovf.size();
+                ovf.resize(ovf.size() + (size_t)excess);
+            }
+        }
+        for (int64_t j = rbeg; j < rend; ++j) {
+            const uint32_t e = net->h_syn[(size_t)j];
+            const int32_t post = syn_post(e), ty = syn_type(e), qb = post / Pr;
+            int32_t qv = (int32_t)std::lrint(net->h_w[(size_t)j] * scale);
+            const int32_t ch = arrival_channel(net->P.model, ty);
+            if (arrival_negated(net->P.model, ty)) qv = -qv;
+            const uint32_t word = (uint32_t)(ch * Pr + post - qb * Pr), dly = (uint32_t)(syn_delay(e) - 1);
+            const int32_t k = fill[(size_t)qb]++;
+            const int32_t c = cnt[(size_t)qb];
+            if (c > kCap && k >= kCap - 1) {  // overflow list, 8-byte entries
+                ovf[(size_t)(ovf_at[(size_t)qb] + (k - (kCap - 1)))] = make_uint2(word | dly << 16, (uint32_t)qv);
+                continue;
+            }
+            uint32_t* lw = slot(qb) + (k / kPer) * 8;
+            const int32_t i = k % kPer;
+            lw[i] = (uint32_t)qv;
+            const uint32_t meta = dly << wbits | word;
+            uint16_t* half = reinterpret_cast<uint16_t*>(lw + 5);  // little endian: meta i is half i
+            half[i] = (uint16_t)meta;
+        }
+    }
+    if ((int64_t)ovf.size() >= ((int64_t)1 << 31)) {
+        set_error("replay slots: overflow list too long");
+        return NBX_ERR_UNSUPPORTED;
+    }
+    NBX_PASS(net->upload(&RS.d_slots, reinterpret_cast<const uint2*>(slots.data()), slots.size() / 2));
+    NBX_PASS(net->upload(&RS.d_ovf, ovf.data(), ovf.size()));
+    RS.ovf_count = (int64_t)ovf.size();
+    RS.m = m;
+    RS.Bq = Bq;
+    RS.packed = true;
+    RS.wbits = wbits;
     RS.built = true;
     return NBX_OK;
 }
@@ -1559,6 +1679,48 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
     if (L.max_row >= (1 << 24)) return r;  // segment lengths share a word with the window step
     if (net->n > (1 << 24)) return r;        // staged ids share a word with the window step
     const double outdeg = (double)net->S / std::max<int32_t>(1, net->n);
+    int cus = 0;  // the streamed kernel runs one workgroup per CU: as many time groups as CUs hold
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int64_t sbudget = (int64_t)max_lds - (int64_t)(kTpCap + 1) * 4 - 2 * kStreamWaves * 4 - 256;
+    auto stream_shape = [&](int32_t m, int64_t words, bool packed, int32_t wbits) {
+        const int64_t Rs = std::min<int64_t>(sbudget / (words * 4), 64);
+        if (Rs < net->D) return false;  // one super step of at least one input step past D - 2
+        r.slots = r.stream = true;
+        r.packed = packed;
+        r.wbits = wbits;
+        r.P = L.P;
+        r.B = L.B;
+        r.m = m;
+        r.R = (int32_t)Rs;
+        // super steps may fill the ring's free steps but two (measured on the 1e5-neuron network
+        // with 20-step delays: R = 30, super steps of 6 steps 2.11 ms, of 5 steps 2.17 ms)
+        r.W = w_req > 0 ? w_req : std::max<int32_t>(1, r.R - net->D - 1);
+        r.lds = Rs * words * 4;
+        r.Bq = (L.B + m - 1) / m;
+        r.G = g_req > 0 ? g_req : std::max<int32_t>(1, cus / r.Bq);
+        r.ok = true;
+        return true;
+    };
+    // Packed slots (20 six-byte entries per 128 bytes) when an entry's LDS word and delay fit
+    // 16 bits: the largest block merge whose mean segment stays below ~13.5 entries (about 1 %
+    // of segments continue in the overflow list), i.e. the fewest slot reads per spike.
+    if (stream_req && env("NBX_REPLAY_PACKED", 1) && slots_req != 0) {
+        auto bits = [](int64_t maxval) {
+            int32_t b = 0;
+            while (((int64_t)1 << b) <= maxval) ++b;
+            return b;
+        };
+        for (int32_t m : {3, 2, 1}) {
+            if ((m_req && m != m_req) || m > L.B) continue;
+            const double mean_seg = outdeg * m * L.P / std::max<int32_t>(1, net->n);
+            if (slots_req < 0 && (mean_seg < 3.0 || mean_seg > 13.5)) continue;
+            const int64_t words = (int64_t)net->nch * m * L.P;
+            const int32_t wbits = bits(words - 1);
+            if (wbits + bits(net->D - 2) > 16) continue;
+            if (stream_shape(m, words, true, wbits)) return r;
+        }
+    }
     for (int32_t m : {3, 2, 1}) {
         if (m_req && m != m_req) continue;
         if (m == 3 && m_req != 3) continue;  // three blocks per workgroup only on request
@@ -1574,25 +1736,8 @@ static ReplayGeometry replay_geometry(const nbx_net* net) {
         const int64_t words = (int64_t)net->nch * m * L.P;
         if (slots && words > 65536) continue;  // a slot entry holds its LDS word in 16 bits
         if (slots && stream_req) {  // ring of as many steps as fit, no staging buffers
-            const int64_t sbudget = (int64_t)max_lds - (int64_t)(kTpCap + 1) * 4 - 2 * kStreamWaves * 4 - 256;
-            const int64_t Rs = std::min<int64_t>(sbudget / (words * 4), 64);
-            if (Rs < net->D) continue;  // one super step of at least one input step past D - 2
-            r.slots = r.stream = true;
-            r.P = L.P;
-            r.B = L.B;
-            r.m = m;
-            r.R = (int32_t)Rs;
-            // super steps of at most half the ring's free steps: waves may drift apart by the rest
-            r.W = w_req > 0 ? w_req : std::max<int32_t>(1, (r.R - net->D + 1) / 2);
-            r.lds = Rs * words * 4;
-            r.Bq = (L.B + m - 1) / m;
-            int cus = 0;  // one workgroup per CU: as many time groups as the CUs hold
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device) != hipSuccess ||
-                cus <= 0)
-                cus = 256;
-            r.G = g_req > 0 ? g_req : std::max<int32_t>(1, cus / r.Bq);
-            r.ok = true;
-            return r;
+            if (stream_shape(m, words, false, 0)) return r;
+            continue;
         }
         const int64_t stage_bytes = slots ? 8 : 16;  // per staged spike, two buffers
         const int32_t steps_fit = (int32_t)((budget - c_min * stage_bytes) / (words * 4));
@@ -1668,15 +1813,22 @@ int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_
             X.ovf = nullptr;
             X.row_ptr = net->d_row_ptr;
             X.err = net->d_err;
+            X.wbits = r.wbits;
             const void* fn = r.lanes == 16 ? (const void*)nbx_replay_ring<16>
                              : r.lanes == 8 ? (const void*)nbx_replay_ring<8>
                                             : (const void*)nbx_replay_ring<4>;
-            if (r.slots) {
+            if (r.packed) {
+                NBX_PASS(build_replay_slots_packed(net, r.m, r.wbits));
+                X.slots = reinterpret_cast<const uint4*>(net->rs.d_slots);
+                X.ovf = net->rs.d_ovf;
+                X.Bq = net->rs.Bq;
+                fn = (const void*)nbx_replay_stream<true>;
+            } else if (r.slots) {
                 NBX_PASS(build_replay_slots(net, r.m));
                 X.slots = reinterpret_cast<const uint4*>(net->rs.d_slots);
                 X.ovf = net->rs.d_ovf;
                 X.Bq = net->rs.Bq;
-                fn = r.stream ? (const void*)nbx_replay_stream : (const void*)nbx_replay_slots;
+                fn = r.stream ? (const void*)nbx_replay_stream<false> : (const void*)nbx_replay_slots;
             }
             if (r.lds > 64 * 1024)
                 NBX_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)r.lds));

@@ -245,22 +245,26 @@ def replay_expect(wl, fixed_shift, spikes):
 
 
 # Replay launch shapes forced through the library's launch-time knobs: (merge factor, window,
-# time groups, staged spikes, slot layout, streamed slot kernel); None = the geometry the library
-# picks. "c64" makes most windows hold more spikes than one staging buffer (the chunked path);
-# the streamed kernel has no windows or staging (its ring holds as many steps as fit).
+# time groups, staged spikes, slot layout, streamed slot kernel, packed 6-byte slot entries);
+# None = the geometry the library picks. "c64" makes most windows hold more spikes than one
+# staging buffer (the chunked path); the streamed kernel has no windows or staging (its ring holds
+# as many steps as fit; W caps its super step).
 REPLAY_KNOBS = ("NBX_REPLAY_M", "NBX_REPLAY_W", "NBX_REPLAY_G", "NBX_REPLAY_C", "NBX_REPLAY_SLOTS",
-                "NBX_REPLAY_STREAM")
+                "NBX_REPLAY_STREAM", "NBX_REPLAY_PACKED")
 REPLAY_SHAPES = {
     "auto": None,
-    "ring-m2w8g2": ("2", "8", "2", "", "0", "0"),
-    "ring-m1w8g3": ("1", "8", "3", "", "0", "0"),
-    "ring-m2w16g5": ("2", "16", "5", "", "0", "0"),
-    "ring-m1w12g2c64": ("1", "12", "2", "64", "0", "0"),
-    "slots-m2w8g2": ("2", "8", "2", "", "1", "0"),
-    "slots-m1w12g3c64": ("1", "12", "3", "64", "1", "0"),
-    "stream-m2g2": ("2", "", "2", "", "1", "1"),
-    "stream-m3g3": ("3", "", "3", "", "1", "1"),
-    "stream-m1g7": ("1", "", "7", "", "1", "1"),
+    "ring-m2w8g2": ("2", "8", "2", "", "0", "0", "0"),
+    "ring-m1w8g3": ("1", "8", "3", "", "0", "0", "0"),
+    "ring-m2w16g5": ("2", "16", "5", "", "0", "0", "0"),
+    "ring-m1w12g2c64": ("1", "12", "2", "64", "0", "0", "0"),
+    "slots-m2w8g2": ("2", "8", "2", "", "1", "0", "0"),
+    "slots-m1w12g3c64": ("1", "12", "3", "64", "1", "0", "0"),
+    "stream-m2g2": ("2", "", "2", "", "1", "1", "0"),
+    "stream-m3g3": ("3", "", "3", "", "1", "1", "0"),
+    "stream-m1g7w1": ("1", "1", "7", "", "1", "1", "0"),
+    "packed-m3g3": ("3", "", "3", "", "1", "1", "1"),
+    "packed-m2g2w2": ("2", "2", "2", "", "1", "1", "1"),
+    "packed-m1g5": ("1", "", "5", "", "1", "1", "1"),
 }
 
 
@@ -293,11 +297,19 @@ def test_replay_delivery_brute_force(gpu, shape, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("stream", ["0", "1"], ids=["window", "stream"])
-def test_replay_dense_steps(gpu, stream, monkeypatch):
+STREAM_VARIANTS = {"window": ("2", "8", "0", "0"), "stream": ("2", "", "1", "0"), "packed": ("3", "", "1", "1")}
+
+
+def stream_shape(variant, g):
+    m, w, stream, packed = STREAM_VARIANTS[variant]
+    return (m, w, g, "", "1", stream, packed)
+
+
+@pytest.mark.parametrize("variant", list(STREAM_VARIANTS))
+def test_replay_dense_steps(gpu, variant, monkeypatch):
     # 600-1300 spikes per step: the streamed kernel takes one input step per super step and
-    # several chunks of 8 rounds per wave; the window kernel stages in several chunks
-    set_replay_shape(monkeypatch, ("2", "8" if stream == "0" else "", "3", "", "1", stream))
+    # several chunks per wave; the window kernel stages in several chunks
+    set_replay_shape(monkeypatch, stream_shape(variant, "3"))
     wl = small_sparse(L.LIF_VJ, L.EULER, n=2500, k_in=80, seed=11)
     net = wl.make_net()
     rng = np.random.default_rng(7)
@@ -310,11 +322,11 @@ def test_replay_dense_steps(gpu, stream, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("stream", ["0", "1"], ids=["window", "stream"])
-def test_replay_slot_overflow(gpu, stream, monkeypatch):
-    # 640 neurons with in-degree 400: a pair of 32-neuron blocks receives ~40 synapses per
+@pytest.mark.parametrize("variant", list(STREAM_VARIANTS))
+def test_replay_slot_overflow(gpu, variant, monkeypatch):
+    # 640 neurons with in-degree 400: two or three 32-neuron blocks receive ~40-60 synapses per
     # presynaptic neuron, so most slots continue in the overflow list
-    set_replay_shape(monkeypatch, ("2", "8" if stream == "0" else "", "2", "", "1", stream))
+    set_replay_shape(monkeypatch, stream_shape(variant, "2"))
     wl = small_sparse(L.LIF_VJ, L.EULER, n=640, k_in=400, seed=4)
     net = wl.make_net()
     rng = np.random.default_rng(2)
@@ -327,10 +339,11 @@ def test_replay_slot_overflow(gpu, stream, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("slots", ["0", "1", "stream"], ids=["ring", "slots", "stream"])
+@pytest.mark.parametrize("slots", ["0", "1", "stream", "packed"], ids=["ring", "slots", "stream", "packed"])
 def test_replay_two_channels(gpu, slots, monkeypatch):
     # conductance Izhikevich: inhibitory arrivals go to their own channel
-    set_replay_shape(monkeypatch, ("2", "", "2", "", "1", "1") if slots == "stream" else ("2", "8", "2", "", slots, "0"))
+    shapes = {"stream": ("2", "", "2", "", "1", "1", "0"), "packed": ("2", "", "2", "", "1", "1", "1")}
+    set_replay_shape(monkeypatch, shapes.get(slots, ("2", "8", "2", "", slots, "0", "0")))
     wl = small_sparse(L.IZH, L.EULER, n=2500, k_in=80, seed=6)
     net = wl.make_net()
     rng = np.random.default_rng(3)
@@ -343,16 +356,20 @@ def test_replay_two_channels(gpu, slots, monkeypatch):
     assert net.events() - e0 == events
 
 
-@pytest.mark.parametrize("slots", ["auto", "0", "window"], ids=["auto", "ring", "window"])
+@pytest.mark.parametrize("slots", ["auto", "0", "window", "stream"], ids=["auto", "ring", "window", "stream"])
 def test_replay_full_size(gpu, slots, monkeypatch):
-    # configuration-size network (1e5 neurons, 1e8 synapses), 96 steps of 100-220 spikes
+    # configuration-size network (1e5 neurons, 1e8 synapses), 96 steps of 100-220 spikes; "auto"
+    # is the streamed kernel on packed slots of three blocks, "stream" on 8-byte slots of two
     if slots == "window":
         monkeypatch.setenv("NBX_REPLAY_STREAM", "0")
+    elif slots == "stream":
+        monkeypatch.setenv("NBX_REPLAY_PACKED", "0")
     elif slots != "auto":
         monkeypatch.setenv("NBX_REPLAY_SLOTS", slots)
     wl = W.sparse_lif()
     net = wl.make_net()
-    expect_path = {"auto": L.REPLAY_STREAM, "0": L.REPLAY_OFFSETS, "window": L.REPLAY_SLOTS}[slots]
+    expect_path = {"auto": L.REPLAY_STREAM, "0": L.REPLAY_OFFSETS, "window": L.REPLAY_SLOTS,
+                   "stream": L.REPLAY_STREAM}[slots]
     assert net.replay_path() == expect_path
     rng = np.random.default_rng(5)
     K = 96
