@@ -453,6 +453,22 @@ typedef struct {
     int32_t neuron_count;
 } nbx_monitor_desc;
 int32_t nbx_net_export_monitor(nbx_net* net, const nbx_monitor_desc* desc);
+/* Time-series monitor (reference: SimulationDebugMonitor::Initialize / CaptureStep / Flush,
+ * SimulationDebugMonitor.cpp:27-96, and DebugMonitorWriter's tables): writes meta.json and the
+ * eight tables' headers into output_dir now; from then on every step t whose update completes
+ * (nbx_net_run on any engine, or nbx_net_commit_firings) with t % sample_interval_steps == 0 is
+ * sampled as one frame, all of whose rows carry time_step t: spikes of the selected neurons at t
+ * (ascending ids), their state after t's update (as nbx_net_export_monitor writes it), the arrivals
+ * already pending for step t + 1 (delay ring, float image; arrivals of plastic synapses are pulled at
+ * their arrival step and are not pending), and every synapse's weight. Frames are captured on the
+ * device and written every flush_interval_steps frames (<= 0: 100), earlier when the device frame
+ * buffers (at most 512 MiB) are full, at nbx_net_monitor_flush and at nbx_net_monitor_stop (also
+ * called by nbx_net_destroy). A run with the monitor on is cut after each sampled step. */
+int32_t nbx_net_monitor_start(nbx_net* net, const nbx_monitor_desc* desc);
+int32_t nbx_net_monitor_flush(nbx_net* net);
+int32_t nbx_net_monitor_stop(nbx_net* net);
+/* Frames written so far and frames captured but not yet written. */
+int32_t nbx_net_monitor_frames(const nbx_net* net, int64_t* written, int32_t* held);
 
 /* Which replay kernel nbx_replay_deliver uses for this network (launch-shape overrides
  * in the environment included): NBX_REPLAY_GLOBAL (global atomics; STDP networks and

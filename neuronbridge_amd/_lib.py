@@ -200,6 +200,10 @@ SIGNATURES = {
     "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],
     "nbx_net_export_monitor": [_NET, C.POINTER(MonitorDesc)],
+    "nbx_net_monitor_start": [_NET, C.POINTER(MonitorDesc)],
+    "nbx_net_monitor_flush": [_NET],
+    "nbx_net_monitor_stop": [_NET],
+    "nbx_net_monitor_frames": [_NET, C.POINTER(C.c_int64), C.POINTER(C.c_int32)],
     "nbx_net_set_rules": [_NET, C.POINTER(RuleDesc), C.c_int32, _i32p, _i32p, _f32p],
     "nbx_net_get_rule_state": [_NET, _f32p, _f32p, _f32p, _i32p, C.c_int64],
     "nbx_net_assign_nbnet_rules": [_NET, C.POINTER(RuleDesc), C.c_int32],

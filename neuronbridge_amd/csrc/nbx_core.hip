@@ -551,6 +551,7 @@ int32_t nbx_net_create_populations(const nbx_model_desc* pops, int32_t count, in
 int32_t nbx_net_destroy(nbx_net* net) {
     if (!net) return NBX_OK;
     hipSetDevice(net->device);
+    if (net->mon.on) (void)nbx_net_monitor_stop(net);  // writes the frames still held
     if (net->stream) hipStreamSynchronize(net->stream);
     net->release();
     delete net;
@@ -659,6 +660,7 @@ const char* rule_desc_error(const nbx_rule_desc& q, bool allow_stdp) {
         default: return "unknown learning rule kind";
     }
 }
+std::vector<float*> state_columns(nbx_net* net, int32_t model) { return model_columns(net, model); }
 }  // namespace nbx
 
 extern "C" {
@@ -1014,6 +1016,15 @@ int32_t nbx_net_pick_engine(const nbx_net* net, int32_t* engine) {
     return NBX_OK;
 }
 
+static int32_t run_engine(nbx_net* net, int32_t nsteps, int32_t engine) {
+    switch (engine) {
+        case NBX_ENGINE_KERNELS: return run_kernels(net, nsteps);
+        case NBX_ENGINE_GRAPH: return run_graph(net, nsteps);
+        case NBX_ENGINE_PERSISTENT: return run_persistent(net, nsteps);
+        default: set_error("unknown engine"); return NBX_ERR_INVALID;
+    }
+}
+
 int32_t nbx_net_run_async(nbx_net* net, int32_t nsteps, int32_t engine) {
     int32_t rc = require_finalized(net);
     if (rc) return rc;
@@ -1021,17 +1032,24 @@ int32_t nbx_net_run_async(nbx_net* net, int32_t nsteps, int32_t engine) {
     NBX_CHECK(net->phase == 0, "a per-phase step is open (finish it with nbx_net_commit_firings)");
     NBX_RC(device_scope(net));
     if (engine == NBX_ENGINE_AUTO) NBX_RC(nbx_net_pick_engine(net, &engine));
-    switch (engine) {
-        case NBX_ENGINE_KERNELS: return run_kernels(net, nsteps);
-        case NBX_ENGINE_GRAPH: return run_graph(net, nsteps);
-        case NBX_ENGINE_PERSISTENT:
-            if (!persistent_supported(net)) {
-                set_error("persistent engine unsupported for this network: " + persistent_reason(net));
-                return NBX_ERR_UNSUPPORTED;
-            }
-            return run_persistent(net, nsteps);
-        default: set_error("unknown engine"); return NBX_ERR_INVALID;
+    NBX_CHECK(engine >= NBX_ENGINE_KERNELS && engine <= NBX_ENGINE_PERSISTENT, "unknown engine");
+    if (engine == NBX_ENGINE_PERSISTENT && !persistent_supported(net)) {
+        set_error("persistent engine unsupported for this network: " + persistent_reason(net));
+        return NBX_ERR_UNSUPPORTED;
     }
+    if (!net->mon.on) return run_engine(net, nsteps, engine);
+    // with the time-series monitor on, the run is cut after every sampled step and the step's frame
+    // is captured there (the same engine runs each piece; pieces compose like separate runs)
+    const int32_t iv = net->mon.interval;
+    for (int32_t done = 0; done < nsteps;) {
+        const int32_t t = net->t;
+        const int32_t s = ((t + iv - 1) / iv) * iv;  // first sampled step >= t
+        const int32_t len = std::min(nsteps - done, s - t + 1);
+        NBX_RC(run_engine(net, len, engine));
+        done += len;
+        if (net->t - 1 == s) NBX_RC(monitor_capture(net, s));
+    }
+    return NBX_OK;
 }
 
 int32_t nbx_net_sync(nbx_net* net) {
@@ -1241,6 +1259,7 @@ int32_t nbx_net_commit_firings(nbx_net* net) {
     NBX_HIP_TRY(hipStreamSynchronize(net->stream));
     net->t += 1;
     net->phase = 0;
+    if (net->mon.on && (net->t - 1) % net->mon.interval == 0) NBX_RC(monitor_capture(net, net->t - 1));
     return NBX_OK;
 }
 

@@ -73,6 +73,26 @@ struct ReplayStdpWork {
     unsigned long long* upd = nullptr;  // update counter of the host entry point
 };
 
+// Time-series monitor (nbx_net_monitor_start): after every sampled step a frame of device copies
+// (state, the step's firing list, pending arrivals, weights) goes into the next free
+// slot of the frame buffers; when they are full, or at a flush, the held frames are written to the
+// DebugMonitor CSV tables in dir.
+struct NetMonitor {
+    bool on = false;
+    std::string dir, name;
+    int32_t interval = 1, flush_every = 100;
+    bool spikes = false, state = false, weights = false, pending = false;
+    std::vector<uint8_t> sel;    // [n] selected neurons
+    int32_t cap = 0, held = 0;   // frames the buffers hold; frames captured and not yet written
+    std::vector<int32_t> steps;  // step of each held frame
+    float* d_state = nullptr;    // [cap][ns][n]
+    int32_t* d_spk = nullptr;    // [cap][n + 1]: the step's firing count, then its ids
+    int32_t* d_pend = nullptr;   // [cap][nch][n] arrivals pending for the next step
+    float* d_w = nullptr;        // [cap][S]
+    std::vector<void*> allocs;   // freed at stop
+    int64_t frames_written = 0;
+};
+
 // One neuron population: neurons [begin, begin + md.n) with one model and its parameters.
 struct NetPop {
     int32_t begin = 0;
@@ -203,6 +223,8 @@ struct nbx_net {
     ReplaySlots rs;
     // replay STDP
     ReplayStdpWork rsw;
+    // time-series monitor
+    NetMonitor mon;
 
     StepArgs args() const {
         StepArgs A{};
@@ -309,6 +331,9 @@ struct nbx_net {
         gexec = nullptr;
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
+        for (void* p : mon.allocs) (void)hipFree(p);
+        mon.allocs.clear();
+        mon.on = false;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -321,6 +346,8 @@ std::string persistent_reason(const nbx_net* net);
 int32_t build_persistent_layout(nbx_net* net);
 int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
+std::vector<float*> state_columns(nbx_net* net, int32_t model);  // device array of each state column
+int32_t monitor_capture(nbx_net* net, int32_t step);  // frame of the completed step (flushes when full)
 std::string small_dense_reason(const nbx_net* net);  // empty when the single-workgroup engine fits
 int32_t run_small_dense(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);

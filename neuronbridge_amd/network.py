@@ -452,10 +452,8 @@ class Net:
                C.byref(upd))
         return upd.value
 
-    def export_monitor(self, output_dir: str, *, sample_interval: int = 1, flush_interval: int = 100,
-                       spikes: bool = True, state: bool = True, weights: bool = False, pending: bool = True,
-                       neuron_ids=None, name: str = "main") -> None:
-        """Writes the DebugMonitor directory (meta.json + CSV tables) the reference's results.py reads."""
+    @staticmethod
+    def _monitor_desc(output_dir, sample_interval, flush_interval, spikes, state, weights, pending, neuron_ids, name):
         ids = None if neuron_ids is None else np.ascontiguousarray(neuron_ids, dtype=np.int32)
         d = L.MonitorDesc(output_dir=os.fsencode(output_dir), component_name=name.encode(),
                           sample_interval_steps=sample_interval, flush_interval_steps=flush_interval,
@@ -463,7 +461,37 @@ class Net:
                           record_pending_channels=int(pending),
                           neuron_ids=_p(ids, C.c_int32) if ids is not None else None,
                           neuron_count=0 if ids is None else int(ids.size))
+        return d, ids
+
+    def export_monitor(self, output_dir: str, *, sample_interval: int = 1, flush_interval: int = 100,
+                       spikes: bool = True, state: bool = True, weights: bool = False, pending: bool = True,
+                       neuron_ids=None, name: str = "main") -> None:
+        """Writes the DebugMonitor directory (meta.json + CSV tables) the reference's results.py reads:
+        spikes from the recorded raster, state / pending / weights as one snapshot."""
+        d, _keep = self._monitor_desc(output_dir, sample_interval, flush_interval, spikes, state, weights, pending,
+                                      neuron_ids, name)
         L.call("nbx_net_export_monitor", self.h, C.byref(d))
+
+    def monitor_start(self, output_dir: str, *, sample_interval: int = 1, flush_interval: int = 100,
+                      spikes: bool = True, state: bool = True, weights: bool = False, pending: bool = True,
+                      neuron_ids=None, name: str = "main") -> None:
+        """Starts the time-series monitor (SimulationDebugMonitor): every later step t with
+        t % sample_interval == 0 is sampled after its update and written to the DebugMonitor tables."""
+        d, _keep = self._monitor_desc(output_dir, sample_interval, flush_interval, spikes, state, weights, pending,
+                                      neuron_ids, name)
+        L.call("nbx_net_monitor_start", self.h, C.byref(d))
+
+    def monitor_flush(self) -> None:
+        L.call("nbx_net_monitor_flush", self.h)
+
+    def monitor_stop(self) -> None:
+        L.call("nbx_net_monitor_stop", self.h)
+
+    def monitor_frames(self) -> tuple:
+        """(frames written, frames held on the device)."""
+        w, h = C.c_int64(), C.c_int32()
+        L.call("nbx_net_monitor_frames", self.h, C.byref(w), C.byref(h))
+        return int(w.value), int(h.value)
 
     def replay_path(self) -> int:
         """Replay kernel this network uses: REPLAY_GLOBAL, REPLAY_OFFSETS, REPLAY_SLOTS or REPLAY_STREAM."""
