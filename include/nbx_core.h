@@ -583,6 +583,15 @@ int32_t nbx_lif_fwd_segment(const nbx_lif_adj_params* p, const float* d_W, const
                             float* d_V, float* d_S, float* d_Vm, int32_t nsteps, void* stream);
 int32_t nbx_lif_bwd_segment(const nbx_lif_adj_params* p, const float* d_WT, const float* d_dI_after, const float* d_R,
                             const float* d_Vm, float* d_dV, float* d_dI, int32_t nsteps, void* stream);
+/* The forward segment for binary spikes (modes 0 and 1): W_rec s_{k-1} as the sum of the rows of
+ * WT = W_rec^T of the neurons that fired at k-1, in ascending neuron order (deterministic, so a
+ * recomputed segment is bit-identical), instead of a dense product. Same results as
+ * nbx_lif_fwd_segment up to the rounding of the recurrent sum. d_bits: caller workspace of
+ * 2 * batch * ceil(n / 32) uint32 (the spikes of consecutive steps as bit words). Mode 2 is
+ * rejected (its spikes are not binary). */
+int32_t nbx_lif_fwd_segment_sparse(const nbx_lif_adj_params* p, const float* d_WT, const float* d_IN,
+                                   const float* d_s_init, float* d_V, float* d_S, float* d_Vm, uint32_t* d_bits,
+                                   int32_t nsteps, void* stream);
 /* Leaky readout y_t = beta*y_{t-1} + (1-beta)*Z_t over [T][m] (state carried in d_state if given),
  * and its adjoint dZ_t = (1-beta)*(gy_t + beta*(...)) as a reverse scan. */
 int32_t nbx_readout_fwd(int32_t T, int64_t m, float beta, const float* d_Z, float* d_y, float* d_state, void* stream);

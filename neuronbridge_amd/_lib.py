@@ -229,6 +229,7 @@ SIGNATURES = {
     "nbx_lif_fwd_step": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp],
     "nbx_lif_bwd_step": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp, _vp],
     "nbx_lif_fwd_segment": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp],
+    "nbx_lif_fwd_segment_sparse": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp],
     "nbx_lif_bwd_segment": [C.POINTER(LifAdjParams), _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, _vp],
     "nbx_readout_fwd": [C.c_int32, C.c_int64, C.c_float, _vp, _vp, _vp, _vp],
     "nbx_readout_bwd": [C.c_int32, C.c_int64, C.c_float, _vp, _vp, _vp, _vp],

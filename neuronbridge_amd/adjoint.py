@@ -2,7 +2,8 @@
 
 Forward, per step t (state [N, B] float32, B trials):
     I_t  = W_in x_t + W_rec s_{t-1}             (the input term for all t in one GEMM; the recurrent
-                                                  term on the matrix cores inside the step kernel)
+                                                  term inside the step kernel: the rows of W_rec^T of
+                                                  the neurons that fired, or MFMA for smooth spikes)
     V_t, s_t = LIF step                          (hand-written HIP kernel: RK4-exact affine update,
                                                   strict threshold, subtractive reset)
     y_t  = beta y_{t-1} + (1 - beta) W_out s_t   (GEMM for all t + HIP scan kernel)
@@ -12,8 +13,10 @@ Only the membrane potential at every `checkpoint`-th step is kept from the forwa
 each segment is recomputed before its backward sweep (checkpoint=0 keeps every step).
 Weight gradients are accumulated per segment as large GEMMs over (steps x trials).
 
-The per-step recurrent products run inside libnbx's fused step kernels (FP32 MFMA tiles,
-nbx_lif_fwd_segment / nbx_lif_bwd_segment, one launch per step from a C loop); the large
+The per-step recurrent products run inside libnbx's step kernels, one launch per step from a C
+loop: forward with binary spikes as a sum of the W_rec^T rows of the neurons that fired
+(nbx_lif_fwd_segment_sparse), with smooth spikes and backward on FP32 MFMA tiles
+(nbx_lif_fwd_segment / nbx_lif_bwd_segment); the large
 GEMMs over all steps (input currents, readout, weight gradients) run through torch (hipBLASLt),
 all on torch's current stream.
 """
@@ -64,13 +67,23 @@ def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _fwd_segment(cp, IN, W_rec, s_init, V, t0, t1, S_out, Vm_out=None):
+def _fwd_segment(cp, IN, W_rec, W_rec_t, bits, s_init, V, t0, t1, S_out, Vm_out=None):
     """Forward steps [t0, t1) from state V (updated in place) and the previous step's spikes
     s_init; spikes of step t go to S_out[t - t0], membrane potentials to Vm_out[t - t0].
-    One fused kernel per step: W_rec s_{t-1} on the matrix cores, then the LIF step."""
+    One kernel per step. Binary spikes (modes 0, 1): W_rec s_{t-1} as the sum of the W_rec^T rows
+    of the neurons that fired (bits: spike-word workspace); smooth spikes (mode 2): the dense
+    product on the matrix cores."""
     assert S_out.is_contiguous() and (Vm_out is None or Vm_out.is_contiguous()) and s_init.is_contiguous()
-    L.call("nbx_lif_fwd_segment", C.byref(cp), _ptr(W_rec), _ptr(IN[t0]), _ptr(s_init), _ptr(V), _ptr(S_out),
-           _ptr(Vm_out), t1 - t0, _stream())
+    if cp.mode == 2:
+        L.call("nbx_lif_fwd_segment", C.byref(cp), _ptr(W_rec), _ptr(IN[t0]), _ptr(s_init), _ptr(V), _ptr(S_out),
+               _ptr(Vm_out), t1 - t0, _stream())
+    else:
+        L.call("nbx_lif_fwd_segment_sparse", C.byref(cp), _ptr(W_rec_t), _ptr(IN[t0]), _ptr(s_init), _ptr(V),
+               _ptr(S_out), _ptr(Vm_out), _ptr(bits), t1 - t0, _stream())
+
+
+def _spike_words(N: int, B: int, device) -> torch.Tensor:
+    return torch.empty(2 * B * ((N + 31) // 32), device=device, dtype=torch.int32)
 
 
 class LifAdjoint(torch.autograd.Function):
@@ -95,13 +108,15 @@ class LifAdjoint(torch.autograd.Function):
         K = checkpoint if checkpoint and checkpoint > 0 else T
         ckpts = []
         zero = torch.zeros_like(V)
+        W_rec_t = W_rec.t().contiguous()
+        bits = _spike_words(N, B, x.device)
         for t0 in range(0, T, K):
             ckpts.append(V.clone())
-            _fwd_segment(cp, IN, W_rec, S[t0 - 1] if t0 > 0 else zero, V, t0, min(T, t0 + K), S[t0:])
+            _fwd_segment(cp, IN, W_rec, W_rec_t, bits, S[t0 - 1] if t0 > 0 else zero, V, t0, min(T, t0 + K), S[t0:])
         Z = torch.matmul(W_out, S)  # [T, M, B]
         y = torch.empty_like(Z)
         L.call("nbx_readout_fwd", T, M * B, p.beta, _ptr(Z), _ptr(y), C.c_void_p(0), _stream())
-        ctx.save_for_backward(W_in, W_rec, W_out, x, S, IN, torch.stack(ckpts))
+        ctx.save_for_backward(W_in, W_rec, W_rec_t, W_out, x, S, IN, torch.stack(ckpts))
         ctx.p = p
         ctx.K = K
         ctx.mark_non_differentiable(S)
@@ -109,7 +124,7 @@ class LifAdjoint(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gy, _gS):
-        W_in, W_rec, W_out, x, S, IN, ckpts = ctx.saved_tensors
+        W_in, W_rec, W_rec_t, W_out, x, S, IN, ckpts = ctx.saved_tensors
         p, K = ctx.p, ctx.K
         T, cin, B = x.shape
         N, M = W_rec.shape[0], W_out.shape[0]
@@ -120,8 +135,8 @@ class LifAdjoint(torch.autograd.Function):
         # dW_out = sum_t dZ_t S_t^T ; readout term of ds: R_t = W_out^T dZ_t
         dW_out = torch.matmul(dZ.permute(1, 0, 2).reshape(M, T * B), S.permute(1, 0, 2).reshape(N, T * B).t())
         W_out_t = W_out.t().contiguous()
-        W_rec_t = W_rec.t().contiguous()
         R = torch.matmul(W_out_t, dZ)  # [T, N, B]
+        bits = _spike_words(N, B, x.device)
         dW_rec = torch.zeros_like(W_rec)
         dW_in = torch.zeros_like(W_in)
         dV = torch.zeros((N, B), device=x.device, dtype=torch.float32)
@@ -133,7 +148,8 @@ class LifAdjoint(torch.autograd.Function):
             V = ckpts[seg].clone()
             Vm = torch.empty((k, N, B), device=x.device, dtype=torch.float32)
             S_scr = torch.empty((k, N, B), device=x.device, dtype=torch.float32)
-            _fwd_segment(cp, IN, W_rec, S[t0 - 1] if t0 > 0 else zeros_s[0], V, t0, t1, S_scr, Vm_out=Vm)
+            _fwd_segment(cp, IN, W_rec, W_rec_t, bits, S[t0 - 1] if t0 > 0 else zeros_s[0], V, t0, t1, S_scr,
+                         Vm_out=Vm)
             dI = torch.empty((k, N, B), device=x.device, dtype=torch.float32)
             # steps t1-1 .. t0, each fusing G = W_rec^T dI_{t+1} with the adjoint step; dI_{t1} is
             # the first step of the later segment (none after the last step)

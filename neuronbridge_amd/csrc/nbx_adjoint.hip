@@ -230,6 +230,121 @@ hipError_t launch_rec(bool bwd, const RecArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- sparse forward step (binary spikes)
+//
+// With Heaviside spikes (modes 0 and 1) W_rec s_{t-1} is a sum of the columns of the neurons that
+// fired at t-1: a trial has ~0.5-2 % of its neurons firing per step, so the product reads those
+// rows of W_rec^T (coalesced, 4 B per neuron each) instead of the whole matrix. Workgroup = 64
+// neurons x 8 trials: wave w takes trial b, lists the trial's fired neurons j in ascending order
+// from its spike words (one word per lane, popcount prefix over the wave), and lane l sums
+// WT[j][i0 + l] over the list in that order (deterministic, so a recomputed segment is
+// bit-identical). The step itself runs on a transposed mapping (8 consecutive threads = one
+// neuron's 8 trials, contiguous in the [n][batch] arrays), and the step's spikes go out as words
+// again, [batch][ceil(n/32)], for the next step.
+constexpr int kSpTrials = 8;                     // trials per workgroup (one wave each)
+constexpr int kSpNeurons = 64;                   // neurons per workgroup (one per lane)
+constexpr int kSpThreads = 64 * kSpTrials;
+constexpr int kSpWords = 64;                     // spike words listed per pass (one per lane)
+constexpr int kSpListCap = kSpWords * 32;        // list entries per wave and pass
+
+struct SpArgs {
+    nbx_lif_adj_params p;
+    const float* WT;         // [n][n]: WT[j * n + i] = W_rec[i][j]
+    const uint32_t* bits;    // spikes of the previous step, [batch][nw]: bit i % 32 of word i / 32
+    const float* IN;         // [n][batch]
+    float* V;                // [n][batch], in/out
+    float* S;                // [n][batch], out
+    float* Vm;               // [n][batch], out or null
+    uint32_t* bits_out;      // spikes of this step, [batch][nw]
+    int32_t nw;
+};
+
+__global__ __launch_bounds__(kSpThreads) void nbx_lif_spk_kernel(SpArgs a) {
+    __shared__ int32_t s_list[kSpTrials][kSpListCap];
+    __shared__ float s_acc[kSpTrials][kSpNeurons];
+    __shared__ uint8_t s_fire[kSpTrials][kSpNeurons];
+    const nbx_lif_adj_params& p = a.p;
+    const int32_t n = p.n, B = p.batch;
+    const int32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int32_t ntg = (B + kSpTrials - 1) / kSpTrials;
+    const int32_t nb = (int32_t)blockIdx.x / ntg, tg = (int32_t)blockIdx.x % ntg;
+    const int32_t i0 = nb * kSpNeurons;
+    const int32_t b = tg * kSpTrials + wave;
+    const int32_t i = i0 + lane;
+    int32_t* list = s_list[wave];
+    float acc = 0.0f;
+    for (int32_t w0 = 0; w0 < a.nw; w0 += kSpWords) {
+        const int32_t wi = w0 + lane;
+        const uint32_t word = (b < B && wi < a.nw) ? a.bits[(int64_t)b * a.nw + wi] : 0u;
+        const int32_t c = __popc(word);
+        int32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t z = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += z;
+        }
+        const int32_t total = __shfl(incl, 63, 64);
+        int32_t pos = incl - c;
+        for (uint32_t m = word; m; m &= m - 1u) list[pos++] = wi * 32 + __builtin_ctz(m);
+        __syncthreads();
+        if (i < n) {
+            const float* col = a.WT + i;
+            int32_t k = 0;
+            for (; k + 8 <= total; k += 8) {
+                float w[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] = col[(int64_t)list[k + u] * n];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += w[u];
+            }
+            for (; k < total; ++k) acc += col[(int64_t)list[k] * n];
+        }
+        __syncthreads();  // the list is read before the next pass overwrites it
+    }
+    s_acc[wave][lane] = acc;
+    __syncthreads();
+    const int32_t il = tid / kSpTrials, bl = tid % kSpTrials;
+    const int32_t ii = i0 + il, bb = tg * kSpTrials + bl;
+    bool fired = false;
+    if (ii < n && bb < B) {  // the forward step of nbx_lif_fwd_kernel on I = IN_t + W_rec s_{t-1}
+        const int64_t j = (int64_t)ii * B + bb;
+        const float I = a.IN[j] + s_acc[bl][il];
+        const float vm = p.g * a.V[j] + p.one_minus_g * (p.v_rest + p.r * I);
+        fired = vm > p.v_th;
+        const float sp = fired ? 1.0f : 0.0f;
+        a.V[j] = vm - p.delta_reset * sp;
+        if (a.Vm) a.Vm[j] = vm;
+        a.S[j] = sp;
+    }
+    s_fire[bl][il] = fired ? 1 : 0;
+    __syncthreads();
+    if (tid < 2 * kSpTrials) {  // this block's two spike words of each of its trials
+        const int32_t bl2 = tid >> 1, h = tid & 1;
+        const int32_t bb2 = tg * kSpTrials + bl2, wi = (i0 >> 5) + h;
+        if (bb2 < B && wi < a.nw) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int q = 0; q < 32; ++q) word |= (uint32_t)s_fire[bl2][h * 32 + q] << q;
+            a.bits_out[(int64_t)bb2 * a.nw + wi] = word;
+        }
+    }
+}
+
+// Spike words [batch][nw] of a float spike matrix s [n][batch] (s > 0: fired); null s: none fired.
+__global__ __launch_bounds__(kThreads) void nbx_spike_words_kernel(const float* __restrict__ s, int32_t n, int32_t B,
+                                                                   int32_t nw, uint32_t* __restrict__ bits) {
+    const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (k >= (int64_t)B * nw) return;
+    const int32_t b = (int32_t)(k / nw), wi = (int32_t)(k % nw);
+    uint32_t word = 0;
+    if (s)
+        for (int q = 0; q < 32; ++q) {
+            const int32_t i = wi * 32 + q;
+            if (i < n && s[(int64_t)i * B + b] > 0.0f) word |= 1u << q;
+        }
+    bits[k] = word;
+}
+
 int blocks_for(int64_t count) { return (int)((count + kThreads - 1) / kThreads); }
 
 int32_t check(const nbx_lif_adj_params* p) {
@@ -292,6 +407,42 @@ int32_t nbx_lif_fwd_segment(const nbx_lif_adj_params* p, const float* d_W, const
         a.out = d_S + k * nb;
         a.Vm_out = d_Vm ? d_Vm + k * nb : nullptr;
         NBX_HIP_TRY(launch_rec(false, a, (hipStream_t)stream));
+    }
+    return NBX_OK;
+}
+
+int32_t nbx_lif_fwd_segment_sparse(const nbx_lif_adj_params* p, const float* d_WT, const float* d_IN,
+                                   const float* d_s_init, float* d_V, float* d_S, float* d_Vm, uint32_t* d_bits,
+                                   int32_t nsteps, void* stream) {
+    int32_t rc = check(p);
+    if (rc) return rc;
+    if (!d_WT || !d_IN || !d_V || !d_S || !d_bits || nsteps < 0) {
+        set_error("nbx_lif_fwd_segment_sparse: bad arguments");
+        return NBX_ERR_INVALID;
+    }
+    if (p->mode != 0 && p->mode != 1) {
+        set_error("nbx_lif_fwd_segment_sparse: binary spikes only (modes 0 and 1); mode 2 uses nbx_lif_fwd_segment");
+        return NBX_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int32_t nw = (p->n + 31) / 32;
+    const int64_t nb = (int64_t)p->n * p->batch, words = (int64_t)p->batch * nw;
+    nbx_spike_words_kernel<<<blocks_for(words), kThreads, 0, st>>>(d_s_init, p->n, p->batch, nw, d_bits);
+    NBX_HIP_TRY(hipGetLastError());
+    SpArgs a{};
+    a.p = *p;
+    a.WT = d_WT;
+    a.nw = nw;
+    const int32_t grid = ((p->n + kSpNeurons - 1) / kSpNeurons) * ((p->batch + kSpTrials - 1) / kSpTrials);
+    for (int32_t k = 0; k < nsteps; ++k) {
+        a.bits = d_bits + (k & 1) * words;
+        a.bits_out = d_bits + ((k + 1) & 1) * words;
+        a.IN = d_IN + k * nb;
+        a.V = d_V;
+        a.S = d_S + k * nb;
+        a.Vm = d_Vm ? d_Vm + k * nb : nullptr;
+        nbx_lif_spk_kernel<<<grid, kSpThreads, 0, st>>>(a);
+        NBX_HIP_TRY(hipGetLastError());
     }
     return NBX_OK;
 }

@@ -127,3 +127,61 @@ def test_fused_recurrent_steps_against_fp64(gpu, n, b):
     got = dI[0].double().cpu().numpy()
     assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.max(np.abs(want)))
     assert np.max(np.abs(dVd.double().cpu().numpy() - p.g * dvm)) <= 1e-5 * max(1.0, np.max(np.abs(dvm)))
+
+
+@pytest.mark.parametrize("n,b", [(2048, 64), (100, 5), (300, 33), (2100, 9)])
+def test_sparse_forward_segment_against_fp64_and_dense(gpu, n, b):
+    # the forward segment for binary spikes (sum of the W_rec^T rows of the neurons that fired)
+    # against the same steps in FP64 and against the dense MFMA segment, over several steps so the
+    # spike words of one step feed the next; recomputation is bit-identical
+    import ctypes as C
+    from neuronbridge_amd import _lib as L
+    rng = np.random.default_rng(n * 7 + b)
+    p = LifParams()
+    cp = p.c_params(n, b)
+    k = 6
+    W = (rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32)
+    s0 = (rng.random((n, b)) < 0.05).astype(np.float32)
+    IN = (rng.standard_normal((k, n, b)) * 0.08 + 0.04).astype(np.float32)
+    V0 = rng.uniform(-0.2, 0.95, (n, b)).astype(np.float32)
+    Wd, WTd, s0d, INd = to_dev(W, np.ascontiguousarray(W.T), s0, IN)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    bits = torch.empty(2 * b * ((n + 31) // 32), device=DEV, dtype=torch.int32)
+    outs = []
+    for rep in range(2):
+        V = torch.tensor(V0, device=DEV)
+        S = torch.empty((k, n, b), device=DEV)
+        Vm = torch.empty((k, n, b), device=DEV)
+        L.call("nbx_lif_fwd_segment_sparse", C.byref(cp), ptr(WTd), ptr(INd), ptr(s0d), ptr(V), ptr(S), ptr(Vm),
+               ptr(bits), k, stream)
+        torch.cuda.synchronize()
+        outs.append((V.cpu().numpy(), S.cpu().numpy(), Vm.cpu().numpy()))
+    for x, y in zip(outs[0], outs[1]):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), "recomputed segment differs"
+    V_g, S_g, Vm_g = outs[0]
+    # FP64 steps fed with the GPU's own spikes (threshold decisions checked away from rounding distance)
+    Wf = W.astype(np.float64)
+    v = V0.astype(np.float64)
+    s_prev = s0.astype(np.float64)
+    for t in range(k):
+        vm64 = p.g * v + (1 - p.g) * (p.v_rest + p.r * (IN[t].astype(np.float64) + Wf @ s_prev))
+        assert np.max(np.abs(Vm_g[t] - vm64)) <= 1e-5 * max(1.0, np.max(np.abs(vm64)))
+        near = np.abs(vm64 - p.v_th) < 1e-4
+        assert np.array_equal((S_g[t] > 0)[~near], (vm64 > p.v_th)[~near])
+        s_prev = S_g[t].astype(np.float64)
+        v = vm64 - (p.v_th - p.v_reset) * s_prev
+    assert S_g.sum() > 0
+    # the dense segment on the same inputs gives the same spikes away from threshold and the same
+    # potentials to rounding
+    V = torch.tensor(V0, device=DEV)
+    S = torch.empty((k, n, b), device=DEV)
+    Vm = torch.empty((k, n, b), device=DEV)
+    L.call("nbx_lif_fwd_segment", C.byref(cp), ptr(Wd), ptr(INd), ptr(s0d), ptr(V), ptr(S), ptr(Vm), k, stream)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(Vm.cpu().numpy()[0] - Vm_g[0])) <= 1e-5
+    # mode 2 is not binary: rejected
+    cp2 = LifParams(mode=2).c_params(n, b)
+    with pytest.raises(L.NbxError):
+        L.call("nbx_lif_fwd_segment_sparse", C.byref(cp2), ptr(WTd), ptr(INd), ptr(s0d), ptr(V), ptr(S), ptr(Vm),
+               ptr(bits), k, stream)
