@@ -496,6 +496,62 @@ def adjoint_bench(reps: int) -> dict:
             "speedup_vs_plain": ms_p / ms, "grad_rel_err_vs_plain": rel, "grad_tol": 1e-4}
 
 
+def adjoint_kernels_bench(steps: int) -> dict:
+    """The C5 step kernels alone, without torch (device buffers through HIP directly, so a counter
+    profile holds libnbx kernels only): one forward segment of `steps` steps (binary spikes, the
+    sum of the W_rec^T rows of the neurons that fired) and one backward segment (W_rec^T dI_{t+1}
+    on FP32 MFMA + the adjoint step) at the configuration's shape, N = 2048, 64 trials, on the
+    configuration's weights and inputs."""
+    import ctypes as C
+    from neuronbridge_amd import _lib as L
+    from neuronbridge_amd import workloads as W
+    from neuronbridge_amd.adjoint import LifParams
+    from neuronbridge_amd.hiptime import DeviceBuffer, Event, hip
+    Wi, Wr, Wo, x, tg = W.diff_lif()
+    N, B = Wr.shape[0], x.shape[2]
+    p = LifParams()
+    cp = p.c_params(N, B)
+    nb = N * B
+    IN = np.matmul(Wi, x[:steps]).astype(np.float32)  # [steps, N, B]
+    d_WT = DeviceBuffer(N * N * 4)
+    d_WT.upload(np.ascontiguousarray(Wr.T).astype(np.float32))
+    d_W = DeviceBuffer(N * N * 4)
+    d_W.upload(np.ascontiguousarray(Wr).astype(np.float32))
+    d_IN = DeviceBuffer(IN.nbytes)
+    d_IN.upload(np.ascontiguousarray(IN))
+    d_V, d_s0 = DeviceBuffer(nb * 4), DeviceBuffer(nb * 4)
+    d_s0.upload(np.zeros(nb, np.float32))
+    d_S, d_Vm, d_dI = DeviceBuffer(steps * nb * 4), DeviceBuffer(steps * nb * 4), DeviceBuffer(steps * nb * 4)
+    d_bits = DeviceBuffer(2 * B * ((N + 31) // 32) * 4)
+    rng = np.random.default_rng(5)
+    d_R, d_dV, d_dIa = DeviceBuffer(steps * nb * 4), DeviceBuffer(nb * 4), DeviceBuffer(nb * 4)
+    d_R.upload((rng.standard_normal(steps * nb) * 1e-3).astype(np.float32))
+    d_dIa.upload((rng.standard_normal(nb) * 1e-3).astype(np.float32))
+    def segments():
+        d_V.upload(np.full(nb, p.v_rest, np.float32))
+        d_dV.upload(np.zeros(nb, np.float32))
+        hip().hipDeviceSynchronize()
+        e0.record(0)
+        L.call("nbx_lif_fwd_segment_sparse", C.byref(cp), d_WT.ptr, d_IN.ptr, d_s0.ptr, d_V.ptr, d_S.ptr, d_Vm.ptr,
+               d_bits.ptr, steps, C.c_void_p(0))
+        e1.record(0)
+        L.call("nbx_lif_bwd_segment", C.byref(cp), d_WT.ptr, d_dIa.ptr, d_R.ptr, d_Vm.ptr, d_dV.ptr, d_dI.ptr, steps,
+               C.c_void_p(0))
+        e2.record(0)
+        e2.synchronize()
+
+    e0, e1, e2 = Event(), Event(), Event()
+    segments()  # warm-up (first launches load the code objects)
+    segments()
+    fwd, bwd = e0.elapsed_ms(e1) / steps, e1.elapsed_ms(e2) / steps
+    flop = 2.0 * N * N * B
+    S = np.zeros(steps * nb, np.float32)
+    d_S.download(S)
+    return {"workload": f"C5 step kernels alone: N={N}, {B} trials, {steps}-step segments, no torch",
+            "fwd_sparse_us_per_step": round(fwd * 1e3, 2), "bwd_mfma_us_per_step": round(bwd * 1e3, 2),
+            "bwd_tflops": flop / (bwd * 1e-3) / 1e12, "spike_rate_hz": float(S.mean()) * 1000.0}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +569,8 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=5000)
     ap.add_argument("--batched-sample", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-kernels", type=int, default=0,
+                    help="also time the C5 step kernels alone over segments of this many steps (no torch)")
     ap.add_argument("--no-stdp", action="store_true")
     ap.add_argument("--stdp-steps", type=int, default=2000)
     ap.add_argument("--no-dense", action="store_true")
@@ -603,6 +661,9 @@ def main() -> int:
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(3)
         log(f"adjoint: {out['adjoint']}")
+    if args.adjoint_kernels > 0:
+        out["adjoint_kernels"] = adjoint_kernels_bench(args.adjoint_kernels)
+        log(f"adjoint kernels: {out['adjoint_kernels']}")
     if not args.no_stdp:
         out["stdp"] = stdp_bench(args.stdp_steps)
         log(f"stdp: {out['stdp']}")

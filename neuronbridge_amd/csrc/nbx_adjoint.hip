@@ -272,6 +272,13 @@ __global__ __launch_bounds__(kSpThreads) void nbx_lif_spk_kernel(SpArgs a) {
     const int32_t b = tg * kSpTrials + wave;
     const int32_t i = i0 + lane;
     int32_t* list = s_list[wave];
+    // this thread's output of the step (transposed mapping below): its input current and potential
+    // are loaded first, so their latency overlaps the list and the row loads
+    const int32_t il = tid / kSpTrials, bl = tid % kSpTrials;
+    const int32_t ii = i0 + il, bb = tg * kSpTrials + bl;
+    const bool mine = ii < n && bb < B;
+    const int64_t jo = (int64_t)ii * B + bb;
+    const float in_j = mine ? a.IN[jo] : 0.0f, v_j = mine ? a.V[jo] : 0.0f;
     float acc = 0.0f;
     for (int32_t w0 = 0; w0 < a.nw; w0 += kSpWords) {
         const int32_t wi = w0 + lane;
@@ -290,12 +297,19 @@ __global__ __launch_bounds__(kSpThreads) void nbx_lif_spk_kernel(SpArgs a) {
         if (i < n) {
             const float* col = a.WT + i;
             int32_t k = 0;
-            for (; k + 8 <= total; k += 8) {
-                float w[8];
+            for (; k + 16 <= total; k += 16) {
+                float w[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) w[u] = col[(int64_t)list[k + u] * n];
+                for (int u = 0; u < 16; ++u) w[u] = col[(int64_t)list[k + u] * n];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) acc += w[u];
+                for (int u = 0; u < 16; ++u) acc += w[u];
+            }
+            for (; k + 4 <= total; k += 4) {
+                float w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) w[u] = col[(int64_t)list[k + u] * n];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc += w[u];
             }
             for (; k < total; ++k) acc += col[(int64_t)list[k] * n];
         }
@@ -303,18 +317,15 @@ __global__ __launch_bounds__(kSpThreads) void nbx_lif_spk_kernel(SpArgs a) {
     }
     s_acc[wave][lane] = acc;
     __syncthreads();
-    const int32_t il = tid / kSpTrials, bl = tid % kSpTrials;
-    const int32_t ii = i0 + il, bb = tg * kSpTrials + bl;
     bool fired = false;
-    if (ii < n && bb < B) {  // the forward step of nbx_lif_fwd_kernel on I = IN_t + W_rec s_{t-1}
-        const int64_t j = (int64_t)ii * B + bb;
-        const float I = a.IN[j] + s_acc[bl][il];
-        const float vm = p.g * a.V[j] + p.one_minus_g * (p.v_rest + p.r * I);
+    if (mine) {  // the forward step of nbx_lif_fwd_kernel on I = IN_t + W_rec s_{t-1}
+        const float I = in_j + s_acc[bl][il];
+        const float vm = p.g * v_j + p.one_minus_g * (p.v_rest + p.r * I);
         fired = vm > p.v_th;
         const float sp = fired ? 1.0f : 0.0f;
-        a.V[j] = vm - p.delta_reset * sp;
-        if (a.Vm) a.Vm[j] = vm;
-        a.S[j] = sp;
+        a.V[jo] = vm - p.delta_reset * sp;
+        if (a.Vm) a.Vm[jo] = vm;
+        a.S[jo] = sp;
     }
     s_fire[bl][il] = fired ? 1 : 0;
     __syncthreads();
@@ -335,14 +346,14 @@ __global__ __launch_bounds__(kThreads) void nbx_spike_words_kernel(const float* 
                                                                    int32_t nw, uint32_t* __restrict__ bits) {
     const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (k >= (int64_t)B * nw) return;
-    const int32_t b = (int32_t)(k / nw), wi = (int32_t)(k % nw);
+    const int32_t wi = (int32_t)(k / B), b = (int32_t)(k % B);  // consecutive threads: consecutive trials
     uint32_t word = 0;
     if (s)
         for (int q = 0; q < 32; ++q) {
             const int32_t i = wi * 32 + q;
             if (i < n && s[(int64_t)i * B + b] > 0.0f) word |= 1u << q;
         }
-    bits[k] = word;
+    bits[(int64_t)b * nw + wi] = word;
 }
 
 int blocks_for(int64_t count) { return (int)((count + kThreads - 1) / kThreads); }
