@@ -111,3 +111,48 @@ def test_large_table_in_several_chunks(gpu, tmp_path):
     assert_bits_equal(a.raster(), b.raster(), "raster")
     assert_bits_equal(a.ring(), b.ring(), "ring")
     assert a.events() == b.events()
+
+
+def test_network_from_nbnet_metadata_matches_oracle(gpu, tmp_path):
+    # a network described by a .nbnet file alone: two layers (LIF voltage jump, conductance
+    # Izhikevich with NMDA synapses) with the reference's parameter names, tagged and plain values and
+    # initial-state offsets; populations, state and output mask come from the metadata
+    from neuronbridge_amd import nbnet_meta as M
+    from neuronbridge_amd.workloads import Workload
+    base = W.populations((L.LIF_VJ, L.IZH), (1600, 400), k_in=100, integrator=L.EULER)
+    tag = lambda kind, v: {M.MARKER: kind, "value": v}
+    meta = dict(META)
+    meta["layers"] = [
+        {"model": "TimeDrivenLIF_Voltage_jump", "count": 1600, "output": True, "parameters": {
+            "V_rest": -65.0, "tau": tag("float32", 20.0), "V_th": -50.0, "R": 1.0, "V_reset": -65.0,
+            "t_ref": tag("int", 20), "random_mu": tag("float32_list", [8.0, 0.0])}},
+        {"model": "TimeDrivenIzhikevic_Exponential_Decay", "count": 400, "monitored": True, "parameters": {
+            "a": 0.1, "b": 0.2, "c": -65.0, "d": 2.0, "V_rest": -65.0, "nmda_tau": 100.0,
+            "random_mu": [5.0, 0.0, 0.0, 0.0, 0.0, 0.0], "random_sigma": [0, 0, 0, 0, 0, 0]}}]
+    n = 2000
+    syn = base.syn
+    pre = np.repeat(np.arange(n, dtype=np.int64), np.diff(syn.row_ptr))
+    recs = records(pre, syn.post, syn.type, syn.weight, syn.delay)
+    path = str(tmp_path / "meta.nbnet")
+    write_nbnet(path, meta, recs[np.random.default_rng(3).permutation(len(recs))])
+    net, desc = M.load_network(path, dt=0.1, integrator=L.EULER, poisson=(base.pois_seed, 1.0, 0.1))
+    assert [ly.count for ly in desc.layers] == [1600, 400] and list(desc.monitored) == list(range(1600, 2000))
+    wl = Workload("from_metadata", desc.spec, syn=syn, pois_seed=base.pois_seed, pois_lambda=1.0, w_bg=0.1,
+                  x0=desc.x0, last0=desc.last0)
+    x, _ = net.get_state()
+    assert_bits_equal(x, desc.x0, "initial state from the metadata")
+    T = 300
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_GRAPH)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    bits = np.unpackbits(raster_o.view(np.uint8), axis=1, bitorder="little")[:, :n]
+    assert bits[:, :1600].sum() > 0 and bits[:, 1600:].sum() > 0, "a layer is silent"
+    compare(net, run, wl, counts_o, raster_o)
+    # the output mask of the layer marked output: one more per-phase step
+    net.begin_step(T)
+    net.run_propagation()
+    out, full = net.neuron_step_outputs()
+    assert set(out.tolist()) == {i for i in full.tolist() if i < 1600}
+    net.commit_firings()
+    net.close()
