@@ -320,6 +320,24 @@ int32_t nbx_net_upload_interface_values_device(nbx_net* net, const float* d_valu
 int32_t nbx_net_add_arrivals(nbx_net* net, const int32_t* targets, const uint8_t* types, const float* weights,
                              int32_t count, int32_t t);
 
+/* Current synapses (synapse type 3; reference: type-3 Interconnections registered as input-current
+ * slots of their target by the model's CheckType, TimeDrivenLIF_Voltage_jump.cpp:127-147, and
+ * CurrentSynapse, Current/src/CurrentSynapes.cpp; filled by PropogatedCurrent events,
+ * Event/src/Current/PropogatedCurrent.cpp:15-25). Declares `count` current synapses src[k] -> dst[k]
+ * with delay[k] (1..63) steps; each target's slots are numbered in declaration order and start at 0.
+ * A current injected at source s for step t (nbx_net_inject_currents; the reference's InputCurrent,
+ * Event/src/Current/InputCurrent.cpp:13-24) reaches each current synapse of s at step t + delay, sets
+ * that synapse's slot to the current (no weight is applied, as in the reference), and the target's
+ * I_ext becomes the float sum of its slots in slot order (TimeDrivenLIF_Voltage_jump.cpp:117-125).
+ * One step's arrivals are applied before its neuron update, in injection order (a later current for
+ * the same synapse wins); a per-step current table (nbx_net_set_current) still wins for the steps it
+ * covers. Runs stop before every step with arrivals. Call after nbx_net_finalize; a new call replaces
+ * the set and clears slots and pending currents; ResetState clears both. */
+int32_t nbx_net_set_current_synapses(nbx_net* net, const int32_t* src, const int32_t* dst, const uint8_t* delay,
+                                     int32_t count);
+/* Source neuron src[k] emits current[k] at step t (t >= the current step). */
+int32_t nbx_net_inject_currents(nbx_net* net, const int32_t* src, const float* current, int32_t count, int32_t t);
+
 /* Back to the state after nbx_net_finalize with the synapses kept (reference:
  * GpuPropagationRuntime::ResetState + DenseUnifiedNeuronRuntime::ResetState): neuron state and
  * LastSpike to their initial values, delay ring, firing history and event count cleared, learning

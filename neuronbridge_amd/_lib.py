@@ -200,6 +200,8 @@ SIGNATURES = {
     "nbx_nbnet_metadata": [C.c_char_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "nbx_net_load_nbnet": [_NET, C.c_char_p, C.c_int32],
     "nbx_net_export_monitor": [_NET, C.POINTER(MonitorDesc)],
+    "nbx_net_set_current_synapses": [_NET, _vp, _vp, _vp, C.c_int32],
+    "nbx_net_inject_currents": [_NET, _vp, _vp, C.c_int32, C.c_int32],
     "nbx_net_monitor_start": [_NET, C.POINTER(MonitorDesc)],
     "nbx_net_monitor_flush": [_NET],
     "nbx_net_monitor_stop": [_NET],

@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libnbx.so")
 ARCH = os.environ.get("NBX_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["nbx_core.hip", "nbx_step_kernels.hip", "nbx_persistent.hip", "nbx_build.hip", "nbx_batch.hip", "nbx_adjoint.hip",
-           "nbx_nbnet.hip", "nbx_monitor.hip", "nbx_replay_stdp.hip",
+           "nbx_nbnet.hip", "nbx_monitor.hip", "nbx_replay_stdp.hip", "nbx_currents.hip",
            "nbx_small.hip"]
 
 HIP_FLAGS = [

@@ -1037,17 +1037,25 @@ int32_t nbx_net_run_async(nbx_net* net, int32_t nsteps, int32_t engine) {
         set_error("persistent engine unsupported for this network: " + persistent_reason(net));
         return NBX_ERR_UNSUPPORTED;
     }
-    if (!net->mon.on) return run_engine(net, nsteps, engine);
-    // with the time-series monitor on, the run is cut after every sampled step and the step's frame
-    // is captured there (the same engine runs each piece; pieces compose like separate runs)
+    if (!net->mon.on && currents_due_next(net, net->t) < 0) return run_engine(net, nsteps, engine);
+    // The run is cut before every step with current-synapse arrivals (applied before it runs) and,
+    // with the time-series monitor on, after every sampled step (its frame is captured there); the
+    // same engine runs each piece, and pieces compose like separate runs.
     const int32_t iv = net->mon.interval;
     for (int32_t done = 0; done < nsteps;) {
         const int32_t t = net->t;
-        const int32_t s = ((t + iv - 1) / iv) * iv;  // first sampled step >= t
-        const int32_t len = std::min(nsteps - done, s - t + 1);
+        if (currents_due_next(net, t) == t) NBX_RC(apply_currents(net, t));
+        int32_t len = nsteps - done;
+        int32_t s = -1;
+        if (net->mon.on) {
+            s = ((t + iv - 1) / iv) * iv;  // first sampled step >= t
+            len = std::min(len, s - t + 1);
+        }
+        const int32_t due = currents_due_next(net, t + 1);
+        if (due >= 0) len = std::min(len, due - t);
         NBX_RC(run_engine(net, len, engine));
         done += len;
-        if (net->t - 1 == s) NBX_RC(monitor_capture(net, s));
+        if (net->mon.on && net->t - 1 == s) NBX_RC(monitor_capture(net, s));
     }
     return NBX_OK;
 }
@@ -1096,6 +1104,10 @@ int32_t nbx_net_begin_step(nbx_net* net, int32_t t) {
     if (t != net->t) {
         set_error("BeginStep time " + std::to_string(t) + " != network time " + std::to_string(net->t));
         return NBX_ERR_STATE;
+    }
+    if (currents_due_next(net, t) == t) {  // current-synapse arrivals of this step, before its update
+        NBX_RC(device_scope(net));
+        NBX_RC(apply_currents(net, t));
     }
     net->phase = 1;
     return NBX_OK;
@@ -1524,6 +1536,7 @@ int32_t nbx_net_reset_state(nbx_net* net) {
     }
     NBX_HIP_TRY(hipStreamSynchronize(st));
     NBX_RC(upload_learning_records(net, false));
+    clear_currents(net);  // current-synapse slots and pending currents
     net->t = 0;
     net->phase = 0;
     return NBX_OK;

@@ -93,6 +93,25 @@ struct NetMonitor {
     int64_t frames_written = 0;
 };
 
+// Current synapses (synapse type 3): per source the synapses it drives, per target its slots in
+// declaration order; currents injected at a source wait on the host, keyed by arrival step, until
+// that step is run.
+struct CurrentSynapses {
+    int32_t count = 0;
+    std::vector<int64_t> out_ptr;   // [n + 1] synapses of each source
+    std::vector<int32_t> out_syn;   // synapse ids by source
+    std::vector<int32_t> delay;     // [count]
+    std::vector<int32_t> target;    // [count]
+    std::vector<int32_t> slot;      // [count] global slot (target-major, declaration order)
+    int32_t* d_in_ptr = nullptr;    // [n + 1] slots of each target (int32: count < 2^31)
+    float* d_slots = nullptr;       // [count] the slot currents
+    std::vector<std::vector<std::pair<int32_t, float>>> pending;  // arrival step - base: (synapse, value)
+    int32_t base = 0;               // step of pending[0]
+    int32_t* d_stage = nullptr;     // staging of one step's (slot, value-bits) pairs and targets
+    int64_t stage_cap = 0;
+    std::vector<void*> allocs;
+};
+
 // One neuron population: neurons [begin, begin + md.n) with one model and its parameters.
 struct NetPop {
     int32_t begin = 0;
@@ -225,6 +244,8 @@ struct nbx_net {
     ReplayStdpWork rsw;
     // time-series monitor
     NetMonitor mon;
+    // current synapses (type 3)
+    CurrentSynapses cs;
 
     StepArgs args() const {
         StepArgs A{};
@@ -334,6 +355,8 @@ struct nbx_net {
         for (void* p : mon.allocs) (void)hipFree(p);
         mon.allocs.clear();
         mon.on = false;
+        for (void* p : cs.allocs) (void)hipFree(p);
+        cs.allocs.clear();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -348,6 +371,9 @@ int32_t refresh_persistent_weights(nbx_net* net);
 int32_t run_persistent(nbx_net* net, int32_t nsteps);
 std::vector<float*> state_columns(nbx_net* net, int32_t model);  // device array of each state column
 int32_t monitor_capture(nbx_net* net, int32_t step);  // frame of the completed step (flushes when full)
+int32_t currents_due_next(const nbx_net* net, int32_t t);  // first step >= t with current arrivals, or -1
+int32_t apply_currents(nbx_net* net, int32_t t);            // arrivals of step t (before its neuron update)
+void clear_currents(nbx_net* net);                          // slots and pending currents (ResetState)
 std::string small_dense_reason(const nbx_net* net);  // empty when the single-workgroup engine fits
 int32_t run_small_dense(nbx_net* net, int32_t nsteps);
 int32_t launch_replay(nbx_net* net, const int32_t* d_ids, const int32_t* d_tick_ptr, int32_t nsteps, int32_t* d_out);

@@ -374,6 +374,23 @@ class Net:
         L.call("nbx_net_add_arrivals", self.h, _p(tg, C.c_int32), _p(ty, C.c_uint8), _p(w, C.c_float),
                int(tg.shape[0]), int(t))
 
+    def set_current_synapses(self, src, dst, delay) -> None:
+        """Type-3 current synapses src[k] -> dst[k] after delay[k] steps (slots in this order)."""
+        self._cs = [np.ascontiguousarray(src, dtype=np.int32), np.ascontiguousarray(dst, dtype=np.int32),
+                    np.ascontiguousarray(delay, dtype=np.uint8)]
+        a, b, d = self._cs
+        assert a.size == b.size == d.size
+        L.call("nbx_net_set_current_synapses", self.h, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+               d.ctypes.data_as(C.c_void_p), int(a.size))
+
+    def inject_currents(self, src, current, t: int) -> None:
+        """Source neurons src[k] emit current[k] at step t (InputCurrent)."""
+        a = np.ascontiguousarray(src, dtype=np.int32)
+        c = np.ascontiguousarray(current, dtype=np.float32)
+        assert a.size == c.size
+        L.call("nbx_net_inject_currents", self.h, a.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+               int(a.size), int(t))
+
     def reset_state(self) -> None:
         L.call("nbx_net_reset_state", self.h)
 
