@@ -34,10 +34,11 @@ struct PersistentLayout {
     int64_t max_row = 0;             // longest presynaptic row
     int32_t* d_off = nullptr;        // [n][nr * gs]
     uint2* d_seg = nullptr;          // {post_local | delay << 16 (6 bits) | channel << 22 (2 bits) | (block & 127) << 25, q}
-    unsigned long long* d_glist = nullptr;  // [D][8][gcap] the steps' spike lists {tag << 32 | id}, one per XCD group
-    unsigned long long* d_gctr = nullptr;   // [D][8][16] their counters {spikes << 32 | arrivals} (one line each)
+    unsigned long long* d_glist = nullptr;  // [D][kGroups][gcap] the steps' spike lists {tag << 32 | id}
+    unsigned long long* d_gctr = nullptr;   // [2 banks][D][kGroups][16] their counters {spikes << 32 | arrivals}
     int32_t gcap = 0;                       // list capacity: the neurons of one group
-    int32_t t_next = -1;                    // step at which the last persistent launch ended (counters clean)
+    int32_t t_next = -1;                    // step at which the last persistent launch ended (-1: none yet)
+    int32_t bank = 0;                       // step-counter bank of the next launch (the other is cleared by it)
     uint32_t epoch = 0;                     // the next launch tags its steps epoch + 1, epoch + 2, ...
     bool launch_checked = false;     // LDS attribute set and co-residency of the B workgroups checked
 };

@@ -3,17 +3,18 @@
 // One cooperative launch runs many steps. Workgroup b owns the neurons
 // [b*P, b*P + P): their state stays in VGPRs and their delay ring
 // ring[D][channel][P] stays in LDS for the whole launch. Per step t a workgroup
-//   1. delivers the spikes of step t-1 that target its own neurons into its LDS
-//      ring (int32 fixed point, ds_add, so the sum is order independent),
-//   2. updates its neurons for step t (same neuron_tick as the per-phase engine)
-//      and compacts its spikes (wavefront ballot + LDS prefix) into the shared
-//      firing-history ring in HBM,
-//   3. joins one grid barrier, after which every workgroup sees all spikes of t.
-// One barrier per step replaces the reference's separate clear/mark/drain/update
-// launches and host round trips (GpuPropagationRuntime.cu:1687-1827,
-// DenseUnifiedNeuronRuntime.cu:284-372). Workgroups are numbered so that
-// consecutive neuron blocks share an XCD and its L2: a presynaptic row's segments
-// for neighbouring blocks are adjacent in memory.
+//   1. waits for the step counters of step t-1 and delivers its spikes that target
+//      its own neurons into its LDS ring (int32 fixed point, ds_add, so the sum is
+//      order independent),
+//   2. updates its neurons for step t (same neuron_tick as the per-phase engine),
+//      compacts its spikes (wavefront ballot + LDS prefix) and publishes them with
+//      one atomic on its group's counter and tagged entries in its group's list,
+//   3. draws the next step's Poisson counts while the others publish.
+// No grid barrier: a workgroup waits only for the counters of the step it reads.
+// This replaces the reference's separate clear/mark/drain/update launches and host
+// round trips (GpuPropagationRuntime.cu:1687-1827, DenseUnifiedNeuronRuntime.cu:284-372).
+// Workgroups are numbered so that consecutive neuron blocks share an XCD and its L2:
+// a presynaptic row's segments for neighbouring blocks are adjacent in memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,24 +72,27 @@ __device__ inline int32_t block_of(int32_t w, int32_t B) {
     return (w % kXcds) * per + (w / kXcds);
 }
 
-// Step synchronisation by counters. Workgroup w belongs to group w % 8 (its XCD); per step and
-// group one 64-bit counter {spikes << 32 | arrivals} sits on its own 128-byte line. Publishing
-// step t, a workgroup adds {its spikes << 32 | 1} to its group's counter of slot t % D with one
-// returning atomic: the old value's high half is where its spike ids go in the group's list, and
-// its arrival is counted. A workgroup needs step t when the 8 counters of the slot all show their
-// group's workgroup count; it polls those 8 words (8 lanes, one load each per round) and then has
-// the step's per-group spike counts too. Few workgroups share one counter (atomics on one address
-// serialise) and nobody polls a line that every workgroup polls more than once per round.
+// Step synchronisation by counters. Workgroup w belongs to group w % kGroups; per ring slot and
+// group one 64-bit counter {spikes << 32 | arrivals} sits on its own 128-byte line. Publishing step
+// t, a workgroup adds {its spikes << 32 | 1} to its group's counter of slot t % D with one returning
+// atomic. Counters are cumulative over a launch and never cleared while it runs: the k-th use of a
+// slot (k = (t - t0) / D + 1) is complete when the group's arrivals reach k times the group's
+// workgroup count, and a step's spike count per group is the difference between the spike totals
+// after this use and after the previous one, which every workgroup keeps in LDS (it reads every
+// step's counters). The atomic's old spike total minus that previous total is where the
+// workgroup's spike ids go in its group's list. A workgroup needs step t when the kGroups counters
+// of the slot all show their use complete; kGroups lanes poll one counter each per round, so few
+// workgroups share one atomic address and nobody polls a line that every workgroup polls twice.
 //
-// Memory ordering: a workgroup's list entries are stored after its atomic, so a reader may see
-// the arrival first; entries are 64-bit {step tag << 32 | id}, written and read with device-scope
-// atomic stores and loads (single-copy atomic), and a reader re-reads an entry until it carries
-// the step's tag. Tags are unique over the network's lifetime (the host advances the epoch by
-// nsteps + 1 per launch). The counters of slot (t + 1) % D are cleared by workgroup 0 in step t,
-// before its own atomic for step t (it waits for the clearing stores), so every atomic for step
-// t + 1 (made after seeing workgroup 0's arrival for step t) finds them cleared; nobody reads
-// them any more (their last readers finished before step t - 1 ended). Spins are bounded: on
-// time-out the error flag's kErrBarrier bit is raised and every waiter that sees it leaves.
+// Memory ordering: every exchange is a single-location atomic (the counters: returning
+// fetch-adds, polled with relaxed loads; each location's modification order alone makes the
+// reader's view consistent) or a self-validating list entry: entries are 64-bit {step tag << 32 |
+// id}, written and read with device-scope atomic stores and loads (single-copy atomic), and a reader
+// re-reads an entry until it carries the step's tag (tags are unique over the network's lifetime:
+// the host advances the epoch by nsteps + 1 per launch). No ordering between different locations is
+// needed. Counters come in two banks, one per launch parity: a launch counts in its bank and clears
+// the other, which the next launch uses after the kernel boundary. Spins are bounded: on time-out
+// the error flag's kErrBarrier bit is raised and every waiter that sees it leaves.
 constexpr int kCtrLine = 16;  // uint64 per 128-byte line
 constexpr int kGroups = 16;   // counter groups: workgroup w belongs to group w % kGroups
 
@@ -99,8 +103,9 @@ struct PersistArgs {
     int32_t B, P;
     int32_t rb, gs, os;  // blocks per region, ints per region group, ints per offset row
     int32_t t0, nsteps;
-    unsigned long long* gctr;   // [D][8][kCtrLine] group counters {spikes << 32 | arrivals}
-    unsigned long long* glist;  // [D][8][gcap] group spike lists {tag << 32 | id}
+    unsigned long long* gctr;   // [D][kGroups][kCtrLine] group counters {spikes << 32 | arrivals}, this launch's bank
+    unsigned long long* gctr_next;  // the other bank, cleared by this launch for the next one
+    unsigned long long* glist;  // [D][kGroups][gcap] group spike lists {tag << 32 | id}
     int32_t gcap;               // list capacity: the neurons of one group
     uint32_t epoch;             // tag of step t0 - 1 (step t0 + k carries epoch + 1 + k)
 };
@@ -144,18 +149,22 @@ __device__ inline bool await_tags(const unsigned long long* const (&p)[N], uint3
     return true;
 }
 
-// Wave 0 of a workgroup: waits until all groups' workgroups have published the step of slot sl
-// and leaves the groups' list starts (step numbering group-major) in s_gstart[0..8], the step's
-// spike count in s_gstart[8]. False if the wait was abandoned.
-__device__ inline bool await_step(const PersistArgs& X, int32_t sl, int32_t lane, int32_t* s_gstart, int32_t* err) {
+// Wave 0 of a workgroup: waits until all groups' workgroups have published step st (ring slot sl),
+// records the groups' spike totals after it in prev[sl] (the totals after the slot's previous use
+// move to prev_old[sl]) and leaves the groups' list starts (step numbering group-major) in
+// s_gstart[0..kGroups), the step's spike count in s_gstart[kGroups]. False if the wait was
+// abandoned.
+__device__ inline bool await_step(const PersistArgs& X, int32_t st, int32_t sl, int32_t lane, int32_t* s_gstart,
+                                  int32_t* prev, int32_t* prev_old, int32_t* err) {
     const int32_t per = X.B / kGroups + ((lane < X.B % kGroups) ? 1 : 0);  // workgroups w with w % kGroups == lane
+    const uint32_t want = (uint32_t)((st - X.t0) / X.A.D + 1) * (uint32_t)per;  // arrivals after this use
     unsigned long long c = 0;
     bool ok = true;
     if (lane < kGroups) {
         const unsigned long long* p = X.gctr + ((int64_t)sl * kGroups + lane) * kCtrLine;
         c = ld_dev64(p);
         uint32_t spins = 0;
-        while ((int32_t)(uint32_t)c != per) {
+        while ((uint32_t)c != want) {
             if ((spins & 63u) == 0 && (ld_dev(err) & kErrBarrier) != 0) {
                 ok = false;
                 break;
@@ -169,8 +178,14 @@ __device__ inline bool await_step(const PersistArgs& X, int32_t sl, int32_t lane
             c = ld_dev64(p);
         }
     }
-    const int32_t gc = (lane < kGroups) ? (int32_t)(c >> 32) : 0;
-    int32_t gp = gc;  // inclusive prefix over lanes 0..7
+    int32_t gc = 0;
+    if (lane < kGroups) {
+        const int32_t total = (int32_t)(c >> 32), before = prev[sl * kGroups + lane];
+        gc = total - before;
+        prev_old[sl * kGroups + lane] = before;
+        prev[sl * kGroups + lane] = total;
+    }
+    int32_t gp = gc;  // inclusive prefix over the kGroups lanes
 #pragma unroll
     for (int o = 1; o < kGroups; o <<= 1) {
         const int32_t z = __shfl_up(gp, o, 64);
@@ -243,6 +258,15 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         s_ev = 0;
         s_abort = 0;
     }
+    // spike totals of each (slot, group) counter after its last use read by this workgroup, and
+    // after the use before that; this launch's bank starts at zero
+    int32_t* s_prev = s_ring + D * nch * P;
+    int32_t* s_prev_old = s_prev + D * kGroups;
+    for (int32_t j = tid; j < 2 * D * kGroups; j += T) s_prev[j] = 0;
+    // the next launch's bank: nobody reads it during this launch; the kernel boundary orders these
+    // stores before the next launch's atomics
+    if (b == 0)
+        for (int32_t j = tid; j < D * kGroups; j += T) X.gctr_next[(int64_t)j * kCtrLine] = 0ull;
     float V[NPT], S1[NPT], GA[NPT], GG[NPT], I[NPT];
     int32_t last[NPT];
 #pragma unroll
@@ -284,7 +308,8 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             // 1. the spikes of step t-1 (published by every workgroup once the slot's counters are full)
             const int32_t ps = (t - 1) % D;
             const uint32_t want = X.epoch + (uint32_t)k;
-            if (wave == 0 && !await_step(X, ps, lane, s_gstart, A.err) && lane == 0) s_abort = 1;
+            if (wave == 0 && !await_step(X, t - 1, ps, lane, s_gstart, s_prev, s_prev_old, A.err) && lane == 0)
+                s_abort = 1;
             __syncthreads();
             if (s_abort) {  // uniform: read after the barrier, set before it
                 aborted = true;
@@ -359,8 +384,6 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
         const int32_t slot = sb;
         const bool rec = A.raster && t >= A.rec_t0 && t < A.rec_t0 + A.rec_n;
         const uint32_t tag = X.epoch + 1u + (uint32_t)k;
-        if (b == 0 && tid < kGroups)  // the next step's counters (see above)
-            st_dev64(X.gctr + ((int64_t)((t + 1) % D) * kGroups + tid) * kCtrLine, 0ull);
         uint64_t msk[NPT];
 #pragma unroll
         for (int u = 0; u < NPT; ++u) {
@@ -391,9 +414,6 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 if (pw + 32 < np) row[((p0 + pw) >> 5) + 1] = (uint32_t)(msk[u] >> 32);
             }
         }
-        // workgroup 0 waits for its clearing stores before its arrival is counted (the only data
-        // ordered by the counters; the list entries carry their tag instead)
-        if (b == 0) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         {
             // this workgroup's spikes, numbered by (neuron chunk, wave, lane), go into its group's
@@ -411,7 +431,7 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
                 const unsigned long long old = __hip_atomic_fetch_add(
                     X.gctr + ((int64_t)slot * kGroups + xg) * kCtrLine, ((unsigned long long)(uint32_t)tot << 32) | 1ull,
                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_base = (int32_t)(old >> 32);
+                s_base = (int32_t)(old >> 32) - s_prev[slot * kGroups + xg];  // within this use of the slot
             }
             __syncthreads();
             unsigned long long* lst = X.glist + ((int64_t)slot * kGroups + xg) * X.gcap + s_base;
@@ -457,7 +477,17 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             const int32_t s = X.t0 + X.nsteps - 1 - j, sl = s % D;
             const uint32_t tag = X.epoch + (uint32_t)(X.nsteps - j);
             __syncthreads();  // s_gstart of the previous step is read
-            if (wave == 0) await_step(X, sl, lane, s_gstart, A.err);
+            if (wave == 0) {  // the step's list starts from the totals its read left (s was the slot's last use)
+                const int32_t gc = (lane < kGroups) ? s_prev[sl * kGroups + lane] - s_prev_old[sl * kGroups + lane] : 0;
+                int32_t gp = gc;
+#pragma unroll
+                for (int o = 1; o < kGroups; o <<= 1) {
+                    const int32_t z = __shfl_up(gp, o, 64);
+                    if (lane >= o) gp += z;
+                }
+                if (lane < kGroups) s_gstart[lane] = gp - gc;
+                if (lane == kGroups - 1) s_gstart[kGroups] = gp;
+            }
             __syncthreads();
             const int32_t cnt = s_gstart[kGroups];
             int32_t gs[kGroups];
@@ -1308,7 +1338,8 @@ Geometry geometry(const nbx_net* net) {
     g.npt = (P + threads - 1) / threads;
     if (g.npt == 3) g.npt = 4;
     g.threads = threads;
-    g.lds = net->D * net->nch * P * (int32_t)sizeof(int32_t);
+    // the delay ring slice, then two spike-total tables of the step counters ([D][kGroups] each)
+    g.lds = (net->D * net->nch * P + 2 * net->D * kGroups) * (int32_t)sizeof(int32_t);
     if (g.npt > kMaxNpt) g.why = "more than 4 neurons per thread";
     else if (g.lds > max_lds - 4096) g.why = "delay ring slice does not fit in LDS";
     if (P > 65535) g.why = "block too large for 16-bit local indices";
@@ -1405,7 +1436,7 @@ int32_t build_persistent_layout(nbx_net* net) {
     L.gcap = (B + kGroups - 1) / kGroups * P;  // the neurons of one group
     rc = net->alloc(&L.d_glist, (size_t)net->D * kGroups * L.gcap);
     if (rc) return rc;
-    rc = net->alloc(&L.d_gctr, (size_t)net->D * kGroups * kCtrLine);
+    rc = net->alloc(&L.d_gctr, (size_t)2 * net->D * kGroups * kCtrLine);  // two banks
     if (rc) return rc;
     L.t_next = -1;  // the counters are cleared before the first launch
     L.epoch = 0;
@@ -1621,19 +1652,20 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     X.os = L.nr * L.gs;
     X.t0 = net->t;
     X.nsteps = nsteps;
-    // the first step's counters must be zero: each launch clears the slot of the step after its
-    // last, so this holds when the previous persistent launch ended at this step; list entries
-    // carry unique tags, so only a tag wrap needs them cleared
+    // this launch's counter bank must be zero: each launch clears the other bank for the next one,
+    // so it holds after a persistent launch; the first launch clears both. List entries carry
+    // unique tags, so only a tag wrap needs them cleared
     const bool wrap = (uint64_t)L.epoch + (uint64_t)nsteps + 2u >= ((uint64_t)1 << 32);
-    if (net->t != L.t_next || wrap)
-        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, (int64_t)2 * net->D * kGroups * kCtrLine,
-                                    net->stream));
+    const int64_t bank_words = (int64_t)net->D * kGroups * kCtrLine;  // uint64 per bank
+    if (L.t_next < 0 || wrap)
+        NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, 2 * 2 * bank_words, net->stream));
     if (wrap) {
         NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_glist), 0, (int64_t)2 * net->D * kGroups * L.gcap,
                                     net->stream));
         L.epoch = 0;
     }
-    X.gctr = L.d_gctr;
+    X.gctr = L.d_gctr + L.bank * bank_words;
+    X.gctr_next = L.d_gctr + (1 - L.bank) * bank_words;
     X.glist = L.d_glist;
     X.gcap = L.gcap;
     X.epoch = L.epoch;
@@ -1643,6 +1675,7 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     NBX_HIP_TRY(launch_coresident(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
     net->t += nsteps;
     L.t_next = net->t;
+    L.bank = 1 - L.bank;
     L.epoch += (uint32_t)nsteps + 1u;  // the launch used tags epoch + 1 .. epoch + nsteps
     return NBX_OK;
 }
