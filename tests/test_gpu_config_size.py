@@ -100,3 +100,48 @@ def test_batched_full_size_vs_plain_and_oracle(gpu):
         assert np.array_equal(b.raster(int(k)), raster), f"instance {k} (n={ins.n}) differs from the oracle"
         v, _ = b.state(int(k))
         assert_bits_equal(v, run.x[:, 0], f"instance {k} V vs oracle")
+
+
+def test_fixed_point_steps_against_float_event_delivery_full_size(gpu):
+    # the sparse LIF network at full size: at several points of a GPU run the oracle takes the GPU's
+    # state and firing history and runs one step with the reference's per-event float delivery;
+    # the GPU's own step (int32 fixed-point arrivals) agrees to 1e-5 relative (threshold decisions
+    # within rounding distance of V_th may differ and are counted, not compared)
+    import oracle as O
+    wl = W.sparse_lif(steps=400)
+    net = wl.make_net()
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    D, n = wl.d_slots, wl.spec.n
+    t, worst, flips, compared = 0, 0.0, 0, 0
+    for target in (5, 40, 120, 250, 390):
+        net.run(target - t, L.ENGINE_PERSISTENT)
+        t = target
+        x, last = net.get_state()
+        run.x[:] = x
+        run.last[:] = last
+        run.t = t
+        run.hist[:] = 0
+        run.hist_cnt[:] = 0
+        for s in range(max(0, t - D + 1), t):
+            ids = np.sort(net.history(s))
+            run.hist[s % D, :ids.size] = ids
+            run.hist_cnt[s % D] = ids.size
+        _, ras = run.run(1, O.ACC_FLOAT_EVENTS, ring=False, raster=True)
+        net.record(t, 1, counts=True, raster=True)
+        net.run(1, L.ENGINE_GRAPH)
+        t += 1
+        xg, lastg = net.get_state()
+        fo = np.unpackbits(ras[0].view(np.uint8), bitorder="little")[:n].astype(bool)
+        fg = np.unpackbits(net.raster()[0].view(np.uint8), bitorder="little")[:n].astype(bool)
+        same = fo == fg
+        flips += int((~same).sum())
+        rel = np.abs(xg[same, 0] - run.x[same, 0]) / np.maximum(np.abs(run.x[same, 0]), 1.0)
+        worst = max(worst, float(rel.max()))
+        compared += int(same.sum())
+        assert fg.sum() > 10
+        assert np.array_equal(lastg[same], run.last[same])
+    print(f"fixed point vs float events at full size: worst relative step difference {worst:.3g}, "
+          f"{flips} threshold decisions of {5 * n} differ")
+    assert worst <= 1e-5
+    assert flips <= 5
+    net.close()
