@@ -341,15 +341,20 @@ __device__ inline void pull_items(const StepArgs& A, int32_t t, const int32_t* s
     }
 }
 
+// s_pref[d] = spikes of the steps t - 1 .. t - d + 1 (d = 1 .. D - 1), s_pref[D] = all of them: lane d
+// of wave 0 loads the count of step t - d (all loads in flight together), then a wave prefix sum
 __device__ inline void pull_prefix(const StepArgs& A, int32_t t, int32_t* s_pref) {
-    if (threadIdx.x == 0) {
-        int32_t tot = 0;
-        s_pref[0] = 0;
-        for (int d = 1; d < A.D; ++d) {
-            s_pref[d] = tot;
-            if (t - d >= 0) tot += A.hist_cnt[(t - d) % A.D];
+    if (threadIdx.x < 64) {
+        const int32_t d = (int32_t)threadIdx.x;
+        const int32_t c = (d >= 1 && d < A.D && t - d >= 0) ? A.hist_cnt[(t - d) % A.D] : 0;
+        int32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t z = __shfl_up(incl, o, 64);
+            if (d >= o) incl += z;
         }
-        s_pref[A.D] = tot;
+        if (d < A.D) s_pref[d] = incl - c;
+        if (d == A.D - 1) s_pref[A.D] = incl;
     }
     __syncthreads();
 }
