@@ -114,6 +114,28 @@ def test_engines_interoperate(gpu):
     assert_history(net, raster_o, wl.spec.n)
 
 
+@pytest.mark.parametrize("integrator", [L.EULER, L.RK4], ids=["euler", "rk4"])
+def test_persistent_burst_steps_bitexact(gpu, integrator):
+    # every neuron starts just under threshold, so the first steps fire thousands of neurons at
+    # once: more spikes than one workgroup's step record holds and than the engine keeps in LDS
+    wl = W.sparse_lif(n=20_000, k_in=100, integrator=integrator, steps=60, seed=17)
+    wl.x0[:, 0] = np.linspace(-50.02, -50.0, wl.spec.n, endpoint=False, dtype=np.float32)
+    net = wl.make_net()
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(60)
+    assert counts_o[:2].min() > 2048, "no burst step"
+    net.record(0, 60, counts=True, raster=True)
+    net.run(6, L.ENGINE_PERSISTENT)
+    bits = np.unpackbits(raster_o.view(np.uint8), axis=1, bitorder="little")[:, :wl.spec.n]
+    for t in range(6):  # the first launch's firing history holds the burst steps
+        assert_bits_equal(np.sort(net.history(t)), np.nonzero(bits[t])[0].astype(np.int32), f"history of step {t}")
+    net.run(31, L.ENGINE_PERSISTENT)
+    net.run(23, L.ENGINE_PERSISTENT)
+    compare(net, run, wl, counts_o, raster_o)
+    assert_history(net, raster_o, wl.spec.n)
+    net.close()
+
+
 def assert_history(net, raster_o, n):
     """Every step the firing-history ring still holds lists exactly the raster's spikes."""
     slots = net.ring().shape[0]
