@@ -100,14 +100,30 @@ __global__ __launch_bounds__(kThreads) void nbx_readout_bwd_kernel(int32_t T, in
 // One step of the recurrence with its product on the matrix cores: out[i][b] = sum_j M[i][j] X[j][b]
 // (M = W_rec forward, W_rec^T backward; X = the previous step's spikes / the next step's dI, both
 // [n][batch]) in FP32 MFMA 16x16x4 tiles, then the step's elementwise update on the tile.
-// Workgroup = one 16 x 16 output tile, eight waves splitting the reduction over j in 64-wide
-// chunks (double-buffered loads), partial tiles summed in a fixed order through LDS (so
-// recomputation is bit-identical).
-// Within a chunk, lane (r, kb) holds M[i0 + r][k0 + 16 kb .. k0 + 16 kb + 16) and lane (c, kb)
-// holds X[k0 + 16 kb .. + 16][b0 + c]; MFMA s (s = 0..15) reduces k = k0 + 16 kb + s over the four
-// lane groups kb. Tiles of one row block are placed on one XCD so its M rows are read into one L2.
+// Workgroup = 16 rows x 32 trials (two MFMA tiles); the reduction runs in chunks of kMmK: each
+// chunk's M block (16 x kMmK) and X block (kMmK x 32) are loaded with coalesced 16-byte loads into
+// registers while the previous chunk is on the matrix cores, then stored to LDS, from which the
+// eight waves each reduce their own kMmK / 8 slice of the chunk. Every M row is read once per
+// trial tile and every X row once per row block, with one load round trip per chunk in flight.
+// The eight partial tiles are summed in a fixed order through LDS (so recomputation is
+// bit-identical). MFMA step s of wave w reduces k = k0 + w * kMmKw + 16 kb + s over the four lane
+// groups kb: lane (r, kb) reads M[i0 + r][that k] (16 consecutive floats per chunk: four LDS
+// 16-byte reads) and lane (c, kb) reads X[that k][b0 + 16 ct + c]. Row blocks are placed on one XCD
+// with their two trial tiles so each M row is read into one L2.
 constexpr int kRecWaves = 8;
 constexpr int kRecThreads = 64 * kRecWaves;
+constexpr int kMmRows = 16;                  // output rows per workgroup
+constexpr int kMmCols = 32;                  // output trials per workgroup
+constexpr int kMmK = 512;                    // reduction chunk
+constexpr int kMmKw = kMmK / kRecWaves;      // per wave and chunk: 16 MFMA steps x 4
+constexpr int kMmAS = kMmK + 4;              // LDS row stride of the M block (16-byte aligned rows)
+constexpr int kMmXS = kMmCols + 1;           // LDS row stride of the X block (odd: lane groups kb 16 rows
+                                             // apart fall on different banks)
+constexpr int kMmA4 = kMmRows * kMmK / 4 / kRecThreads;  // 16-byte M loads per thread and chunk
+constexpr int kMmX4 = kMmK * kMmCols / 4 / kRecThreads;  // 16-byte X loads per thread and chunk
+constexpr int kMmLds = (kMmRows * kMmAS + kMmK * kMmXS) * (int)sizeof(float);
+static_assert(kMmRows * kMmCols == kRecThreads, "one output per thread in the epilogue");
+static_assert(kRecWaves * kMmRows * kMmCols * (int)sizeof(float) <= kMmLds, "the partial tiles reuse the chunk LDS");
 
 struct RecArgs {
     nbx_lif_adj_params p;
@@ -118,81 +134,132 @@ struct RecArgs {
     float* V;         // forward: V (in/out); backward: dV (in/out)
     float* out;       // forward: s_t; backward: dI_t
     float* Vm_out;    // forward: Vm_t or null
-    int32_t tiles_c;
+    int32_t tiles_c;  // trial tiles: ceil(batch / kMmCols)
 };
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ inline void load16(const float* __restrict__ p, int32_t valid, float (&v)[16]) {
-    if (valid >= 16 && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
-        const float4* q = reinterpret_cast<const float4*>(p);
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const float4 x = q[h];
-            v[4 * h] = x.x;
-            v[4 * h + 1] = x.y;
-            v[4 * h + 2] = x.z;
-            v[4 * h + 3] = x.w;
-        }
-    } else {
-#pragma unroll
-        for (int s = 0; s < 16; ++s) v[s] = s < valid ? p[s] : 0.0f;
+// Operand loads are raw buffer loads: an offset at or past the resource's byte count (kMmOff for
+// an element outside the matrix) reads zeros without a memory request, so the loads need no
+// branches and all of a chunk's loads stay in flight together.
+typedef uint32_t mm_u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kMmOff = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t mm_rsrc(const float* base, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0,
+                                             (int32_t)(bytes < 0x7FFFFFFF ? bytes : 0x7FFFFFFF), 0x00020000);
+}
+// Four consecutive floats at element e of the resource: one 16-byte load when the rows are 16-byte
+// aligned (AL), else four 4-byte loads; an element at or past `lim` (the end of its row) reads 0.
+template <bool AL>
+__device__ inline float4 mm_load4(__amdgpu_buffer_rsrc_t r, uint32_t e, int32_t first, int32_t lim) {
+    if (AL) {
+        const mm_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, first + 3 < lim ? 4u * e : kMmOff, 0, 0);
+        return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
     }
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        v[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, first + q < lim ? 4u * (e + q) : kMmOff, 0, 0));
+    return make_float4(v[0], v[1], v[2], v[3]);
 }
 
-template <bool BWD>
+template <bool BWD, bool AL>
 __global__ __launch_bounds__(kRecThreads) void nbx_lif_rec_kernel(RecArgs a) {
-    __shared__ float red[kRecWaves][256];
+    extern __shared__ float s_mm[];
+    float* sA = s_mm;                      // [kMmRows][kMmAS]
+    float* sX = s_mm + kMmRows * kMmAS;    // [kMmK][kMmXS]
     const nbx_lif_adj_params& p = a.p;
     const int32_t n = p.n, B = p.batch;
     const int32_t G = gridDim.x;
     const int32_t lin = (G % 8 == 0) ? (int32_t)(blockIdx.x % 8) * (G / 8) + (int32_t)blockIdx.x / 8
                                      : (int32_t)blockIdx.x;
-    const int32_t i0 = (lin / a.tiles_c) * 16, b0 = (lin % a.tiles_c) * 16;
+    const int32_t i0 = (lin / a.tiles_c) * kMmRows, b0 = (lin % a.tiles_c) * kMmCols;
     const int32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int32_t r = lane & 15, kb = lane >> 4;
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    f4 acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
     if (a.X) {
-        // reduction range per wave, in chunks of 64; the next chunk's operands are loaded while
-        // the current one runs through the matrix cores
-        const int32_t kw = (n + kRecWaves * 64 - 1) / (kRecWaves * 64) * 64;
-        const int32_t kbeg = wave * kw, kend = min(n, kbeg + kw);
-        const bool row_ok = i0 + r < n, col_ok = b0 + r < B;
-        const float* mrow = a.M + (int64_t)(row_ok ? i0 + r : 0) * n;
-        auto load = [&](int32_t k0, float (&av)[16], float (&xv)[16]) {
-            const int32_t kk = k0 + 16 * kb;
-            const int32_t valid = min(16, max(0, kend - kk));
-            load16(mrow + kk, row_ok ? valid : 0, av);
+        // this workgroup's rows of M, and (per chunk) the chunk's rows of X: rows past n read zeros
+        const __amdgpu_buffer_rsrc_t ma = mm_rsrc(a.M + (int64_t)i0 * n, (int64_t)min(kMmRows, n - i0) * n * 4);
+        float4 ra[kMmA4], rx[kMmX4];
+        auto fetch = [&](int32_t k0) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) xv[s] = (col_ok && s < valid) ? a.X[(int64_t)(kk + s) * B + b0 + r] : 0.0f;
+            for (int h = 0; h < kMmA4; ++h) {
+                const int32_t idx = tid + h * kRecThreads;
+                const int32_t row = idx / (kMmK / 4), k = k0 + 4 * (idx % (kMmK / 4));
+                ra[h] = mm_load4<AL>(ma, (uint32_t)(row * n + k), k, n);
+            }
+            const __amdgpu_buffer_rsrc_t xa = mm_rsrc(a.X + (int64_t)k0 * B, (int64_t)min(kMmK, n - k0) * B * 4);
+#pragma unroll
+            for (int h = 0; h < kMmX4; ++h) {
+                const int32_t idx = tid + h * kRecThreads;
+                const int32_t kr = idx / (kMmCols / 4), c = b0 + 4 * (idx % (kMmCols / 4));
+                rx[h] = mm_load4<AL>(xa, (uint32_t)(kr * B + c), c, B);
+            }
         };
-        float a0[16], x0[16], a1[16], x1[16];
-        int32_t k0 = kbeg;
-        if (k0 < kend) load(k0, a0, x0);
-        while (k0 < kend) {
-            const int32_t k1 = k0 + 64;
-            if (k1 < kend) load(k1, a1, x1);
+        auto stage = [&]() {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[s], x0[s], acc, 0, 0, 0);
-            if (k1 >= kend) break;
-            const int32_t k2 = k1 + 64;
-            if (k2 < kend) load(k2, a0, x0);
+            for (int h = 0; h < kMmA4; ++h) {
+                const int32_t idx = tid + h * kRecThreads;
+                *reinterpret_cast<float4*>(sA + (idx / (kMmK / 4)) * kMmAS + 4 * (idx % (kMmK / 4))) = ra[h];
+            }
 #pragma unroll
-            for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[s], x1[s], acc, 0, 0, 0);
-            k0 = k2;
+            for (int h = 0; h < kMmX4; ++h) {
+                const int32_t idx = tid + h * kRecThreads;
+                float* q = sX + (idx / (kMmCols / 4)) * kMmXS + 4 * (idx % (kMmCols / 4));
+                q[0] = rx[h].x;
+                q[1] = rx[h].y;
+                q[2] = rx[h].z;
+                q[3] = rx[h].w;
+            }
+        };
+        fetch(0);
+        for (int32_t k0 = 0; k0 < n; k0 += kMmK) {
+            if (k0 > 0) __syncthreads();  // the previous chunk's LDS reads are done
+            stage();
+            __syncthreads();
+            if (k0 + kMmK < n) fetch(k0 + kMmK);  // in flight while this chunk is reduced
+            const int32_t kw = wave * kMmKw + 16 * kb;
+            float am[16];
+            {
+                const float4* q = reinterpret_cast<const float4*>(sA + r * kMmAS + kw);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float4 v = q[h];
+                    am[4 * h] = v.x;
+                    am[4 * h + 1] = v.y;
+                    am[4 * h + 2] = v.z;
+                    am[4 * h + 3] = v.w;
+                }
+            }
+            float xv[16][2];  // every LDS read of the slice is issued before the first MFMA
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const float* xr = sX + (kw + s) * kMmXS + r;
+                xv[s][0] = xr[0];
+                xv[s][1] = xr[16];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s], xv[s][0], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s], xv[s][1], acc[1], 0, 0, 0);
+            }
         }
+        __syncthreads();  // the chunk LDS is reused for the partial tiles
     }
-    // lane holds rows 4 kb .. 4 kb + 3 of column r of this wave's partial tile
+    // lane holds rows 4 kb .. 4 kb + 3 of columns r and 16 + r of this wave's partial tile
+    float* red = s_mm;  // [kRecWaves][kMmRows][kMmCols]
 #pragma unroll
-    for (int e = 0; e < 4; ++e) red[wave][(4 * kb + e) * 16 + r] = acc[e];
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) red[(wave * kMmRows + 4 * kb + e) * kMmCols + 16 * ct + r] = acc[ct][e];
     __syncthreads();
-    if (tid >= 256) return;
-    const int32_t row = tid >> 4, col = tid & 15;
+    const int32_t row = tid / kMmCols, col = tid % kMmCols;
     const int32_t i = i0 + row, b = b0 + col;
     if (i >= n || b >= B) return;
-    float prod = red[0][tid];
+    float prod = red[tid];
 #pragma unroll
-    for (int w = 1; w < kRecWaves; ++w) prod += red[w][tid];  // fixed order: recomputation is bit-identical
+    for (int w = 1; w < kRecWaves; ++w) prod += red[w * kMmRows * kMmCols + tid];  // fixed order
     const int64_t j = (int64_t)i * B + b;
     if (!BWD) {  // the forward step of nbx_lif_fwd_kernel on I = IN_t + W_rec s_{t-1}
         const float I = a.add[j] + prod;
@@ -222,11 +289,26 @@ __global__ __launch_bounds__(kRecThreads) void nbx_lif_rec_kernel(RecArgs a) {
     }
 }
 
-hipError_t launch_rec(bool bwd, const RecArgs& a, hipStream_t st) {
-    const int32_t tr = (a.p.n + 15) / 16;
-    const int32_t G = tr * a.tiles_c;
-    if (bwd) nbx_lif_rec_kernel<true><<<G, kRecThreads, 0, st>>>(a);
-    else nbx_lif_rec_kernel<false><<<G, kRecThreads, 0, st>>>(a);
+hipError_t launch_rec(bool bwd, RecArgs a, hipStream_t st) {
+    static bool lds_set = false;
+    if (!lds_set) {  // more than the default 64 KB of dynamic LDS
+        for (const void* f : {(const void*)nbx_lif_rec_kernel<true, true>, (const void*)nbx_lif_rec_kernel<false, true>,
+                              (const void*)nbx_lif_rec_kernel<true, false>,
+                              (const void*)nbx_lif_rec_kernel<false, false>}) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMmLds);
+            if (e != hipSuccess) return e;
+        }
+        lds_set = true;
+    }
+    a.tiles_c = (a.p.batch + kMmCols - 1) / kMmCols;
+    const int32_t G = (a.p.n + kMmRows - 1) / kMmRows * a.tiles_c;
+    // 16-byte loads need 16-byte aligned rows of both operands
+    const bool al = (a.p.n % 4) == 0 && (a.p.batch % 4) == 0 && (reinterpret_cast<uintptr_t>(a.M) & 15u) == 0 &&
+                    (!a.X || (reinterpret_cast<uintptr_t>(a.X) & 15u) == 0);
+    if (bwd && al) nbx_lif_rec_kernel<true, true><<<G, kRecThreads, kMmLds, st>>>(a);
+    else if (bwd) nbx_lif_rec_kernel<true, false><<<G, kRecThreads, kMmLds, st>>>(a);
+    else if (al) nbx_lif_rec_kernel<false, true><<<G, kRecThreads, kMmLds, st>>>(a);
+    else nbx_lif_rec_kernel<false, false><<<G, kRecThreads, kMmLds, st>>>(a);
     return hipGetLastError();
 }
 
@@ -410,7 +492,6 @@ int32_t nbx_lif_fwd_segment(const nbx_lif_adj_params* p, const float* d_W, const
     RecArgs a{};
     a.p = *p;
     a.M = d_W;
-    a.tiles_c = (p->batch + 15) / 16;
     for (int32_t k = 0; k < nsteps; ++k) {
         a.X = k == 0 ? d_s_init : d_S + (k - 1) * nb;
         a.add = d_IN + k * nb;
@@ -470,7 +551,6 @@ int32_t nbx_lif_bwd_segment(const nbx_lif_adj_params* p, const float* d_WT, cons
     RecArgs a{};
     a.p = *p;
     a.M = d_WT;
-    a.tiles_c = (p->batch + 15) / 16;
     for (int32_t k = nsteps - 1; k >= 0; --k) {
         a.X = k == nsteps - 1 ? d_dI_after : d_dI + (k + 1) * nb;
         a.add = d_R ? d_R + k * nb : nullptr;

@@ -83,10 +83,11 @@ def test_full_size_configuration(gpu):
     print(f"full size: relative gradient errors {errs}, spike decisions differing {flips}, rate {rate * 1000:.1f} Hz")
 
 
-@pytest.mark.parametrize("n,b", [(2048, 64), (100, 5), (96, 4), (300, 33)])
+@pytest.mark.parametrize("n,b", [(2048, 64), (100, 5), (96, 4), (300, 33), (2050, 37), (1030, 64)])
 def test_fused_recurrent_steps_against_fp64(gpu, n, b):
     # the fused step kernels (W s on FP32 matrix cores + the step) against the same step in FP64,
-    # at the configuration's shape and at shapes that exercise partial tiles
+    # at the configuration's shape and at shapes that exercise partial tiles, a partial last
+    # reduction chunk and unaligned rows
     import ctypes as C
     from neuronbridge_amd import _lib as L
     rng = np.random.default_rng(n + b)
