@@ -328,6 +328,7 @@ constexpr int kSpNeurons = 64;                   // neurons per workgroup (one p
 constexpr int kSpThreads = 64 * kSpTrials;
 constexpr int kSpWords = 64;                     // spike words listed per pass (one per lane)
 constexpr int kSpListCap = kSpWords * 32;        // list entries per wave and pass
+constexpr int kSpRows = 32;                      // rows of W_rec^T loaded together per lane
 
 struct SpArgs {
     nbx_lif_adj_params p;
@@ -376,24 +377,18 @@ __global__ __launch_bounds__(kSpThreads) void nbx_lif_spk_kernel(SpArgs a) {
         int32_t pos = incl - c;
         for (uint32_t m = word; m; m &= m - 1u) list[pos++] = wi * 32 + __builtin_ctz(m);
         __syncthreads();
-        if (i < n) {
+        if (i < n && total > 0) {
+            // kSpRows rows per pass, all loads in flight together (a typical step's ~13 rows per
+            // trial take one round trip): entries past the list re-read its last row, and their
+            // values are not added, so the sum keeps ascending order exactly
             const float* col = a.WT + i;
-            int32_t k = 0;
-            for (; k + 16 <= total; k += 16) {
-                float w[16];
+            for (int32_t k = 0; k < total; k += kSpRows) {
+                float w[kSpRows];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) w[u] = col[(int64_t)list[k + u] * n];
+                for (int u = 0; u < kSpRows; ++u) w[u] = col[(int64_t)list[min(k + u, total - 1)] * n];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) acc += w[u];
+                for (int u = 0; u < kSpRows; ++u) acc = (k + u < total) ? acc + w[u] : acc;
             }
-            for (; k + 4 <= total; k += 4) {
-                float w[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) w[u] = col[(int64_t)list[k + u] * n];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) acc += w[u];
-            }
-            for (; k < total; ++k) acc += col[(int64_t)list[k] * n];
         }
         __syncthreads();  // the list is read before the next pass overwrites it
     }
