@@ -327,6 +327,11 @@ int32_t check_device_error(nbx_net* net) {
         set_error("device interface values could overflow the int32 arrival sums; those values were dropped");
         return NBX_ERR_STATE;
     }
+    if (e & kErrResidency) {
+        set_error("persistent engine: its workgroups could not all be resident at once (other work held compute "
+                  "units); the run took no step on the device; nbx_net_reset_state clears the error");
+        return NBX_ERR_STATE;
+    }
     if (e & kErrRaster) {  // an argument error of one replay call: reported once
         NBX_PASS(clear_raster_error(net));
         set_error("replay STDP: invalid raster (a spike id out of range, a neuron listed twice in one step, "
@@ -1528,6 +1533,7 @@ int32_t nbx_net_reset_state(nbx_net* net) {
     if (net->d_fmask) NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(net->d_fmask), 0, 2 * (int64_t)n, st));
     NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(net->d_events), 0, 2, st));
     NBX_HIP_TRY(launch_fill_i32(net->d_err, 0, 1, st));
+    net->pl.t_next = -1;  // the persistent engine's counter banks are cleared before its next launch
     if (!net->rules.empty()) {
         NBX_HIP_TRY(launch_fill_i32(net->d_tstamp, -1, net->S, st));
         NBX_HIP_TRY(launch_fill_i32(net->d_tpost_stamp, -1, net->n, st));

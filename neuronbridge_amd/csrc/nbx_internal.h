@@ -123,6 +123,7 @@ struct RuleDev {
 };
 // Device error codes (nbx_net::d_err; the batched engine raises kErrExchange, replay STDP kErrRaster)
 constexpr int32_t kErrBarrier = 1, kErrHistory = 2, kErrExchange = 4, kErrRaster = 8, kErrInput = 16;
+constexpr int32_t kErrResidency = 32;  // a persistent launch's workgroups were not all resident: no step taken
 
 // One external input of a step: an arrival of synapse type kind (0..2) or (kInputCurrent) a new
 // I_ext; value is the applied value, or the scale of d_values[src] when src >= 0.

@@ -96,6 +96,18 @@ __device__ inline int32_t block_of(int32_t w, int32_t B) {
 constexpr int kCtrLine = 16;  // uint64 per 128-byte line
 constexpr int kGroups = 16;   // counter groups: workgroup w belongs to group w % kGroups
 
+// Residency agreement. The step exchange waits on every workgroup, so all B must be resident at
+// once; the kernel is launched as an ordinary dispatch (a cooperative launch, which guarantees
+// residency, costs ~12 µs more per launch) and the workgroups agree on residency before any of
+// them writes global state. Each adds 1 to its launch's start counter (zero at launch: it sits
+// in the counter bank) and waits until the counter reads B. A workgroup that has waited
+// start_spins polls replaces the value v < B it last read with v + kStartPoison by a
+// compare-and-swap, after which B can never be read: every workgroup then leaves without taking a
+// step, and kErrResidency is raised. Once the counter reads B no compare-and-swap can succeed (they
+// expect a value below B), so every workgroup reaches the same decision. A launch finding
+// kErrResidency already raised takes no step either.
+constexpr unsigned long long kStartPoison = 1ull << 40;
+
 struct PersistArgs {
     StepArgs A;
     const int32_t* off;
@@ -105,6 +117,9 @@ struct PersistArgs {
     int32_t t0, nsteps;
     unsigned long long* gctr;   // [D][kGroups][kCtrLine] group counters {spikes << 32 | arrivals}, this launch's bank
     unsigned long long* gctr_next;  // the other bank, cleared by this launch for the next one
+    unsigned long long* gstart;     // this bank's start counter (residency agreement)
+    unsigned long long start_want;  // the workgroups that must arrive: B
+    uint32_t start_spins;           // polls before a workgroup gives the agreement up
     unsigned long long* glist;  // [D][kGroups][gcap] group spike lists {tag << 32 | id}
     int32_t gcap;               // list capacity: the neurons of one group
     uint32_t epoch;             // tag of step t0 - 1 (step t0 + k carries epoch + 1 + k)
@@ -230,6 +245,15 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     const int32_t gl = tid % kLanes, grp = tid / kLanes, ngrp = T / kLanes;
     const int32_t i0 = (b / X.rb) * X.gs + b % X.rb;  // this block's entry in an offset row; i0 + 1 ends it
 
+    // residency agreement (above): arrive now, decide after the ring load
+    __shared__ int32_t s_go;
+    bool stale = false;  // thread 0: kErrResidency was raised before this launch
+    unsigned long long arrived = 0;
+    if (tid == 0) {
+        stale = (ld_dev(A.err) & kErrResidency) != 0;
+        if (!stale)
+            arrived = __hip_atomic_fetch_add(X.gstart, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1ull;
+    }
     // load this block's slice of the delay ring (rows of P neurons, many loads in flight), the
     // Poisson thresholds and its neurons' state
     {
@@ -263,10 +287,41 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     int32_t* s_prev = s_ring + D * nch * P;
     int32_t* s_prev_old = s_prev + D * kGroups;
     for (int32_t j = tid; j < 2 * D * kGroups; j += T) s_prev[j] = 0;
+    if (tid == 0) {
+        int32_t go = 0;
+        if (!stale) {
+            unsigned long long v = arrived;
+            uint32_t spins = 0;
+            for (;;) {
+                if (v == X.start_want) {
+                    go = 1;
+                    break;
+                }
+                if (v >= kStartPoison) break;
+                if (++spins > X.start_spins) {  // give up unless everyone has arrived meanwhile
+                    unsigned long long seen = v;
+                    if (__hip_atomic_compare_exchange_strong(X.gstart, &seen, v + kStartPoison, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        atomicOr(A.err, kErrResidency);
+                        break;
+                    }
+                    v = seen;  // the counter moved: decide on its new value
+                    continue;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                v = ld_dev64(X.gstart);
+            }
+        }
+        s_go = go;
+    }
+    __syncthreads();
+    if (!s_go) return;  // uniform; nothing global was written
     // the next launch's bank: nobody reads it during this launch; the kernel boundary orders these
     // stores before the next launch's atomics
-    if (b == 0)
+    if (b == 0) {
         for (int32_t j = tid; j < D * kGroups; j += T) X.gctr_next[(int64_t)j * kCtrLine] = 0ull;
+        if (tid == 0) X.gctr_next[(int64_t)D * kGroups * kCtrLine] = 0ull;  // its start counter
+    }
     float V[NPT], S1[NPT], GA[NPT], GG[NPT], I[NPT];
     int32_t last[NPT];
 #pragma unroll
@@ -1436,7 +1491,7 @@ int32_t build_persistent_layout(nbx_net* net) {
     L.gcap = (B + kGroups - 1) / kGroups * P;  // the neurons of one group
     rc = net->alloc(&L.d_glist, (size_t)net->D * kGroups * L.gcap);
     if (rc) return rc;
-    rc = net->alloc(&L.d_gctr, (size_t)2 * net->D * kGroups * kCtrLine);  // two banks
+    rc = net->alloc(&L.d_gctr, (size_t)2 * (net->D * kGroups + 1) * kCtrLine);  // two banks, each with its start line
     if (rc) return rc;
     L.t_next = -1;  // the counters are cleared before the first launch
     L.epoch = 0;
@@ -1622,6 +1677,11 @@ hipError_t launch_coresident(const void* fn, dim3 grid, dim3 block, void** args,
     return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
 }
 
+static int32_t env_int(const char* name, int32_t def) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : def;
+}
+
 int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     if (net->has_dense && !net->has_sparse) return run_small_dense(net, nsteps);
     int32_t rc = build_persistent_layout(net);
@@ -1656,7 +1716,8 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     // so it holds after a persistent launch; the first launch clears both. List entries carry
     // unique tags, so only a tag wrap needs them cleared
     const bool wrap = (uint64_t)L.epoch + (uint64_t)nsteps + 2u >= ((uint64_t)1 << 32);
-    const int64_t bank_words = (int64_t)net->D * kGroups * kCtrLine;  // uint64 per bank
+    // uint64 per bank: the step counters, then one line holding the start counter
+    const int64_t bank_words = ((int64_t)net->D * kGroups + 1) * kCtrLine;
     if (L.t_next < 0 || wrap)
         NBX_HIP_TRY(launch_fill_i32(reinterpret_cast<int32_t*>(L.d_gctr), 0, 2 * 2 * bank_words, net->stream));
     if (wrap) {
@@ -1666,13 +1727,20 @@ int32_t run_persistent(nbx_net* net, int32_t nsteps) {
     }
     X.gctr = L.d_gctr + L.bank * bank_words;
     X.gctr_next = L.d_gctr + (1 - L.bank) * bank_words;
+    X.gstart = X.gctr + (int64_t)net->D * kGroups * kCtrLine;
+    // NBX_PERSISTENT_TEST_ABSENT=k (tests of the failure path only) makes the launch wait for k
+    // workgroups that do not exist, with a short wait: the agreement fails as it would if k
+    // workgroups could not be resident
+    const int32_t absent = std::max(0, env_int("NBX_PERSISTENT_TEST_ABSENT", 0));
+    X.start_want = (unsigned long long)L.B + (unsigned long long)absent;
+    X.start_spins = absent ? 256u : (1u << 16);
     X.glist = L.d_glist;
     X.gcap = L.gcap;
     X.epoch = L.epoch;
     void* kargs[] = {&X};
-    // Co-residency of all L.B workgroups is checked above (occupancy x CUs); the cooperative
-    // launch additionally has the runtime guarantee it, which the step counters rely on.
-    NBX_HIP_TRY(launch_coresident(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
+    // An ordinary dispatch: co-residency of all L.B workgroups is possible (occupancy x CUs,
+    // checked above) and the workgroups agree on it before taking a step (residency agreement).
+    NBX_HIP_TRY(hipLaunchKernel(fn, dim3(L.B), dim3(L.threads), kargs, (size_t)L.lds_bytes, net->stream));
     net->t += nsteps;
     L.t_next = net->t;
     L.bank = 1 - L.bank;

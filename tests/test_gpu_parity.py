@@ -136,6 +136,35 @@ def test_persistent_burst_steps_bitexact(gpu, integrator):
     net.close()
 
 
+def test_persistent_residency_failure_takes_no_step(gpu, monkeypatch):
+    # a launch whose workgroups cannot all be resident (simulated from the host: the agreement waits
+    # for one workgroup more than the grid has, with a short wait) takes no step and reports an
+    # error; after a reset the engine runs bit-exact again
+    wl = small_sparse(L.LIF_VJ, L.RK4)
+    net = wl.make_net()
+    x_init, l_init = net.get_state()
+    net.run(40, L.ENGINE_PERSISTENT)
+    x0, l0 = net.get_state()
+    monkeypatch.setenv("NBX_PERSISTENT_TEST_ABSENT", "1")
+    with pytest.raises(Exception, match="resident"):
+        net.run(10, L.ENGINE_PERSISTENT)
+    x1, l1 = net.get_state()
+    assert_bits_equal(x1, x0, "state after the failed launch")
+    assert_bits_equal(l1, l0, "LastSpike after the failed launch")
+    monkeypatch.delenv("NBX_PERSISTENT_TEST_ABSENT")
+    with pytest.raises(Exception, match="resident"):  # the error stays until a reset
+        net.run(5, L.ENGINE_PERSISTENT)
+    net.reset_state()
+    net.set_state(x_init, l_init)
+    T = 300
+    net.record(0, T, counts=True, raster=True)
+    net.run(T, L.ENGINE_PERSISTENT)
+    run = OracleRun(wl, fixed_shift=net.fixed_shift)
+    counts_o, raster_o = run.run(T)
+    compare(net, run, wl, counts_o, raster_o)
+    net.close()
+
+
 def assert_history(net, raster_o, n):
     """Every step the firing-history ring still holds lists exactly the raster's spikes."""
     slots = net.ring().shape[0]
