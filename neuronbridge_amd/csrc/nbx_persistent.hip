@@ -523,12 +523,14 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
     }
     if (ev) atomicAdd(&s_ev, ev);
     // The firing-history ring for the steps it still holds (the launch's last D - 1), from the group
-    // lists: workgroup j takes the step t_end - 1 - j (every workgroup saw every step complete, and
-    // an entry not yet seen with its tag is waited for). Each history entry is written by one
-    // workgroup.
+    // lists: the workgroups share them out, B / (D - 1) per step, each taking every (B / (D - 1))-th
+    // chunk of the step's list (every workgroup saw every step complete, and an entry not yet seen
+    // with its tag is waited for). Each history entry is written by one workgroup.
     if (!aborted) {
         const int32_t nh = min(X.nsteps, D - 1);
-        for (int32_t j = b; j < nh; j += X.B) {
+        const int32_t parts = max(1, X.B / max(nh, 1));  // workgroups sharing one step's list
+        for (int32_t item = b; item < nh * parts; item += X.B) {
+            const int32_t j = item % nh, part = item / nh;
             const int32_t s = X.t0 + X.nsteps - 1 - j, sl = s % D;
             const uint32_t tag = X.epoch + (uint32_t)(X.nsteps - j);
             __syncthreads();  // s_gstart of the previous step is read
@@ -548,13 +550,13 @@ __global__ __launch_bounds__(kMaxThreads) void nbx_persistent(PersistArgs X) {
             int32_t gs[kGroups];
 #pragma unroll
             for (int q = 0; q < kGroups; ++q) gs[q] = s_gstart[q];
-            for (int32_t f = tid; f < cnt; f += T) {
+            for (int32_t f = part * T + tid; f < cnt; f += parts * T) {
                 const unsigned long long* sp[1] = {spike_ptr(X, sl, gs, f)};
                 unsigned long long x[1] = {ld_dev64(sp[0])};
                 await_tags(sp, tag, x, A.err);
                 A.hist[(int64_t)sl * n + f] = (int32_t)(uint32_t)x[0];
             }
-            if (tid == 0) A.hist_cnt[sl] = cnt;
+            if (part == 0 && tid == 0) A.hist_cnt[sl] = cnt;
         }
         // the next step's updates start counting from zero (the slot held step t_end - D)
         if (b == 0 && tid == 0) A.hist_cnt[(X.t0 + X.nsteps) % D] = 0;
